@@ -1,0 +1,91 @@
+/*
+ * CPU oracle for the vkzg hot path -- TEST INFRASTRUCTURE ONLY.
+ *
+ * A plain-C restatement of what the reference computes for its two batched
+ * entry points (FK20 cell proofs and cell recovery), used by tests/, by the
+ * smoke check and by bench.py's CPU-baseline leg as the thing the GPU path is
+ * checked against.  Nothing in the shipped library links or calls this code.
+ *
+ * Arithmetic is exact: Fp/Fr in Montgomery form with 64-bit limbs and
+ * unsigned __int128 products.  Byte formats are the ones of the reference's
+ * public header (include/vkzg.h:32-38): 32-byte big-endian field elements,
+ * 48-byte ZCash-compressed G1 points.
+ */
+#ifndef VKZG_ORACLE_H
+#define VKZG_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Opaque FK20 setup: decompressed SRS, x_ext_fft and the phase-A position table. */
+typedef struct or_setup or_setup;
+
+/* ---- field element byte helpers (exposed for unit tests) ---- */
+/* r (the BLS12-381 group order) and p (the base field prime) as 32/48 big-endian bytes. */
+void or_modulus_r(uint8_t out[32]);
+void or_modulus_p(uint8_t out[48]);
+/* out = a*b mod r for 32-byte big-endian canonical inputs (Montgomery round trip). */
+void or_fr_mul_be(uint8_t out[32], const uint8_t a[32], const uint8_t b[32]);
+/* out = a*b mod p for 48-byte big-endian canonical inputs. */
+void or_fp_mul_be(uint8_t out[48], const uint8_t a[48], const uint8_t b[48]);
+/* out = a^-1 mod r (0 -> 0). */
+void or_fr_inv_be(uint8_t out[32], const uint8_t a[32]);
+/* k-th power of the primitive 8192-th root of unity 7^((r-1)/8192), big-endian. */
+void or_root_8192_be(uint8_t out[32], uint32_t k);
+
+/* ---- G1 helpers ---- */
+/* Decompress then recompress (returns 0 on success, -1 on invalid encoding). */
+int or_g1_roundtrip(uint8_t out[48], const uint8_t in[48]);
+/* out = compress(k * G1) for a 32-byte big-endian scalar (any 256-bit value). */
+void or_g1_mul_gen(uint8_t out[48], const uint8_t k[32]);
+/* out = compress(k * P) for a compressed point P. Returns -1 on a bad encoding. */
+int or_g1_mul(uint8_t out[48], const uint8_t p[48], const uint8_t k[32]);
+/* out = compress(P + Q). */
+int or_g1_add(uint8_t out[48], const uint8_t p[48], const uint8_t q[48]);
+
+/* ---- polynomial / cells ---- */
+/* Blob (4096 x 32-byte BE) -> monomial coefficients (4096 x 32-byte BE canonical).
+ * Returns 0, or -1 when an element is not canonical (>= r). */
+int or_blob_to_poly(uint8_t coeffs[4096 * 32], const uint8_t blob[131072]);
+/* Blob -> 128 cells (128 x 64 x 32 bytes), the 2x Reed-Solomon extension. -1 if non-canonical. */
+int or_blob_to_cells(uint8_t cells[128 * 2048], const uint8_t blob[131072]);
+/* Forward (inverse=0) or inverse (inverse=1, scaled by 1/n) NTT over Fr of size n | 8192,
+ * natural order in and out, on 32-byte BE canonical elements (reduced mod r on input). */
+void or_fr_ntt(uint8_t *data, uint32_t n, int inverse);
+
+/* ---- recovery ---- */
+/* Recover all 128 cells of `num_blobs` blobs.  Same buffer contract as
+ * vkzg_recover_cells_batch.  Returns 0, or 1 (BADARGS) when a blob has fewer
+ * than 64 present cells. */
+int or_recover_cells(uint8_t *cells_out, const uint8_t *cells, const uint8_t *present,
+                     size_t num_blobs);
+
+/* ---- FK20 ---- */
+/* Build the FK20 setup from 4096 compressed G1 monomial SRS points.  NULL on a bad point. */
+or_setup *or_setup_new(const uint8_t *g1_monomial_compressed /* 4096*48 */);
+void or_setup_free(or_setup *s);
+/* 128 compressed cell proofs per blob.  Returns 0, or 6 (INVALID_BLOB) when a
+ * blob element is not canonical (then no proofs of that blob are written). */
+int or_compute_proofs(const or_setup *s, uint8_t *proofs, const uint8_t *blobs, size_t num_blobs);
+/* The x_ext_fft point for (j, i) as compressed bytes (for table-layout tests). */
+void or_setup_xext(const or_setup *s, uint32_t j, uint32_t i, uint8_t out[48]);
+
+/* ---- seeded-secret setups and the closed form ---- */
+/* 4096 compressed G1 points [tau^i]G1 for a 32-byte BE tau. */
+void or_setup_g1_from_tau(uint8_t *out /* 4096*48 */, const uint8_t tau[32]);
+/* Proofs by definition: proof[c] = [q_c(tau)]G1 with q_c = p div (X^64 - w^(64*brev7(c))).
+ * Returns 0 or 6 like or_compute_proofs. */
+int or_proofs_from_tau(uint8_t *proofs, const uint8_t *blob, const uint8_t tau[32]);
+
+/* ---- reference-style direct H_m form (independent FK20 cross-check) ---- */
+/* h[m-1] = sum_k c_{k+64m} s_k for m=1..63, then proofs = brp7(FFT_128(h)).  Slow. */
+int or_compute_proofs_direct(const or_setup *s, uint8_t *proofs, const uint8_t *blob);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

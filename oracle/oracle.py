@@ -1,0 +1,176 @@
+"""ctypes wrapper around oracle/_build/liboracle.so (test infrastructure only).
+
+Each function mirrors a C entry point of oracle/oracle.h; byte buffers in and
+out use the reference's public formats (include/vkzg.h:32-38).
+"""
+import ctypes
+import os
+import subprocess
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+BYTES_PER_BLOB = 131072
+BYTES_PER_CELL = 2048
+CELLS_PER_BLOB = 128
+BYTES_PER_PROOF = 48
+PROOF_BYTES_PER_BLOB = CELLS_PER_BLOB * BYTES_PER_PROOF
+CELL_BYTES_PER_BLOB = CELLS_PER_BLOB * BYTES_PER_CELL
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        c = ctypes.c_char_p
+        u8p = ctypes.c_char_p
+        L.or_setup_new.restype = ctypes.c_void_p
+        L.or_setup_new.argtypes = [u8p]
+        L.or_setup_free.argtypes = [ctypes.c_void_p]
+        L.or_compute_proofs.argtypes = [ctypes.c_void_p, c, u8p, ctypes.c_size_t]
+        L.or_compute_proofs_direct.argtypes = [ctypes.c_void_p, c, u8p]
+        L.or_setup_xext.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, c]
+        L.or_recover_cells.argtypes = [c, u8p, u8p, ctypes.c_size_t]
+        L.or_blob_to_cells.argtypes = [c, u8p]
+        L.or_blob_to_poly.argtypes = [c, u8p]
+        L.or_fr_ntt.argtypes = [c, ctypes.c_uint32, ctypes.c_int]
+        L.or_setup_g1_from_tau.argtypes = [c, u8p]
+        L.or_proofs_from_tau.argtypes = [c, u8p, u8p]
+        L.or_g1_roundtrip.argtypes = [c, u8p]
+        L.or_g1_mul_gen.argtypes = [c, u8p]
+        L.or_g1_mul.argtypes = [c, u8p, u8p]
+        L.or_g1_add.argtypes = [c, u8p, u8p]
+        L.or_fr_mul_be.argtypes = [c, u8p, u8p]
+        L.or_fp_mul_be.argtypes = [c, u8p, u8p]
+        L.or_fr_inv_be.argtypes = [c, u8p]
+        L.or_root_8192_be.argtypes = [c, ctypes.c_uint32]
+        _lib = L
+    return _lib
+
+
+R = 0x73EDA753299D7D483339D80809A1D80553BDA402FFFE5BFEFFFFFFFF00000001
+P = 0x1A0111EA397FE69A4B1BA7B6434BACD764774B84F38512BF6730D2A0F6B0F6241EABFFFEB153FFFFB9FEFFFFFFFFAAAB
+
+
+class Setup:
+    """FK20 setup built from 4096 compressed G1 monomial points."""
+
+    def __init__(self, g1_monomial: bytes):
+        assert len(g1_monomial) == 4096 * 48
+        self._h = lib().or_setup_new(g1_monomial)
+        if not self._h:
+            raise ValueError("invalid setup point")
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().or_setup_free(self._h)
+            self._h = None
+
+    def compute_proofs(self, blobs: bytes, n: int):
+        """Returns (rc, proofs); rc 0 = ok, 6 = invalid blob."""
+        out = ctypes.create_string_buffer(PROOF_BYTES_PER_BLOB * n)
+        rc = lib().or_compute_proofs(self._h, out, blobs, n)
+        return rc, out.raw
+
+    def compute_proofs_direct(self, blob: bytes):
+        out = ctypes.create_string_buffer(PROOF_BYTES_PER_BLOB)
+        rc = lib().or_compute_proofs_direct(self._h, out, blob)
+        return rc, out.raw
+
+    def xext(self, j: int, i: int) -> bytes:
+        out = ctypes.create_string_buffer(48)
+        lib().or_setup_xext(self._h, j, i, out)
+        return out.raw
+
+
+def recover_cells(cells: bytes, present: bytes, n: int):
+    out = ctypes.create_string_buffer(CELL_BYTES_PER_BLOB * n)
+    rc = lib().or_recover_cells(out, cells, present, n)
+    return rc, out.raw
+
+
+def blob_to_cells(blob: bytes):
+    out = ctypes.create_string_buffer(CELL_BYTES_PER_BLOB)
+    rc = lib().or_blob_to_cells(out, blob)
+    return rc, out.raw
+
+
+def blob_to_poly(blob: bytes):
+    out = ctypes.create_string_buffer(4096 * 32)
+    rc = lib().or_blob_to_poly(out, blob)
+    return rc, out.raw
+
+
+def fr_ntt(data: bytes, n: int, inverse: bool) -> bytes:
+    buf = ctypes.create_string_buffer(bytes(data), len(data))
+    lib().or_fr_ntt(buf, n, 1 if inverse else 0)
+    return buf.raw[: len(data)]
+
+
+def setup_g1_from_tau(tau: int) -> bytes:
+    out = ctypes.create_string_buffer(4096 * 48)
+    lib().or_setup_g1_from_tau(out, tau.to_bytes(32, "big"))
+    return out.raw
+
+
+def proofs_from_tau(blob: bytes, tau: int):
+    out = ctypes.create_string_buffer(PROOF_BYTES_PER_BLOB)
+    rc = lib().or_proofs_from_tau(out, blob, tau.to_bytes(32, "big"))
+    return rc, out.raw
+
+
+def g1_roundtrip(p: bytes):
+    out = ctypes.create_string_buffer(48)
+    rc = lib().or_g1_roundtrip(out, p)
+    return rc, out.raw
+
+
+def g1_mul_gen(k: int) -> bytes:
+    out = ctypes.create_string_buffer(48)
+    lib().or_g1_mul_gen(out, (k % (1 << 256)).to_bytes(32, "big"))
+    return out.raw
+
+
+def g1_mul(p: bytes, k: int) -> bytes:
+    out = ctypes.create_string_buffer(48)
+    if lib().or_g1_mul(out, p, (k % (1 << 256)).to_bytes(32, "big")):
+        raise ValueError("bad point")
+    return out.raw
+
+
+def g1_add(p: bytes, q: bytes) -> bytes:
+    out = ctypes.create_string_buffer(48)
+    if lib().or_g1_add(out, p, q):
+        raise ValueError("bad point")
+    return out.raw
+
+
+def fr_mul(a: int, b: int) -> int:
+    out = ctypes.create_string_buffer(32)
+    lib().or_fr_mul_be(out, a.to_bytes(32, "big"), b.to_bytes(32, "big"))
+    return int.from_bytes(out.raw, "big")
+
+
+def fp_mul(a: int, b: int) -> int:
+    out = ctypes.create_string_buffer(48)
+    lib().or_fp_mul_be(out, a.to_bytes(48, "big"), b.to_bytes(48, "big"))
+    return int.from_bytes(out.raw, "big")
+
+
+def fr_inv(a: int) -> int:
+    out = ctypes.create_string_buffer(32)
+    lib().or_fr_inv_be(out, a.to_bytes(32, "big"))
+    return int.from_bytes(out.raw, "big")
+
+
+def root_8192(k: int) -> int:
+    out = ctypes.create_string_buffer(32)
+    lib().or_root_8192_be(out, k)
+    return int.from_bytes(out.raw, "big")
