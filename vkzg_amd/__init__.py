@@ -1,0 +1,204 @@
+"""vkzg for AMD Instinct MI355X: EIP-7594 FK20 cell proofs and cell recovery.
+
+The computation runs in libvkzg.so (hand-written gfx950 HIP kernels behind the
+reference's C ABI, include/vkzg.h).  This module is the Python host mirror of
+the reference's Java binding (bindings/java/src/main/java/vkzg/Vkzg.java):
+
+  * module-level ``init`` / ``deinit`` / ``compute_cell_kzg_proofs_batch`` /
+    ``recover_cells_batch`` / ``device_name`` / ``gpu_core_count`` with the
+    same argument checks and errors as Vkzg.java:164-305, and
+  * a ``Prover`` class for callers that want several provers (custom setups,
+    different batch sizes).
+"""
+import ctypes
+import threading
+
+from ._lib import LIB_PATH, VkzgAmdInfo, VkzgOptions, load
+
+FIELD_ELEMENTS_PER_BLOB = 4096
+BYTES_PER_FIELD_ELEMENT = 32
+BYTES_PER_BLOB = FIELD_ELEMENTS_PER_BLOB * BYTES_PER_FIELD_ELEMENT
+CELL_PROOFS_PER_BLOB = 128
+BYTES_PER_PROOF = 48
+FIELD_ELEMENTS_PER_CELL = 64
+BYTES_PER_CELL = FIELD_ELEMENTS_PER_CELL * BYTES_PER_FIELD_ELEMENT
+
+OK, ERR_BADARGS, ERR_MALLOC, ERR_IO, ERR_SETUP, ERR_GPU, ERR_INVALID_BLOB = range(7)
+
+
+class VkzgError(Exception):
+    """A nonzero vkzg_result (mirrors VkzgException: code + vkzg_error_string)."""
+
+    def __init__(self, code):
+        self.code = code
+        super().__init__("vkzg error %d: %s" % (code, error_string(code)))
+
+
+def error_string(code):
+    return load().vkzg_error_string(code).decode()
+
+
+def _check(rc):
+    if rc != OK:
+        raise VkzgError(rc)
+
+
+class Prover:
+    """One vkzg_prover on the current HIP device."""
+
+    def __init__(self, max_batch_size=0, setup_g1_monomial=None):
+        L = load()
+        opts = VkzgOptions()
+        L.vkzg_options_default(ctypes.byref(opts))
+        if max_batch_size:
+            opts.max_batch_size = max_batch_size
+        h = ctypes.c_void_p()
+        if setup_g1_monomial is None:
+            rc = L.vkzg_prover_new(ctypes.byref(h), ctypes.byref(opts))
+        else:
+            rc = L.vkzg_amd_prover_new_with_setup(ctypes.byref(h), ctypes.byref(opts),
+                                                  bytes(setup_g1_monomial),
+                                                  len(setup_g1_monomial))
+        _check(rc)
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().vkzg_prover_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    def device_name(self):
+        return load().vkzg_prover_device_name(self._h).decode()
+
+    def gpu_core_count(self):
+        return int(load().vkzg_prover_gpu_core_count(self._h))
+
+    def info(self):
+        out = VkzgAmdInfo()
+        _check(load().vkzg_amd_prover_info(self._h, ctypes.byref(out)))
+        return {f: getattr(out, f) for f, _ in VkzgAmdInfo._fields_}
+
+    def compute_cell_kzg_proofs_batch(self, blobs, blob_count):
+        if len(blobs) != blob_count * BYTES_PER_BLOB:
+            raise ValueError("blobs.length must be blobCount * %d" % BYTES_PER_BLOB)
+        nbytes = blob_count * CELL_PROOFS_PER_BLOB * BYTES_PER_PROOF
+        out = ctypes.create_string_buffer(max(1, nbytes))
+        src = ctypes.create_string_buffer(bytes(blobs), max(1, len(blobs)))
+        _check(load().vkzg_compute_proofs_batch(self._h, out, src, blob_count))
+        return out.raw[:nbytes]
+
+    def recover_cells_batch(self, cells, cell_present, blob_count):
+        per_blob = CELL_PROOFS_PER_BLOB * BYTES_PER_CELL
+        if len(cells) != blob_count * per_blob:
+            raise ValueError("cells.length must be blobCount * %d" % per_blob)
+        if len(cell_present) != blob_count * CELL_PROOFS_PER_BLOB:
+            raise ValueError("cellPresent.length must be blobCount * %d" % CELL_PROOFS_PER_BLOB)
+        out = ctypes.create_string_buffer(max(1, len(cells)))
+        c = ctypes.create_string_buffer(bytes(cells), max(1, len(cells)))
+        pm = ctypes.create_string_buffer(bytes(cell_present), max(1, len(cell_present)))
+        _check(load().vkzg_recover_cells_batch(self._h, out, c, pm, blob_count))
+        return out.raw[: len(cells)]
+
+    def set_stage_timing(self, enable=True):
+        _check(load().vkzg_amd_set_stage_timing(self._h, 1 if enable else 0))
+
+    def last_stage_times(self):
+        """{stage name: ms} of the last call (HIP events on the kernels' stream)."""
+        ms = (ctypes.c_float * 16)()
+        names = (ctypes.c_char_p * 16)()
+        n = load().vkzg_amd_last_stage_times(self._h, ms, names, 16)
+        return {names[i].decode(): float(ms[i]) for i in range(n)}
+
+    # Device-pointer entry points (device addresses as integers, e.g. from vkzg_amd.hip).
+    def compute_proofs_device(self, d_proofs, d_blobs, blob_count, stream=None):
+        _check(load().vkzg_amd_compute_proofs_device(self._h, d_proofs, d_blobs, blob_count,
+                                                     stream))
+
+    def recover_cells_device(self, d_out, d_cells, d_present, h_present, blob_count, stream=None):
+        pm = ctypes.create_string_buffer(bytes(h_present), max(1, len(h_present)))
+        _check(load().vkzg_amd_recover_cells_device(self._h, d_out, d_cells, d_present, pm,
+                                                    blob_count, stream))
+
+
+# ---- module-level singleton, as in Vkzg.java ----
+_lock = threading.Lock()
+_prover = None
+
+
+def init(max_batch_size=0):
+    global _prover
+    with _lock:
+        if _prover is not None:
+            raise RuntimeError("already initialized")
+        _prover = Prover(max_batch_size)
+
+
+def deinit():
+    global _prover
+    with _lock:
+        _loaded().close()
+        _prover = None
+
+
+def _loaded():
+    if _prover is None:
+        raise RuntimeError("not initialized; call init() first")
+    return _prover
+
+
+def compute_cell_kzg_proofs_batch(blobs, blob_count):
+    with _lock:
+        return _loaded().compute_cell_kzg_proofs_batch(blobs, blob_count)
+
+
+def recover_cells_batch(cells, cell_present, blob_count):
+    with _lock:
+        return _loaded().recover_cells_batch(cells, cell_present, blob_count)
+
+
+def device_name():
+    with _lock:
+        return _loaded().device_name()
+
+
+def gpu_core_count():
+    with _lock:
+        return _loaded().gpu_core_count()
+
+
+def debug_fp_mul(a_limbs, b_limbs, prover=None):
+    """Products of canonical Fp values given as lists of 12-limb tuples (test hook)."""
+    return _debug_mul(a_limbs, b_limbs, 12, prover, load().vkzg_amd_debug_fp_mul)
+
+
+def debug_fr_mul(a_limbs, b_limbs, prover=None):
+    """Products mod r of 256-bit values given as lists of 8-limb tuples (test hook)."""
+    return _debug_mul(a_limbs, b_limbs, 8, prover, load().vkzg_amd_debug_fr_mul)
+
+
+def _debug_mul(a, b, limbs, prover, fn):
+    n = len(a)
+    A = (ctypes.c_uint32 * (n * limbs))(*[x for v in a for x in v])
+    B = (ctypes.c_uint32 * (n * limbs))(*[x for v in b for x in v])
+    C = (ctypes.c_uint32 * (n * limbs))()
+    _check(fn(prover.handle if prover else None, C, A, B, n))
+    return [tuple(C[i * limbs:(i + 1) * limbs]) for i in range(n)]
+
+
+__all__ = [
+    "Prover", "VkzgError", "init", "deinit", "compute_cell_kzg_proofs_batch",
+    "recover_cells_batch", "device_name", "gpu_core_count", "error_string", "LIB_PATH",
+]

@@ -1,0 +1,100 @@
+"""ctypes bindings to libvkzg.so, the gfx950 build of the vkzg C ABI.
+
+The shared library is loaded from this package directory only; when it is
+missing the import fails loudly (there is no CPU fallback).
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libvkzg.so")
+
+# The eight symbols of include/vkzg.h (reference include/vkzg.h:53-121)
+# and the additive ones of include/vkzg_amd.h.
+ABI_SYMBOLS = [
+    "vkzg_error_string",
+    "vkzg_options_default",
+    "vkzg_prover_new",
+    "vkzg_prover_free",
+    "vkzg_prover_device_name",
+    "vkzg_prover_gpu_core_count",
+    "vkzg_compute_proofs_batch",
+    "vkzg_recover_cells_batch",
+]
+EXT_SYMBOLS = [
+    "vkzg_amd_prover_new_with_setup",
+    "vkzg_amd_compute_proofs_device",
+    "vkzg_amd_recover_cells_device",
+    "vkzg_amd_prover_info",
+    "vkzg_amd_set_stage_timing",
+    "vkzg_amd_last_stage_times",
+    "vkzg_amd_debug_fp_mul",
+    "vkzg_amd_debug_fr_mul",
+]
+
+
+class VkzgOptions(ctypes.Structure):
+    _fields_ = [("max_batch_size", ctypes.c_uint32)]
+
+
+class VkzgAmdInfo(ctypes.Structure):
+    _fields_ = [
+        ("window_bits", ctypes.c_uint32),
+        ("windows", ctypes.c_uint32),
+        ("table_bytes", ctypes.c_uint64),
+        ("working_bytes", ctypes.c_uint64),
+        ("max_batch_size", ctypes.c_uint32),
+        ("setup_ms", ctypes.c_double),
+    ]
+
+
+assert ctypes.sizeof(VkzgOptions) == 4
+
+_lib = None
+
+
+def load():
+    """Loads and prototypes libvkzg.so (raises OSError when it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise OSError("libvkzg.so is not built (run python3 vkzg_amd/build.py): " + LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    vp, sz, u8p = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_char_p
+    L.vkzg_error_string.restype = ctypes.c_char_p
+    L.vkzg_error_string.argtypes = [ctypes.c_int]
+    L.vkzg_options_default.restype = None
+    L.vkzg_options_default.argtypes = [ctypes.POINTER(VkzgOptions)]
+    L.vkzg_prover_new.restype = ctypes.c_int
+    L.vkzg_prover_new.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(VkzgOptions)]
+    L.vkzg_prover_free.restype = None
+    L.vkzg_prover_free.argtypes = [vp]
+    L.vkzg_prover_device_name.restype = ctypes.c_char_p
+    L.vkzg_prover_device_name.argtypes = [vp]
+    L.vkzg_prover_gpu_core_count.restype = ctypes.c_uint32
+    L.vkzg_prover_gpu_core_count.argtypes = [vp]
+    L.vkzg_compute_proofs_batch.restype = ctypes.c_int
+    L.vkzg_compute_proofs_batch.argtypes = [vp, vp, vp, sz]
+    L.vkzg_recover_cells_batch.restype = ctypes.c_int
+    L.vkzg_recover_cells_batch.argtypes = [vp, vp, vp, vp, sz]
+    L.vkzg_amd_prover_new_with_setup.restype = ctypes.c_int
+    L.vkzg_amd_prover_new_with_setup.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(VkzgOptions),
+                                                 u8p, sz]
+    L.vkzg_amd_compute_proofs_device.restype = ctypes.c_int
+    L.vkzg_amd_compute_proofs_device.argtypes = [vp, vp, vp, sz, vp]
+    L.vkzg_amd_recover_cells_device.restype = ctypes.c_int
+    L.vkzg_amd_recover_cells_device.argtypes = [vp, vp, vp, vp, vp, sz, vp]
+    L.vkzg_amd_prover_info.restype = ctypes.c_int
+    L.vkzg_amd_prover_info.argtypes = [vp, ctypes.POINTER(VkzgAmdInfo)]
+    L.vkzg_amd_set_stage_timing.restype = ctypes.c_int
+    L.vkzg_amd_set_stage_timing.argtypes = [vp, ctypes.c_int]
+    L.vkzg_amd_last_stage_times.restype = ctypes.c_int
+    L.vkzg_amd_last_stage_times.argtypes = [vp, ctypes.POINTER(ctypes.c_float),
+                                            ctypes.POINTER(ctypes.c_char_p), ctypes.c_int]
+    for name in ("vkzg_amd_debug_fp_mul", "vkzg_amd_debug_fr_mul"):
+        f = getattr(L, name)
+        f.restype = ctypes.c_int
+        f.argtypes = [vp, vp, vp, vp, sz]
+    _lib = L
+    return L

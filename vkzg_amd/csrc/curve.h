@@ -1,0 +1,283 @@
+// BLS12-381 G1 (y^2 = x^3 + 4) point arithmetic for gfx950.
+//
+// Three representations:
+//   G1A  affine (x, y), with (0, 0) standing for the identity (it is not on
+//        the curve), as in the reference (src/shaders/field.glsl.inc:304-320);
+//   G1J  Jacobian (X, Y, Z), x = X/Z^2, y = Y/Z^3, Z = 0 is the identity --
+//        used for doubling chains (dbl-2009-l, 2M+5S);
+//   G1X  extended Jacobian "XYZZ" (X, Y, ZZ, ZZZ), x = X/ZZ, y = Y/ZZZ,
+//        ZZ = 0 is the identity -- used for accumulation (madd-2008-s 8M+2S,
+//        add-2008-s 12M+2S), cheaper than the reference's Jacobian madd
+//        (7M+4S) and add (11M+5S).
+// Every addition handles P == Q (falls back to doubling) and P == -Q.
+#pragma once
+
+#include "field.h"
+
+// Point operations are out-of-line device functions: each contains 8-15
+// inlined field multiplications, and inlining them at every call site made
+// kernels large enough to slow compilation by minutes for no measurable gain.
+#define VKZG_DEV_CALL __device__ __attribute__((noinline))
+
+namespace vkzg_amd {
+
+struct G1A {
+    Fp x, y;
+};
+struct G1J {
+    Fp x, y, z;
+};
+struct G1X {
+    Fp x, y, zz, zzz;
+};
+
+VKZG_DEV G1A g1a_identity() {
+    G1A r;
+    r.x = fp_zero();
+    r.y = fp_zero();
+    return r;
+}
+VKZG_DEV bool g1a_is_identity(const G1A &p) { return fp_is_zero(p.x) && fp_is_zero(p.y); }
+VKZG_DEV G1A g1a_neg_if(const G1A &p, bool neg) {
+    G1A r;
+    r.x = p.x;
+    // The identity (0, 0) stays (0, 0): fp_neg(0) == 0.
+    r.y = fp_cond_neg(p.y, neg);
+    return r;
+}
+
+// ---------------------------------------------------------------- Jacobian
+
+VKZG_DEV G1J g1j_identity() {
+    G1J r;
+    r.x = fp_one();
+    r.y = fp_one();
+    r.z = fp_zero();
+    return r;
+}
+VKZG_DEV bool g1j_is_identity(const G1J &p) { return fp_is_zero(p.z); }
+VKZG_DEV G1J g1j_from_affine(const G1A &a) {
+    if (g1a_is_identity(a)) return g1j_identity();
+    G1J r;
+    r.x = a.x;
+    r.y = a.y;
+    r.z = fp_one();
+    return r;
+}
+
+VKZG_DEV_CALL G1J g1j_dbl(const G1J &p) {
+    if (fp_is_zero(p.z)) return p;
+    const Fp A = fp_sqr(p.x);
+    const Fp B = fp_sqr(p.y);
+    const Fp C = fp_sqr(B);
+    Fp t = fp_add(p.x, B);
+    t = fp_sqr(t);
+    t = fp_sub(t, A);
+    t = fp_sub(t, C);
+    const Fp D = fp_dbl(t);
+    const Fp E = fp_add(fp_dbl(A), A);
+    const Fp F = fp_sqr(E);
+    G1J r;
+    r.x = fp_sub(F, fp_dbl(D));
+    const Fp C8 = fp_dbl(fp_dbl(fp_dbl(C)));
+    r.y = fp_sub(fp_mul(E, fp_sub(D, r.x)), C8);
+    r.z = fp_dbl(fp_mul(p.y, p.z));
+    return r;
+}
+
+// add-2007-bl
+VKZG_DEV_CALL G1J g1j_add(const G1J &a, const G1J &b) {
+    if (fp_is_zero(a.z)) return b;
+    if (fp_is_zero(b.z)) return a;
+    const Fp Z1Z1 = fp_sqr(a.z);
+    const Fp Z2Z2 = fp_sqr(b.z);
+    const Fp U1 = fp_mul(a.x, Z2Z2);
+    const Fp U2 = fp_mul(b.x, Z1Z1);
+    const Fp S1 = fp_mul(a.y, fp_mul(b.z, Z2Z2));
+    const Fp S2 = fp_mul(b.y, fp_mul(a.z, Z1Z1));
+    if (fp_eq(U1, U2)) {
+        if (fp_eq(S1, S2)) return g1j_dbl(a);
+        return g1j_identity();
+    }
+    const Fp H = fp_sub(U2, U1);
+    const Fp I = fp_sqr(fp_dbl(H));
+    const Fp J = fp_mul(H, I);
+    const Fp K = fp_dbl(fp_sub(S2, S1));
+    const Fp V = fp_mul(U1, I);
+    G1J r;
+    r.x = fp_sub(fp_sub(fp_sqr(K), J), fp_dbl(V));
+    r.y = fp_sub(fp_mul(K, fp_sub(V, r.x)), fp_dbl(fp_mul(S1, J)));
+    Fp z = fp_sqr(fp_add(a.z, b.z));
+    z = fp_sub(fp_sub(z, Z1Z1), Z2Z2);
+    r.z = fp_mul(z, H);
+    return r;
+}
+
+// madd-2007-bl (b affine, may be the identity)
+VKZG_DEV_CALL G1J g1j_madd(const G1J &a, const G1A &b) {
+    if (g1a_is_identity(b)) return a;
+    if (fp_is_zero(a.z)) return g1j_from_affine(b);
+    const Fp Z1Z1 = fp_sqr(a.z);
+    const Fp U2 = fp_mul(b.x, Z1Z1);
+    const Fp S2 = fp_mul(b.y, fp_mul(a.z, Z1Z1));
+    if (fp_eq(a.x, U2)) {
+        if (fp_eq(a.y, S2)) return g1j_dbl(a);
+        return g1j_identity();
+    }
+    const Fp H = fp_sub(U2, a.x);
+    const Fp HH = fp_sqr(H);
+    const Fp I = fp_dbl(fp_dbl(HH));
+    const Fp J = fp_mul(H, I);
+    const Fp K = fp_dbl(fp_sub(S2, a.y));
+    const Fp V = fp_mul(a.x, I);
+    G1J r;
+    r.x = fp_sub(fp_sub(fp_sqr(K), J), fp_dbl(V));
+    r.y = fp_sub(fp_mul(K, fp_sub(V, r.x)), fp_dbl(fp_mul(a.y, J)));
+    Fp z = fp_sqr(fp_add(a.z, H));
+    r.z = fp_sub(fp_sub(z, Z1Z1), HH);
+    return r;
+}
+
+VKZG_DEV G1J g1j_neg(const G1J &p) {
+    G1J r = p;
+    r.y = fp_neg(p.y);
+    return r;
+}
+
+// ---------------------------------------------------------------- XYZZ
+
+VKZG_DEV G1X g1x_identity() {
+    G1X r;
+    r.x = fp_one();
+    r.y = fp_one();
+    r.zz = fp_zero();
+    r.zzz = fp_zero();
+    return r;
+}
+VKZG_DEV bool g1x_is_identity(const G1X &p) { return fp_is_zero(p.zz); }
+VKZG_DEV G1X g1x_from_affine(const G1A &a) {
+    if (g1a_is_identity(a)) return g1x_identity();
+    G1X r;
+    r.x = a.x;
+    r.y = a.y;
+    r.zz = fp_one();
+    r.zzz = fp_one();
+    return r;
+}
+VKZG_DEV G1X g1x_from_jacobian(const G1J &p) {
+    if (fp_is_zero(p.z)) return g1x_identity();
+    G1X r;
+    r.x = p.x;
+    r.y = p.y;
+    r.zz = fp_sqr(p.z);
+    r.zzz = fp_mul(r.zz, p.z);
+    return r;
+}
+// (X, Y, ZZ, ZZZ) with ZZ^3 == ZZZ^2 -> Jacobian with Z = ZZ*ZZZ:
+// X_J = X*ZZ^4, Y_J = Y*ZZZ^4 (no inversion).
+VKZG_DEV_CALL G1J g1j_from_xyzz(const G1X &p) {
+    if (fp_is_zero(p.zz)) return g1j_identity();
+    G1J r;
+    const Fp zz2 = fp_sqr(p.zz);
+    const Fp zzz2 = fp_sqr(p.zzz);
+    r.x = fp_mul(p.x, fp_sqr(zz2));
+    r.y = fp_mul(p.y, fp_sqr(zzz2));
+    r.z = fp_mul(p.zz, p.zzz);
+    return r;
+}
+
+// dbl-2008-s-1
+VKZG_DEV_CALL G1X g1x_dbl(const G1X &p) {
+    if (fp_is_zero(p.zz)) return p;
+    const Fp U = fp_dbl(p.y);
+    const Fp V = fp_sqr(U);
+    const Fp W = fp_mul(U, V);
+    const Fp S = fp_mul(p.x, V);
+    const Fp X2 = fp_sqr(p.x);
+    const Fp M = fp_add(fp_dbl(X2), X2);
+    G1X r;
+    r.x = fp_sub(fp_sqr(M), fp_dbl(S));
+    r.y = fp_sub(fp_mul(M, fp_sub(S, r.x)), fp_mul(W, p.y));
+    r.zz = fp_mul(V, p.zz);
+    r.zzz = fp_mul(W, p.zzz);
+    return r;
+}
+
+// madd-2008-s: XYZZ + affine (b may be the identity)
+VKZG_DEV_CALL G1X g1x_madd(const G1X &a, const G1A &b) {
+    if (g1a_is_identity(b)) return a;
+    if (fp_is_zero(a.zz)) return g1x_from_affine(b);
+    const Fp U2 = fp_mul(b.x, a.zz);
+    const Fp S2 = fp_mul(b.y, a.zzz);
+    const Fp P = fp_sub(U2, a.x);
+    const Fp R = fp_sub(S2, a.y);
+    if (fp_is_zero(P)) {
+        if (fp_is_zero(R)) return g1x_dbl(g1x_from_affine(b));
+        return g1x_identity();
+    }
+    const Fp PP = fp_sqr(P);
+    const Fp PPP = fp_mul(P, PP);
+    const Fp Q = fp_mul(a.x, PP);
+    G1X r;
+    r.x = fp_sub(fp_sub(fp_sqr(R), PPP), fp_dbl(Q));
+    r.y = fp_sub(fp_mul(R, fp_sub(Q, r.x)), fp_mul(a.y, PPP));
+    r.zz = fp_mul(a.zz, PP);
+    r.zzz = fp_mul(a.zzz, PPP);
+    return r;
+}
+
+// add-2008-s: XYZZ + XYZZ
+VKZG_DEV_CALL G1X g1x_add(const G1X &a, const G1X &b) {
+    if (fp_is_zero(a.zz)) return b;
+    if (fp_is_zero(b.zz)) return a;
+    const Fp U1 = fp_mul(a.x, b.zz);
+    const Fp U2 = fp_mul(b.x, a.zz);
+    const Fp S1 = fp_mul(a.y, b.zzz);
+    const Fp S2 = fp_mul(b.y, a.zzz);
+    const Fp P = fp_sub(U2, U1);
+    const Fp R = fp_sub(S2, S1);
+    if (fp_is_zero(P)) {
+        if (fp_is_zero(R)) return g1x_dbl(a);
+        return g1x_identity();
+    }
+    const Fp PP = fp_sqr(P);
+    const Fp PPP = fp_mul(P, PP);
+    const Fp Q = fp_mul(U1, PP);
+    G1X r;
+    r.x = fp_sub(fp_sub(fp_sqr(R), PPP), fp_dbl(Q));
+    r.y = fp_sub(fp_mul(R, fp_sub(Q, r.x)), fp_mul(S1, PPP));
+    r.zz = fp_mul(fp_mul(a.zz, b.zz), PP);
+    r.zzz = fp_mul(fp_mul(a.zzz, b.zzz), PPP);
+    return r;
+}
+
+VKZG_DEV G1X g1x_neg(const G1X &p) {
+    G1X r = p;
+    r.y = fp_neg(p.y);
+    return r;
+}
+
+// --------------------------------------------------------------- encoding
+
+// Affine (Montgomery) -> 48-byte ZCash compressed encoding as 12 big-endian
+// words: x big-endian, 0x80 compressed flag, 0x20 when y > (p-1)/2, identity
+// = 0xc0 followed by zeros (k_compress_proofs.comp:21-46).
+VKZG_DEV_CALL void g1a_compress_words(uint32_t *w, const G1A &p) {
+    if (g1a_is_identity(p)) {
+        w[0] = __builtin_bswap32(0xc0000000u);
+#pragma unroll
+        for (int i = 1; i < 12; i++) w[i] = 0;
+        return;
+    }
+    const Fp x = fp_to_canonical(p.x);
+    const Fp y = fp_to_canonical(p.y);
+#pragma unroll
+    for (int i = 0; i < 12; i++) w[11 - i] = x.v[i];
+    uint32_t flags = 0x80000000u;
+    if (fp_is_lex_largest(y)) flags |= 0x20000000u;
+    w[0] |= flags;
+#pragma unroll
+    for (int i = 0; i < 12; i++) w[i] = __builtin_bswap32(w[i]);
+}
+
+}  // namespace vkzg_amd

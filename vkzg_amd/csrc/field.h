@@ -1,0 +1,308 @@
+// BLS12-381 Fp (12 x 32-bit limbs) and Fr (8 x 32-bit limbs) Montgomery
+// arithmetic for gfx950, held in VGPRs.
+//
+// Multiplication is CIOS with the "no final carry word" simplification: both
+// moduli have a clear top bit in their top limb (p: 0x1a0111ea..., r:
+// 0x73eda753...), so the running value t stays below 2^(32N) and the two
+// inner chains (a*b_i and m*p) can share one pass over the limbs.  Each limb
+// step is a 32x32->64 multiply-add (v_mad_u64_u32) plus a carry add; inputs
+// below the modulus give an output below twice the modulus, reduced once.
+//
+// Semantics follow the reference's field library (src/shaders/field.glsl.inc
+// :69-297): little-endian limbs, Montgomery radix 2^(32N), canonical results.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "constants.h"
+
+#define VKZG_DEV __device__ __forceinline__
+
+namespace vkzg_amd {
+
+struct Fr {
+    uint32_t v[8];
+};
+struct Fp {
+    uint32_t v[12];
+};
+
+// ------------------------------------------------------------ limb helpers
+
+// r = t - m when t >= m, else t.
+template <int N>
+VKZG_DEV void reduce_once(uint32_t *r, const uint32_t *t, const uint32_t (&m)[N]) {
+    uint32_t d[N];
+    uint32_t br = 0;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        const uint64_t s = (uint64_t)t[i] - m[i] - br;
+        d[i] = (uint32_t)s;
+        br = (uint32_t)(s >> 32) & 1u;
+    }
+#pragma unroll
+    for (int i = 0; i < N; i++) r[i] = br ? t[i] : d[i];
+}
+
+template <int N>
+VKZG_DEV void add_mod(uint32_t *r, const uint32_t *a, const uint32_t *b, const uint32_t (&m)[N]) {
+    uint32_t t[N];
+    uint32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        const uint64_t s = (uint64_t)a[i] + b[i] + c;
+        t[i] = (uint32_t)s;
+        c = (uint32_t)(s >> 32);
+    }
+    // a + b < 2m < 2^(32N): no carry out of the top limb.
+    reduce_once<N>(r, t, m);
+}
+
+template <int N>
+VKZG_DEV void sub_mod(uint32_t *r, const uint32_t *a, const uint32_t *b, const uint32_t (&m)[N]) {
+    uint32_t t[N];
+    uint32_t br = 0;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        const uint64_t s = (uint64_t)a[i] - b[i] - br;
+        t[i] = (uint32_t)s;
+        br = (uint32_t)(s >> 32) & 1u;
+    }
+    const uint32_t mask = 0u - br;
+    uint32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        const uint64_t s = (uint64_t)t[i] + (m[i] & mask) + c;
+        r[i] = (uint32_t)s;
+        c = (uint32_t)(s >> 32);
+    }
+}
+
+template <int N>
+VKZG_DEV void mont_mul(uint32_t *r, const uint32_t *a, const uint32_t *b, const uint32_t (&m)[N],
+                       uint32_t n0) {
+    uint32_t t[N];
+#pragma unroll
+    for (int j = 0; j < N; j++) t[j] = 0;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        const uint32_t bi = b[i];
+        uint64_t s = (uint64_t)a[0] * bi + t[0];
+        t[0] = (uint32_t)s;
+        uint32_t A = (uint32_t)(s >> 32);
+        const uint32_t mm = t[0] * n0;
+        uint64_t q = (uint64_t)mm * m[0] + t[0];
+        uint32_t C = (uint32_t)(q >> 32);
+#pragma unroll
+        for (int j = 1; j < N; j++) {
+            s = (uint64_t)a[j] * bi + t[j] + A;
+            t[j] = (uint32_t)s;
+            A = (uint32_t)(s >> 32);
+            q = (uint64_t)mm * m[j] + t[j] + C;
+            t[j - 1] = (uint32_t)q;
+            C = (uint32_t)(q >> 32);
+        }
+        t[N - 1] = C + A;
+    }
+    reduce_once<N>(r, t, m);
+}
+
+template <int N>
+VKZG_DEV bool limbs_zero(const uint32_t *a) {
+    uint32_t acc = 0;
+#pragma unroll
+    for (int i = 0; i < N; i++) acc |= a[i];
+    return acc == 0;
+}
+
+template <int N>
+VKZG_DEV bool limbs_eq(const uint32_t *a, const uint32_t *b) {
+    uint32_t acc = 0;
+#pragma unroll
+    for (int i = 0; i < N; i++) acc |= a[i] ^ b[i];
+    return acc == 0;
+}
+
+// a < m as plain integers
+template <int N>
+VKZG_DEV bool limbs_lt(const uint32_t *a, const uint32_t (&m)[N]) {
+    uint32_t br = 0;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        const uint64_t s = (uint64_t)a[i] - m[i] - br;
+        br = (uint32_t)(s >> 32) & 1u;
+    }
+    return br != 0;
+}
+
+// ------------------------------------------------------------------ Fr
+
+VKZG_DEV Fr fr_zero() {
+    Fr r;
+#pragma unroll
+    for (int i = 0; i < 8; i++) r.v[i] = 0;
+    return r;
+}
+VKZG_DEV Fr fr_one() {
+    Fr r;
+#pragma unroll
+    for (int i = 0; i < 8; i++) r.v[i] = cfr::ONE[i];
+    return r;
+}
+VKZG_DEV Fr fr_const(const uint32_t (&c)[8]) {
+    Fr r;
+#pragma unroll
+    for (int i = 0; i < 8; i++) r.v[i] = c[i];
+    return r;
+}
+VKZG_DEV Fr fr_mul(const Fr &a, const Fr &b) {
+    Fr r;
+    mont_mul<8>(r.v, a.v, b.v, cfr::P, cfr::N0);
+    return r;
+}
+VKZG_DEV Fr fr_sqr(const Fr &a) { return fr_mul(a, a); }
+VKZG_DEV Fr fr_add(const Fr &a, const Fr &b) {
+    Fr r;
+    add_mod<8>(r.v, a.v, b.v, cfr::P);
+    return r;
+}
+VKZG_DEV Fr fr_sub(const Fr &a, const Fr &b) {
+    Fr r;
+    sub_mod<8>(r.v, a.v, b.v, cfr::P);
+    return r;
+}
+VKZG_DEV Fr fr_neg(const Fr &a) { return fr_sub(fr_zero(), a); }
+VKZG_DEV bool fr_is_zero(const Fr &a) { return limbs_zero<8>(a.v); }
+
+// Montgomery -> canonical (multiply by plain 1).
+VKZG_DEV Fr fr_to_canonical(const Fr &a) {
+    Fr one = fr_zero();
+    one.v[0] = 1;
+    return fr_mul(a, one);
+}
+// canonical (< r) -> Montgomery.
+VKZG_DEV Fr fr_from_canonical(const Fr &a) { return fr_mul(a, fr_const(cfr::R2)); }
+// Any 256-bit value -> Montgomery form of (value mod r).  2^256 < 3r, so at
+// most two subtractions bring it below r first (k_cells_to_fr.comp:5-6 reduces
+// non-canonical cell bytes mod r).
+VKZG_DEV Fr fr_from_any(const Fr &a) {
+    Fr t;
+    reduce_once<8>(t.v, a.v, cfr::P);
+    reduce_once<8>(t.v, t.v, cfr::P);
+    return fr_from_canonical(t);
+}
+
+template <int NE>
+VKZG_DEV Fr fr_pow_const(const Fr &a, const uint32_t (&e)[NE]) {
+    Fr acc = fr_one();
+#pragma unroll 1
+    for (int i = NE - 1; i >= 0; i--) {
+        const uint32_t limb = e[i];
+#pragma unroll 1
+        for (int b = 31; b >= 0; b--) {
+            acc = fr_sqr(acc);
+            if ((limb >> b) & 1u) acc = fr_mul(acc, a);
+        }
+    }
+    return acc;
+}
+VKZG_DEV Fr fr_inv(const Fr &a) { return fr_pow_const<8>(a, cfr::P_MINUS_2); }
+
+// 32 big-endian bytes (as 8 big-endian words) -> little-endian limbs
+VKZG_DEV Fr fr_from_be_words(const uint32_t *w) {
+    Fr r;
+#pragma unroll
+    for (int i = 0; i < 8; i++) r.v[i] = __builtin_bswap32(w[7 - i]);
+    return r;
+}
+VKZG_DEV void fr_to_be_words(uint32_t *w, const Fr &a) {
+#pragma unroll
+    for (int i = 0; i < 8; i++) w[7 - i] = __builtin_bswap32(a.v[i]);
+}
+
+// ------------------------------------------------------------------ Fp
+
+VKZG_DEV Fp fp_zero() {
+    Fp r;
+#pragma unroll
+    for (int i = 0; i < 12; i++) r.v[i] = 0;
+    return r;
+}
+VKZG_DEV Fp fp_one() {
+    Fp r;
+#pragma unroll
+    for (int i = 0; i < 12; i++) r.v[i] = cfp::ONE[i];
+    return r;
+}
+VKZG_DEV Fp fp_const(const uint32_t (&c)[12]) {
+    Fp r;
+#pragma unroll
+    for (int i = 0; i < 12; i++) r.v[i] = c[i];
+    return r;
+}
+VKZG_DEV Fp fp_mul(const Fp &a, const Fp &b) {
+    Fp r;
+    mont_mul<12>(r.v, a.v, b.v, cfp::P, cfp::N0);
+    return r;
+}
+VKZG_DEV Fp fp_sqr(const Fp &a) { return fp_mul(a, a); }
+VKZG_DEV Fp fp_add(const Fp &a, const Fp &b) {
+    Fp r;
+    add_mod<12>(r.v, a.v, b.v, cfp::P);
+    return r;
+}
+VKZG_DEV Fp fp_dbl(const Fp &a) { return fp_add(a, a); }
+VKZG_DEV Fp fp_sub(const Fp &a, const Fp &b) {
+    Fp r;
+    sub_mod<12>(r.v, a.v, b.v, cfp::P);
+    return r;
+}
+VKZG_DEV Fp fp_neg(const Fp &a) { return fp_sub(fp_zero(), a); }
+VKZG_DEV bool fp_is_zero(const Fp &a) { return limbs_zero<12>(a.v); }
+VKZG_DEV bool fp_eq(const Fp &a, const Fp &b) { return limbs_eq<12>(a.v, b.v); }
+VKZG_DEV Fp fp_cond_neg(const Fp &a, bool neg) {
+    const Fp n = fp_neg(a);
+    Fp r;
+#pragma unroll
+    for (int i = 0; i < 12; i++) r.v[i] = neg ? n.v[i] : a.v[i];
+    return r;
+}
+
+VKZG_DEV Fp fp_to_canonical(const Fp &a) {
+    Fp one = fp_zero();
+    one.v[0] = 1;
+    return fp_mul(a, one);
+}
+VKZG_DEV Fp fp_from_canonical(const Fp &a) { return fp_mul(a, fp_const(cfp::R2)); }
+
+template <int NE>
+VKZG_DEV Fp fp_pow_const(const Fp &a, const uint32_t (&e)[NE]) {
+    Fp acc = fp_one();
+#pragma unroll 1
+    for (int i = NE - 1; i >= 0; i--) {
+        const uint32_t limb = e[i];
+#pragma unroll 1
+        for (int b = 31; b >= 0; b--) {
+            acc = fp_sqr(acc);
+            if ((limb >> b) & 1u) acc = fp_mul(acc, a);
+        }
+    }
+    return acc;
+}
+VKZG_DEV Fp fp_inv(const Fp &a) { return fp_pow_const<12>(a, cfp::P_MINUS_2); }
+
+// canonical value > (p-1)/2 (the sign flag of the compressed encoding)
+VKZG_DEV bool fp_is_lex_largest(const Fp &canonical) {
+    // (p-1)/2 < x  <=>  borrow out of (HALF - x)
+    uint32_t br = 0;
+#pragma unroll
+    for (int i = 0; i < 12; i++) {
+        const uint64_t s = (uint64_t)cfp::HALF[i] - canonical.v[i] - br;
+        br = (uint32_t)(s >> 32) & 1u;
+    }
+    return br != 0;
+}
+
+}  // namespace vkzg_amd

@@ -1,0 +1,76 @@
+// Host-side launch wrappers for the gfx950 kernels, and the device layouts
+// they share.  Every wrapper enqueues on `stream` and returns the launch
+// status; none synchronises.
+//
+// Layouts (all arrays of structs, little-endian 32-bit limbs, Montgomery form
+// unless stated):
+//   Fr roots[8192]            omega_8192^k;  roots_inv: omega_8192^-k
+//   Fr shift_fwd[8192]        7^k / 8192;    shift_inv: 7^-k / 8192
+//   G1A srs[4096]             the G1 monomial setup, affine
+//   G1A xext[128][64]         x_ext_fft[j][i] (FK20 setup in the Fourier domain)
+//   G1A table[128][64][W][H]  table[j][i][w][d-1] = d * 2^(c*w) * xext[j][i],
+//                             c = window bits, W = ceil(256/c), H = 2^(c-1)
+//   Fr scal[B][128][64]       phase-A scalars, CANONICAL (not Montgomery)
+//   G1X u[B][128]             phase-A outputs
+//   G1J lad[B][32][128]       2^(8d) * u[j]
+//   G1J fold[B][2][64][32]    L+ / L- ladders, normalised into G1A lad_aff with
+//                             the same layout
+//   G1X buckets[B][128][128]  phase-B buckets, [output a][bucket k]
+//   G1X cvals[B][128]         C+ (a < 64) and C- (a >= 64)
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace vkzg_amd {
+
+struct Fr;
+struct Fp;
+struct G1A;
+struct G1J;
+struct G1X;
+
+// ------------------------------------------------------------------ setup
+hipError_t launch_roots(Fr *roots, Fr *roots_inv, Fr *shift_fwd, Fr *shift_inv, hipStream_t s);
+hipError_t launch_decompress(G1A *out, const uint8_t *in, uint32_t n, uint32_t *err, hipStream_t s);
+hipError_t launch_xext_fft(G1J *out, const G1A *srs, const Fr *roots, hipStream_t s);
+hipError_t launch_jac_to_affine(G1A *out, const G1J *in, Fp *prefix, uint64_t count, uint32_t chunk,
+                                hipStream_t s);
+hipError_t launch_table_chains(G1J *out, const G1A *xext, uint32_t c, uint32_t W, hipStream_t s);
+hipError_t launch_table_multiples(G1A *table, const G1A *pw, uint32_t c, uint32_t W,
+                                  uint64_t thread_begin, uint32_t thread_count, G1J *scratch,
+                                  Fp *prefix, hipStream_t s);
+hipError_t launch_phase_b_scalars(Fr *out, const Fr *roots, hipStream_t s);
+
+// ----------------------------------------------------------------- proofs
+hipError_t launch_fk20_scalars(Fr *scal, const uint8_t *blobs, uint32_t *err, const Fr *roots,
+                               const Fr *roots_inv, uint32_t nblobs, hipStream_t s);
+hipError_t launch_phase_a(G1X *u, const Fr *scal, const G1A *table, uint32_t c, uint32_t W,
+                          uint32_t nblobs, hipStream_t s);
+hipError_t launch_ladder(G1J *lad, const G1X *u, uint32_t nblobs, hipStream_t s);
+hipError_t launch_fold(G1J *fold, const G1J *lad, uint32_t nblobs, hipStream_t s);
+hipError_t launch_phase_b(G1X *buckets, const G1A *lad_aff, const uint32_t *items,
+                          const uint32_t *offsets, uint32_t nblobs, hipStream_t s);
+hipError_t launch_bucket_reduce(G1X *cvals, const G1X *buckets, uint32_t nblobs, hipStream_t s);
+hipError_t launch_finalize(uint8_t *proofs, const G1X *cvals, uint32_t nblobs, hipStream_t s);
+
+// --------------------------------------------------------------- recovery
+hipError_t launch_rec_vanishing(Fr *zd, Fr *izc, const uint8_t *present, const Fr *roots,
+                                uint32_t nblobs, hipStream_t s);
+// in_mode 0: `in` is cell bytes, multiplied by zd on load; 1: `in` is Fr.
+hipError_t launch_rec_pass_a(Fr *out, const void *in, int in_mode, const Fr *zd, bool inverse,
+                             const Fr *roots, const Fr *roots_inv, uint32_t nblobs, hipStream_t s);
+// out_mode 0: multiply by mul_table[k], store Fr; 1: multiply by izc[k % 128],
+// store Fr; 2: store canonical big-endian bytes at the cell wire position.
+hipError_t launch_rec_pass_b(void *out, const Fr *in, int out_mode, const Fr *mul_table,
+                             const Fr *izc, bool inverse, const Fr *roots, const Fr *roots_inv,
+                             uint32_t nblobs, hipStream_t s);
+
+// ------------------------------------------------------------ test hooks
+hipError_t launch_debug_fp_mul(uint32_t *out, const uint32_t *a, const uint32_t *b, uint32_t n,
+                               hipStream_t s);
+hipError_t launch_debug_fr_mul(uint32_t *out, const uint32_t *a, const uint32_t *b, uint32_t n,
+                               hipStream_t s);
+
+}  // namespace vkzg_amd

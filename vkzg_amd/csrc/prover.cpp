@@ -1,0 +1,672 @@
+// Host runtime behind the C ABI of include/vkzg.h and include/vkzg_amd.h.
+//
+// One prover = one HIP device, one stream, resident tables and a working set
+// sized for max_batch_size blobs.  Calls are serialised by a mutex (reference
+// src/vkzg.cpp:159,1188,1206).  Every HIP failure maps to VKZG_ERR_GPU.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/vkzg.h"
+#include "../../include/vkzg_amd.h"
+#include "curve.h"
+#include "kernels.h"
+
+extern "C" const uint8_t vkzg_amd_mainnet_g1[];
+extern "C" const uint8_t vkzg_amd_mainnet_g1_end[];
+
+using namespace vkzg_amd;
+
+namespace {
+
+constexpr size_t kBlobBytes = VKZG_BYTES_PER_BLOB;
+constexpr size_t kProofBytes = (size_t)VKZG_NUM_CELL_PROOFS * VKZG_BYTES_PER_PROOF;
+constexpr size_t kCellsBytes = (size_t)VKZG_NUM_CELL_PROOFS * VKZG_BYTES_PER_CELL;
+constexpr size_t kSetupBytes = 4096 * 48;
+constexpr uint32_t kDefaultBatch = 256;
+
+static_assert(sizeof(Fr) == 32 && sizeof(Fp) == 48, "field layout");
+static_assert(sizeof(G1A) == 96 && sizeof(G1J) == 144 && sizeof(G1X) == 192, "point layout");
+
+struct HipFail {};
+
+inline void ck(hipError_t e) {
+    if (e != hipSuccess) throw HipFail{};
+}
+
+
+}  // namespace
+
+struct vkzg_prover {
+    int device = 0;
+    std::string name;
+    uint32_t cus = 0;
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    uint32_t max_batch = kDefaultBatch;
+    double setup_ms = 0;
+    uint64_t working_bytes = 0;
+
+    // Resident tables.
+    Fr *roots = nullptr, *roots_inv = nullptr, *shift_fwd = nullptr, *shift_inv = nullptr;
+    G1A *srs = nullptr, *xext = nullptr, *table = nullptr;
+    uint32_t c = 0, W = 0, H = 0;
+    uint64_t table_bytes = 0;
+    uint32_t *kb_items = nullptr, *kb_offsets = nullptr;
+
+    // Proof working set.
+    uint8_t *d_blobs = nullptr, *d_proofs = nullptr;
+    uint32_t *d_err = nullptr;
+    Fr *d_scal = nullptr;
+    G1X *d_u = nullptr, *d_buckets = nullptr, *d_cvals = nullptr;
+    G1J *d_lad = nullptr, *d_fold = nullptr;
+    G1A *d_lad_aff = nullptr;
+    Fp *d_prefix = nullptr;
+
+    // Recovery working set.
+    uint8_t *d_cells = nullptr, *d_present = nullptr, *d_cells_out = nullptr;
+    Fr *d_zd = nullptr, *d_izc = nullptr, *d_bufA = nullptr, *d_bufB = nullptr;
+
+    // Pinned staging for the host-pointer entry points.
+    uint8_t *h_in = nullptr, *h_out = nullptr, *h_present = nullptr;
+    uint32_t *h_err = nullptr;
+
+    // Optional per-stage timing: events recorded between the launches of the
+    // last call on the stream the kernels run on, accumulated over its chunks.
+    bool timing = false;
+    static constexpr int kMaxStages = 16;
+    hipEvent_t ev[kMaxStages + 1] = {};
+    float stage_ms[kMaxStages] = {};
+    int nstages = 0;
+    const char *stage_names[kMaxStages] = {};
+
+    std::vector<void *> dev_allocs;
+    std::vector<void *> host_allocs;
+
+    template <typename T>
+    T *dalloc(size_t count) {
+        void *ptr = nullptr;
+        ck(hipMalloc(&ptr, count * sizeof(T) ? count * sizeof(T) : 16));
+        dev_allocs.push_back(ptr);
+        return static_cast<T *>(ptr);
+    }
+    template <typename T>
+    T *halloc(size_t count) {
+        void *ptr = nullptr;
+        ck(hipHostMalloc(&ptr, count * sizeof(T) ? count * sizeof(T) : 16, hipHostMallocDefault));
+        host_allocs.push_back(ptr);
+        return static_cast<T *>(ptr);
+    }
+    void dfree(void *ptr) {
+        for (auto &a : dev_allocs) {
+            if (a == ptr) {
+                (void)hipFree(a);
+                a = nullptr;
+            }
+        }
+    }
+
+    ~vkzg_prover() {
+        int cur = 0;
+        (void)hipGetDevice(&cur);
+        (void)hipSetDevice(device);
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (void *a : dev_allocs)
+            if (a) (void)hipFree(a);
+        for (void *a : host_allocs)
+            if (a) (void)hipHostFree(a);
+        for (auto &e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+        (void)hipSetDevice(cur);
+    }
+};
+
+namespace {
+
+uint64_t table_bytes_for(uint32_t c) {
+    const uint64_t W = (256 + c - 1) / c;
+    const uint64_t H = 1ull << (c - 1);
+    return 8192ull * W * H * sizeof(G1A);
+}
+
+// Window bits of the resident phase-A table: $VKZG_AMD_WINDOW if set (6..14),
+// else the largest c <= 13 whose table fits in a third of the free memory.
+uint32_t choose_window(size_t free_bytes) {
+    if (const char *env = std::getenv("VKZG_AMD_WINDOW")) {
+        const int c = std::atoi(env);
+        if (c >= 6 && c <= 14) return (uint32_t)c;
+    }
+    for (uint32_t c = 13; c > 8; c--) {
+        if (table_bytes_for(c) <= free_bytes / 3) return c;
+    }
+    return 8;
+}
+
+// Signed 8-bit digits of the 66 phase-B scalars, grouped by bucket |v|-1:
+// item = e | d << 8 | (v < 0) << 16, offsets[half][129] (absolute).
+void build_phase_b_items(const Fr *kpm, std::vector<uint32_t> &items, std::vector<uint32_t> &offsets) {
+    std::vector<uint32_t> lists[2][128];
+    for (int half = 0; half < 2; half++) {
+        for (uint32_t e = 0; e < 64; e++) {
+            const Fr &s = kpm[half * 64 + e];
+            int carry = 0;
+            for (uint32_t d = 0; d < 32; d++) {
+                int v = (int)((s.v[d / 4] >> (8 * (d % 4))) & 0xffu) + carry;
+                carry = 0;
+                if (v > 128) {
+                    v -= 256;
+                    carry = 1;
+                }
+                if (v == 0) continue;
+                const uint32_t mag = (uint32_t)(v < 0 ? -v : v);
+                lists[half][mag - 1].push_back(e | (d << 8) | ((v < 0 ? 1u : 0u) << 16));
+            }
+        }
+    }
+    items.clear();
+    offsets.assign(2 * 129, 0);
+    for (int half = 0; half < 2; half++) {
+        for (int k = 0; k < 128; k++) {
+            offsets[half * 129 + k] = (uint32_t)items.size();
+            items.insert(items.end(), lists[half][k].begin(), lists[half][k].end());
+        }
+        offsets[half * 129 + 128] = (uint32_t)items.size();
+    }
+}
+
+void build_tables(vkzg_prover *p, const uint8_t *setup) {
+    hipStream_t s = p->stream;
+    p->roots = p->dalloc<Fr>(8192);
+    p->roots_inv = p->dalloc<Fr>(8192);
+    p->shift_fwd = p->dalloc<Fr>(8192);
+    p->shift_inv = p->dalloc<Fr>(8192);
+    ck(launch_roots(p->roots, p->roots_inv, p->shift_fwd, p->shift_inv, s));
+
+    // Decompress the setup.
+    uint8_t *d_setup = p->dalloc<uint8_t>(kSetupBytes);
+    uint32_t *d_flag = p->dalloc<uint32_t>(1);
+    ck(hipMemcpyAsync(d_setup, setup, kSetupBytes, hipMemcpyHostToDevice, s));
+    ck(hipMemsetAsync(d_flag, 0, 4, s));
+    p->srs = p->dalloc<G1A>(4096);
+    ck(launch_decompress(p->srs, d_setup, 4096, d_flag, s));
+    uint32_t flag = 0;
+    ck(hipMemcpyAsync(&flag, d_flag, 4, hipMemcpyDeviceToHost, s));
+    ck(hipStreamSynchronize(s));
+    p->dfree(d_setup);
+    if (flag) throw VKZG_ERR_SETUP;
+
+    // x_ext_fft.
+    G1J *xj = p->dalloc<G1J>(8192);
+    Fp *prefix = p->dalloc<Fp>(8192);
+    p->xext = p->dalloc<G1A>(8192);
+    ck(launch_xext_fft(xj, p->srs, p->roots, s));
+    ck(launch_jac_to_affine(p->xext, xj, prefix, 8192, 8, s));
+
+    // Phase-B scalars -> item lists.
+    Fr *d_kpm = p->dalloc<Fr>(128);
+    ck(launch_phase_b_scalars(d_kpm, p->roots, s));
+    std::vector<Fr> kpm(128);
+    ck(hipMemcpyAsync(kpm.data(), d_kpm, 128 * sizeof(Fr), hipMemcpyDeviceToHost, s));
+    ck(hipStreamSynchronize(s));
+    std::vector<uint32_t> items, offsets;
+    build_phase_b_items(kpm.data(), items, offsets);
+    p->kb_items = p->dalloc<uint32_t>(items.size());
+    p->kb_offsets = p->dalloc<uint32_t>(offsets.size());
+    ck(hipMemcpyAsync(p->kb_items, items.data(), items.size() * 4, hipMemcpyHostToDevice, s));
+    ck(hipMemcpyAsync(p->kb_offsets, offsets.data(), offsets.size() * 4, hipMemcpyHostToDevice, s));
+
+    // Phase-A multiples table.
+    size_t free_b = 0, total_b = 0;
+    ck(hipMemGetInfo(&free_b, &total_b));
+    p->c = choose_window(free_b);
+    p->W = (256 + p->c - 1) / p->c;
+    p->H = 1u << (p->c - 1);
+    p->table_bytes = table_bytes_for(p->c);
+    p->table = p->dalloc<G1A>(p->table_bytes / sizeof(G1A));
+
+    G1J *pwj = p->dalloc<G1J>((size_t)p->W * 8192);
+    Fp *pw_prefix = p->dalloc<Fp>((size_t)p->W * 8192);
+    G1A *pw = p->dalloc<G1A>((size_t)p->W * 8192);
+    ck(launch_table_chains(pwj, p->xext, p->c, p->W, s));
+    ck(launch_jac_to_affine(pw, pwj, pw_prefix, (uint64_t)p->W * 8192, 16, s));
+    ck(hipStreamSynchronize(s));
+    p->dfree(pwj);
+    p->dfree(pw_prefix);
+
+    const uint32_t S = p->H < 64 ? p->H : 64;
+    const uint64_t total_threads = 8192ull * p->W * (p->H / S);
+    const uint32_t slice = 1u << 18;
+    G1J *scratch = p->dalloc<G1J>((size_t)S * slice);
+    Fp *pref = p->dalloc<Fp>((size_t)S * slice);
+    for (uint64_t t0 = 0; t0 < total_threads; t0 += slice) {
+        const uint32_t n = (uint32_t)std::min<uint64_t>(slice, total_threads - t0);
+        ck(launch_table_multiples(p->table, pw, p->c, p->W, t0, n, scratch, pref, s));
+    }
+    ck(hipStreamSynchronize(s));
+    p->dfree(scratch);
+    p->dfree(pref);
+    p->dfree(pw);
+    p->dfree(xj);
+    p->dfree(prefix);
+    p->dfree(d_kpm);
+    p->dfree(d_flag);
+}
+
+void alloc_working_set(vkzg_prover *p) {
+    const size_t B = p->max_batch;
+    uint64_t before = 0;
+    size_t f0 = 0, t0 = 0;
+    ck(hipMemGetInfo(&f0, &t0));
+    before = f0;
+    p->d_blobs = p->dalloc<uint8_t>(B * kBlobBytes);
+    p->d_proofs = p->dalloc<uint8_t>(B * kProofBytes);
+    p->d_err = p->dalloc<uint32_t>(B);
+    p->d_scal = p->dalloc<Fr>(B * 128 * 64);
+    p->d_u = p->dalloc<G1X>(B * 128);
+    p->d_lad = p->dalloc<G1J>(B * 128 * 32);
+    p->d_fold = p->dalloc<G1J>(B * 128 * 32);
+    p->d_lad_aff = p->dalloc<G1A>(B * 128 * 32);
+    p->d_prefix = p->dalloc<Fp>(B * 128 * 32);
+    p->d_buckets = p->dalloc<G1X>(B * 128 * 128);
+    p->d_cvals = p->dalloc<G1X>(B * 128);
+    p->d_cells = p->dalloc<uint8_t>(B * kCellsBytes);
+    p->d_cells_out = p->dalloc<uint8_t>(B * kCellsBytes);
+    p->d_present = p->dalloc<uint8_t>(B * 128);
+    p->d_zd = p->dalloc<Fr>(B * 128);
+    p->d_izc = p->dalloc<Fr>(B * 128);
+    p->d_bufA = p->dalloc<Fr>(B * 8192);
+    p->d_bufB = p->dalloc<Fr>(B * 8192);
+    size_t f1 = 0;
+    ck(hipMemGetInfo(&f1, &t0));
+    p->working_bytes = before > f1 ? before - f1 : 0;
+    p->h_in = p->halloc<uint8_t>(B * kCellsBytes);
+    p->h_out = p->halloc<uint8_t>(B * kCellsBytes);
+    p->h_present = p->halloc<uint8_t>(B * 128);
+    p->h_err = p->halloc<uint32_t>(B);
+}
+
+vkzg_result create(vkzg_prover **out, const vkzg_options *opts, const uint8_t *setup) {
+    const auto t0 = std::chrono::steady_clock::now();
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return VKZG_ERR_GPU;
+    vkzg_prover *p = new (std::nothrow) vkzg_prover();
+    if (!p) return VKZG_ERR_MALLOC;
+    try {
+        ck(hipGetDevice(&p->device));
+        hipDeviceProp_t prop;
+        ck(hipGetDeviceProperties(&prop, p->device));
+        p->name = prop.name;
+        if (p->name.empty()) {
+            // Some driver stacks leave the marketing name blank; fall back to the ISA.
+            std::string arch = prop.gcnArchName;
+            p->name = "AMD " + arch.substr(0, arch.find(':')) + " (" +
+                      std::to_string(prop.multiProcessorCount) + " CUs)";
+        }
+        p->cus = (uint32_t)prop.multiProcessorCount;
+        ck(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+        for (auto &e : p->ev) ck(hipEventCreate(&e));
+        if (opts && opts->max_batch_size) p->max_batch = opts->max_batch_size;
+        build_tables(p, setup);
+        alloc_working_set(p);
+        ck(hipStreamSynchronize(p->stream));
+    } catch (HipFail &) {
+        delete p;
+        return VKZG_ERR_GPU;
+    } catch (vkzg_result r) {
+        delete p;
+        return r;
+    }
+    p->setup_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    *out = p;
+    return VKZG_OK;
+}
+
+// Records stage boundaries when timing is on.  mark(i, name) closes stage i-1
+// and opens stage i; finish() folds the elapsed times into stage_ms.
+struct StageClock {
+    vkzg_prover *p;
+    hipStream_t s;
+    int n = 0;
+    StageClock(vkzg_prover *pp, hipStream_t ss) : p(pp), s(ss) {
+        if (p->timing) ck(hipEventRecord(p->ev[0], s));
+    }
+    void mark(const char *name) {
+        if (!p->timing || n >= vkzg_prover::kMaxStages) return;
+        p->stage_names[n] = name;
+        ck(hipEventRecord(p->ev[++n], s));
+    }
+    // Call after the stream has been synchronised.
+    void finish() {
+        if (!p->timing) return;
+        p->nstages = n;
+        for (int i = 0; i < n; i++) {
+            float ms = 0;
+            ck(hipEventElapsedTime(&ms, p->ev[i], p->ev[i + 1]));
+            p->stage_ms[i] += ms;
+        }
+    }
+};
+
+void reset_stage_times(vkzg_prover *p) {
+    for (float &v : p->stage_ms) v = 0;
+    p->nstages = 0;
+}
+
+// Enqueues the proof pipeline for `nb` blobs at d_blobs -> d_proofs.
+void enqueue_proofs(vkzg_prover *p, uint8_t *d_proofs, const uint8_t *d_blobs, uint32_t nb,
+                    hipStream_t s, StageClock &clk) {
+    ck(hipMemsetAsync(p->d_err, 0, nb * sizeof(uint32_t), s));
+    ck(launch_fk20_scalars(p->d_scal, d_blobs, p->d_err, p->roots, p->roots_inv, nb, s));
+    clk.mark("fk20_scalars");
+    ck(launch_phase_a(p->d_u, p->d_scal, p->table, p->c, p->W, nb, s));
+    clk.mark("phase_a");
+    ck(launch_ladder(p->d_lad, p->d_u, nb, s));
+    clk.mark("ladder");
+    ck(launch_fold(p->d_fold, p->d_lad, nb, s));
+    clk.mark("fold");
+    ck(launch_jac_to_affine(p->d_lad_aff, p->d_fold, p->d_prefix, (uint64_t)nb * 4096, 32, s));
+    clk.mark("normalize");
+    ck(launch_phase_b(p->d_buckets, p->d_lad_aff, p->kb_items, p->kb_offsets, nb, s));
+    clk.mark("phase_b");
+    ck(launch_bucket_reduce(p->d_cvals, p->d_buckets, nb, s));
+    clk.mark("bucket_reduce");
+    ck(launch_finalize(d_proofs, p->d_cvals, nb, s));
+    clk.mark("finalize");
+}
+
+// Enqueues recovery for `nb` blobs (cells and mask already on the device).
+void enqueue_recovery(vkzg_prover *p, uint8_t *d_out, const uint8_t *d_cells,
+                      const uint8_t *d_present, uint32_t nb, hipStream_t s, StageClock &clk) {
+    ck(launch_rec_vanishing(p->d_zd, p->d_izc, d_present, p->roots, nb, s));
+    clk.mark("vanishing");
+    // (E * Z) -> coefficients, times 7^k / 8192
+    ck(launch_rec_pass_a(p->d_bufA, d_cells, 0, p->d_zd, true, p->roots, p->roots_inv, nb, s));
+    clk.mark("inv_pass_a");
+    ck(launch_rec_pass_b(p->d_bufB, p->d_bufA, 0, p->shift_fwd, nullptr, true, p->roots,
+                         p->roots_inv, nb, s));
+    clk.mark("inv_pass_b");
+    // coset evaluations, divided by Z on the coset
+    ck(launch_rec_pass_a(p->d_bufA, p->d_bufB, 1, nullptr, false, p->roots, p->roots_inv, nb, s));
+    clk.mark("coset_pass_a");
+    ck(launch_rec_pass_b(p->d_bufB, p->d_bufA, 1, nullptr, p->d_izc, false, p->roots,
+                         p->roots_inv, nb, s));
+    clk.mark("coset_pass_b");
+    // coset interpolation: coefficients of P, times 7^-k / 8192
+    ck(launch_rec_pass_a(p->d_bufA, p->d_bufB, 1, nullptr, true, p->roots, p->roots_inv, nb, s));
+    clk.mark("coset_inv_pass_a");
+    ck(launch_rec_pass_b(p->d_bufB, p->d_bufA, 0, p->shift_inv, nullptr, true, p->roots,
+                         p->roots_inv, nb, s));
+    clk.mark("coset_inv_pass_b");
+    // evaluate P on the whole domain, write cells
+    ck(launch_rec_pass_a(p->d_bufA, p->d_bufB, 1, nullptr, false, p->roots, p->roots_inv, nb, s));
+    clk.mark("eval_pass_a");
+    ck(launch_rec_pass_b(d_out, p->d_bufA, 2, nullptr, nullptr, false, p->roots, p->roots_inv, nb,
+                         s));
+    clk.mark("eval_pass_b");
+}
+
+bool chunk_missing_ok(const uint8_t *present, uint32_t nb) {
+    for (uint32_t b = 0; b < nb; b++) {
+        uint32_t missing = 0;
+        for (uint32_t i = 0; i < 128; i++) missing += present[(size_t)b * 128 + i] == 0;
+        if (missing > 64) return false;
+    }
+    return true;
+}
+
+struct DeviceGuard {
+    int prev = 0;
+    explicit DeviceGuard(int dev) {
+        (void)hipGetDevice(&prev);
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard() { (void)hipSetDevice(prev); }
+};
+
+}  // namespace
+
+// ------------------------------------------------------------------ C ABI
+
+extern "C" {
+
+const char *vkzg_error_string(vkzg_result r) {
+    switch (r) {
+        case VKZG_OK: return "ok";
+        case VKZG_ERR_BADARGS: return "invalid argument";
+        case VKZG_ERR_MALLOC: return "allocation failed";
+        case VKZG_ERR_IO: return "i/o error";
+        case VKZG_ERR_SETUP: return "malformed trusted setup";
+        case VKZG_ERR_GPU: return "gpu error";
+        case VKZG_ERR_INVALID_BLOB: return "blob contains a non-canonical field element";
+    }
+    return "unknown error";
+}
+
+void vkzg_options_default(vkzg_options *opts) {
+    if (!opts) return;
+    opts->max_batch_size = 0;
+}
+
+vkzg_result vkzg_prover_new(vkzg_prover **out, const vkzg_options *opts) {
+    if (!out) return VKZG_ERR_BADARGS;
+    *out = nullptr;
+    if ((size_t)(vkzg_amd_mainnet_g1_end - vkzg_amd_mainnet_g1) != kSetupBytes) return VKZG_ERR_SETUP;
+    return create(out, opts, vkzg_amd_mainnet_g1);
+}
+
+vkzg_result vkzg_amd_prover_new_with_setup(vkzg_prover **out, const vkzg_options *opts,
+                                           const uint8_t *g1, size_t len) {
+    if (!out) return VKZG_ERR_BADARGS;
+    *out = nullptr;
+    if (!g1 || len != kSetupBytes) return VKZG_ERR_BADARGS;
+    return create(out, opts, g1);
+}
+
+void vkzg_prover_free(vkzg_prover *p) { delete p; }
+
+const char *vkzg_prover_device_name(const vkzg_prover *p) { return p ? p->name.c_str() : ""; }
+
+uint32_t vkzg_prover_gpu_core_count(const vkzg_prover *p) { return p ? p->cus : 0; }
+
+vkzg_result vkzg_compute_proofs_batch(vkzg_prover *p, uint8_t *proofs, const uint8_t *blobs,
+                                      size_t num_blobs) {
+    if (!p || !blobs || !proofs) return VKZG_ERR_BADARGS;
+    if (num_blobs == 0) return VKZG_OK;
+    std::lock_guard<std::mutex> lock(p->mu);
+    DeviceGuard guard(p->device);
+    reset_stage_times(p);
+    try {
+        for (size_t done = 0; done < num_blobs;) {
+            const uint32_t nb = (uint32_t)std::min<size_t>(p->max_batch, num_blobs - done);
+            hipStream_t s = p->stream;
+            std::memcpy(p->h_in, blobs + done * kBlobBytes, nb * kBlobBytes);
+            ck(hipMemcpyAsync(p->d_blobs, p->h_in, nb * kBlobBytes, hipMemcpyHostToDevice, s));
+            StageClock clk(p, s);
+            enqueue_proofs(p, p->d_proofs, p->d_blobs, nb, s, clk);
+            ck(hipMemcpyAsync(p->h_err, p->d_err, nb * 4, hipMemcpyDeviceToHost, s));
+            ck(hipMemcpyAsync(p->h_out, p->d_proofs, nb * kProofBytes, hipMemcpyDeviceToHost, s));
+            ck(hipStreamSynchronize(s));
+            clk.finish();
+            for (uint32_t b = 0; b < nb; b++)
+                if (p->h_err[b]) return VKZG_ERR_INVALID_BLOB;
+            std::memcpy(proofs + done * kProofBytes, p->h_out, nb * kProofBytes);
+            done += nb;
+        }
+    } catch (HipFail &) {
+        return VKZG_ERR_GPU;
+    }
+    return VKZG_OK;
+}
+
+vkzg_result vkzg_recover_cells_batch(vkzg_prover *p, uint8_t *cells_out, const uint8_t *cells,
+                                     const uint8_t *cell_present, size_t num_blobs) {
+    if (!p || !cells_out || !cells || !cell_present) return VKZG_ERR_BADARGS;
+    if (num_blobs == 0) return VKZG_OK;
+    std::lock_guard<std::mutex> lock(p->mu);
+    DeviceGuard guard(p->device);
+    reset_stage_times(p);
+    try {
+        for (size_t done = 0; done < num_blobs;) {
+            const uint32_t nb = (uint32_t)std::min<size_t>(p->max_batch, num_blobs - done);
+            const uint8_t *pm = cell_present + done * 128;
+            if (!chunk_missing_ok(pm, nb)) return VKZG_ERR_BADARGS;
+            hipStream_t s = p->stream;
+            std::memcpy(p->h_in, cells + done * kCellsBytes, nb * kCellsBytes);
+            std::memcpy(p->h_present, pm, nb * 128);
+            ck(hipMemcpyAsync(p->d_cells, p->h_in, nb * kCellsBytes, hipMemcpyHostToDevice, s));
+            ck(hipMemcpyAsync(p->d_present, p->h_present, nb * 128, hipMemcpyHostToDevice, s));
+            StageClock clk(p, s);
+            enqueue_recovery(p, p->d_cells_out, p->d_cells, p->d_present, nb, s, clk);
+            ck(hipMemcpyAsync(p->h_out, p->d_cells_out, nb * kCellsBytes, hipMemcpyDeviceToHost, s));
+            ck(hipStreamSynchronize(s));
+            clk.finish();
+            std::memcpy(cells_out + done * kCellsBytes, p->h_out, nb * kCellsBytes);
+            done += nb;
+        }
+    } catch (HipFail &) {
+        return VKZG_ERR_GPU;
+    }
+    return VKZG_OK;
+}
+
+vkzg_result vkzg_amd_compute_proofs_device(vkzg_prover *p, void *d_proofs, const void *d_blobs,
+                                           size_t num_blobs, void *hip_stream) {
+    if (!p || !d_proofs || !d_blobs) return VKZG_ERR_BADARGS;
+    if (num_blobs == 0) return VKZG_OK;
+    std::lock_guard<std::mutex> lock(p->mu);
+    DeviceGuard guard(p->device);
+    reset_stage_times(p);
+    hipStream_t s = hip_stream ? (hipStream_t)hip_stream : p->stream;
+    try {
+        for (size_t done = 0; done < num_blobs;) {
+            const uint32_t nb = (uint32_t)std::min<size_t>(p->max_batch, num_blobs - done);
+            StageClock clk(p, s);
+            enqueue_proofs(p, (uint8_t *)d_proofs + done * kProofBytes,
+                           (const uint8_t *)d_blobs + done * kBlobBytes, nb, s, clk);
+            ck(hipMemcpyAsync(p->h_err, p->d_err, nb * 4, hipMemcpyDeviceToHost, s));
+            ck(hipStreamSynchronize(s));
+            clk.finish();
+            for (uint32_t b = 0; b < nb; b++)
+                if (p->h_err[b]) return VKZG_ERR_INVALID_BLOB;
+            done += nb;
+        }
+    } catch (HipFail &) {
+        return VKZG_ERR_GPU;
+    }
+    return VKZG_OK;
+}
+
+vkzg_result vkzg_amd_recover_cells_device(vkzg_prover *p, void *d_cells_out, const void *d_cells,
+                                          const void *d_present, const uint8_t *h_present,
+                                          size_t num_blobs, void *hip_stream) {
+    if (!p || !d_cells_out || !d_cells || !d_present || !h_present) return VKZG_ERR_BADARGS;
+    if (num_blobs == 0) return VKZG_OK;
+    std::lock_guard<std::mutex> lock(p->mu);
+    DeviceGuard guard(p->device);
+    reset_stage_times(p);
+    hipStream_t s = hip_stream ? (hipStream_t)hip_stream : p->stream;
+    try {
+        for (size_t done = 0; done < num_blobs;) {
+            const uint32_t nb = (uint32_t)std::min<size_t>(p->max_batch, num_blobs - done);
+            if (!chunk_missing_ok(h_present + done * 128, nb)) return VKZG_ERR_BADARGS;
+            StageClock clk(p, s);
+            enqueue_recovery(p, (uint8_t *)d_cells_out + done * kCellsBytes,
+                             (const uint8_t *)d_cells + done * kCellsBytes,
+                             (const uint8_t *)d_present + done * 128, nb, s, clk);
+            ck(hipStreamSynchronize(s));
+            clk.finish();
+            done += nb;
+        }
+    } catch (HipFail &) {
+        return VKZG_ERR_GPU;
+    }
+    return VKZG_OK;
+}
+
+vkzg_result vkzg_amd_prover_info(const vkzg_prover *p, vkzg_amd_info *out) {
+    if (!p || !out) return VKZG_ERR_BADARGS;
+    out->window_bits = p->c;
+    out->windows = p->W;
+    out->table_bytes = p->table_bytes;
+    out->working_bytes = p->working_bytes;
+    out->max_batch_size = p->max_batch;
+    out->setup_ms = p->setup_ms;
+    return VKZG_OK;
+}
+
+vkzg_result vkzg_amd_set_stage_timing(vkzg_prover *p, int enable) {
+    if (!p) return VKZG_ERR_BADARGS;
+    std::lock_guard<std::mutex> lock(p->mu);
+    p->timing = enable != 0;
+    return VKZG_OK;
+}
+
+int vkzg_amd_last_stage_times(vkzg_prover *p, float *ms, const char **names, int max) {
+    if (!p) return 0;
+    std::lock_guard<std::mutex> lock(p->mu);
+    const int n = p->nstages < max ? p->nstages : max;
+    for (int i = 0; i < n; i++) {
+        if (ms) ms[i] = p->stage_ms[i];
+        if (names) names[i] = p->stage_names[i];
+    }
+    return n;
+}
+
+static vkzg_result debug_mul(vkzg_prover *p, uint32_t *out, const uint32_t *a, const uint32_t *b,
+                             size_t n, int limbs) {
+    if (!out || !a || !b) return VKZG_ERR_BADARGS;
+    if (n == 0) return VKZG_OK;
+    // A prover is optional: without one the current device and its null
+    // stream are used.
+    std::unique_lock<std::mutex> lock;
+    if (p) lock = std::unique_lock<std::mutex>(p->mu);
+    int dev = 0;
+    if (p) dev = p->device;
+    else if (hipGetDevice(&dev) != hipSuccess) return VKZG_ERR_GPU;
+    DeviceGuard guard(dev);
+    hipStream_t stream = p ? p->stream : nullptr;
+    void *da = nullptr, *db = nullptr, *dc = nullptr;
+    const size_t bytes = n * limbs * 4;
+    vkzg_result rc = VKZG_OK;
+    try {
+        ck(hipMalloc(&da, bytes));
+        ck(hipMalloc(&db, bytes));
+        ck(hipMalloc(&dc, bytes));
+        ck(hipMemcpyAsync(da, a, bytes, hipMemcpyHostToDevice, stream));
+        ck(hipMemcpyAsync(db, b, bytes, hipMemcpyHostToDevice, stream));
+        if (limbs == 12)
+            ck(launch_debug_fp_mul((uint32_t *)dc, (const uint32_t *)da, (const uint32_t *)db,
+                                   (uint32_t)n, stream));
+        else
+            ck(launch_debug_fr_mul((uint32_t *)dc, (const uint32_t *)da, (const uint32_t *)db,
+                                   (uint32_t)n, stream));
+        ck(hipMemcpyAsync(out, dc, bytes, hipMemcpyDeviceToHost, stream));
+        ck(hipStreamSynchronize(stream));
+    } catch (HipFail &) {
+        rc = VKZG_ERR_GPU;
+    }
+    (void)hipFree(da);
+    (void)hipFree(db);
+    (void)hipFree(dc);
+    return rc;
+}
+
+vkzg_result vkzg_amd_debug_fp_mul(vkzg_prover *p, uint32_t *out, const uint32_t *a,
+                                  const uint32_t *b, size_t n) {
+    return debug_mul(p, out, a, b, n, 12);
+}
+
+vkzg_result vkzg_amd_debug_fr_mul(vkzg_prover *p, uint32_t *out, const uint32_t *a,
+                                  const uint32_t *b, size_t n) {
+    return debug_mul(p, out, a, b, n, 8);
+}
+
+}  // extern "C"

@@ -1,0 +1,167 @@
+// Cell-recovery kernels (the reference's recoverBatch, src/vkzg.cpp:1086-1179).
+//
+// The reference computes P = coset_iNTT(coset_NTT(iNTT(E * NTT(Z))) / coset_NTT(Z))
+// and evaluates NTT(P).  Z(X) = z(X^64) with z of degree <= 64 in X^64, so both
+// of Z's 8192-point evaluations take only 128 distinct values:
+//   Z(omega^k)     = z(omega_128^(k mod 128))            (zero on missing cells)
+//   Z(7 omega^k)   = z(7^64 omega_128^(k mod 128))
+// k_rec_vanishing computes those 2 x 128 values (and the 128 inverses) from
+// the product form of z, which removes two of the six transforms and the
+// per-element Fermat inversion of k_ew_div.comp.  The remaining four
+// transforms run as two four-step passes each (8192 = 64 x 128) with every
+// pointwise product fused into a pass:
+//   pass A/B inverse   : load cells * Z(omega^k)          -> * 7^k / 8192
+//   pass A/B forward   :                                  -> * 1 / Z(7 omega^k)
+//   pass A/B inverse   :                                  -> * 7^-k / 8192
+//   pass A/B forward   :                                  -> canonical cell bytes
+#include "kernels.h"
+#include "ntt.h"
+
+namespace vkzg_amd {
+
+namespace {
+
+// Per blob: zd[k] = z(omega_128^k), izc[k] = 1 / z(7^64 omega_128^k),
+// z(y) = prod over missing wire cells c of (y - omega_128^brev7(c))
+// (k_vanishing_poly.comp, src/vkzg.cpp:1043-1050).
+__global__ void __launch_bounds__(256)
+    k_rec_vanishing(Fr *zd, Fr *izc, const uint8_t *present, const Fr *roots) {
+    const uint32_t blob = blockIdx.x;
+    const uint32_t t = threadIdx.x;
+    const uint32_t k = t & 127u;
+    const bool coset = t >= 128;
+    Fr x = roots[(64u * k) & 8191u];
+    if (coset) x = fr_mul(x, fr_const(cfr::SEVEN_POW64_M));
+    const uint8_t *pm = present + 128ull * blob;
+    Fr prod = fr_one();
+#pragma unroll 1
+    for (uint32_t c = 0; c < 128; c++) {
+        if (pm[c]) continue;
+        const Fr root = roots[(64u * brev_bits(c, 7)) & 8191u];
+        prod = fr_mul(prod, fr_sub(x, root));
+    }
+    if (coset) izc[blob * 128 + k] = fr_inv(prod);
+    else zd[blob * 128 + k] = prod;
+}
+
+constexpr int kCols = 8;       // sub-transforms per block
+constexpr int kThreads = 256;
+
+// Pass A: for each n1 < 64, DFT_128 over n2 of x[n1 + 64 n2], times
+// omega^(+-n1 k2), stored at y[n1 * 128 + k2].  Block = 8 consecutive n1.
+template <int IN_MODE, bool INVERSE>
+__global__ void __launch_bounds__(kThreads)
+    k_rec_pass_a(Fr *out, const void *in, const Fr *zd, const Fr *roots_dir) {
+    constexpr int n = 128;
+    __shared__ uint32_t lds[8 * kCols * n];
+    constexpr int pitch = kCols * n;
+    const uint32_t blob = blockIdx.y;
+    const uint32_t n1_0 = blockIdx.x * kCols;
+    const int tid = threadIdx.x;
+#pragma unroll 1
+    for (int e = tid; e < kCols * n; e += kThreads) {
+        const int c = e % kCols, n2 = e / kCols;
+        const uint32_t idx = n1_0 + c + 64u * n2;  // natural evaluation index
+        Fr v;
+        if (IN_MODE == 0) {
+            // wire element brev13(idx) of the blob's cells, reduced mod r, times z
+            const uint32_t *cw = reinterpret_cast<const uint32_t *>(in) +
+                                 ((size_t)blob * 8192 + brev_bits(idx, 13)) * 8;
+            uint32_t w[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) w[k] = cw[k];
+            v = fr_mul(fr_from_any(fr_from_be_words(w)), zd[blob * 128 + (idx & 127u)]);
+        } else {
+            v = reinterpret_cast<const Fr *>(in)[(size_t)blob * 8192 + idx];
+        }
+        lds_store_fr(lds, pitch, c * n + (int)brev_bits(n2, 7), v);
+    }
+    __syncthreads();
+    lds_ntt_dit<7>(lds, pitch, kCols, roots_dir, tid, kThreads);
+#pragma unroll 1
+    for (int e = tid; e < kCols * n; e += kThreads) {
+        const int k2 = e % n, c = e / n;
+        const uint32_t n1 = n1_0 + c;
+        const Fr v = fr_mul(lds_load_fr(lds, pitch, c * n + k2), roots_dir[(n1 * k2) & 8191u]);
+        out[(size_t)blob * 8192 + n1 * 128 + k2] = v;
+    }
+}
+
+// Pass B: for each k2 < 128, DFT_64 over n1 of y[n1 * 128 + k2], giving
+// X[k2 + 128 k1].  Block = 8 consecutive k2.
+template <int OUT_MODE, bool INVERSE>
+__global__ void __launch_bounds__(kThreads)
+    k_rec_pass_b(void *out, const Fr *in, const Fr *mul_table, const Fr *izc, const Fr *roots_dir) {
+    constexpr int n = 64;
+    __shared__ uint32_t lds[8 * kCols * n];
+    constexpr int pitch = kCols * n;
+    const uint32_t blob = blockIdx.y;
+    const uint32_t k2_0 = blockIdx.x * kCols;
+    const int tid = threadIdx.x;
+#pragma unroll 1
+    for (int e = tid; e < kCols * n; e += kThreads) {
+        const int c = e % kCols, n1 = e / kCols;
+        const Fr v = in[(size_t)blob * 8192 + n1 * 128 + k2_0 + c];
+        lds_store_fr(lds, pitch, c * n + (int)brev_bits(n1, 6), v);
+    }
+    __syncthreads();
+    lds_ntt_dit<6>(lds, pitch, kCols, roots_dir, tid, kThreads);
+#pragma unroll 1
+    for (int e = tid; e < kCols * n; e += kThreads) {
+        const int c = e % kCols, k1 = e / kCols;
+        const uint32_t k = k2_0 + c + 128u * k1;  // natural output index
+        Fr v = lds_load_fr(lds, pitch, c * n + k1);
+        if (OUT_MODE == 0) {
+            reinterpret_cast<Fr *>(out)[(size_t)blob * 8192 + k] = fr_mul(v, mul_table[k]);
+        } else if (OUT_MODE == 1) {
+            reinterpret_cast<Fr *>(out)[(size_t)blob * 8192 + k] =
+                fr_mul(v, izc[blob * 128 + (k & 127u)]);
+        } else {
+            uint32_t w[8];
+            fr_to_be_words(w, fr_to_canonical(v));
+            uint32_t *dst = reinterpret_cast<uint32_t *>(out) +
+                            ((size_t)blob * 8192 + brev_bits(k, 13)) * 8;
+#pragma unroll
+            for (int q = 0; q < 8; q++) dst[q] = w[q];
+        }
+    }
+}
+
+}  // namespace
+
+hipError_t launch_rec_vanishing(Fr *zd, Fr *izc, const uint8_t *present, const Fr *roots,
+                                uint32_t nblobs, hipStream_t s) {
+    hipLaunchKernelGGL(k_rec_vanishing, dim3(nblobs), dim3(256), 0, s, zd, izc, present, roots);
+    return hipGetLastError();
+}
+
+hipError_t launch_rec_pass_a(Fr *out, const void *in, int in_mode, const Fr *zd, bool inverse,
+                             const Fr *roots, const Fr *roots_inv, uint32_t nblobs, hipStream_t s) {
+    const dim3 grid(64 / kCols, nblobs), block(kThreads);
+    const Fr *r = inverse ? roots_inv : roots;
+    if (in_mode == 0 && inverse)
+        hipLaunchKernelGGL((k_rec_pass_a<0, true>), grid, block, 0, s, out, in, zd, r);
+    else if (in_mode == 0)
+        hipLaunchKernelGGL((k_rec_pass_a<0, false>), grid, block, 0, s, out, in, zd, r);
+    else if (inverse)
+        hipLaunchKernelGGL((k_rec_pass_a<1, true>), grid, block, 0, s, out, in, zd, r);
+    else
+        hipLaunchKernelGGL((k_rec_pass_a<1, false>), grid, block, 0, s, out, in, zd, r);
+    return hipGetLastError();
+}
+
+hipError_t launch_rec_pass_b(void *out, const Fr *in, int out_mode, const Fr *mul_table,
+                             const Fr *izc, bool inverse, const Fr *roots, const Fr *roots_inv,
+                             uint32_t nblobs, hipStream_t s) {
+    const dim3 grid(128 / kCols, nblobs), block(kThreads);
+    const Fr *r = inverse ? roots_inv : roots;
+    if (out_mode == 0)
+        hipLaunchKernelGGL((k_rec_pass_b<0, true>), grid, block, 0, s, out, in, mul_table, izc, r);
+    else if (out_mode == 1)
+        hipLaunchKernelGGL((k_rec_pass_b<1, false>), grid, block, 0, s, out, in, mul_table, izc, r);
+    else
+        hipLaunchKernelGGL((k_rec_pass_b<2, false>), grid, block, 0, s, out, in, mul_table, izc, r);
+    return hipGetLastError();
+}
+
+}  // namespace vkzg_amd
