@@ -76,7 +76,18 @@ def main():
     ap.add_argument("--cpu-sample-blobs", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run 3 device-to-device copies of MIB MiB (counter calibration)")
     a = ap.parse_args()
+    if a.calibrate_copy:
+        from vkzg_amd import hip
+        n = a.calibrate_copy << 20
+        src, dst = hip.DeviceBuffer(n), hip.DeviceBuffer(n)
+        src.zero()
+        for _ in range(3):
+            hip.check(hip.lib().hipMemcpy(dst.ptr, src.ptr, n, hip.D2D))
+        print(json.dumps({"calibrate_copy_bytes": n, "copies": 3}))
+        return
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if a.gpus != 1 or world != 1:
         print("bench.py measures one GPU only: this build runs the whole batch on a single "

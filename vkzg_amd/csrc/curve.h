@@ -14,10 +14,12 @@
 
 #include "field.h"
 
-// Point operations are out-of-line device functions: each contains 8-15
-// inlined field multiplications, and inlining them at every call site made
-// kernels large enough to slow compilation by minutes for no measurable gain.
-#define VKZG_DEV_CALL __device__ __attribute__((noinline))
+// Point additions and doublings are inlined into the kernels (passing 100-200
+// byte points to an out-of-line function goes through the stack in scratch
+// memory, which measured at several times the kernels' useful HBM traffic).
+// The P == Q fallbacks inside additions are rare and stay out of line.
+#define VKZG_DEV_CALL __device__ __forceinline__
+#define VKZG_DEV_COLD __device__ __attribute__((noinline))
 
 namespace vkzg_amd {
 
@@ -85,6 +87,8 @@ VKZG_DEV_CALL G1J g1j_dbl(const G1J &p) {
     return r;
 }
 
+VKZG_DEV_COLD G1J g1j_dbl_cold(const G1J &p);
+
 // add-2007-bl
 VKZG_DEV_CALL G1J g1j_add(const G1J &a, const G1J &b) {
     if (fp_is_zero(a.z)) return b;
@@ -96,7 +100,7 @@ VKZG_DEV_CALL G1J g1j_add(const G1J &a, const G1J &b) {
     const Fp S1 = fp_mul(a.y, fp_mul(b.z, Z2Z2));
     const Fp S2 = fp_mul(b.y, fp_mul(a.z, Z1Z1));
     if (fp_eq(U1, U2)) {
-        if (fp_eq(S1, S2)) return g1j_dbl(a);
+        if (fp_eq(S1, S2)) return g1j_dbl_cold(a);
         return g1j_identity();
     }
     const Fp H = fp_sub(U2, U1);
@@ -113,6 +117,8 @@ VKZG_DEV_CALL G1J g1j_add(const G1J &a, const G1J &b) {
     return r;
 }
 
+VKZG_DEV_COLD G1J g1j_dbl_cold(const G1J &p) { return g1j_dbl(p); }
+
 // madd-2007-bl (b affine, may be the identity)
 VKZG_DEV_CALL G1J g1j_madd(const G1J &a, const G1A &b) {
     if (g1a_is_identity(b)) return a;
@@ -121,7 +127,7 @@ VKZG_DEV_CALL G1J g1j_madd(const G1J &a, const G1A &b) {
     const Fp U2 = fp_mul(b.x, Z1Z1);
     const Fp S2 = fp_mul(b.y, fp_mul(a.z, Z1Z1));
     if (fp_eq(a.x, U2)) {
-        if (fp_eq(a.y, S2)) return g1j_dbl(a);
+        if (fp_eq(a.y, S2)) return g1j_dbl_cold(a);
         return g1j_identity();
     }
     const Fp H = fp_sub(U2, a.x);
@@ -203,6 +209,8 @@ VKZG_DEV_CALL G1X g1x_dbl(const G1X &p) {
     return r;
 }
 
+VKZG_DEV_COLD G1X g1x_dbl_cold(const G1X &p) { return g1x_dbl(p); }
+
 // madd-2008-s: XYZZ + affine (b may be the identity)
 VKZG_DEV_CALL G1X g1x_madd(const G1X &a, const G1A &b) {
     if (g1a_is_identity(b)) return a;
@@ -212,7 +220,7 @@ VKZG_DEV_CALL G1X g1x_madd(const G1X &a, const G1A &b) {
     const Fp P = fp_sub(U2, a.x);
     const Fp R = fp_sub(S2, a.y);
     if (fp_is_zero(P)) {
-        if (fp_is_zero(R)) return g1x_dbl(g1x_from_affine(b));
+        if (fp_is_zero(R)) return g1x_dbl_cold(g1x_from_affine(b));
         return g1x_identity();
     }
     const Fp PP = fp_sqr(P);
@@ -237,7 +245,7 @@ VKZG_DEV_CALL G1X g1x_add(const G1X &a, const G1X &b) {
     const Fp P = fp_sub(U2, U1);
     const Fp R = fp_sub(S2, S1);
     if (fp_is_zero(P)) {
-        if (fp_is_zero(R)) return g1x_dbl(a);
+        if (fp_is_zero(R)) return g1x_dbl_cold(a);
         return g1x_identity();
     }
     const Fp PP = fp_sqr(P);

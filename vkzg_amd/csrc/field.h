@@ -1,12 +1,12 @@
 // BLS12-381 Fp (12 x 32-bit limbs) and Fr (8 x 32-bit limbs) Montgomery
 // arithmetic for gfx950, held in VGPRs.
 //
-// Multiplication is CIOS with the "no final carry word" simplification: both
-// moduli have a clear top bit in their top limb (p: 0x1a0111ea..., r:
-// 0x73eda753...), so the running value t stays below 2^(32N) and the two
-// inner chains (a*b_i and m*p) can share one pass over the limbs.  Each limb
-// step is a 32x32->64 multiply-add (v_mad_u64_u32) plus a carry add; inputs
-// below the modulus give an output below twice the modulus, reduced once.
+// Multiplication is Montgomery by product scanning (mont_mul below): each
+// 32x32->64 limb product is one v_mad_u64_u32 whose carry-out feeds a
+// v_addc_co_u32 into a 96-bit column accumulator.  Both moduli have a clear
+// top bit in their top limb (p: 0x1a0111ea..., r: 0x73eda753...), so inputs
+// below the modulus give a product below twice the modulus in N words,
+// reduced by one conditional subtraction.
 //
 // Semantics follow the reference's field library (src/shaders/field.glsl.inc
 // :69-297): little-endian limbs, Montgomery radix 2^(32N), canonical results.
@@ -80,9 +80,68 @@ VKZG_DEV void sub_mod(uint32_t *r, const uint32_t *a, const uint32_t *b, const u
     }
 }
 
+// acc (64-bit) += x * y and hi += carry-out: v_mad_u64_u32 leaves its carry
+// in VCC and v_addc folds it into the third accumulator word, two VALU
+// instructions per limb product (the compiler's own lowering of the same
+// expression needs five to six).  `y` is an SGPR operand when it is a
+// wave-uniform modulus limb.
+VKZG_DEV void mac_vv(uint64_t &acc, uint32_t &hi, uint32_t x, uint32_t y) {
+    asm("v_mad_u64_u32 %0, vcc, %2, %3, %0\n\tv_addc_co_u32_e32 %1, vcc, 0, %1, vcc"
+        : "+v"(acc), "+v"(hi)
+        : "v"(x), "v"(y)
+        : "vcc");
+}
+VKZG_DEV void mac_vs(uint64_t &acc, uint32_t &hi, uint32_t x, uint32_t y) {
+    asm("v_mad_u64_u32 %0, vcc, %2, %3, %0\n\tv_addc_co_u32_e32 %1, vcc, 0, %1, vcc"
+        : "+v"(acc), "+v"(hi)
+        : "v"(x), "s"(y)
+        : "vcc");
+}
+
+// Montgomery multiplication by finely integrated product scanning (FIPS):
+// column k of a*b and of m*p are summed together in a 96-bit accumulator,
+// m_k = low word * n0 makes the low word vanish, and the accumulator shifts
+// down one word per column.  Inputs below the modulus give a result below
+// twice the modulus (the modulus' top bit is clear), reduced once.
 template <int N>
-VKZG_DEV void mont_mul(uint32_t *r, const uint32_t *a, const uint32_t *b, const uint32_t (&m)[N],
+VKZG_DEV void mont_mul(uint32_t *r, const uint32_t *a, const uint32_t *b, const uint32_t (&p)[N],
                        uint32_t n0) {
+    uint32_t m[N];
+    uint64_t acc = 0;
+    uint32_t hi = 0;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+#pragma unroll
+        for (int j = 0; j < i; j++) {
+            mac_vv(acc, hi, a[j], b[i - j]);
+            mac_vs(acc, hi, m[j], p[i - j]);
+        }
+        mac_vv(acc, hi, a[i], b[0]);
+        m[i] = (uint32_t)acc * n0;
+        mac_vs(acc, hi, m[i], p[0]);
+        acc = (acc >> 32) | ((uint64_t)hi << 32);
+        hi = 0;
+    }
+    uint32_t t[N];
+#pragma unroll
+    for (int i = N; i < 2 * N; i++) {
+#pragma unroll
+        for (int j = i - N + 1; j < N; j++) {
+            mac_vv(acc, hi, a[j], b[i - j]);
+            mac_vs(acc, hi, m[j], p[i - j]);
+        }
+        t[i - N] = (uint32_t)acc;
+        acc = (acc >> 32) | ((uint64_t)hi << 32);
+        hi = 0;
+    }
+    reduce_once<N>(r, t, p);
+}
+
+// The same product by operand scanning (CIOS) in plain C++; the test hooks
+// compute both forms and flag any disagreement.
+template <int N>
+VKZG_DEV void mont_mul_cios(uint32_t *r, const uint32_t *a, const uint32_t *b,
+                            const uint32_t (&m)[N], uint32_t n0) {
     uint32_t t[N];
 #pragma unroll
     for (int j = 0; j < N; j++) t[j] = 0;

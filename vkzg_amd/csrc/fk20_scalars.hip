@@ -1,0 +1,107 @@
+// FK20 cell-proof kernels (the reference's computeBatch, src/vkzg.cpp:899-1039).
+//
+//   k_fk20_scalars   blob bytes -> canonical check -> iNTT_4096 -> circulant
+//                    columns -> NTT_128 -> phase-A scalars (one block per blob,
+//                    the whole polynomial resident in LDS)
+//   k_phase_a        u[j] = sum_i scal[j][i] * xext[j][i]: fixed-base windowed
+//                    MSM against the resident multiples table, 16 lanes per MSM
+//   k_ladder/k_fold  2^(8d) * u[j] and the split-circulant inputs L+-
+//   k_phase_b        phase-B buckets, one lane per (blob, output, bucket)
+//   k_bucket_reduce  sum_k k * B[k-1] per output
+//   k_finalize       C+ +- C-, affine, compressed, bit-reversed order
+
+#include "proof_common.h"
+
+namespace vkzg_amd {
+
+namespace {
+constexpr int kScalarThreads = 256;
+constexpr int kPolyPitch = 4096;          // LDS pitch of the polynomial (SoA limbs)
+constexpr int kColPitch = 4 * 128;        // LDS pitch of the 4 per-wave circulant columns
+
+// One block per blob.  LDS: 8 x 4096 words of polynomial + 8 x 512 words of
+// circulant columns = 144 KiB (gfx950 has 160 KiB per CU).
+__global__ void __launch_bounds__(kScalarThreads)
+    k_fk20_scalars(Fr *scal, const uint8_t *blobs, uint32_t *err, const Fr *roots,
+                   const Fr *roots_inv) {
+    __shared__ uint32_t poly[8 * kPolyPitch];
+    __shared__ uint32_t cols[8 * kColPitch];
+    const uint32_t blob = blockIdx.x;
+    const int tid = threadIdx.x;
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(blobs + 131072ull * blob);
+
+    // 1. Parse, check canonical, enter Montgomery form (k_blob_to_fr.comp:23-37).
+    //    blob[i] = p(omega_4096^brev12(i)), i.e. the bit-reversed evaluation
+    //    order a decimation-in-time transform takes as input.
+    bool bad = false;
+#pragma unroll 1
+    for (int i = tid; i < 4096; i += kScalarThreads) {
+        uint32_t w[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) w[k] = src[8 * i + k];
+        const Fr v = fr_from_be_words(w);
+        if (!limbs_lt<8>(v.v, cfr::P)) bad = true;
+        lds_store_fr(poly, kPolyPitch, i, fr_from_canonical(v));
+    }
+    if (bad) atomicOr(&err[blob], 1u);
+    __syncthreads();
+
+    // 2. Inverse NTT_4096 -> monomial coefficients, scaled by 1/4096.
+    lds_ntt_dit<12>(poly, kPolyPitch, 1, roots_inv, tid, kScalarThreads);
+    const Fr inv_n = fr_const(cfr::INV_4096_M);
+#pragma unroll 1
+    for (int i = tid; i < 4096; i += kScalarThreads)
+        lds_store_fr(poly, kPolyPitch, i, fr_mul(lds_load_fr(poly, kPolyPitch, i), inv_n));
+    __syncthreads();
+
+    // 3. 64 circulant columns, 4 at a time (one per wave): tc_i[0] = c[4095-i],
+    //    tc_i[128-m] = c[4095-i-64m] (m = 1..62), zero elsewhere; NTT_128;
+    //    scal[j][i] = canonical(result[j]) (k_build_circulant.comp:30-72).
+    const int wave = tid >> 6;
+    const int lane = tid & 63;
+    uint32_t *col = cols + wave * 128;
+    Fr *out = scal + (size_t)blob * 128 * 64;
+#pragma unroll 1
+    for (int round = 0; round < 16; round++) {
+        const int i = round * 4 + wave;
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int t = lane + 64 * h;
+            Fr v = fr_zero();
+            if (t == 0) v = lds_load_fr(poly, kPolyPitch, 4095 - i);
+            else if (t >= 66) v = lds_load_fr(poly, kPolyPitch, 4095 - i - 64 * (128 - t));
+            lds_store_fr(col, kColPitch, (int)brev_bits(t, 7), v);
+        }
+        __syncthreads();
+        // NTT_128 of this wave's column; all four waves step together.
+#pragma unroll 1
+        for (int s = 1; s <= 7; s++) {
+            const int half = 1 << (s - 1);
+            const int j = lane & (half - 1);
+            const int i0 = ((lane >> (s - 1)) << s) + j;
+            const int i1 = i0 + half;
+            const Fr u = lds_load_fr(col, kColPitch, i0);
+            const Fr vv = fr_mul(lds_load_fr(col, kColPitch, i1), roots[j << (13 - s)]);
+            lds_store_fr(col, kColPitch, i0, fr_add(u, vv));
+            lds_store_fr(col, kColPitch, i1, fr_sub(u, vv));
+            __syncthreads();
+        }
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int j = lane + 64 * h;
+            out[j * 64 + i] = fr_to_canonical(lds_load_fr(col, kColPitch, j));
+        }
+        __syncthreads();
+    }
+}
+
+}  // namespace
+
+hipError_t launch_fk20_scalars(Fr *scal, const uint8_t *blobs, uint32_t *err, const Fr *roots,
+                               const Fr *roots_inv, uint32_t nblobs, hipStream_t s) {
+    hipLaunchKernelGGL(k_fk20_scalars, dim3(nblobs), dim3(kScalarThreads), 0, s, scal, blobs, err,
+                       roots, roots_inv);
+    return hipGetLastError();
+}
+
+}  // namespace vkzg_amd

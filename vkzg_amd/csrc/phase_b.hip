@@ -1,0 +1,42 @@
+// FK20 phase B buckets (split circulant, fixed scalars, variable points).
+#include "proof_common.h"
+
+namespace vkzg_amd {
+
+namespace {
+// Lane (blob, a, k): bucket k of output a.  Output a < 64 is the cyclic half
+// over L+, a >= 64 the negacyclic half over L- with a sign flip when a tap
+// wraps (k_phase_b_split.comp:25-61).  The item list depends only on (half,
+// k), so it is uniform across a wave.
+__global__ void __launch_bounds__(64)
+    k_phase_b(G1X *buckets, const G1A *lad_aff, const uint32_t *items, const uint32_t *offsets) {
+    const uint32_t k = blockIdx.x;                       // bucket, 0..127
+    const uint32_t ba = blockIdx.y * 64 + threadIdx.x;   // blob * 128 + a
+    const uint32_t blob = ba / 128, a = ba % 128;
+    const uint32_t half = a >> 6, aa = a & 63u;
+    const uint32_t lo = offsets[half * 129 + k], hi = offsets[half * 129 + k + 1];
+    const G1A *L = lad_aff + (size_t)(blob * 2 + half) * 64 * 32;
+    G1X acc = g1x_identity();
+#pragma unroll 1
+    for (uint32_t it = lo; it < hi; it++) {
+        const uint32_t packed = items[it];
+        const uint32_t e = packed & 0xffu;
+        const uint32_t d = (packed >> 8) & 0xffu;
+        bool neg = ((packed >> 16) & 1u) != 0;
+        if (half && e > aa) neg = !neg;
+        const uint32_t src = (aa - e) & 63u;
+        acc = g1x_madd(acc, g1a_neg_if(L[src * 32 + d], neg));
+    }
+    buckets[(size_t)ba * 128 + k] = acc;
+}
+
+}  // namespace
+
+hipError_t launch_phase_b(G1X *buckets, const G1A *lad_aff, const uint32_t *items,
+                          const uint32_t *offsets, uint32_t nblobs, hipStream_t s) {
+    hipLaunchKernelGGL(k_phase_b, dim3(128, nblobs * 2), dim3(64), 0, s, buckets, lad_aff, items,
+                       offsets);
+    return hipGetLastError();
+}
+
+}  // namespace vkzg_amd

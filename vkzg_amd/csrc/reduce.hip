@@ -1,0 +1,98 @@
+// FK20 phase B reduction, recombination and proof encoding.
+#include "proof_common.h"
+
+namespace vkzg_amd {
+
+namespace {
+constexpr int kReduceLanes = 8;
+
+// cvals[blob][a] = sum_{k=1..128} k * buckets[blob][a][k-1]
+// (field.glsl.inc:478-517).  Lane l of a team owns buckets [16l, 16l+16):
+// rs_l = sum_i (i+1) B[16l+i] by running sums, s_l = sum_i B[16l+i]; the
+// output is sum_l (rs_l + 16 l * s_l).
+__global__ void __launch_bounds__(256) k_bucket_reduce(G1X *cvals, const G1X *buckets, uint32_t nout) {
+    const uint32_t o = blockIdx.x * (256 / kReduceLanes) + threadIdx.x / kReduceLanes;
+    const uint32_t l = threadIdx.x % kReduceLanes;
+    constexpr uint32_t per = 128 / kReduceLanes;
+    G1X s = g1x_identity(), rs = g1x_identity();
+    if (o < nout) {
+        const G1X *B = buckets + (size_t)o * 128 + l * per;
+#pragma unroll 1
+        for (int i = per - 1; i >= 0; i--) {
+            s = g1x_add(s, B[i]);
+            rs = g1x_add(rs, s);
+        }
+    }
+    // t = (16 l) * s by double-and-add over the bits of 16 l.
+    const uint32_t mult = per * l;
+    G1X t = g1x_identity();
+    if (mult) {
+#pragma unroll 1
+        for (int bit = 31 - __builtin_clz(mult); bit >= 0; bit--) {
+            t = g1x_dbl(t);
+            if ((mult >> bit) & 1u) t = g1x_add(t, s);
+        }
+    }
+    G1X v = g1x_add(rs, t);
+#pragma unroll 1
+    for (int m = 1; m < kReduceLanes; m <<= 1) v = g1x_add(v, g1x_shfl_xor(v, m));
+    if (o < nout && l == 0) cvals[o] = v;
+}
+
+// XYZZ -> affine for two points with one inversion.
+VKZG_DEV void g1x_pair_to_affine(G1A &ra, G1A &rb, const G1X &a, const G1X &b) {
+    const bool ia = fp_is_zero(a.zz), ib = fp_is_zero(b.zz);
+    const Fp za = ia ? fp_one() : a.zzz;
+    const Fp zb = ib ? fp_one() : b.zzz;
+    const Fp inv = fp_inv(fp_mul(za, zb));
+    const Fp iza = fp_mul(inv, zb);  // 1/zzz_a
+    const Fp izb = fp_mul(inv, za);  // 1/zzz_b
+    ra = g1a_identity();
+    rb = g1a_identity();
+    if (!ia) {
+        ra.y = fp_mul(a.y, iza);
+        ra.x = fp_mul(a.x, fp_sqr(fp_mul(a.zz, iza)));  // 1/zz = (zz/zzz)^2
+    }
+    if (!ib) {
+        rb.y = fp_mul(b.y, izb);
+        rb.x = fp_mul(b.x, fp_sqr(fp_mul(b.zz, izb)));
+    }
+}
+
+// out[a] = C+[a] + C-[a], out[a+64] = C+[a] - C-[a] (k_combine_split.comp),
+// compressed and written at brev7(k) (k_compress_proofs.comp).
+__global__ void k_finalize(uint8_t *proofs, const G1X *cvals, uint32_t nblobs) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nblobs * 64) return;
+    const uint32_t blob = g / 64, a = g % 64;
+    const G1X cp = cvals[blob * 128 + a];
+    const G1X cm = cvals[blob * 128 + a + 64];
+    G1A p0, p1;
+    g1x_pair_to_affine(p0, p1, g1x_add(cp, cm), g1x_add(cp, g1x_neg(cm)));
+    uint32_t *dst = reinterpret_cast<uint32_t *>(proofs + 6144ull * blob);
+    uint32_t w[12];
+    g1a_compress_words(w, p0);
+    const uint32_t k0 = brev_bits(a, 7), k1 = brev_bits(a + 64, 7);
+#pragma unroll
+    for (int i = 0; i < 12; i++) dst[k0 * 12 + i] = w[i];
+    g1a_compress_words(w, p1);
+#pragma unroll
+    for (int i = 0; i < 12; i++) dst[k1 * 12 + i] = w[i];
+}
+
+}  // namespace
+
+hipError_t launch_bucket_reduce(G1X *cvals, const G1X *buckets, uint32_t nblobs, hipStream_t s) {
+    const uint32_t nout = nblobs * 128;
+    hipLaunchKernelGGL(k_bucket_reduce, dim3(blocks_for(nout, 256 / kReduceLanes)), dim3(256), 0, s,
+                       cvals, buckets, nout);
+    return hipGetLastError();
+}
+
+hipError_t launch_finalize(uint8_t *proofs, const G1X *cvals, uint32_t nblobs, hipStream_t s) {
+    hipLaunchKernelGGL(k_finalize, dim3(blocks_for(nblobs * 64ull, 64)), dim3(64), 0, s, proofs,
+                       cvals, nblobs);
+    return hipGetLastError();
+}
+
+}  // namespace vkzg_amd

@@ -1,0 +1,132 @@
+// Prover creation: batched normalisation and the phase-A multiples table.
+#include "proof_common.h"
+
+namespace vkzg_amd {
+
+namespace {
+
+// Batched Jacobian -> affine (Montgomery's trick, k_normalize.comp:29-56).
+// Thread t converts the points t, t+T, t+2T, ... (T = thread count) so that
+// neighbouring lanes touch neighbouring points.
+__global__ void k_jac_to_affine(G1A *out, const G1J *in, Fp *prefix, uint64_t count, uint32_t chunk,
+                                uint64_t nthreads) {
+    const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nthreads) return;
+    Fp acc = fp_one();
+    uint32_t m = 0;
+#pragma unroll 1
+    for (uint64_t k = t; m < chunk && k < count; k += nthreads, m++) {
+        prefix[k] = acc;
+        const Fp z = in[k].z;
+        if (!fp_is_zero(z)) acc = fp_mul(acc, z);
+    }
+    Fp inv = fp_inv(acc);
+#pragma unroll 1
+    while (m-- > 0) {
+        const uint64_t k = t + (uint64_t)m * nthreads;
+        const G1J p = in[k];
+        if (fp_is_zero(p.z)) {
+            out[k] = g1a_identity();
+            continue;
+        }
+        const Fp zi = fp_mul(inv, prefix[k]);
+        inv = fp_mul(inv, p.z);
+        const Fp zi2 = fp_sqr(zi);
+        G1A r;
+        r.x = fp_mul(p.x, zi2);
+        r.y = fp_mul(p.y, fp_mul(zi2, zi));
+        out[k] = r;
+    }
+}
+
+// out[w][b] = 2^(c*w) * xext[b] for the 8192 bases.
+__global__ void k_table_chains(G1J *out, const G1A *xext, uint32_t c, uint32_t W) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= 8192) return;
+    G1J p = g1j_from_affine(xext[b]);
+#pragma unroll 1
+    for (uint32_t w = 0; w < W; w++) {
+        out[(size_t)w * 8192 + b] = p;
+#pragma unroll 1
+        for (uint32_t k = 0; k < c; k++) p = g1j_dbl(p);
+    }
+}
+
+// Thread g = (b*W + w)*nseg + seg writes table[b][w][d-1] for d in
+// [seg*S+1, seg*S+S]: a running sum of the affine base P = pw[w][b],
+// normalised with one inversion per thread.
+__global__ void k_table_multiples(G1A *table, const G1A *pw, uint32_t c, uint32_t W,
+                                  uint64_t thread_begin, uint32_t thread_count, G1J *scratch,
+                                  Fp *prefix) {
+    const uint32_t lt = blockIdx.x * blockDim.x + threadIdx.x;
+    if (lt >= thread_count) return;
+    const uint64_t g = thread_begin + lt;
+    const uint32_t H = 1u << (c - 1);
+    const uint32_t S = H < 64u ? H : 64u;
+    const uint32_t nseg = H / S;
+    const uint32_t seg = (uint32_t)(g % nseg);
+    const uint64_t bw = g / nseg;  // b*W + w
+    const uint32_t w = (uint32_t)(bw % W);
+    const uint32_t b = (uint32_t)(bw / W);
+    const G1A P = pw[(size_t)w * 8192 + b];
+    const uint32_t d0 = seg * S + 1;
+
+    G1J q = g1j_identity();
+#pragma unroll 1
+    for (int bit = 31 - __builtin_clz(d0); bit >= 0; bit--) {
+        q = g1j_dbl(q);
+        if ((d0 >> bit) & 1u) q = g1j_madd(q, P);
+    }
+    Fp acc = fp_one();
+#pragma unroll 1
+    for (uint32_t m = 0; m < S; m++) {
+        const size_t slot = (size_t)m * thread_count + lt;
+        scratch[slot] = q;
+        prefix[slot] = acc;
+        if (!fp_is_zero(q.z)) acc = fp_mul(acc, q.z);
+        q = g1j_madd(q, P);
+    }
+    Fp inv = fp_inv(acc);
+    G1A *dst = table + bw * H + (d0 - 1);
+#pragma unroll 1
+    for (int m = (int)S - 1; m >= 0; m--) {
+        const size_t slot = (size_t)m * thread_count + lt;
+        const G1J p = scratch[slot];
+        if (fp_is_zero(p.z)) {
+            dst[m] = g1a_identity();
+            continue;
+        }
+        const Fp zi = fp_mul(inv, prefix[slot]);
+        inv = fp_mul(inv, p.z);
+        const Fp zi2 = fp_sqr(zi);
+        G1A r;
+        r.x = fp_mul(p.x, zi2);
+        r.y = fp_mul(p.y, fp_mul(zi2, zi));
+        dst[m] = r;
+    }
+}
+
+}  // namespace
+
+hipError_t launch_jac_to_affine(G1A *out, const G1J *in, Fp *prefix, uint64_t count, uint32_t chunk,
+                                hipStream_t s) {
+    const uint64_t nthreads = (count + chunk - 1) / chunk;
+    hipLaunchKernelGGL(k_jac_to_affine, dim3(blocks_for(nthreads, 64)), dim3(64), 0, s, out, in,
+                       prefix, count, chunk, nthreads);
+    return hipGetLastError();
+}
+
+hipError_t launch_table_chains(G1J *out, const G1A *xext, uint32_t c, uint32_t W, hipStream_t s) {
+    hipLaunchKernelGGL(k_table_chains, dim3(8192 / 64), dim3(64), 0, s, out, xext, c, W);
+    return hipGetLastError();
+}
+
+hipError_t launch_table_multiples(G1A *table, const G1A *pw, uint32_t c, uint32_t W,
+                                  uint64_t thread_begin, uint32_t thread_count, G1J *scratch,
+                                  Fp *prefix, hipStream_t s) {
+    hipLaunchKernelGGL(k_table_multiples, dim3(blocks_for(thread_count, 64)), dim3(64), 0, s, table,
+                       pw, c, W, thread_begin, thread_count, scratch, prefix);
+    return hipGetLastError();
+}
+
+}  // namespace vkzg_amd
