@@ -144,6 +144,27 @@ VKZG_DEV_CALL G1J g1j_madd(const G1J &a, const G1A &b) {
     return r;
 }
 
+// Jacobian + affine for hot loops (see g1x_madd_fast for the contract).
+VKZG_DEV G1J g1j_madd_fast(const G1J &a, const G1A &b, bool &exc) {
+    if (g1a_is_identity(b)) return a;
+    if (fp_is_zero(a.z)) return g1j_from_affine(b);
+    const Fp Z1Z1 = fp_sqr(a.z);
+    const Fp U2 = fp_mul(b.x, Z1Z1);
+    const Fp S2 = fp_mul(b.y, fp_mul(a.z, Z1Z1));
+    const Fp H = fp_sub(U2, a.x);
+    exc |= fp_is_zero(H);
+    const Fp HH = fp_sqr(H);
+    const Fp I = fp_dbl(fp_dbl(HH));
+    const Fp J = fp_mul(H, I);
+    const Fp K = fp_dbl(fp_sub(S2, a.y));
+    const Fp V = fp_mul(a.x, I);
+    G1J r;
+    r.x = fp_sub(fp_sub(fp_sqr(K), J), fp_dbl(V));
+    r.y = fp_sub(fp_mul(K, fp_sub(V, r.x)), fp_dbl(fp_mul(a.y, J)));
+    r.z = fp_sub(fp_sub(fp_sqr(fp_add(a.z, H)), Z1Z1), HH);
+    return r;
+}
+
 VKZG_DEV G1J g1j_neg(const G1J &p) {
     G1J r = p;
     r.y = fp_neg(p.y);
@@ -231,6 +252,53 @@ VKZG_DEV_CALL G1X g1x_madd(const G1X &a, const G1A &b) {
     r.y = fp_sub(fp_mul(R, fp_sub(Q, r.x)), fp_mul(a.y, PPP));
     r.zz = fp_mul(a.zz, PP);
     r.zzz = fp_mul(a.zzz, PPP);
+    return r;
+}
+
+// Hot-loop variants.  They handle either operand being the identity, and
+// when the inputs are equal or opposite (the doubling / cancellation cases,
+// which random data never reaches) they set `exc` and return an unspecified
+// point; callers then recompute that result with the checked forms above
+// outside their loops.  Keeping the rare path out of the loop body is what
+// keeps the accumulator in VGPRs (a call inside the loop that takes the
+// point by address pins it to scratch memory).
+VKZG_DEV G1X g1x_madd_fast(const G1X &a, const G1A &b, bool &exc) {
+    if (g1a_is_identity(b)) return a;
+    if (fp_is_zero(a.zz)) return g1x_from_affine(b);
+    const Fp U2 = fp_mul(b.x, a.zz);
+    const Fp S2 = fp_mul(b.y, a.zzz);
+    const Fp P = fp_sub(U2, a.x);
+    const Fp R = fp_sub(S2, a.y);
+    exc |= fp_is_zero(P);
+    const Fp PP = fp_sqr(P);
+    const Fp PPP = fp_mul(P, PP);
+    const Fp Q = fp_mul(a.x, PP);
+    G1X r;
+    r.x = fp_sub(fp_sub(fp_sqr(R), PPP), fp_dbl(Q));
+    r.y = fp_sub(fp_mul(R, fp_sub(Q, r.x)), fp_mul(a.y, PPP));
+    r.zz = fp_mul(a.zz, PP);
+    r.zzz = fp_mul(a.zzz, PPP);
+    return r;
+}
+
+VKZG_DEV G1X g1x_add_fast(const G1X &a, const G1X &b, bool &exc) {
+    if (fp_is_zero(a.zz)) return b;
+    if (fp_is_zero(b.zz)) return a;
+    const Fp U1 = fp_mul(a.x, b.zz);
+    const Fp U2 = fp_mul(b.x, a.zz);
+    const Fp S1 = fp_mul(a.y, b.zzz);
+    const Fp S2 = fp_mul(b.y, a.zzz);
+    const Fp P = fp_sub(U2, U1);
+    const Fp R = fp_sub(S2, S1);
+    exc |= fp_is_zero(P);
+    const Fp PP = fp_sqr(P);
+    const Fp PPP = fp_mul(P, PP);
+    const Fp Q = fp_mul(U1, PP);
+    G1X r;
+    r.x = fp_sub(fp_sub(fp_sqr(R), PPP), fp_dbl(Q));
+    r.y = fp_sub(fp_mul(R, fp_sub(Q, r.x)), fp_mul(S1, PPP));
+    r.zz = fp_mul(fp_mul(a.zz, b.zz), PP);
+    r.zzz = fp_mul(fp_mul(a.zzz, b.zzz), PPP);
     return r;
 }
 

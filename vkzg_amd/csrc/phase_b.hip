@@ -4,11 +4,26 @@
 namespace vkzg_amd {
 
 namespace {
+// One bucket recomputed with the checked point operations (see k_phase_b).
+__device__ __attribute__((noinline)) void phase_b_slow(G1X *out, const G1A *L, const uint32_t *items,
+                                                       uint32_t lo, uint32_t hi, uint32_t half,
+                                                       uint32_t aa) {
+    G1X acc = g1x_identity();
+    for (uint32_t it = lo; it < hi; it++) {
+        const uint32_t packed = items[it];
+        const uint32_t e = packed & 0xffu, d = (packed >> 8) & 0xffu;
+        bool neg = ((packed >> 16) & 1u) != 0;
+        if (half && e > aa) neg = !neg;
+        acc = g1x_madd(acc, g1a_neg_if(L[((aa - e) & 63u) * 32 + d], neg));
+    }
+    *out = acc;
+}
+
 // Lane (blob, a, k): bucket k of output a.  Output a < 64 is the cyclic half
 // over L+, a >= 64 the negacyclic half over L- with a sign flip when a tap
 // wraps (k_phase_b_split.comp:25-61).  The item list depends only on (half,
 // k), so it is uniform across a wave.
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(64, 2)
     k_phase_b(G1X *buckets, const G1A *lad_aff, const uint32_t *items, const uint32_t *offsets) {
     const uint32_t k = blockIdx.x;                       // bucket, 0..127
     const uint32_t ba = blockIdx.y * 64 + threadIdx.x;   // blob * 128 + a
@@ -17,6 +32,7 @@ __global__ void __launch_bounds__(64)
     const uint32_t lo = offsets[half * 129 + k], hi = offsets[half * 129 + k + 1];
     const G1A *L = lad_aff + (size_t)(blob * 2 + half) * 64 * 32;
     G1X acc = g1x_identity();
+    bool exc = false;
 #pragma unroll 1
     for (uint32_t it = lo; it < hi; it++) {
         const uint32_t packed = items[it];
@@ -25,9 +41,10 @@ __global__ void __launch_bounds__(64)
         bool neg = ((packed >> 16) & 1u) != 0;
         if (half && e > aa) neg = !neg;
         const uint32_t src = (aa - e) & 63u;
-        acc = g1x_madd(acc, g1a_neg_if(L[src * 32 + d], neg));
+        acc = g1x_madd_fast(acc, g1a_neg_if(L[src * 32 + d], neg), exc);
     }
-    buckets[(size_t)ba * 128 + k] = acc;
+    if (exc) phase_b_slow(&buckets[(size_t)ba * 128 + k], L, items, lo, hi, half, aa);
+    else buckets[(size_t)ba * 128 + k] = acc;
 }
 
 }  // namespace

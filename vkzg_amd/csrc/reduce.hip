@@ -6,36 +6,55 @@ namespace vkzg_amd {
 namespace {
 constexpr int kReduceLanes = 8;
 
+// One output's reduction recomputed with the checked point operations.
+__device__ __attribute__((noinline)) void reduce_slow(G1X *out, const G1X *B) {
+    G1X run = g1x_identity(), sum = g1x_identity();
+    for (int k = 127; k >= 0; k--) {
+        run = g1x_add(run, B[k]);
+        sum = g1x_add(sum, run);
+    }
+    *out = sum;
+}
+
 // cvals[blob][a] = sum_{k=1..128} k * buckets[blob][a][k-1]
 // (field.glsl.inc:478-517).  Lane l of a team owns buckets [16l, 16l+16):
 // rs_l = sum_i (i+1) B[16l+i] by running sums, s_l = sum_i B[16l+i]; the
 // output is sum_l (rs_l + 16 l * s_l).
-__global__ void __launch_bounds__(256) k_bucket_reduce(G1X *cvals, const G1X *buckets, uint32_t nout) {
+__global__ void __launch_bounds__(256, 2) k_bucket_reduce(G1X *cvals, const G1X *buckets, uint32_t nout) {
     const uint32_t o = blockIdx.x * (256 / kReduceLanes) + threadIdx.x / kReduceLanes;
     const uint32_t l = threadIdx.x % kReduceLanes;
     constexpr uint32_t per = 128 / kReduceLanes;
     G1X s = g1x_identity(), rs = g1x_identity();
+    bool exc = false;
     if (o < nout) {
         const G1X *B = buckets + (size_t)o * 128 + l * per;
 #pragma unroll 1
         for (int i = per - 1; i >= 0; i--) {
-            s = g1x_add(s, B[i]);
-            rs = g1x_add(rs, s);
+            s = g1x_add_fast(s, B[i], exc);
+            rs = g1x_add_fast(rs, s, exc);
         }
     }
-    // t = (16 l) * s by double-and-add over the bits of 16 l.
+    // t = (16 l) * s by double-and-add over the bits of 16 l (t never equals
+    // +-s when it is added: it is an even multiple of s there).
     const uint32_t mult = per * l;
     G1X t = g1x_identity();
     if (mult) {
 #pragma unroll 1
         for (int bit = 31 - __builtin_clz(mult); bit >= 0; bit--) {
             t = g1x_dbl(t);
-            if ((mult >> bit) & 1u) t = g1x_add(t, s);
+            if ((mult >> bit) & 1u) t = g1x_add_fast(t, s, exc);
         }
     }
-    G1X v = g1x_add(rs, t);
+    G1X v = g1x_add_fast(rs, t, exc);
 #pragma unroll 1
-    for (int m = 1; m < kReduceLanes; m <<= 1) v = g1x_add(v, g1x_shfl_xor(v, m));
+    for (int m = 1; m < kReduceLanes; m <<= 1) v = g1x_add_fast(v, g1x_shfl_xor(v, m), exc);
+    const uint64_t flagged = __ballot(exc);
+    const uint32_t team = (threadIdx.x & 63u) / kReduceLanes;
+    const bool redo = ((flagged >> (team * kReduceLanes)) & ((1ull << kReduceLanes) - 1)) != 0;
+    if (o < nout && l == 0 && redo) {
+        reduce_slow(&cvals[o], buckets + (size_t)o * 128);
+        return;
+    }
     if (o < nout && l == 0) cvals[o] = v;
 }
 

@@ -52,6 +52,26 @@ __global__ void k_table_chains(G1J *out, const G1A *xext, uint32_t c, uint32_t W
     }
 }
 
+// The running sum d0*P, (d0+1)*P, ... with checked additions (rare fallback).
+__device__ __attribute__((noinline)) void table_run_slow(G1J *scratch, Fp *prefix, Fp *acc_out,
+                                                         G1A P, uint32_t d0, uint32_t S,
+                                                         uint32_t thread_count, uint32_t lt) {
+    G1J q = g1j_identity();
+    for (int bit = 31 - __builtin_clz(d0); bit >= 0; bit--) {
+        q = g1j_dbl(q);
+        if ((d0 >> bit) & 1u) q = g1j_madd(q, P);
+    }
+    Fp acc = fp_one();
+    for (uint32_t m = 0; m < S; m++) {
+        const size_t slot = (size_t)m * thread_count + lt;
+        scratch[slot] = q;
+        prefix[slot] = acc;
+        if (!fp_is_zero(q.z)) acc = fp_mul(acc, q.z);
+        q = g1j_madd(q, P);
+    }
+    *acc_out = acc;
+}
+
 // Thread g = (b*W + w)*nseg + seg writes table[b][w][d-1] for d in
 // [seg*S+1, seg*S+S]: a running sum of the affine base P = pw[w][b],
 // normalised with one inversion per thread.
@@ -71,11 +91,14 @@ __global__ void k_table_multiples(G1A *table, const G1A *pw, uint32_t c, uint32_
     const G1A P = pw[(size_t)w * 8192 + b];
     const uint32_t d0 = seg * S + 1;
 
+    // d0 * P by double-and-add (no addition there meets equal points), then
+    // the running sum; the one doubling case, 1*P + P, is taken explicitly.
+    bool exc = false;
     G1J q = g1j_identity();
 #pragma unroll 1
     for (int bit = 31 - __builtin_clz(d0); bit >= 0; bit--) {
         q = g1j_dbl(q);
-        if ((d0 >> bit) & 1u) q = g1j_madd(q, P);
+        if ((d0 >> bit) & 1u) q = g1j_madd_fast(q, P, exc);
     }
     Fp acc = fp_one();
 #pragma unroll 1
@@ -84,8 +107,10 @@ __global__ void k_table_multiples(G1A *table, const G1A *pw, uint32_t c, uint32_
         scratch[slot] = q;
         prefix[slot] = acc;
         if (!fp_is_zero(q.z)) acc = fp_mul(acc, q.z);
-        q = g1j_madd(q, P);
+        if (d0 + m == 1) q = g1j_dbl(q);
+        else q = g1j_madd_fast(q, P, exc);
     }
+    if (exc) table_run_slow(scratch, prefix, &acc, P, d0, S, thread_count, lt);
     Fp inv = fp_inv(acc);
     G1A *dst = table + bw * H + (d0 - 1);
 #pragma unroll 1
