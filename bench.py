@@ -25,6 +25,7 @@ FP_M = 300                    # limb-MACs per Fp multiplication (12-limb CIOS)
 FR_M = 136                    # limb-MACs per Fr multiplication (8-limb CIOS)
 W_PROOF = 1.734e9             # limb-MACs per blob of the reference's FK20 as written
 METRIC = "proof blobs/s @256-blob batch; recovery blobs/s; p50/p99 serve latency"
+HERE = os.path.dirname(os.path.abspath(__file__))
 
 
 def seeded_blobs(n, seed):
@@ -76,6 +77,8 @@ def main():
     ap.add_argument("--cpu-sample-blobs", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
+    ap.add_argument("--traffic-file", default=os.path.join(HERE, "profiles", "phase_a_traffic.json"),
+                    help="per-launch HBM traffic measured by rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 3 device-to-device copies of MIB MiB (counter calibration)")
     a = ap.parse_args()
@@ -127,6 +130,11 @@ def main():
     frac = {k: (work[k] / (stage_ms[k] * 1e-3)) / PEAK_INT for k in work if stage_ms.get(k)}
     dom = max(stage_ms, key=stage_ms.get)
     dom_work = work.get(dom)
+    traffic = None
+    if os.path.exists(a.traffic_file):
+        t = json.load(open(a.traffic_file))
+        if t.get("kernel") == dom and t.get("blobs") == nb:
+            traffic = t["fetch_bytes"] + t["write_bytes"]
     roofline = {
         "bound": "int-valu",
         "kernel": dom,
@@ -134,7 +142,7 @@ def main():
         "peak": PEAK_INT / 1e12,
         "unit": "T limb-MAC/s",
         "frac": frac.get(dom),
-        "traffic": None,
+        "traffic": traffic,
     }
 
     secondary = {"stage_ms": stage_ms, "stage_frac_of_int_peak": frac,
@@ -171,6 +179,25 @@ def main():
                                     stream.handle)
         secondary["recovery_stage_ms"] = prover.last_stage_times()
         prover.set_stage_timing(False)
+        # uneven recovery: call sizes U[1,128], per-blob present count U[64,127]
+        calls = [rng.randint(1, 128) for _ in range(12)]
+        maxn = max(calls)
+        ucells = rng.randbytes(maxn * 128 * BYTES_PER_CELL)
+        d_ucells = hip.DeviceBuffer.from_bytes(ucells)
+        d_uout = hip.DeviceBuffer(len(ucells))
+        masks = []
+        for _ in range(maxn):
+            keep = set(rng.sample(range(128), rng.randint(64, 127)))
+            masks.append(bytes(1 if k in keep else 0 for k in range(128)))
+        umask = b"".join(masks)
+        d_umask = hip.DeviceBuffer.from_bytes(umask)
+        prover.recover_cells_device(d_uout.ptr, d_ucells.ptr, d_umask.ptr, umask, maxn, stream.handle)
+        t0 = time.perf_counter()
+        for n in calls:
+            prover.recover_cells_device(d_uout.ptr, d_ucells.ptr, d_umask.ptr, umask[:128 * n], n,
+                                        stream.handle)
+        secondary["uneven_recovery_blobs_per_s"] = sum(calls) / (time.perf_counter() - t0)
+        secondary["uneven_recovery_calls"] = calls
         # plain form: one blob per call through the host C ABI
         one = blobs[:131072]
         prover.compute_cell_kzg_proofs_batch(one, 1)

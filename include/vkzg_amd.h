@@ -43,6 +43,7 @@ typedef struct {
     uint64_t working_bytes;    /* per-prover working set */
     uint32_t max_batch_size;   /* chunk size actually used */
     double setup_ms;           /* wall time of prover creation */
+    uint32_t last_splits;      /* sub-streams the last call's batches were spread over */
 } vkzg_amd_info;
 
 /* Fills `out` (never fails for a non-NULL prover). */

@@ -45,6 +45,7 @@ class VkzgAmdInfo(ctypes.Structure):
         ("working_bytes", ctypes.c_uint64),
         ("max_batch_size", ctypes.c_uint32),
         ("setup_ms", ctypes.c_double),
+        ("last_splits", ctypes.c_uint32),
     ]
 
 

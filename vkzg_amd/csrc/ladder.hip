@@ -6,6 +6,9 @@ namespace vkzg_amd {
 namespace {
 // lad[blob][d][j] = 2^(8d) * u[blob][j], d < 32.
 __global__ void k_ladder(G1J *lad, const G1X *u, uint32_t nblobs) {
+    // A long dependent chain: give these waves issue priority over the
+    // throughput kernels of another sub-batch sharing the SIMD.
+    __builtin_amdgcn_s_setprio(3);
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= nblobs * 128) return;
     const uint32_t blob = g / 128, j = g % 128;

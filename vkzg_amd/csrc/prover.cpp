@@ -1,134 +1,30 @@
 // Host runtime behind the C ABI of include/vkzg.h and include/vkzg_amd.h.
 //
-// One prover = one HIP device, one stream, resident tables and a working set
-// sized for max_batch_size blobs.  Calls are serialised by a mutex (reference
-// src/vkzg.cpp:159,1188,1206).  Every HIP failure maps to VKZG_ERR_GPU.
-#include <hip/hip_runtime.h>
-
+// One prover = one HIP device, one stream (plus sub-streams a large batch is
+// spread over), resident tables and a working set sized for max_batch_size
+// blobs.  Calls are serialised by a mutex (reference src/vkzg.cpp:159,1188,
+// 1206).  Every HIP failure maps to VKZG_ERR_GPU.
+#include <algorithm>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
-#include <mutex>
-#include <string>
-#include <vector>
 
-#include "../../include/vkzg.h"
 #include "../../include/vkzg_amd.h"
-#include "curve.h"
-#include "kernels.h"
+#include "runtime.h"
 
 extern "C" const uint8_t vkzg_amd_mainnet_g1[];
 extern "C" const uint8_t vkzg_amd_mainnet_g1_end[];
 
 using namespace vkzg_amd;
 
-namespace {
-
-constexpr size_t kBlobBytes = VKZG_BYTES_PER_BLOB;
-constexpr size_t kProofBytes = (size_t)VKZG_NUM_CELL_PROOFS * VKZG_BYTES_PER_PROOF;
-constexpr size_t kCellsBytes = (size_t)VKZG_NUM_CELL_PROOFS * VKZG_BYTES_PER_CELL;
-constexpr size_t kSetupBytes = 4096 * 48;
-constexpr uint32_t kDefaultBatch = 256;
-
 static_assert(sizeof(Fr) == 32 && sizeof(Fp) == 48, "field layout");
 static_assert(sizeof(G1A) == 96 && sizeof(G1J) == 144 && sizeof(G1X) == 192, "point layout");
 
-struct HipFail {};
+namespace vkzg_amd {
 
-inline void ck(hipError_t e) {
-    if (e != hipSuccess) throw HipFail{};
-}
-
-
-}  // namespace
-
-struct vkzg_prover {
-    int device = 0;
-    std::string name;
-    uint32_t cus = 0;
-    hipStream_t stream = nullptr;
-    std::mutex mu;
-    uint32_t max_batch = kDefaultBatch;
-    double setup_ms = 0;
-    uint64_t working_bytes = 0;
-
-    // Resident tables.
-    Fr *roots = nullptr, *roots_inv = nullptr, *shift_fwd = nullptr, *shift_inv = nullptr;
-    G1A *srs = nullptr, *xext = nullptr, *table = nullptr;
-    uint32_t c = 0, W = 0, H = 0;
-    uint64_t table_bytes = 0;
-    uint32_t *kb_items = nullptr, *kb_offsets = nullptr;
-
-    // Proof working set.
-    uint8_t *d_blobs = nullptr, *d_proofs = nullptr;
-    uint32_t *d_err = nullptr;
-    Fr *d_scal = nullptr;
-    G1X *d_u = nullptr, *d_buckets = nullptr, *d_cvals = nullptr;
-    G1J *d_lad = nullptr, *d_fold = nullptr;
-    G1A *d_lad_aff = nullptr;
-    Fp *d_prefix = nullptr;
-
-    // Recovery working set.
-    uint8_t *d_cells = nullptr, *d_present = nullptr, *d_cells_out = nullptr;
-    Fr *d_zd = nullptr, *d_izc = nullptr, *d_bufA = nullptr, *d_bufB = nullptr;
-
-    // Pinned staging for the host-pointer entry points.
-    uint8_t *h_in = nullptr, *h_out = nullptr, *h_present = nullptr;
-    uint32_t *h_err = nullptr;
-
-    // Optional per-stage timing: events recorded between the launches of the
-    // last call on the stream the kernels run on, accumulated over its chunks.
-    bool timing = false;
-    static constexpr int kMaxStages = 16;
-    hipEvent_t ev[kMaxStages + 1] = {};
-    float stage_ms[kMaxStages] = {};
-    int nstages = 0;
-    const char *stage_names[kMaxStages] = {};
-
-    std::vector<void *> dev_allocs;
-    std::vector<void *> host_allocs;
-
-    template <typename T>
-    T *dalloc(size_t count) {
-        void *ptr = nullptr;
-        ck(hipMalloc(&ptr, count * sizeof(T) ? count * sizeof(T) : 16));
-        dev_allocs.push_back(ptr);
-        return static_cast<T *>(ptr);
-    }
-    template <typename T>
-    T *halloc(size_t count) {
-        void *ptr = nullptr;
-        ck(hipHostMalloc(&ptr, count * sizeof(T) ? count * sizeof(T) : 16, hipHostMallocDefault));
-        host_allocs.push_back(ptr);
-        return static_cast<T *>(ptr);
-    }
-    void dfree(void *ptr) {
-        for (auto &a : dev_allocs) {
-            if (a == ptr) {
-                (void)hipFree(a);
-                a = nullptr;
-            }
-        }
-    }
-
-    ~vkzg_prover() {
-        int cur = 0;
-        (void)hipGetDevice(&cur);
-        (void)hipSetDevice(device);
-        if (stream) (void)hipStreamSynchronize(stream);
-        for (void *a : dev_allocs)
-            if (a) (void)hipFree(a);
-        for (void *a : host_allocs)
-            if (a) (void)hipHostFree(a);
-        for (auto &e : ev)
-            if (e) (void)hipEventDestroy(e);
-        if (stream) (void)hipStreamDestroy(stream);
-        (void)hipSetDevice(cur);
-    }
-};
-
-namespace {
+constexpr size_t kSetupBytes = 4096 * 48;
+constexpr uint32_t kDefaultBatch = 256;
 
 uint64_t table_bytes_for(uint32_t c) {
     const uint64_t W = (256 + c - 1) / c;
@@ -183,42 +79,42 @@ void build_phase_b_items(const Fr *kpm, std::vector<uint32_t> &items, std::vecto
 
 void build_tables(vkzg_prover *p, const uint8_t *setup) {
     hipStream_t s = p->stream;
-    p->roots = p->dalloc<Fr>(8192);
-    p->roots_inv = p->dalloc<Fr>(8192);
-    p->shift_fwd = p->dalloc<Fr>(8192);
-    p->shift_inv = p->dalloc<Fr>(8192);
+    p->roots = p->arena.dalloc<Fr>(8192);
+    p->roots_inv = p->arena.dalloc<Fr>(8192);
+    p->shift_fwd = p->arena.dalloc<Fr>(8192);
+    p->shift_inv = p->arena.dalloc<Fr>(8192);
     ck(launch_roots(p->roots, p->roots_inv, p->shift_fwd, p->shift_inv, s));
 
     // Decompress the setup.
-    uint8_t *d_setup = p->dalloc<uint8_t>(kSetupBytes);
-    uint32_t *d_flag = p->dalloc<uint32_t>(1);
+    uint8_t *d_setup = p->arena.dalloc<uint8_t>(kSetupBytes);
+    uint32_t *d_flag = p->arena.dalloc<uint32_t>(1);
     ck(hipMemcpyAsync(d_setup, setup, kSetupBytes, hipMemcpyHostToDevice, s));
     ck(hipMemsetAsync(d_flag, 0, 4, s));
-    p->srs = p->dalloc<G1A>(4096);
+    p->srs = p->arena.dalloc<G1A>(4096);
     ck(launch_decompress(p->srs, d_setup, 4096, d_flag, s));
     uint32_t flag = 0;
     ck(hipMemcpyAsync(&flag, d_flag, 4, hipMemcpyDeviceToHost, s));
     ck(hipStreamSynchronize(s));
-    p->dfree(d_setup);
+    p->arena.dfree(d_setup);
     if (flag) throw VKZG_ERR_SETUP;
 
     // x_ext_fft.
-    G1J *xj = p->dalloc<G1J>(8192);
-    Fp *prefix = p->dalloc<Fp>(8192);
-    p->xext = p->dalloc<G1A>(8192);
+    G1J *xj = p->arena.dalloc<G1J>(8192);
+    Fp *prefix = p->arena.dalloc<Fp>(8192);
+    p->xext = p->arena.dalloc<G1A>(8192);
     ck(launch_xext_fft(xj, p->srs, p->roots, s));
     ck(launch_jac_to_affine(p->xext, xj, prefix, 8192, 8, s));
 
     // Phase-B scalars -> item lists.
-    Fr *d_kpm = p->dalloc<Fr>(128);
+    Fr *d_kpm = p->arena.dalloc<Fr>(128);
     ck(launch_phase_b_scalars(d_kpm, p->roots, s));
     std::vector<Fr> kpm(128);
     ck(hipMemcpyAsync(kpm.data(), d_kpm, 128 * sizeof(Fr), hipMemcpyDeviceToHost, s));
     ck(hipStreamSynchronize(s));
     std::vector<uint32_t> items, offsets;
     build_phase_b_items(kpm.data(), items, offsets);
-    p->kb_items = p->dalloc<uint32_t>(items.size());
-    p->kb_offsets = p->dalloc<uint32_t>(offsets.size());
+    p->kb_items = p->arena.dalloc<uint32_t>(items.size());
+    p->kb_offsets = p->arena.dalloc<uint32_t>(offsets.size());
     ck(hipMemcpyAsync(p->kb_items, items.data(), items.size() * 4, hipMemcpyHostToDevice, s));
     ck(hipMemcpyAsync(p->kb_offsets, offsets.data(), offsets.size() * 4, hipMemcpyHostToDevice, s));
 
@@ -229,68 +125,201 @@ void build_tables(vkzg_prover *p, const uint8_t *setup) {
     p->W = (256 + p->c - 1) / p->c;
     p->H = 1u << (p->c - 1);
     p->table_bytes = table_bytes_for(p->c);
-    p->table = p->dalloc<G1A>(p->table_bytes / sizeof(G1A));
+    p->table = p->arena.dalloc<G1A>(p->table_bytes / sizeof(G1A));
 
-    G1J *pwj = p->dalloc<G1J>((size_t)p->W * 8192);
-    Fp *pw_prefix = p->dalloc<Fp>((size_t)p->W * 8192);
-    G1A *pw = p->dalloc<G1A>((size_t)p->W * 8192);
+    G1J *pwj = p->arena.dalloc<G1J>((size_t)p->W * 8192);
+    Fp *pw_prefix = p->arena.dalloc<Fp>((size_t)p->W * 8192);
+    G1A *pw = p->arena.dalloc<G1A>((size_t)p->W * 8192);
     ck(launch_table_chains(pwj, p->xext, p->c, p->W, s));
     ck(launch_jac_to_affine(pw, pwj, pw_prefix, (uint64_t)p->W * 8192, 16, s));
     ck(hipStreamSynchronize(s));
-    p->dfree(pwj);
-    p->dfree(pw_prefix);
+    p->arena.dfree(pwj);
+    p->arena.dfree(pw_prefix);
 
     const uint32_t S = p->H < 64 ? p->H : 64;
     const uint64_t total_threads = 8192ull * p->W * (p->H / S);
     const uint32_t slice = 1u << 18;
-    G1J *scratch = p->dalloc<G1J>((size_t)S * slice);
-    Fp *pref = p->dalloc<Fp>((size_t)S * slice);
+    G1J *scratch = p->arena.dalloc<G1J>((size_t)S * slice);
+    Fp *pref = p->arena.dalloc<Fp>((size_t)S * slice);
     for (uint64_t t0 = 0; t0 < total_threads; t0 += slice) {
         const uint32_t n = (uint32_t)std::min<uint64_t>(slice, total_threads - t0);
         ck(launch_table_multiples(p->table, pw, p->c, p->W, t0, n, scratch, pref, s));
     }
     ck(hipStreamSynchronize(s));
-    p->dfree(scratch);
-    p->dfree(pref);
-    p->dfree(pw);
-    p->dfree(xj);
-    p->dfree(prefix);
-    p->dfree(d_kpm);
-    p->dfree(d_flag);
+    p->arena.dfree(scratch);
+    p->arena.dfree(pref);
+    p->arena.dfree(pw);
+    p->arena.dfree(xj);
+    p->arena.dfree(prefix);
+    p->arena.dfree(d_kpm);
+    p->arena.dfree(d_flag);
 }
 
-void alloc_working_set(vkzg_prover *p) {
-    const size_t B = p->max_batch;
-    uint64_t before = 0;
-    size_t f0 = 0, t0 = 0;
-    ck(hipMemGetInfo(&f0, &t0));
-    before = f0;
-    p->d_blobs = p->dalloc<uint8_t>(B * kBlobBytes);
-    p->d_proofs = p->dalloc<uint8_t>(B * kProofBytes);
-    p->d_err = p->dalloc<uint32_t>(B);
-    p->d_scal = p->dalloc<Fr>(B * 128 * 64);
-    p->d_u = p->dalloc<G1X>(B * 128);
-    p->d_lad = p->dalloc<G1J>(B * 128 * 32);
-    p->d_fold = p->dalloc<G1J>(B * 128 * 32);
-    p->d_lad_aff = p->dalloc<G1A>(B * 128 * 32);
-    p->d_prefix = p->dalloc<Fp>(B * 128 * 32);
-    p->d_buckets = p->dalloc<G1X>(B * 128 * 128);
-    p->d_cvals = p->dalloc<G1X>(B * 128);
-    p->d_cells = p->dalloc<uint8_t>(B * kCellsBytes);
-    p->d_cells_out = p->dalloc<uint8_t>(B * kCellsBytes);
-    p->d_present = p->dalloc<uint8_t>(B * 128);
-    p->d_zd = p->dalloc<Fr>(B * 128);
-    p->d_izc = p->dalloc<Fr>(B * 128);
-    p->d_bufA = p->dalloc<Fr>(B * 8192);
-    p->d_bufB = p->dalloc<Fr>(B * 8192);
-    size_t f1 = 0;
-    ck(hipMemGetInfo(&f1, &t0));
-    p->working_bytes = before > f1 ? before - f1 : 0;
-    p->h_in = p->halloc<uint8_t>(B * kCellsBytes);
-    p->h_out = p->halloc<uint8_t>(B * kCellsBytes);
-    p->h_present = p->halloc<uint8_t>(B * 128);
-    p->h_err = p->halloc<uint32_t>(B);
+
+uint64_t WorkSet::bytes_per_blob() {
+    return kBlobBytes + kProofBytes + 4 + 128 * 64 * sizeof(Fr) + 128 * sizeof(G1X) +
+           2 * 4096 * sizeof(G1J) + 4096 * sizeof(G1A) + 4096 * sizeof(Fp) +
+           128 * 128 * sizeof(G1X) + 128 * sizeof(G1X) + 2 * kCellsBytes + 128 +
+           2 * 128 * sizeof(Fr) + 2 * 8192 * sizeof(Fr);
 }
+
+void WorkSet::allocate(uint32_t B) {
+    cap = B;
+    blobs = arena.dalloc<uint8_t>((size_t)B * kBlobBytes);
+    proofs = arena.dalloc<uint8_t>((size_t)B * kProofBytes);
+    err = arena.dalloc<uint32_t>(B);
+    scal = arena.dalloc<Fr>((size_t)B * 128 * 64);
+    u = arena.dalloc<G1X>((size_t)B * 128);
+    lad = arena.dalloc<G1J>((size_t)B * 4096);
+    fold = arena.dalloc<G1J>((size_t)B * 4096);
+    lad_aff = arena.dalloc<G1A>((size_t)B * 4096);
+    prefix = arena.dalloc<Fp>((size_t)B * 4096);
+    buckets = arena.dalloc<G1X>((size_t)B * 128 * 128);
+    cvals = arena.dalloc<G1X>((size_t)B * 128);
+    cells = arena.dalloc<uint8_t>((size_t)B * kCellsBytes);
+    cells_out = arena.dalloc<uint8_t>((size_t)B * kCellsBytes);
+    present = arena.dalloc<uint8_t>((size_t)B * 128);
+    zd = arena.dalloc<Fr>((size_t)B * 128);
+    izc = arena.dalloc<Fr>((size_t)B * 128);
+    bufA = arena.dalloc<Fr>((size_t)B * 8192);
+    bufB = arena.dalloc<Fr>((size_t)B * 8192);
+}
+
+// Spreading a batch over sub-streams was meant to overlap one sub-batch's
+// latency-bound stages (ladder, inversions) with another's phase A.  Measured
+// at 256 blobs it loses (28.8 ms in one stream, 32.4 ms over two, 33.1 ms over
+// three): the ladder's workgroups wait for slots held by phase A, so the
+// dependent chains get longer, not hidden.  One stream is the default;
+// $VKZG_AMD_SPLITS (1-4) selects more.
+uint32_t choose_splits(const vkzg_prover *p, uint32_t nb) {
+    uint32_t s = p->forced_splits > 0 ? (uint32_t)p->forced_splits : 1u;
+    if (s > (uint32_t)kMaxSplits) s = kMaxSplits;
+    if (s > nb) s = nb;
+    return s < 1 ? 1 : s;
+}
+
+namespace {
+
+// One sub-batch: blobs [off, off + nb) of working set ws.
+void proofs_range(vkzg_prover *p, WorkSet &ws, uint32_t off, uint32_t nb, uint8_t *d_proofs,
+                  const uint8_t *d_blobs, hipStream_t s, StageClock &clk) {
+    uint32_t *err = ws.err + off;
+    Fr *scal = ws.scal + (size_t)off * 8192;
+    G1X *u = ws.u + (size_t)off * 128;
+    G1J *lad = ws.lad + (size_t)off * 4096, *fold = ws.fold + (size_t)off * 4096;
+    G1A *lad_aff = ws.lad_aff + (size_t)off * 4096;
+    Fp *prefix = ws.prefix + (size_t)off * 4096;
+    G1X *buckets = ws.buckets + (size_t)off * 128 * 128, *cvals = ws.cvals + (size_t)off * 128;
+    ck(hipMemsetAsync(err, 0, nb * sizeof(uint32_t), s));
+    ck(launch_fk20_scalars(scal, d_blobs, err, p->roots, p->roots_inv, nb, s));
+    clk.mark("fk20_scalars");
+    ck(launch_phase_a(u, scal, p->table, p->c, p->W, nb, s));
+    clk.mark("phase_a");
+    ck(launch_ladder(lad, u, nb, s));
+    clk.mark("ladder");
+    ck(launch_fold(fold, lad, nb, s));
+    clk.mark("fold");
+    ck(launch_jac_to_affine(lad_aff, fold, prefix, (uint64_t)nb * 4096, 32, s));
+    clk.mark("normalize");
+    ck(launch_phase_b(buckets, lad_aff, p->kb_items, p->kb_offsets, nb, s));
+    clk.mark("phase_b");
+    ck(launch_bucket_reduce(cvals, buckets, nb, s));
+    clk.mark("bucket_reduce");
+    ck(launch_finalize(d_proofs, cvals, nb, s));
+    clk.mark("finalize");
+}
+
+void recovery_range(vkzg_prover *p, WorkSet &ws, uint32_t off, uint32_t nb, uint8_t *d_out,
+                    const uint8_t *d_cells, const uint8_t *d_present, hipStream_t s,
+                    StageClock &clk) {
+    Fr *zd = ws.zd + (size_t)off * 128, *izc = ws.izc + (size_t)off * 128;
+    Fr *A = ws.bufA + (size_t)off * 8192, *B = ws.bufB + (size_t)off * 8192;
+    const Fr *r = p->roots, *ri = p->roots_inv;
+    ck(launch_rec_vanishing(zd, izc, d_present, r, nb, s));
+    clk.mark("vanishing");
+    // (E * Z) -> coefficients, times 7^k / 8192
+    ck(launch_rec_pass_a(A, d_cells, 0, zd, true, r, ri, nb, s));
+    clk.mark("inv_pass_a");
+    ck(launch_rec_pass_b(B, A, 0, p->shift_fwd, nullptr, true, r, ri, nb, s));
+    clk.mark("inv_pass_b");
+    // coset evaluations, divided by Z on the coset
+    ck(launch_rec_pass_a(A, B, 1, nullptr, false, r, ri, nb, s));
+    clk.mark("coset_pass_a");
+    ck(launch_rec_pass_b(B, A, 1, nullptr, izc, false, r, ri, nb, s));
+    clk.mark("coset_pass_b");
+    // coset interpolation: coefficients of P, times 7^-k / 8192
+    ck(launch_rec_pass_a(A, B, 1, nullptr, true, r, ri, nb, s));
+    clk.mark("coset_inv_pass_a");
+    ck(launch_rec_pass_b(B, A, 0, p->shift_inv, nullptr, true, r, ri, nb, s));
+    clk.mark("coset_inv_pass_b");
+    // evaluate P on the whole domain, write cells
+    ck(launch_rec_pass_a(A, B, 1, nullptr, false, r, ri, nb, s));
+    clk.mark("eval_pass_a");
+    ck(launch_rec_pass_b(d_out, A, 2, nullptr, nullptr, false, r, ri, nb, s));
+    clk.mark("eval_pass_b");
+}
+
+// Runs body(k, off, n, stream) for each of `splits` sub-batches, forked from
+// and joined back into s.  The clock follows sub-batch 0.
+template <typename Body>
+void split_over_streams(StreamSet &ss, uint32_t nb, uint32_t splits, hipStream_t s,
+                        StageClock &clk, Body body) {
+    if (splits <= 1) {
+        body(0u, 0u, nb, s);
+        return;
+    }
+    ck(hipEventRecord(ss.fork, s));
+    uint32_t off = 0;
+    for (uint32_t k = 0; k < splits; k++) {
+        const uint32_t n = nb / splits + (k < nb % splits ? 1u : 0u);
+        hipStream_t q = ss.sub[k];
+        ck(hipStreamWaitEvent(q, ss.fork, 0));
+        if (k == 0 && clk.on) {
+            clk.s = q;
+            ck(hipEventRecord(clk.p->ev[0], q));
+        }
+        body(k, off, n, q);
+        ck(hipEventRecord(ss.join[k], q));
+        off += n;
+    }
+    for (uint32_t k = 0; k < splits; k++) ck(hipStreamWaitEvent(s, ss.join[k], 0));
+}
+
+}  // namespace
+
+void enqueue_proofs(vkzg_prover *p, WorkSet &ws, StreamSet &ss, uint8_t *d_proofs,
+                    const uint8_t *d_blobs, uint32_t nb, hipStream_t s, StageClock &clk) {
+    const uint32_t splits = choose_splits(p, nb);
+    p->last_splits = splits;
+    StageClock off_clock(p, s, false);
+    split_over_streams(ss, nb, splits, s, clk, [&](uint32_t k, uint32_t off, uint32_t n, hipStream_t q) {
+        proofs_range(p, ws, off, n, d_proofs + (size_t)off * kProofBytes,
+                     d_blobs + (size_t)off * kBlobBytes, q, k == 0 ? clk : off_clock);
+    });
+}
+
+void enqueue_recovery(vkzg_prover *p, WorkSet &ws, StreamSet &ss, uint8_t *d_out,
+                      const uint8_t *d_cells, const uint8_t *d_present, uint32_t nb, hipStream_t s,
+                      StageClock &clk) {
+    const uint32_t splits = choose_splits(p, nb);
+    p->last_splits = splits;
+    StageClock off_clock(p, s, false);
+    split_over_streams(ss, nb, splits, s, clk, [&](uint32_t k, uint32_t off, uint32_t n, hipStream_t q) {
+        recovery_range(p, ws, off, n, d_out + (size_t)off * kCellsBytes,
+                       d_cells + (size_t)off * kCellsBytes, d_present + (size_t)off * 128, q,
+                       k == 0 ? clk : off_clock);
+    });
+}
+
+bool chunk_missing_ok(const uint8_t *present, uint32_t nb) {
+    for (uint32_t b = 0; b < nb; b++) {
+        uint32_t missing = 0;
+        for (uint32_t i = 0; i < 128; i++) missing += present[(size_t)b * 128 + i] == 0;
+        if (missing > 64) return false;
+    }
+    return true;
+}
+
 
 vkzg_result create(vkzg_prover **out, const vkzg_options *opts, const uint8_t *setup) {
     const auto t0 = std::chrono::steady_clock::now();
@@ -310,11 +339,18 @@ vkzg_result create(vkzg_prover **out, const vkzg_options *opts, const uint8_t *s
                       std::to_string(prop.multiProcessorCount) + " CUs)";
         }
         p->cus = (uint32_t)prop.multiProcessorCount;
+        if (const char *env = std::getenv("VKZG_AMD_SPLITS")) p->forced_splits = std::atoi(env);
         ck(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+        p->streams.create();
         for (auto &e : p->ev) ck(hipEventCreate(&e));
-        if (opts && opts->max_batch_size) p->max_batch = opts->max_batch_size;
+        p->max_batch = opts && opts->max_batch_size ? opts->max_batch_size : kDefaultBatch;
         build_tables(p, setup);
-        alloc_working_set(p);
+        p->ws.allocate(p->max_batch);
+        const size_t B = p->max_batch;
+        p->h_in = p->arena.halloc<uint8_t>(B * kCellsBytes);
+        p->h_out = p->arena.halloc<uint8_t>(B * kCellsBytes);
+        p->h_present = p->arena.halloc<uint8_t>(B * 128);
+        p->h_err = p->arena.halloc<uint32_t>(B);
         ck(hipStreamSynchronize(p->stream));
     } catch (HipFail &) {
         delete p;
@@ -329,109 +365,27 @@ vkzg_result create(vkzg_prover **out, const vkzg_options *opts, const uint8_t *s
     return VKZG_OK;
 }
 
-// Records stage boundaries when timing is on.  mark(i, name) closes stage i-1
-// and opens stage i; finish() folds the elapsed times into stage_ms.
-struct StageClock {
-    vkzg_prover *p;
-    hipStream_t s;
-    int n = 0;
-    StageClock(vkzg_prover *pp, hipStream_t ss) : p(pp), s(ss) {
-        if (p->timing) ck(hipEventRecord(p->ev[0], s));
-    }
-    void mark(const char *name) {
-        if (!p->timing || n >= vkzg_prover::kMaxStages) return;
-        p->stage_names[n] = name;
-        ck(hipEventRecord(p->ev[++n], s));
-    }
-    // Call after the stream has been synchronised.
-    void finish() {
-        if (!p->timing) return;
-        p->nstages = n;
-        for (int i = 0; i < n; i++) {
-            float ms = 0;
-            ck(hipEventElapsedTime(&ms, p->ev[i], p->ev[i + 1]));
-            p->stage_ms[i] += ms;
-        }
-    }
-};
-
 void reset_stage_times(vkzg_prover *p) {
     for (float &v : p->stage_ms) v = 0;
     p->nstages = 0;
 }
 
-// Enqueues the proof pipeline for `nb` blobs at d_blobs -> d_proofs.
-void enqueue_proofs(vkzg_prover *p, uint8_t *d_proofs, const uint8_t *d_blobs, uint32_t nb,
-                    hipStream_t s, StageClock &clk) {
-    ck(hipMemsetAsync(p->d_err, 0, nb * sizeof(uint32_t), s));
-    ck(launch_fk20_scalars(p->d_scal, d_blobs, p->d_err, p->roots, p->roots_inv, nb, s));
-    clk.mark("fk20_scalars");
-    ck(launch_phase_a(p->d_u, p->d_scal, p->table, p->c, p->W, nb, s));
-    clk.mark("phase_a");
-    ck(launch_ladder(p->d_lad, p->d_u, nb, s));
-    clk.mark("ladder");
-    ck(launch_fold(p->d_fold, p->d_lad, nb, s));
-    clk.mark("fold");
-    ck(launch_jac_to_affine(p->d_lad_aff, p->d_fold, p->d_prefix, (uint64_t)nb * 4096, 32, s));
-    clk.mark("normalize");
-    ck(launch_phase_b(p->d_buckets, p->d_lad_aff, p->kb_items, p->kb_offsets, nb, s));
-    clk.mark("phase_b");
-    ck(launch_bucket_reduce(p->d_cvals, p->d_buckets, nb, s));
-    clk.mark("bucket_reduce");
-    ck(launch_finalize(d_proofs, p->d_cvals, nb, s));
-    clk.mark("finalize");
+}  // namespace vkzg_amd
+
+vkzg_prover::~vkzg_prover() {
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    (void)hipSetDevice(device);
+    if (stream) (void)hipStreamSynchronize(stream);
+    (void)hipDeviceSynchronize();
+    ws.arena.release();
+    arena.release();
+    for (auto &e : ev)
+        if (e) (void)hipEventDestroy(e);
+    streams.destroy();
+    if (stream) (void)hipStreamDestroy(stream);
+    (void)hipSetDevice(cur);
 }
-
-// Enqueues recovery for `nb` blobs (cells and mask already on the device).
-void enqueue_recovery(vkzg_prover *p, uint8_t *d_out, const uint8_t *d_cells,
-                      const uint8_t *d_present, uint32_t nb, hipStream_t s, StageClock &clk) {
-    ck(launch_rec_vanishing(p->d_zd, p->d_izc, d_present, p->roots, nb, s));
-    clk.mark("vanishing");
-    // (E * Z) -> coefficients, times 7^k / 8192
-    ck(launch_rec_pass_a(p->d_bufA, d_cells, 0, p->d_zd, true, p->roots, p->roots_inv, nb, s));
-    clk.mark("inv_pass_a");
-    ck(launch_rec_pass_b(p->d_bufB, p->d_bufA, 0, p->shift_fwd, nullptr, true, p->roots,
-                         p->roots_inv, nb, s));
-    clk.mark("inv_pass_b");
-    // coset evaluations, divided by Z on the coset
-    ck(launch_rec_pass_a(p->d_bufA, p->d_bufB, 1, nullptr, false, p->roots, p->roots_inv, nb, s));
-    clk.mark("coset_pass_a");
-    ck(launch_rec_pass_b(p->d_bufB, p->d_bufA, 1, nullptr, p->d_izc, false, p->roots,
-                         p->roots_inv, nb, s));
-    clk.mark("coset_pass_b");
-    // coset interpolation: coefficients of P, times 7^-k / 8192
-    ck(launch_rec_pass_a(p->d_bufA, p->d_bufB, 1, nullptr, true, p->roots, p->roots_inv, nb, s));
-    clk.mark("coset_inv_pass_a");
-    ck(launch_rec_pass_b(p->d_bufB, p->d_bufA, 0, p->shift_inv, nullptr, true, p->roots,
-                         p->roots_inv, nb, s));
-    clk.mark("coset_inv_pass_b");
-    // evaluate P on the whole domain, write cells
-    ck(launch_rec_pass_a(p->d_bufA, p->d_bufB, 1, nullptr, false, p->roots, p->roots_inv, nb, s));
-    clk.mark("eval_pass_a");
-    ck(launch_rec_pass_b(d_out, p->d_bufA, 2, nullptr, nullptr, false, p->roots, p->roots_inv, nb,
-                         s));
-    clk.mark("eval_pass_b");
-}
-
-bool chunk_missing_ok(const uint8_t *present, uint32_t nb) {
-    for (uint32_t b = 0; b < nb; b++) {
-        uint32_t missing = 0;
-        for (uint32_t i = 0; i < 128; i++) missing += present[(size_t)b * 128 + i] == 0;
-        if (missing > 64) return false;
-    }
-    return true;
-}
-
-struct DeviceGuard {
-    int prev = 0;
-    explicit DeviceGuard(int dev) {
-        (void)hipGetDevice(&prev);
-        if (prev != dev) (void)hipSetDevice(dev);
-    }
-    ~DeviceGuard() { (void)hipSetDevice(prev); }
-};
-
-}  // namespace
 
 // ------------------------------------------------------------------ C ABI
 
@@ -488,11 +442,11 @@ vkzg_result vkzg_compute_proofs_batch(vkzg_prover *p, uint8_t *proofs, const uin
             const uint32_t nb = (uint32_t)std::min<size_t>(p->max_batch, num_blobs - done);
             hipStream_t s = p->stream;
             std::memcpy(p->h_in, blobs + done * kBlobBytes, nb * kBlobBytes);
-            ck(hipMemcpyAsync(p->d_blobs, p->h_in, nb * kBlobBytes, hipMemcpyHostToDevice, s));
-            StageClock clk(p, s);
-            enqueue_proofs(p, p->d_proofs, p->d_blobs, nb, s, clk);
-            ck(hipMemcpyAsync(p->h_err, p->d_err, nb * 4, hipMemcpyDeviceToHost, s));
-            ck(hipMemcpyAsync(p->h_out, p->d_proofs, nb * kProofBytes, hipMemcpyDeviceToHost, s));
+            ck(hipMemcpyAsync(p->ws.blobs, p->h_in, nb * kBlobBytes, hipMemcpyHostToDevice, s));
+            StageClock clk(p, s, true);
+            enqueue_proofs(p, p->ws, p->streams, p->ws.proofs, p->ws.blobs, nb, s, clk);
+            ck(hipMemcpyAsync(p->h_err, p->ws.err, nb * 4, hipMemcpyDeviceToHost, s));
+            ck(hipMemcpyAsync(p->h_out, p->ws.proofs, nb * kProofBytes, hipMemcpyDeviceToHost, s));
             ck(hipStreamSynchronize(s));
             clk.finish();
             for (uint32_t b = 0; b < nb; b++)
@@ -521,11 +475,12 @@ vkzg_result vkzg_recover_cells_batch(vkzg_prover *p, uint8_t *cells_out, const u
             hipStream_t s = p->stream;
             std::memcpy(p->h_in, cells + done * kCellsBytes, nb * kCellsBytes);
             std::memcpy(p->h_present, pm, nb * 128);
-            ck(hipMemcpyAsync(p->d_cells, p->h_in, nb * kCellsBytes, hipMemcpyHostToDevice, s));
-            ck(hipMemcpyAsync(p->d_present, p->h_present, nb * 128, hipMemcpyHostToDevice, s));
-            StageClock clk(p, s);
-            enqueue_recovery(p, p->d_cells_out, p->d_cells, p->d_present, nb, s, clk);
-            ck(hipMemcpyAsync(p->h_out, p->d_cells_out, nb * kCellsBytes, hipMemcpyDeviceToHost, s));
+            ck(hipMemcpyAsync(p->ws.cells, p->h_in, nb * kCellsBytes, hipMemcpyHostToDevice, s));
+            ck(hipMemcpyAsync(p->ws.present, p->h_present, nb * 128, hipMemcpyHostToDevice, s));
+            StageClock clk(p, s, true);
+            enqueue_recovery(p, p->ws, p->streams, p->ws.cells_out, p->ws.cells, p->ws.present, nb,
+                             s, clk);
+            ck(hipMemcpyAsync(p->h_out, p->ws.cells_out, nb * kCellsBytes, hipMemcpyDeviceToHost, s));
             ck(hipStreamSynchronize(s));
             clk.finish();
             std::memcpy(cells_out + done * kCellsBytes, p->h_out, nb * kCellsBytes);
@@ -548,10 +503,10 @@ vkzg_result vkzg_amd_compute_proofs_device(vkzg_prover *p, void *d_proofs, const
     try {
         for (size_t done = 0; done < num_blobs;) {
             const uint32_t nb = (uint32_t)std::min<size_t>(p->max_batch, num_blobs - done);
-            StageClock clk(p, s);
-            enqueue_proofs(p, (uint8_t *)d_proofs + done * kProofBytes,
+            StageClock clk(p, s, true);
+            enqueue_proofs(p, p->ws, p->streams, (uint8_t *)d_proofs + done * kProofBytes,
                            (const uint8_t *)d_blobs + done * kBlobBytes, nb, s, clk);
-            ck(hipMemcpyAsync(p->h_err, p->d_err, nb * 4, hipMemcpyDeviceToHost, s));
+            ck(hipMemcpyAsync(p->h_err, p->ws.err, nb * 4, hipMemcpyDeviceToHost, s));
             ck(hipStreamSynchronize(s));
             clk.finish();
             for (uint32_t b = 0; b < nb; b++)
@@ -577,8 +532,8 @@ vkzg_result vkzg_amd_recover_cells_device(vkzg_prover *p, void *d_cells_out, con
         for (size_t done = 0; done < num_blobs;) {
             const uint32_t nb = (uint32_t)std::min<size_t>(p->max_batch, num_blobs - done);
             if (!chunk_missing_ok(h_present + done * 128, nb)) return VKZG_ERR_BADARGS;
-            StageClock clk(p, s);
-            enqueue_recovery(p, (uint8_t *)d_cells_out + done * kCellsBytes,
+            StageClock clk(p, s, true);
+            enqueue_recovery(p, p->ws, p->streams, (uint8_t *)d_cells_out + done * kCellsBytes,
                              (const uint8_t *)d_cells + done * kCellsBytes,
                              (const uint8_t *)d_present + done * 128, nb, s, clk);
             ck(hipStreamSynchronize(s));
@@ -596,7 +551,8 @@ vkzg_result vkzg_amd_prover_info(const vkzg_prover *p, vkzg_amd_info *out) {
     out->window_bits = p->c;
     out->windows = p->W;
     out->table_bytes = p->table_bytes;
-    out->working_bytes = p->working_bytes;
+    out->working_bytes = WorkSet::bytes_per_blob() * p->max_batch;
+    out->last_splits = p->last_splits;
     out->max_batch_size = p->max_batch;
     out->setup_ms = p->setup_ms;
     return VKZG_OK;

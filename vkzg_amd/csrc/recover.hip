@@ -26,6 +26,7 @@ namespace {
 // (k_vanishing_poly.comp, src/vkzg.cpp:1043-1050).
 __global__ void __launch_bounds__(256)
     k_rec_vanishing(Fr *zd, Fr *izc, const uint8_t *present, const Fr *roots) {
+    __builtin_amdgcn_s_setprio(3);  // dependent inversion chain: issue first
     const uint32_t blob = blockIdx.x;
     const uint32_t t = threadIdx.x;
     const uint32_t k = t & 127u;

@@ -1,0 +1,195 @@
+// Internal host-side types shared by the prover (prover.cpp) and the request
+// server (server.cpp).  Not part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/vkzg.h"
+#include "curve.h"
+#include "kernels.h"
+
+namespace vkzg_amd {
+
+constexpr size_t kBlobBytes = VKZG_BYTES_PER_BLOB;
+constexpr size_t kProofBytes = (size_t)VKZG_NUM_CELL_PROOFS * VKZG_BYTES_PER_PROOF;
+constexpr size_t kCellsBytes = (size_t)VKZG_NUM_CELL_PROOFS * VKZG_BYTES_PER_CELL;
+constexpr int kMaxSplits = 4;  // sub-streams one batch may be spread over
+
+struct HipFail {};
+
+inline void ck(hipError_t e) {
+    if (e != hipSuccess) throw HipFail{};
+}
+
+struct DeviceGuard {
+    int prev = 0;
+    explicit DeviceGuard(int dev) {
+        (void)hipGetDevice(&prev);
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard() { (void)hipSetDevice(prev); }
+};
+
+// Device allocations owned by one object, freed together.
+struct DeviceArena {
+    std::vector<void *> dev, host;
+    template <typename T>
+    T *dalloc(size_t count) {
+        void *ptr = nullptr;
+        ck(hipMalloc(&ptr, count * sizeof(T) ? count * sizeof(T) : 16));
+        dev.push_back(ptr);
+        return static_cast<T *>(ptr);
+    }
+    template <typename T>
+    T *halloc(size_t count) {
+        void *ptr = nullptr;
+        ck(hipHostMalloc(&ptr, count * sizeof(T) ? count * sizeof(T) : 16, hipHostMallocDefault));
+        host.push_back(ptr);
+        return static_cast<T *>(ptr);
+    }
+    void dfree(void *ptr) {
+        for (auto &a : dev)
+            if (a == ptr) {
+                (void)hipFree(a);
+                a = nullptr;
+            }
+    }
+    void release() {
+        for (void *a : dev)
+            if (a) (void)hipFree(a);
+        for (void *a : host)
+            if (a) (void)hipHostFree(a);
+        dev.clear();
+        host.clear();
+    }
+};
+
+// Per-batch device buffers for `cap` blobs.  Every buffer is indexed by blob,
+// so a sub-batch at blob offset `off` uses slices at that offset.
+struct WorkSet {
+    uint32_t cap = 0;
+    DeviceArena arena;
+    // proofs
+    uint8_t *blobs = nullptr, *proofs = nullptr;
+    uint32_t *err = nullptr;
+    Fr *scal = nullptr;
+    G1X *u = nullptr, *buckets = nullptr, *cvals = nullptr;
+    G1J *lad = nullptr, *fold = nullptr;
+    G1A *lad_aff = nullptr;
+    Fp *prefix = nullptr;
+    // recovery
+    uint8_t *cells = nullptr, *present = nullptr, *cells_out = nullptr;
+    Fr *zd = nullptr, *izc = nullptr, *bufA = nullptr, *bufB = nullptr;
+
+    void allocate(uint32_t blobs_cap);
+    ~WorkSet() { arena.release(); }
+    static uint64_t bytes_per_blob();
+};
+
+// Sub-streams a batch is spread over, with the events that fork them from
+// and join them back into the caller's stream.
+struct StreamSet {
+    hipStream_t sub[kMaxSplits] = {};
+    hipEvent_t fork = nullptr, join[kMaxSplits] = {};
+    void create() {
+        for (auto &q : sub) ck(hipStreamCreateWithFlags(&q, hipStreamNonBlocking));
+        ck(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+        for (auto &e : join) ck(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    void destroy() {
+        for (auto &e : join)
+            if (e) (void)hipEventDestroy(e);
+        if (fork) (void)hipEventDestroy(fork);
+        for (auto &q : sub)
+            if (q) (void)hipStreamDestroy(q);
+    }
+};
+
+}  // namespace vkzg_amd
+
+struct vkzg_prover {
+    int device = 0;
+    std::string name;
+    uint32_t cus = 0;
+    hipStream_t stream = nullptr;
+    vkzg_amd::StreamSet streams;
+    std::mutex mu;
+    uint32_t max_batch = 256;
+    double setup_ms = 0;
+    uint32_t last_splits = 1;
+    int forced_splits = 0;  // $VKZG_AMD_SPLITS, 0 = automatic
+
+    // Resident tables.
+    vkzg_amd::DeviceArena arena;
+    vkzg_amd::Fr *roots = nullptr, *roots_inv = nullptr, *shift_fwd = nullptr, *shift_inv = nullptr;
+    vkzg_amd::G1A *srs = nullptr, *xext = nullptr, *table = nullptr;
+    uint32_t c = 0, W = 0, H = 0;
+    uint64_t table_bytes = 0;
+    uint32_t *kb_items = nullptr, *kb_offsets = nullptr;
+
+    // Working set of the synchronous entry points and its pinned staging.
+    vkzg_amd::WorkSet ws;
+    uint8_t *h_in = nullptr, *h_out = nullptr, *h_present = nullptr;
+    uint32_t *h_err = nullptr;
+
+    // Per-stage timing of the last call (events on the stream of the first
+    // sub-batch), accumulated over its chunks.
+    bool timing = false;
+    static constexpr int kMaxStages = 16;
+    hipEvent_t ev[kMaxStages + 1] = {};
+    float stage_ms[kMaxStages] = {};
+    int nstages = 0;
+    const char *stage_names[kMaxStages] = {};
+
+    ~vkzg_prover();
+};
+
+namespace vkzg_amd {
+
+// Stage boundaries of one sub-batch pipeline when timing is on.
+struct StageClock {
+    vkzg_prover *p;
+    hipStream_t s;
+    bool on;
+    int n = 0;
+    StageClock(vkzg_prover *pp, hipStream_t ss, bool enable) : p(pp), s(ss), on(enable && pp->timing) {
+        if (on) ck(hipEventRecord(p->ev[0], s));
+    }
+    void mark(const char *name) {
+        if (!on || n >= vkzg_prover::kMaxStages) return;
+        p->stage_names[n] = name;
+        ck(hipEventRecord(p->ev[++n], s));
+    }
+    // After the stream has been synchronised.
+    void finish() {
+        if (!on) return;
+        p->nstages = n;
+        for (int i = 0; i < n; i++) {
+            float ms = 0;
+            ck(hipEventElapsedTime(&ms, p->ev[i], p->ev[i + 1]));
+            p->stage_ms[i] += ms;
+        }
+    }
+};
+
+// Number of sub-streams a batch of nb blobs is spread over.
+uint32_t choose_splits(const vkzg_prover *p, uint32_t nb);
+
+// Enqueue the whole proof / recovery pipeline for nb blobs on stream s, using
+// working set ws (capacity >= nb).  The batch is split over sub-streams when
+// that overlaps latency-bound stages with throughput-bound ones; the split is
+// forked from and joined back into s.  `clk` times the first sub-batch.
+void enqueue_proofs(vkzg_prover *p, WorkSet &ws, StreamSet &ss, uint8_t *d_proofs,
+                    const uint8_t *d_blobs, uint32_t nb, hipStream_t s, StageClock &clk);
+void enqueue_recovery(vkzg_prover *p, WorkSet &ws, StreamSet &ss, uint8_t *d_out,
+                      const uint8_t *d_cells, const uint8_t *d_present, uint32_t nb, hipStream_t s,
+                      StageClock &clk);
+
+bool chunk_missing_ok(const uint8_t *present, uint32_t nb);
+
+}  // namespace vkzg_amd

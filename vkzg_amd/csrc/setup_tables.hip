@@ -10,6 +10,7 @@ namespace {
 // neighbouring lanes touch neighbouring points.
 __global__ void k_jac_to_affine(G1A *out, const G1J *in, Fp *prefix, uint64_t count, uint32_t chunk,
                                 uint64_t nthreads) {
+    __builtin_amdgcn_s_setprio(3);  // dependent inversion chain: issue first
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nthreads) return;
     Fp acc = fp_one();
