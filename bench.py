@@ -67,6 +67,109 @@ def cpu_baseline(sample_blobs, seed):
                       "Vulkan/lavapipe, absent here" % (sample_blobs, dt)}
 
 
+def serve_bench(prover, rng, seconds, rate, probe_rate, slots):
+    """Poisson request stream through vkzg_amd.Server (70/30 proof/recovery mix) plus
+    6-blob proof probes; then the same requests replayed one call each
+    through the synchronous C ABI (the plain per-call form)."""
+    import ctypes
+    import threading
+    import vkzg_amd
+    from vkzg_amd import BYTES_PER_BLOB, BYTES_PER_CELL
+
+    cells_per_blob = 128 * BYTES_PER_CELL
+    blob_pool = ctypes.create_string_buffer(seeded_blobs(48, rng.randrange(1 << 30)), 48 * BYTES_PER_BLOB)
+    raw = bytearray(rng.randbytes(64 * cells_per_blob))
+    raw[0::32] = bytes(b & 0x3F for b in raw[0::32])  # canonical field elements (< r)
+    cell_pool = ctypes.create_string_buffer(bytes(raw), 64 * cells_per_blob)
+    # schedule
+    reqs, t = [], 0.0
+    while t < seconds:
+        t += rng.expovariate(rate)
+        if rng.random() < 0.7:
+            reqs.append([t, "p", rng.randint(1, 48), None])
+        else:
+            k = rng.randint(1, 64)
+            masks = []
+            for _ in range(k):
+                keep = set(rng.sample(range(128), rng.randint(64, 127)))
+                masks.append(bytes(1 if q in keep else 0 for q in range(128)))
+            reqs.append([t, "r", k, ctypes.create_string_buffer(b"".join(masks), 128 * k)])
+    t = 0.0
+    while t < seconds:
+        t += rng.expovariate(probe_rate)
+        reqs.append([t, "probe", 6, None])
+    reqs.sort(key=lambda r: r[0])
+    n_out = 128
+    out_bufs = [ctypes.create_string_buffer(64 * cells_per_blob) for _ in range(n_out)]
+
+    def submit(srv, i, r):
+        out = ctypes.addressof(out_bufs[i % n_out])
+        if r[1] == "r":
+            return srv.submit_recovery_raw(out, ctypes.addressof(cell_pool), ctypes.addressof(r[3]), r[2])
+        return srv.submit_proofs_raw(out, ctypes.addressof(blob_pool), r[2])
+
+    def run(srv, paced):
+        tickets = [None] * len(reqs)
+        infos = [None] * len(reqs)
+        ready = threading.Semaphore(0)
+
+        def waiter():
+            for i in range(len(reqs)):
+                ready.acquire()
+                rc, info = srv.wait_raw(tickets[i])
+                assert rc == 0, rc
+                infos[i] = info
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        t0 = time.perf_counter()
+        for i, r in enumerate(reqs):
+            if paced:
+                delay = r[0] - (time.perf_counter() - t0)
+                if delay > 0:
+                    time.sleep(delay)
+            while i >= n_out and infos[i - n_out] is None:  # output buffer reuse
+                time.sleep(0.0002)
+            tickets[i] = submit(srv, i, r)
+            ready.release()
+        th.join()
+        first = min(x["submit_ns"] for x in infos)
+        last = max(x["done_ns"] for x in infos)
+        return infos, (last - first) * 1e-9
+
+    blobs = sum(r[2] for r in reqs)
+    with vkzg_amd.Server(prover, slots=slots) as srv:
+        run(srv, False)  # warm-up
+        infos, wall = run(srv, True)
+        sat_infos, sat_wall = run(srv, False)
+        st = srv.stats()
+    lat = sorted((x["done_ns"] - x["submit_ns"]) * 1e-6 for x in infos)
+    probe = sorted((x["done_ns"] - x["submit_ns"]) * 1e-6 for x, r in zip(infos, reqs) if r[1] == "probe")
+    pct = lambda v, q: v[min(len(v) - 1, int(q * len(v)))]
+    # plain per-call form: the same requests, one synchronous call each
+    t0 = time.perf_counter()
+    for i, r in enumerate(reqs):
+        out = out_bufs[i % n_out]
+        if r[1] == "r":
+            rc = vkzg_amd.load().vkzg_recover_cells_batch(prover.handle, out, cell_pool, r[3], r[2])
+        else:
+            rc = vkzg_amd.load().vkzg_compute_proofs_batch(prover.handle, out, blob_pool, r[2])
+        assert rc == 0
+    plain_wall = time.perf_counter() - t0
+    return {
+        "offered_requests_per_s": rate + probe_rate, "requests": len(reqs), "blobs": blobs,
+        "served_blobs_per_s": blobs / wall, "p50_ms": pct(lat, 0.50), "p99_ms": pct(lat, 0.99),
+        "probe_6blob_proof_p50_ms": pct(probe, 0.50), "probe_6blob_proof_p99_ms": pct(probe, 0.99),
+        "saturated_served_blobs_per_s": blobs / sat_wall,
+        "plain_per_call_blobs_per_s": blobs / plain_wall,
+        "served_over_plain": (blobs / sat_wall) / (blobs / plain_wall),
+        "server": st,
+        "mix": "70%% proofs k~U[1,48], 30%% recovery k~U[1,64] with 64-127 of 128 cells present, "
+               "plus 6-blob proof probes at %g/s; Poisson arrivals at %g requests/s"
+               % (probe_rate, rate),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +180,10 @@ def main():
     ap.add_argument("--cpu-sample-blobs", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=4.0)
+    ap.add_argument("--serve-rate", type=float, default=150.0, help="mix requests per second")
+    ap.add_argument("--probe-rate", type=float, default=25.0, help="6-blob proof probes per second")
+    ap.add_argument("--serve-slots", type=int, default=2)
     ap.add_argument("--traffic-file", default=os.path.join(HERE, "profiles", "phase_a_traffic.json"),
                     help="per-launch HBM traffic measured by rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -206,6 +313,10 @@ def main():
         for _ in range(calls):
             prover.compute_cell_kzg_proofs_batch(one, 1)
         secondary["plain_per_call_1_blob_blobs_per_s"] = calls / (time.perf_counter() - t0)
+
+    if not a.no_secondary and a.serve_seconds > 0:
+        secondary["serve"] = serve_bench(prover, random.Random(a.seed + 3), a.serve_seconds,
+                                         a.serve_rate, a.probe_rate, a.serve_slots)
 
     cpu = None
     if not a.no_cpu_baseline:

@@ -49,6 +49,53 @@ typedef struct {
 /* Fills `out` (never fails for a non-NULL prover). */
 vkzg_result vkzg_amd_prover_info(const vkzg_prover *p, vkzg_amd_info *out);
 
+/* ------------------------------------------------------------------ server
+ * Continuous batching of proof and recovery requests on one prover.  A
+ * dispatcher packs queued requests (oldest first, one kind per batch) into
+ * device batches of up to `batch_blobs` blobs and runs up to `slots` batches
+ * at once, each with its own device working set; requests arriving while
+ * batches run join the next free slot.  `hbm_budget_bytes` caps the working
+ * sets (fewer slots, then smaller batches).  Request buffers belong to the
+ * caller and must stay valid until vkzg_amd_server_wait returns.  A request
+ * with an invalid blob fails alone (VKZG_ERR_INVALID_BLOB, its outputs
+ * unspecified); more than 64 missing cells is rejected at submission.
+ * Thread-safe. */
+typedef struct vkzg_amd_server vkzg_amd_server;
+
+typedef struct {
+    uint32_t batch_blobs;      /* blobs per device batch; 0 = the prover's max_batch_size */
+    uint32_t slots;            /* batches in flight; 0 = 2 */
+    uint64_t hbm_budget_bytes; /* device memory for the slots; 0 = no limit */
+} vkzg_amd_server_options;
+
+typedef struct {
+    uint64_t submit_ns, start_ns, done_ns; /* steady-clock times */
+    uint32_t batches;                      /* device batches the request was spread over */
+} vkzg_amd_request_info;
+
+typedef struct {
+    uint64_t requests, blobs, batches;
+    uint32_t slots, batch_blobs;
+    uint64_t slot_bytes;
+} vkzg_amd_server_counters;
+
+void vkzg_amd_server_options_default(vkzg_amd_server_options *opts);
+vkzg_result vkzg_amd_server_new(vkzg_amd_server **out, vkzg_prover *p,
+                                const vkzg_amd_server_options *opts);
+/* Drains every queued request, then frees the server. */
+void vkzg_amd_server_free(vkzg_amd_server *s);
+vkzg_result vkzg_amd_server_submit_proofs(vkzg_amd_server *s, uint8_t *proofs, const uint8_t *blobs,
+                                          size_t num_blobs, uint64_t *ticket);
+vkzg_result vkzg_amd_server_submit_recovery(vkzg_amd_server *s, uint8_t *cells_out,
+                                            const uint8_t *cells, const uint8_t *cell_present,
+                                            size_t num_blobs, uint64_t *ticket);
+/* *done = 1 once the request has completed. */
+vkzg_result vkzg_amd_server_poll(vkzg_amd_server *s, uint64_t ticket, int *done);
+/* Blocks until the request completes, returns its result and retires the
+ * ticket; `info` (may be NULL) receives its timestamps. */
+vkzg_result vkzg_amd_server_wait(vkzg_amd_server *s, uint64_t ticket, vkzg_amd_request_info *info);
+vkzg_result vkzg_amd_server_stats(vkzg_amd_server *s, vkzg_amd_server_counters *out);
+
 /* Per-stage timing: when enabled, the prover records HIP events between its
  * kernel launches on the stream they run on; vkzg_amd_last_stage_times then
  * returns the elapsed milliseconds of each stage of the last call (summed over

@@ -13,7 +13,7 @@ the reference's Java binding (bindings/java/src/main/java/vkzg/Vkzg.java):
 import ctypes
 import threading
 
-from ._lib import LIB_PATH, VkzgAmdInfo, VkzgOptions, load
+from ._lib import LIB_PATH, RequestInfo, ServerOptions, ServerStats, VkzgAmdInfo, VkzgOptions, load
 
 FIELD_ELEMENTS_PER_BLOB = 4096
 BYTES_PER_FIELD_ELEMENT = 32
@@ -133,6 +133,101 @@ class Prover:
                                                     blob_count, stream))
 
 
+class Server:
+    """Continuous-batching request server on a Prover (vkzg_amd_server_*).
+
+    submit_* return a ticket immediately; the request's input and output
+    buffers are held by the ticket until wait() returns."""
+
+    def __init__(self, prover, batch_blobs=0, slots=0, hbm_budget_bytes=0):
+        L = load()
+        o = ServerOptions()
+        L.vkzg_amd_server_options_default(ctypes.byref(o))
+        o.batch_blobs, o.slots, o.hbm_budget_bytes = batch_blobs, slots, hbm_budget_bytes
+        h = ctypes.c_void_p()
+        _check(L.vkzg_amd_server_new(ctypes.byref(h), prover.handle, ctypes.byref(o)))
+        self._h = h
+        self._prover = prover
+        self._held = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().vkzg_amd_server_free(self._h)
+            self._h = None
+            self._held.clear()
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def submit_proofs(self, blobs, n):
+        if len(blobs) != n * BYTES_PER_BLOB:
+            raise ValueError("blobs.length must be blobCount * %d" % BYTES_PER_BLOB)
+        src = ctypes.create_string_buffer(bytes(blobs), max(1, len(blobs)))
+        out = ctypes.create_string_buffer(max(1, n * CELL_PROOFS_PER_BLOB * BYTES_PER_PROOF))
+        t = ctypes.c_uint64()
+        _check(load().vkzg_amd_server_submit_proofs(self._h, out, src, n, ctypes.byref(t)))
+        self._held[t.value] = (src, out, n * CELL_PROOFS_PER_BLOB * BYTES_PER_PROOF)
+        return t.value
+
+    def submit_recovery(self, cells, present, n):
+        per_blob = CELL_PROOFS_PER_BLOB * BYTES_PER_CELL
+        if len(cells) != n * per_blob or len(present) != n * CELL_PROOFS_PER_BLOB:
+            raise ValueError("cells/present length mismatch")
+        src = ctypes.create_string_buffer(bytes(cells), max(1, len(cells)))
+        pm = ctypes.create_string_buffer(bytes(present), max(1, len(present)))
+        out = ctypes.create_string_buffer(max(1, len(cells)))
+        t = ctypes.c_uint64()
+        _check(load().vkzg_amd_server_submit_recovery(self._h, out, src, pm, n, ctypes.byref(t)))
+        self._held[t.value] = (src, pm, out, len(cells))
+        return t.value
+
+    # Raw-pointer submissions (addresses of caller-owned host buffers that
+    # must outlive the request); used by benchmarks to avoid copies.
+    def submit_proofs_raw(self, out_addr, in_addr, n):
+        t = ctypes.c_uint64()
+        _check(load().vkzg_amd_server_submit_proofs(self._h, out_addr, in_addr, n, ctypes.byref(t)))
+        return t.value
+
+    def submit_recovery_raw(self, out_addr, in_addr, present_addr, n):
+        t = ctypes.c_uint64()
+        _check(load().vkzg_amd_server_submit_recovery(self._h, out_addr, in_addr, present_addr, n,
+                                                      ctypes.byref(t)))
+        return t.value
+
+    def wait_raw(self, ticket):
+        """(result code, info dict) without raising."""
+        info = RequestInfo()
+        rc = load().vkzg_amd_server_wait(self._h, ticket, ctypes.byref(info))
+        return rc, {f: getattr(info, f) for f, _ in RequestInfo._fields_}
+
+    def poll(self, ticket):
+        d = ctypes.c_int()
+        _check(load().vkzg_amd_server_poll(self._h, ticket, ctypes.byref(d)))
+        return bool(d.value)
+
+    def wait(self, ticket):
+        """Returns (result bytes, info dict); raises VkzgError on a failed request."""
+        info = RequestInfo()
+        rc = load().vkzg_amd_server_wait(self._h, ticket, ctypes.byref(info))
+        held = self._held.pop(ticket)
+        out, nbytes = held[-2], held[-1]
+        meta = {f: getattr(info, f) for f, _ in RequestInfo._fields_}
+        if rc != OK:
+            raise VkzgError(rc)
+        return out.raw[:nbytes], meta
+
+    def stats(self):
+        st = ServerStats()
+        _check(load().vkzg_amd_server_stats(self._h, ctypes.byref(st)))
+        return {f: getattr(st, f) for f, _ in ServerStats._fields_}
+
+
 # ---- module-level singleton, as in Vkzg.java ----
 _lock = threading.Lock()
 _prover = None
@@ -199,6 +294,6 @@ def _debug_mul(a, b, limbs, prover, fn):
 
 
 __all__ = [
-    "Prover", "VkzgError", "init", "deinit", "compute_cell_kzg_proofs_batch",
+    "Prover", "Server", "VkzgError", "init", "deinit", "compute_cell_kzg_proofs_batch",
     "recover_cells_batch", "device_name", "gpu_core_count", "error_string", "LIB_PATH",
 ]

@@ -30,6 +30,14 @@ EXT_SYMBOLS = [
     "vkzg_amd_last_stage_times",
     "vkzg_amd_debug_fp_mul",
     "vkzg_amd_debug_fr_mul",
+    "vkzg_amd_server_options_default",
+    "vkzg_amd_server_new",
+    "vkzg_amd_server_free",
+    "vkzg_amd_server_submit_proofs",
+    "vkzg_amd_server_submit_recovery",
+    "vkzg_amd_server_poll",
+    "vkzg_amd_server_wait",
+    "vkzg_amd_server_stats",
 ]
 
 
@@ -47,6 +55,22 @@ class VkzgAmdInfo(ctypes.Structure):
         ("setup_ms", ctypes.c_double),
         ("last_splits", ctypes.c_uint32),
     ]
+
+
+class ServerOptions(ctypes.Structure):
+    _fields_ = [("batch_blobs", ctypes.c_uint32), ("slots", ctypes.c_uint32),
+                ("hbm_budget_bytes", ctypes.c_uint64)]
+
+
+class RequestInfo(ctypes.Structure):
+    _fields_ = [("submit_ns", ctypes.c_uint64), ("start_ns", ctypes.c_uint64),
+                ("done_ns", ctypes.c_uint64), ("batches", ctypes.c_uint32)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("requests", ctypes.c_uint64), ("blobs", ctypes.c_uint64),
+                ("batches", ctypes.c_uint64), ("slots", ctypes.c_uint32),
+                ("batch_blobs", ctypes.c_uint32), ("slot_bytes", ctypes.c_uint64)]
 
 
 assert ctypes.sizeof(VkzgOptions) == 4
@@ -93,6 +117,23 @@ def load():
     L.vkzg_amd_last_stage_times.restype = ctypes.c_int
     L.vkzg_amd_last_stage_times.argtypes = [vp, ctypes.POINTER(ctypes.c_float),
                                             ctypes.POINTER(ctypes.c_char_p), ctypes.c_int]
+    L.vkzg_amd_server_options_default.restype = None
+    L.vkzg_amd_server_options_default.argtypes = [ctypes.POINTER(ServerOptions)]
+    L.vkzg_amd_server_new.restype = ctypes.c_int
+    L.vkzg_amd_server_new.argtypes = [ctypes.POINTER(vp), vp, ctypes.POINTER(ServerOptions)]
+    L.vkzg_amd_server_free.restype = None
+    L.vkzg_amd_server_free.argtypes = [vp]
+    u64p = ctypes.POINTER(ctypes.c_uint64)
+    L.vkzg_amd_server_submit_proofs.restype = ctypes.c_int
+    L.vkzg_amd_server_submit_proofs.argtypes = [vp, vp, vp, sz, u64p]
+    L.vkzg_amd_server_submit_recovery.restype = ctypes.c_int
+    L.vkzg_amd_server_submit_recovery.argtypes = [vp, vp, vp, vp, sz, u64p]
+    L.vkzg_amd_server_poll.restype = ctypes.c_int
+    L.vkzg_amd_server_poll.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int)]
+    L.vkzg_amd_server_wait.restype = ctypes.c_int
+    L.vkzg_amd_server_wait.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(RequestInfo)]
+    L.vkzg_amd_server_stats.restype = ctypes.c_int
+    L.vkzg_amd_server_stats.argtypes = [vp, ctypes.POINTER(ServerStats)]
     for name in ("vkzg_amd_debug_fp_mul", "vkzg_amd_debug_fr_mul"):
         f = getattr(L, name)
         f.restype = ctypes.c_int

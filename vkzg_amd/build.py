@@ -19,7 +19,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("VKZG_AMD_ARCH", "gfx950")
 
 SOURCES = ["setup.hip", "setup_fft.hip", "setup_tables.hip", "fk20_scalars.hip", "phase_a.hip",
-           "ladder.hip", "phase_b.hip", "reduce.hip", "recover.hip", "prover.cpp"]
+           "ladder.hip", "phase_b.hip", "reduce.hip", "recover.hip", "prover.cpp", "server.cpp"]
 HIP_FLAGS = [
     "--offload-arch=" + ARCH,
     "-O3",

@@ -288,9 +288,8 @@ void split_over_streams(StreamSet &ss, uint32_t nb, uint32_t splits, hipStream_t
 }  // namespace
 
 void enqueue_proofs(vkzg_prover *p, WorkSet &ws, StreamSet &ss, uint8_t *d_proofs,
-                    const uint8_t *d_blobs, uint32_t nb, hipStream_t s, StageClock &clk) {
-    const uint32_t splits = choose_splits(p, nb);
-    p->last_splits = splits;
+                    const uint8_t *d_blobs, uint32_t nb, hipStream_t s, StageClock &clk,
+                    uint32_t splits) {
     StageClock off_clock(p, s, false);
     split_over_streams(ss, nb, splits, s, clk, [&](uint32_t k, uint32_t off, uint32_t n, hipStream_t q) {
         proofs_range(p, ws, off, n, d_proofs + (size_t)off * kProofBytes,
@@ -300,9 +299,7 @@ void enqueue_proofs(vkzg_prover *p, WorkSet &ws, StreamSet &ss, uint8_t *d_proof
 
 void enqueue_recovery(vkzg_prover *p, WorkSet &ws, StreamSet &ss, uint8_t *d_out,
                       const uint8_t *d_cells, const uint8_t *d_present, uint32_t nb, hipStream_t s,
-                      StageClock &clk) {
-    const uint32_t splits = choose_splits(p, nb);
-    p->last_splits = splits;
+                      StageClock &clk, uint32_t splits) {
     StageClock off_clock(p, s, false);
     split_over_streams(ss, nb, splits, s, clk, [&](uint32_t k, uint32_t off, uint32_t n, hipStream_t q) {
         recovery_range(p, ws, off, n, d_out + (size_t)off * kCellsBytes,
@@ -444,7 +441,9 @@ vkzg_result vkzg_compute_proofs_batch(vkzg_prover *p, uint8_t *proofs, const uin
             std::memcpy(p->h_in, blobs + done * kBlobBytes, nb * kBlobBytes);
             ck(hipMemcpyAsync(p->ws.blobs, p->h_in, nb * kBlobBytes, hipMemcpyHostToDevice, s));
             StageClock clk(p, s, true);
-            enqueue_proofs(p, p->ws, p->streams, p->ws.proofs, p->ws.blobs, nb, s, clk);
+            p->last_splits = choose_splits(p, nb);
+            enqueue_proofs(p, p->ws, p->streams, p->ws.proofs, p->ws.blobs, nb, s, clk,
+                           p->last_splits);
             ck(hipMemcpyAsync(p->h_err, p->ws.err, nb * 4, hipMemcpyDeviceToHost, s));
             ck(hipMemcpyAsync(p->h_out, p->ws.proofs, nb * kProofBytes, hipMemcpyDeviceToHost, s));
             ck(hipStreamSynchronize(s));
@@ -478,8 +477,9 @@ vkzg_result vkzg_recover_cells_batch(vkzg_prover *p, uint8_t *cells_out, const u
             ck(hipMemcpyAsync(p->ws.cells, p->h_in, nb * kCellsBytes, hipMemcpyHostToDevice, s));
             ck(hipMemcpyAsync(p->ws.present, p->h_present, nb * 128, hipMemcpyHostToDevice, s));
             StageClock clk(p, s, true);
+            p->last_splits = choose_splits(p, nb);
             enqueue_recovery(p, p->ws, p->streams, p->ws.cells_out, p->ws.cells, p->ws.present, nb,
-                             s, clk);
+                             s, clk, p->last_splits);
             ck(hipMemcpyAsync(p->h_out, p->ws.cells_out, nb * kCellsBytes, hipMemcpyDeviceToHost, s));
             ck(hipStreamSynchronize(s));
             clk.finish();
@@ -504,8 +504,10 @@ vkzg_result vkzg_amd_compute_proofs_device(vkzg_prover *p, void *d_proofs, const
         for (size_t done = 0; done < num_blobs;) {
             const uint32_t nb = (uint32_t)std::min<size_t>(p->max_batch, num_blobs - done);
             StageClock clk(p, s, true);
+            p->last_splits = choose_splits(p, nb);
             enqueue_proofs(p, p->ws, p->streams, (uint8_t *)d_proofs + done * kProofBytes,
-                           (const uint8_t *)d_blobs + done * kBlobBytes, nb, s, clk);
+                           (const uint8_t *)d_blobs + done * kBlobBytes, nb, s, clk,
+                           p->last_splits);
             ck(hipMemcpyAsync(p->h_err, p->ws.err, nb * 4, hipMemcpyDeviceToHost, s));
             ck(hipStreamSynchronize(s));
             clk.finish();
@@ -533,9 +535,10 @@ vkzg_result vkzg_amd_recover_cells_device(vkzg_prover *p, void *d_cells_out, con
             const uint32_t nb = (uint32_t)std::min<size_t>(p->max_batch, num_blobs - done);
             if (!chunk_missing_ok(h_present + done * 128, nb)) return VKZG_ERR_BADARGS;
             StageClock clk(p, s, true);
+            p->last_splits = choose_splits(p, nb);
             enqueue_recovery(p, p->ws, p->streams, (uint8_t *)d_cells_out + done * kCellsBytes,
                              (const uint8_t *)d_cells + done * kCellsBytes,
-                             (const uint8_t *)d_present + done * 128, nb, s, clk);
+                             (const uint8_t *)d_present + done * 128, nb, s, clk, p->last_splits);
             ck(hipStreamSynchronize(s));
             clk.finish();
             done += nb;

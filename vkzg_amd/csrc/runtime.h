@@ -185,10 +185,11 @@ uint32_t choose_splits(const vkzg_prover *p, uint32_t nb);
 // that overlaps latency-bound stages with throughput-bound ones; the split is
 // forked from and joined back into s.  `clk` times the first sub-batch.
 void enqueue_proofs(vkzg_prover *p, WorkSet &ws, StreamSet &ss, uint8_t *d_proofs,
-                    const uint8_t *d_blobs, uint32_t nb, hipStream_t s, StageClock &clk);
+                    const uint8_t *d_blobs, uint32_t nb, hipStream_t s, StageClock &clk,
+                    uint32_t splits);
 void enqueue_recovery(vkzg_prover *p, WorkSet &ws, StreamSet &ss, uint8_t *d_out,
                       const uint8_t *d_cells, const uint8_t *d_present, uint32_t nb, hipStream_t s,
-                      StageClock &clk);
+                      StageClock &clk, uint32_t splits);
 
 bool chunk_missing_ok(const uint8_t *present, uint32_t nb);
 
