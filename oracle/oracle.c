@@ -1054,3 +1054,660 @@ int or_proofs_from_tau(uint8_t *proofs, const uint8_t *blob, const uint8_t tau[3
     free(c);
     return 0;
 }
+
+/* ============================================================================
+ * Pairing and batch cell-proof verification (test-side check of the proofs;
+ * the reference has no verifier, include/vkzg.h:14-18).
+ *
+ * Written from the mathematics of BLS12-381, not from any library:
+ *   Fp2 = Fp[u]/(u^2 + 1), Fp6 = Fp2[v]/(v^3 - xi) with xi = 1 + u,
+ *   Fp12 = Fp6[w]/(w^2 - v), so w^6 = xi.
+ *   G2 lives on the twist E': y^2 = x^3 + 4 xi over Fp2; the untwist
+ *   psi(x, y) = (x w^-2, y w^-3) maps it into E(Fp12): y^2 = x^3 + 4.
+ *   Miller loop over |z| = 0xd201000000010000 (the curve parameter z is
+ *   negative; inverting every Miller value does not change whether a product
+ *   of pairings is 1, nor bilinearity), with lines computed in affine
+ *   coordinates over Fp12.  Vertical lines take values in Fp6 and are dropped:
+ *   the final exponentiation (p^12 - 1)/r is a multiple of p^6 - 1.
+ *   Final exponentiation: f^(p^6 - 1) = conj(f) / f (the p^6-power Frobenius
+ *   negates w), then the power (p^6 + 1)/r by square-and-multiply.
+ * Every part is checked by the tests through bilinearity, non-degeneracy and
+ * group orders rather than by known answers, since no G2 data ships with the
+ * reference.
+ * ========================================================================== */
+
+typedef struct { fp c0, c1; } fp2;
+typedef struct { fp2 c0, c1, c2; } fp6;
+typedef struct { fp6 d0, d1; } fp12;
+typedef struct { fp2 x, y; int inf; } g2a;
+
+/* (p^6 + 1) / r, little-endian 64-bit limbs. */
+static const uint64_t FINAL_EXP_HARD[32] = {
+    0x8739e1cdc0705d6aull, 0x09a5256de0381a16ull, 0x9cf0f70a61c791e2ull, 0x3a09c4497903f76eull, 0x2d7271563890f133ull, 0x224741b36fec7760ull, 0x338259c22a12bd40ull, 0x38ee1cd4778e0de7ull, 0xc3b5ef4b188a20b0ull, 0x1d615d49e2764d7bull, 0x816101ddd076117dull, 0xf007c01e7ebe3afcull, 0x27d7bd90935021c3ull, 0xc3b5e2f557c0b15full, 0x5e886c94c4f82384ull, 0xee6a95db11e63f56ull, 0x2b822f514a9c4f6full, 0x12d6a874d21b73daull, 0x1304275ef499dffbull, 0x967878febcb95d1full, 0x4744497f8b2f2922ull, 0x85a2e707f0841855ull, 0x9f0c50126c802eecull, 0xfb46e197bd2fa489ull, 0x548ce0809bc5f61aull, 0xcf56fb1573beaa8cull, 0xad7375a3763bdf7cull, 0xe0ec9031179bdeccull, 0x6579aea83c48c1daull, 0xdbf85ae664cf5bb3ull, 0x7b6f235c55ca7566ull, 0x000028b314877503ull};
+/* G2 cofactor h2 = (z^8 - 4z^7 + 5z^6 - 4z^4 + 6z^3 - 4z^2 - 4z + 13) / 9, z = -0xd201000000010000. */
+static const uint64_t G2_COFACTOR[8] = {
+    0xcf1c38e31c7238e5ull, 0x1616ec6e786f0c70ull, 0x21537e293a6691aeull, 0xa628f1cb4d9e82efull, 0xa68a205b2e5a7ddfull, 0xcd91de4547085abaull, 0x091d50792876a202ull, 0x05d543a95414e7f1ull};
+static const uint64_t ATE_LOOP = 0xd201000000010000ULL;
+
+static void fp2_add(fp2 *r, const fp2 *a, const fp2 *b) { fp_add(&r->c0, &a->c0, &b->c0); fp_add(&r->c1, &a->c1, &b->c1); }
+static void fp2_sub(fp2 *r, const fp2 *a, const fp2 *b) { fp_sub(&r->c0, &a->c0, &b->c0); fp_sub(&r->c1, &a->c1, &b->c1); }
+static void fp2_neg(fp2 *r, const fp2 *a) { fp_neg(&r->c0, &a->c0); fp_neg(&r->c1, &a->c1); }
+static int fp2_is_zero(const fp2 *a) { return fp_is_zero(&a->c0) && fp_is_zero(&a->c1); }
+static int fp2_eq(const fp2 *a, const fp2 *b) { return fp_eq(&a->c0, &b->c0) && fp_eq(&a->c1, &b->c1); }
+static void fp2_zero(fp2 *r) { memset(r, 0, sizeof(*r)); }
+static void fp2_one(fp2 *r) { fp2_zero(r); r->c0 = FP_ONE; }
+
+static void fp2_mul(fp2 *r, const fp2 *a, const fp2 *b) {
+    fp t0, t1, t2, t3;
+    fp_mul(&t0, &a->c0, &b->c0);
+    fp_mul(&t1, &a->c1, &b->c1);
+    fp_mul(&t2, &a->c0, &b->c1);
+    fp_mul(&t3, &a->c1, &b->c0);
+    fp_sub(&r->c0, &t0, &t1); /* u^2 = -1 */
+    fp_add(&r->c1, &t2, &t3);
+}
+static void fp2_sqr(fp2 *r, const fp2 *a) { fp2_mul(r, a, a); }
+static void fp2_mul_fp(fp2 *r, const fp2 *a, const fp *b) { fp_mul(&r->c0, &a->c0, b); fp_mul(&r->c1, &a->c1, b); }
+/* a * xi = (a0 - a1) + (a0 + a1) u */
+static void fp2_mul_xi(fp2 *r, const fp2 *a) {
+    fp t0, t1;
+    fp_sub(&t0, &a->c0, &a->c1);
+    fp_add(&t1, &a->c0, &a->c1);
+    r->c0 = t0;
+    r->c1 = t1;
+}
+static void fp2_inv(fp2 *r, const fp2 *a) {
+    fp n, t;
+    fp_sqr(&n, &a->c0);
+    fp_sqr(&t, &a->c1);
+    fp_add(&n, &n, &t);
+    fp_inv(&n, &n);
+    fp_mul(&r->c0, &a->c0, &n);
+    fp_mul(&t, &a->c1, &n);
+    fp_neg(&r->c1, &t);
+}
+/* Square root through the norm: for a = a0 + a1 u with norm n^2 = a0^2 + a1^2,
+ * x0^2 = (a0 +- n)/2 and x1 = a1 / (2 x0).  Returns 0 and the root, or -1. */
+static int fp2_sqrt(fp2 *r, const fp2 *a) {
+    fp two, inv2;
+    fp_add(&two, &FP_ONE, &FP_ONE);
+    fp_inv(&inv2, &two);
+    fp2 cand;
+    if (fp_is_zero(&a->c1)) {
+        fp s, na;
+        if (!fp_sqrt(&s, &a->c0)) { cand.c0 = s; memset(&cand.c1, 0, sizeof(fp)); }
+        else {
+            fp_neg(&na, &a->c0);
+            if (fp_sqrt(&s, &na)) return -1;
+            memset(&cand.c0, 0, sizeof(fp));
+            cand.c1 = s;
+        }
+    } else {
+        fp n2, t, n, x0sq, x0, d;
+        fp_sqr(&n2, &a->c0);
+        fp_sqr(&t, &a->c1);
+        fp_add(&n2, &n2, &t);
+        if (fp_sqrt(&n, &n2)) return -1;
+        fp_add(&x0sq, &a->c0, &n);
+        fp_mul(&x0sq, &x0sq, &inv2);
+        if (fp_sqrt(&x0, &x0sq)) {
+            fp_sub(&x0sq, &a->c0, &n);
+            fp_mul(&x0sq, &x0sq, &inv2);
+            if (fp_sqrt(&x0, &x0sq)) return -1;
+        }
+        fp_add(&d, &x0, &x0);
+        fp_inv(&d, &d);
+        cand.c0 = x0;
+        fp_mul(&cand.c1, &a->c1, &d);
+    }
+    fp2 chk;
+    fp2_sqr(&chk, &cand);
+    if (!fp2_eq(&chk, a)) return -1;
+    *r = cand;
+    return 0;
+}
+
+static void fp6_add(fp6 *r, const fp6 *a, const fp6 *b) { fp2_add(&r->c0, &a->c0, &b->c0); fp2_add(&r->c1, &a->c1, &b->c1); fp2_add(&r->c2, &a->c2, &b->c2); }
+static void fp6_sub(fp6 *r, const fp6 *a, const fp6 *b) { fp2_sub(&r->c0, &a->c0, &b->c0); fp2_sub(&r->c1, &a->c1, &b->c1); fp2_sub(&r->c2, &a->c2, &b->c2); }
+static void fp6_neg(fp6 *r, const fp6 *a) { fp2_neg(&r->c0, &a->c0); fp2_neg(&r->c1, &a->c1); fp2_neg(&r->c2, &a->c2); }
+static void fp6_zero(fp6 *r) { memset(r, 0, sizeof(*r)); }
+
+static void fp6_mul(fp6 *r, const fp6 *a, const fp6 *b) {
+    fp2 t, s, c0, c1, c2;
+    /* c0 = a0 b0 + xi (a1 b2 + a2 b1) */
+    fp2_mul(&t, &a->c1, &b->c2);
+    fp2_mul(&s, &a->c2, &b->c1);
+    fp2_add(&t, &t, &s);
+    fp2_mul_xi(&t, &t);
+    fp2_mul(&s, &a->c0, &b->c0);
+    fp2_add(&c0, &s, &t);
+    /* c1 = a0 b1 + a1 b0 + xi a2 b2 */
+    fp2_mul(&t, &a->c2, &b->c2);
+    fp2_mul_xi(&t, &t);
+    fp2_mul(&s, &a->c0, &b->c1);
+    fp2_add(&t, &t, &s);
+    fp2_mul(&s, &a->c1, &b->c0);
+    fp2_add(&c1, &t, &s);
+    /* c2 = a0 b2 + a1 b1 + a2 b0 */
+    fp2_mul(&t, &a->c0, &b->c2);
+    fp2_mul(&s, &a->c1, &b->c1);
+    fp2_add(&t, &t, &s);
+    fp2_mul(&s, &a->c2, &b->c0);
+    fp2_add(&c2, &t, &s);
+    r->c0 = c0;
+    r->c1 = c1;
+    r->c2 = c2;
+}
+/* a * v = xi a2 + a0 v + a1 v^2 */
+static void fp6_mul_v(fp6 *r, const fp6 *a) {
+    fp2 t;
+    fp2_mul_xi(&t, &a->c2);
+    r->c2 = a->c1;
+    r->c1 = a->c0;
+    r->c0 = t;
+}
+static void fp6_inv(fp6 *r, const fp6 *a) {
+    fp2 t0, t1, t2, s, n;
+    fp2_sqr(&t0, &a->c0);
+    fp2_mul(&s, &a->c1, &a->c2);
+    fp2_mul_xi(&s, &s);
+    fp2_sub(&t0, &t0, &s); /* a0^2 - xi a1 a2 */
+    fp2_sqr(&t1, &a->c2);
+    fp2_mul_xi(&t1, &t1);
+    fp2_mul(&s, &a->c0, &a->c1);
+    fp2_sub(&t1, &t1, &s); /* xi a2^2 - a0 a1 */
+    fp2_sqr(&t2, &a->c1);
+    fp2_mul(&s, &a->c0, &a->c2);
+    fp2_sub(&t2, &t2, &s); /* a1^2 - a0 a2 */
+    fp2_mul(&n, &a->c2, &t1);
+    fp2_mul(&s, &a->c1, &t2);
+    fp2_add(&n, &n, &s);
+    fp2_mul_xi(&n, &n);
+    fp2_mul(&s, &a->c0, &t0);
+    fp2_add(&n, &n, &s);
+    fp2_inv(&n, &n);
+    fp2_mul(&r->c0, &t0, &n);
+    fp2_mul(&r->c1, &t1, &n);
+    fp2_mul(&r->c2, &t2, &n);
+}
+
+static void fp12_one(fp12 *r) { memset(r, 0, sizeof(*r)); r->d0.c0.c0 = FP_ONE; }
+static int fp12_eq(const fp12 *a, const fp12 *b) { return memcmp(a, b, sizeof(fp12)) == 0; }
+static void fp12_add(fp12 *r, const fp12 *a, const fp12 *b) { fp6_add(&r->d0, &a->d0, &b->d0); fp6_add(&r->d1, &a->d1, &b->d1); }
+static void fp12_sub(fp12 *r, const fp12 *a, const fp12 *b) { fp6_sub(&r->d0, &a->d0, &b->d0); fp6_sub(&r->d1, &a->d1, &b->d1); }
+static void fp12_mul(fp12 *r, const fp12 *a, const fp12 *b) {
+    fp6 t0, t1, t2, s;
+    fp6_mul(&t0, &a->d0, &b->d0);
+    fp6_mul(&t1, &a->d1, &b->d1);
+    fp6_mul(&t2, &a->d0, &b->d1);
+    fp6_mul(&s, &a->d1, &b->d0);
+    fp6_add(&r->d1, &t2, &s);
+    fp6_mul_v(&t1, &t1);
+    fp6_add(&r->d0, &t0, &t1);
+}
+static void fp12_sqr(fp12 *r, const fp12 *a) { fp12_mul(r, a, a); }
+static void fp12_conj(fp12 *r, const fp12 *a) { r->d0 = a->d0; fp6_neg(&r->d1, &a->d1); }
+/* 1/(a + b w) = (a - b w) / (a^2 - b^2 v) */
+static void fp12_inv(fp12 *r, const fp12 *a) {
+    fp6 t0, t1;
+    fp6_mul(&t0, &a->d0, &a->d0);
+    fp6_mul(&t1, &a->d1, &a->d1);
+    fp6_mul_v(&t1, &t1);
+    fp6_sub(&t0, &t0, &t1);
+    fp6_inv(&t0, &t0);
+    fp6_mul(&r->d0, &a->d0, &t0);
+    fp6_mul(&t1, &a->d1, &t0);
+    fp6_neg(&r->d1, &t1);
+}
+static void fp12_pow_limbs(fp12 *r, const fp12 *a, const uint64_t *e, int nl) {
+    fp12 acc;
+    fp12_one(&acc);
+    for (int i = nl * 64 - 1; i >= 0; i--) {
+        fp12_sqr(&acc, &acc);
+        if ((e[i / 64] >> (i % 64)) & 1) fp12_mul(&acc, &acc, a);
+    }
+    *r = acc;
+}
+static void fp12_from_fp(fp12 *r, const fp *a) { memset(r, 0, sizeof(*r)); r->d0.c0.c0 = *a; }
+static void fp12_from_fp2(fp12 *r, const fp2 *a) { memset(r, 0, sizeof(*r)); r->d0.c0 = *a; }
+
+/* ---- G2 on the twist (affine; speed is irrelevant here) ---- */
+static void g2_b(fp2 *b) { fp2 four; fp2_zero(&four); four.c0 = FP_FOUR; fp2_mul_xi(b, &four); }
+static int g2_on_curve(const g2a *p) {
+    if (p->inf) return 1;
+    fp2 l, rr, b;
+    fp2_sqr(&l, &p->y);
+    fp2_sqr(&rr, &p->x);
+    fp2_mul(&rr, &rr, &p->x);
+    g2_b(&b);
+    fp2_add(&rr, &rr, &b);
+    return fp2_eq(&l, &rr);
+}
+static void g2_add(g2a *r, const g2a *a, const g2a *b) {
+    if (a->inf) { *r = *b; return; }
+    if (b->inf) { *r = *a; return; }
+    fp2 lam, t, x3, y3;
+    if (fp2_eq(&a->x, &b->x)) {
+        fp2 sy;
+        fp2_add(&sy, &a->y, &b->y);
+        if (fp2_is_zero(&sy)) { memset(r, 0, sizeof(*r)); r->inf = 1; return; }
+        /* doubling: lam = 3 x^2 / (2 y) */
+        fp2_sqr(&lam, &a->x);
+        fp2_add(&t, &lam, &lam);
+        fp2_add(&lam, &lam, &t);
+        fp2_add(&t, &a->y, &a->y);
+    } else {
+        fp2_sub(&lam, &b->y, &a->y);
+        fp2_sub(&t, &b->x, &a->x);
+    }
+    fp2_inv(&t, &t);
+    fp2_mul(&lam, &lam, &t);
+    fp2_sqr(&x3, &lam);
+    fp2_sub(&x3, &x3, &a->x);
+    fp2_sub(&x3, &x3, &b->x);
+    fp2_sub(&y3, &a->x, &x3);
+    fp2_mul(&y3, &y3, &lam);
+    fp2_sub(&y3, &y3, &a->y);
+    r->x = x3;
+    r->y = y3;
+    r->inf = 0;
+}
+static void g2_mul_limbs(g2a *r, const g2a *p, const uint64_t *k, int nl) {
+    g2a acc;
+    memset(&acc, 0, sizeof(acc));
+    acc.inf = 1;
+    for (int i = nl * 64 - 1; i >= 0; i--) {
+        g2_add(&acc, &acc, &acc);
+        if ((k[i / 64] >> (i % 64)) & 1) g2_add(&acc, &acc, p);
+    }
+    *r = acc;
+}
+
+/* 192-byte encoding used by this oracle: x.c0, x.c1, y.c0, y.c1, each 48-byte
+ * big-endian canonical; all-zero bytes = infinity. */
+static void g2_to_bytes(uint8_t out[192], const g2a *p) {
+    if (p->inf) { memset(out, 0, 192); return; }
+    const fp *c[4] = {&p->x.c0, &p->x.c1, &p->y.c0, &p->y.c1};
+    for (int k = 0; k < 4; k++) {
+        uint64_t plain[6];
+        fp_to_plain(plain, c[k]);
+        limbs_to_be(out + 48 * k, plain, 6);
+    }
+}
+static int g2_from_bytes(g2a *p, const uint8_t in[192]) {
+    int zero = 1;
+    for (int k = 0; k < 192; k++) if (in[k]) { zero = 0; break; }
+    memset(p, 0, sizeof(*p));
+    if (zero) { p->inf = 1; return 0; }
+    fp *c[4] = {&p->x.c0, &p->x.c1, &p->y.c0, &p->y.c1};
+    for (int k = 0; k < 4; k++) {
+        uint64_t plain[6];
+        be_to_limbs(plain, in + 48 * k, 6);
+        if (cmp_n(plain, P_MOD, 6) >= 0) return -1;
+        fp_from_plain(c[k], plain);
+    }
+    return g2_on_curve(p) ? 0 : -1;
+}
+
+/* A point of order r on E'(Fp2): the first x = i + u (i = 1, 2, ...) with a
+ * square right-hand side, times the cofactor. */
+static void g2_generator(g2a *q) {
+    for (uint64_t i = 1;; i++) {
+        uint64_t pi[6] = {i, 0, 0, 0, 0, 0};
+        g2a p;
+        memset(&p, 0, sizeof(p));
+        fp_from_plain(&p.x.c0, pi);
+        p.x.c1 = FP_ONE;
+        fp2 rhs, b;
+        fp2_sqr(&rhs, &p.x);
+        fp2_mul(&rhs, &rhs, &p.x);
+        g2_b(&b);
+        fp2_add(&rhs, &rhs, &b);
+        if (fp2_sqrt(&p.y, &rhs)) continue;
+        g2_mul_limbs(q, &p, G2_COFACTOR, 8);
+        if (!q->inf) return;
+    }
+}
+
+/* ---- Miller loop ---- */
+typedef struct { fp12 x, y; } e12pt;
+
+/* line through t with slope lam, evaluated at P = (xp, yp): (yp - yt) - lam (xp - xt) */
+static void line_eval(fp12 *l, const e12pt *t, const fp12 *lam, const fp12 *xp, const fp12 *yp) {
+    fp12 a, b;
+    fp12_sub(&a, yp, &t->y);
+    fp12_sub(&b, xp, &t->x);
+    fp12_mul(&b, &b, lam);
+    fp12_sub(l, &a, &b);
+}
+
+static void miller(fp12 *f, const g1a *p, const g2a *q) {
+    fp12_one(f);
+    if (p->inf || q->inf) return;
+    /* untwist Q into E(Fp12) */
+    fp12 w, winv, w2inv, w3inv;
+    memset(&w, 0, sizeof(w));
+    w.d1.c0.c0 = FP_ONE;
+    fp12_inv(&winv, &w);
+    fp12_sqr(&w2inv, &winv);
+    fp12_mul(&w3inv, &w2inv, &winv);
+    e12pt Q, T;
+    fp12_from_fp2(&Q.x, &q->x);
+    fp12_mul(&Q.x, &Q.x, &w2inv);
+    fp12_from_fp2(&Q.y, &q->y);
+    fp12_mul(&Q.y, &Q.y, &w3inv);
+    T = Q;
+    fp12 xp, yp;
+    fp12_from_fp(&xp, &p->x);
+    fp12_from_fp(&yp, &p->y);
+    int top = 63;
+    while (!((ATE_LOOP >> top) & 1)) top--;
+    for (int i = top - 1; i >= 0; i--) {
+        /* doubling step: lam = 3 X^2 / (2 Y) */
+        fp12 lam, t, l, x3, y3;
+        fp12_sqr(&lam, &T.x);
+        fp12_add(&t, &lam, &lam);
+        fp12_add(&lam, &lam, &t);
+        fp12_add(&t, &T.y, &T.y);
+        fp12_inv(&t, &t);
+        fp12_mul(&lam, &lam, &t);
+        line_eval(&l, &T, &lam, &xp, &yp);
+        fp12_sqr(f, f);
+        fp12_mul(f, f, &l);
+        fp12_sqr(&x3, &lam);
+        fp12_sub(&x3, &x3, &T.x);
+        fp12_sub(&x3, &x3, &T.x);
+        fp12_sub(&y3, &T.x, &x3);
+        fp12_mul(&y3, &y3, &lam);
+        fp12_sub(&y3, &y3, &T.y);
+        T.x = x3;
+        T.y = y3;
+        if ((ATE_LOOP >> i) & 1) {
+            /* addition step with Q: lam = (YQ - Y) / (XQ - X) */
+            fp12_sub(&lam, &Q.y, &T.y);
+            fp12_sub(&t, &Q.x, &T.x);
+            fp12_inv(&t, &t);
+            fp12_mul(&lam, &lam, &t);
+            line_eval(&l, &T, &lam, &xp, &yp);
+            fp12_mul(f, f, &l);
+            fp12_sqr(&x3, &lam);
+            fp12_sub(&x3, &x3, &T.x);
+            fp12_sub(&x3, &x3, &Q.x);
+            fp12_sub(&y3, &T.x, &x3);
+            fp12_mul(&y3, &y3, &lam);
+            fp12_sub(&y3, &y3, &T.y);
+            T.x = x3;
+            T.y = y3;
+        }
+    }
+}
+
+static void final_exp(fp12 *r, const fp12 *f) {
+    fp12 c, inv;
+    fp12_conj(&c, f);
+    fp12_inv(&inv, f);
+    fp12_mul(&c, &c, &inv); /* f^(p^6 - 1) */
+    fp12_pow_limbs(r, &c, FINAL_EXP_HARD, 32);
+}
+
+static void fp12_to_bytes(uint8_t out[576], const fp12 *a) {
+    const fp *c[12] = {&a->d0.c0.c0, &a->d0.c0.c1, &a->d0.c1.c0, &a->d0.c1.c1, &a->d0.c2.c0, &a->d0.c2.c1,
+                       &a->d1.c0.c0, &a->d1.c0.c1, &a->d1.c1.c0, &a->d1.c1.c1, &a->d1.c2.c0, &a->d1.c2.c1};
+    for (int k = 0; k < 12; k++) {
+        uint64_t plain[6];
+        fp_to_plain(plain, c[k]);
+        limbs_to_be(out + 48 * k, plain, 6);
+    }
+}
+
+void or_g2_generator(uint8_t out[192]) {
+    g2a q;
+    g2_generator(&q);
+    g2_to_bytes(out, &q);
+}
+
+void or_g2_cofactor(uint8_t out[64]) { limbs_to_be(out, G2_COFACTOR, 8); }
+void or_final_exp_hard(uint8_t out[256]) { limbs_to_be(out, FINAL_EXP_HARD, 32); }
+
+int or_g2_on_curve(const uint8_t p[192]) {
+    g2a a;
+    return g2_from_bytes(&a, p) == 0;
+}
+
+int or_g2_mul(uint8_t out[192], const uint8_t p[192], const uint8_t *k_be, uint32_t k_len) {
+    g2a a, r;
+    if (g2_from_bytes(&a, p)) return -1;
+    const int nl = (int)((k_len + 7) / 8);
+    uint64_t *k = (uint64_t *)calloc((size_t)nl, 8);
+    for (uint32_t b = 0; b < k_len; b++) {
+        const uint32_t pos = k_len - 1 - b;  /* byte index from the least significant end */
+        k[pos / 8] |= (uint64_t)k_be[b] << (8 * (pos % 8));
+    }
+    g2_mul_limbs(&r, &a, k, nl);
+    free(k);
+    g2_to_bytes(out, &r);
+    return 0;
+}
+
+int or_g2_add(uint8_t out[192], const uint8_t p[192], const uint8_t q[192]) {
+    g2a a, b, r;
+    if (g2_from_bytes(&a, p) || g2_from_bytes(&b, q)) return -1;
+    g2_add(&r, &a, &b);
+    g2_to_bytes(out, &r);
+    return 0;
+}
+
+/* A point on E'(Fp2) that is not cleared by the cofactor (for order tests). */
+int or_g2_raw_point(uint8_t out[192], uint64_t seed) {
+    for (uint64_t i = seed + 1;; i++) {
+        uint64_t pi[6] = {i, 0, 0, 0, 0, 0};
+        g2a p;
+        memset(&p, 0, sizeof(p));
+        fp_from_plain(&p.x.c0, pi);
+        p.x.c1 = FP_ONE;
+        fp2 rhs, b;
+        fp2_sqr(&rhs, &p.x);
+        fp2_mul(&rhs, &rhs, &p.x);
+        g2_b(&b);
+        fp2_add(&rhs, &rhs, &b);
+        if (fp2_sqrt(&p.y, &rhs)) continue;
+        g2_to_bytes(out, &p);
+        return 0;
+    }
+}
+
+/* [tau^i] Q for i = 0..n-1, Q = or_g2_generator. */
+void or_g2_powers_from_tau(uint8_t *out, const uint8_t tau[32], uint32_t n) {
+    fr t;
+    fr_from_be_reduce(&t, tau);
+    g2a q;
+    g2_generator(&q);
+    fr pw = FR_ONE;
+    fr *pws = (fr *)malloc(sizeof(fr) * n);
+    for (uint32_t i = 0; i < n; i++) { pws[i] = pw; fr_mul(&pw, &pw, &t); }
+#pragma omp parallel for schedule(dynamic)
+    for (int i = 0; i < (int)n; i++) {
+        uint64_t plain[4];
+        fr_to_plain(plain, &pws[i]);
+        g2a r;
+        g2_mul_limbs(&r, &q, plain, 4);
+        g2_to_bytes(out + 192 * (size_t)i, &r);
+    }
+    free(pws);
+}
+
+/* e(P, Q) = final_exp(miller(P, Q)) as 576 bytes (12 Fp, tower order). */
+int or_pairing(uint8_t out[576], const uint8_t p[48], const uint8_t q[192]) {
+    g1a a;
+    g2a b;
+    if (g1a_decompress(&a, p) || g2_from_bytes(&b, q)) return -1;
+    fp12 f, e;
+    miller(&f, &a, &b);
+    final_exp(&e, &f);
+    fp12_to_bytes(out, &e);
+    return 0;
+}
+
+static int pairing_product_is_one(const g1a *ps, const g2a *qs, size_t n) {
+    fp12 *fs = (fp12 *)malloc(sizeof(fp12) * (n ? n : 1));
+#pragma omp parallel for schedule(dynamic)
+    for (int k = 0; k < (int)n; k++) miller(&fs[k], &ps[k], &qs[k]);
+    fp12 acc, e, one;
+    fp12_one(&acc);
+    for (size_t k = 0; k < n; k++) fp12_mul(&acc, &acc, &fs[k]);
+    free(fs);
+    final_exp(&e, &acc);
+    fp12_one(&one);
+    return fp12_eq(&e, &one);
+}
+
+/* 1 when prod_k e(P_k, Q_k) == 1, 0 when not, -1 on a bad encoding. */
+int or_pairing_check(const uint8_t *g1s, const uint8_t *g2s, size_t n) {
+    g1a *ps = (g1a *)malloc(sizeof(g1a) * (n ? n : 1));
+    g2a *qs = (g2a *)malloc(sizeof(g2a) * (n ? n : 1));
+    int rc = 0;
+    for (size_t k = 0; k < n && !rc; k++)
+        if (g1a_decompress(&ps[k], g1s + 48 * k) || g2_from_bytes(&qs[k], g2s + 192 * k)) rc = -1;
+    if (!rc) rc = pairing_product_is_one(ps, qs, n);
+    free(ps);
+    free(qs);
+    return rc;
+}
+
+/* ---- batch verification of cell proofs ---- */
+struct or_vsetup {
+    g1a srs[4096];
+    g2a g2_gen, g2_tau64;
+};
+
+or_vsetup *or_vsetup_new(const uint8_t *g1_monomial, const uint8_t *g2_powers) {
+    or_vsetup *s = (or_vsetup *)calloc(1, sizeof(or_vsetup));
+    int bad = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad)
+    for (int k = 0; k < 4096; k++) bad |= g1a_decompress(&s->srs[k], g1_monomial + 48 * k) != 0;
+    if (bad || g2_from_bytes(&s->g2_gen, g2_powers) || g2_from_bytes(&s->g2_tau64, g2_powers + 192 * 64)) {
+        free(s);
+        return NULL;
+    }
+    return s;
+}
+
+void or_vsetup_free(or_vsetup *s) { free(s); }
+
+/* Commitment [p(tau)]G1 = sum_i c_i [tau^i]G1 from the monomial setup. */
+int or_commit(const or_vsetup *s, uint8_t out[48], const uint8_t blob[131072]) {
+    fr *c = (fr *)malloc(sizeof(fr) * 4096);
+    if (blob_to_coeffs(c, blob)) { free(c); return 6; }
+    g1j r;
+    msm_pippenger(&r, s->srs, c, 4096);
+    g1_compress(out, &r);
+    free(c);
+    return 0;
+}
+
+/* Commitments of n blobs, in parallel over blobs; 0, or 6 if any blob is non-canonical. */
+int or_commit_batch(const or_vsetup *s, uint8_t *out, const uint8_t *blobs, size_t n) {
+    int bad = 0;
+#pragma omp parallel for schedule(dynamic) reduction(| : bad)
+    for (int b = 0; b < (int)n; b++) bad |= or_commit(s, out + 48 * (size_t)b, blobs + (size_t)131072 * b) != 0;
+    return bad ? 6 : 0;
+}
+
+static uint64_t splitmix64(uint64_t *x) {
+    uint64_t z = (*x += 0x9e3779b97f4a7c15ULL);
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+    return z ^ (z >> 31);
+}
+
+/* Cell proof k claims p(X) = q(X)(X^64 - z_k) + I_k(X), where cell c holds p
+ * on the coset h_c H_64 (h_c = omega_8192^brev7(c), element j at
+ * h_c omega_64^brev6(j)), z_k = h_c^64 and I_k interpolates the cell.  With
+ * random weights r_k the 128-per-blob equations
+ *   e(pi_k, [tau^64]Q - z_k Q) = e(C_k - [I_k(tau)]G1, Q)
+ * fold into one product of two pairings:
+ *   e(sum r_k pi_k, [tau^64]Q) * e(-(sum r_k (C_k + z_k pi_k) - [sum r_k I_k](tau)), Q) == 1.
+ * This is the check of the consensus specs' verify_cell_kzg_proof_batch with
+ * seeded weights instead of Fiat-Shamir ones.  Returns 1 (valid), 0 (invalid),
+ * -1 (bad encoding, non-canonical cell element or cell index >= 128). */
+int or_verify_cell_proofs(const or_vsetup *s, const uint8_t *commitments, const uint32_t *cell_indices,
+                          const uint8_t *cells, const uint8_t *proofs, size_t n, uint64_t seed) {
+    if (n == 0) return 1;
+    g1a *pi = (g1a *)malloc(sizeof(g1a) * n);
+    fr *w = (fr *)malloc(sizeof(fr) * n);      /* r_k */
+    fr *wz = (fr *)malloc(sizeof(fr) * n);     /* r_k z_k */
+    fr (*ik)[64] = (fr (*)[64])malloc(sizeof(fr) * 64 * n);
+    int bad = 0;
+    uint64_t st = seed;
+    for (size_t k = 0; k < n; k++) {
+        uint64_t plain[4];
+        for (int q = 0; q < 4; q++) plain[q] = splitmix64(&st);
+        plain[3] &= (1ULL << 62) - 1;  /* < 2^254 < r */
+        fr_from_plain(&w[k], plain);
+        if (cell_indices[k] >= 128) bad = 1;
+    }
+    if (bad) goto out;
+#pragma omp parallel for schedule(static) reduction(| : bad)
+    for (int k = 0; k < (int)n; k++) {
+        if (g1a_decompress(&pi[k], proofs + 48 * (size_t)k)) { bad |= 1; continue; }
+        const uint32_t hc = brev(cell_indices[k], 7);
+        const fr z = ROOTS[(64u * hc) & 8191u];
+        fr_mul(&wz[k], &w[k], &z);
+        /* natural-order values on h H_64, inverse NTT_64 -> coefficients of I(h X) */
+        fr y[64];
+        for (uint32_t j = 0; j < 64; j++)
+            if (fr_from_be_canonical(&y[brev(j, 6)], cells + 2048 * (size_t)k + 32 * j)) bad |= 1;
+        fr_ntt_inplace(y, 64, 1);
+        fr_scale_inv_n(y, 64);
+        const fr hinv = ROOTS[(8192u - hc) & 8191u];
+        fr hp = w[k];  /* r_k h^-i */
+        for (int i = 0; i < 64; i++) {
+            fr_mul(&ik[k][i], &y[i], &hp);
+            fr_mul(&hp, &hp, &hinv);
+        }
+    }
+    if (bad) goto out;
+    {
+        fr agg[64];
+        memset(agg, 0, sizeof(agg));
+        for (size_t k = 0; k < n; k++)
+            for (int i = 0; i < 64; i++) fr_add(&agg[i], &agg[i], &ik[k][i]);
+        /* commitments: runs of equal bytes share one point and a summed weight */
+        g1a *cs = (g1a *)malloc(sizeof(g1a) * n);
+        fr *cw = (fr *)malloc(sizeof(fr) * n);
+        size_t nc = 0;
+        for (size_t k = 0; k < n; k++) {
+            if (k && !memcmp(commitments + 48 * k, commitments + 48 * (k - 1), 48)) {
+                fr_add(&cw[nc - 1], &cw[nc - 1], &w[k]);
+                continue;
+            }
+            if (g1a_decompress(&cs[nc], commitments + 48 * k)) { bad = 1; break; }
+            cw[nc++] = w[k];
+        }
+        if (!bad) {
+            g1j A, B, t;
+            msm_pippenger(&A, pi, w, n);
+            msm_pippenger(&B, pi, wz, n);
+            msm_pippenger(&t, cs, cw, nc);
+            g1_add(&B, &B, &t);
+            msm_pippenger(&t, s->srs, agg, 64);
+            g1_neg(&t, &t);
+            g1_add(&B, &B, &t);
+            g1_neg(&B, &B);
+            g1a ps[2];
+            g2a qs[2] = {s->g2_tau64, s->g2_gen};
+            g1_to_affine(&ps[0], &A);
+            g1_to_affine(&ps[1], &B);
+            bad = pairing_product_is_one(ps, qs, 2) ? 0 : 2;
+        }
+        free(cs);
+        free(cw);
+    }
+out:
+    free(pi);
+    free(w);
+    free(wz);
+    free(ik);
+    if (bad == 1) return -1;
+    return bad ? 0 : 1;
+}

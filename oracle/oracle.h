@@ -85,6 +85,43 @@ int or_proofs_from_tau(uint8_t *proofs, const uint8_t *blob, const uint8_t tau[3
 /* h[m-1] = sum_k c_{k+64m} s_k for m=1..63, then proofs = brp7(FFT_128(h)).  Slow. */
 int or_compute_proofs_direct(const or_setup *s, uint8_t *proofs, const uint8_t *blob);
 
+/* ---- pairing and batch cell-proof verification ---- */
+/* G2 points use a 192-byte encoding: x.c0, x.c1, y.c0, y.c1 (48-byte BE each),
+ * Fp2 = Fp[u]/(u^2+1); all-zero = infinity.  No G2 data ships with the
+ * reference, so G2 uses its own point Q of order r (any nonzero point of the
+ * cyclic group G2 gives the same verification equations). */
+void or_g2_generator(uint8_t out[192]);
+/* A point of E'(Fp2) before cofactor clearing (x = seed+1+i + u for the first
+ * square right-hand side). */
+int or_g2_raw_point(uint8_t out[192], uint64_t seed);
+int or_g2_on_curve(const uint8_t p[192]);
+/* out = k * P for a big-endian scalar of k_len bytes (any length). */
+int or_g2_mul(uint8_t out[192], const uint8_t p[192], const uint8_t *k_be, uint32_t k_len);
+int or_g2_add(uint8_t out[192], const uint8_t p[192], const uint8_t q[192]);
+/* [tau^i]Q for i < n. */
+void or_g2_powers_from_tau(uint8_t *out /* n*192 */, const uint8_t tau[32], uint32_t n);
+/* Constants exposed for tests: G2 cofactor (64 B BE) and (p^6+1)/r (256 B BE). */
+void or_g2_cofactor(uint8_t out[64]);
+void or_final_exp_hard(uint8_t out[256]);
+/* Optimal-ate pairing value (after final exponentiation), 12 Fp elements in
+ * tower order d0.c0.c0, d0.c0.c1, d0.c1.c0, ..., d1.c2.c1, 48-byte BE each. */
+int or_pairing(uint8_t out[576], const uint8_t p[48], const uint8_t q[192]);
+/* 1 when prod_k e(P_k, Q_k) == 1, 0 otherwise, -1 on a bad encoding. */
+int or_pairing_check(const uint8_t *g1s /* n*48 */, const uint8_t *g2s /* n*192 */, size_t n);
+
+/* Verification setup: 4096 compressed G1 monomial points and >= 65 G2 powers [tau^i]Q. */
+typedef struct or_vsetup or_vsetup;
+or_vsetup *or_vsetup_new(const uint8_t *g1_monomial /* 4096*48 */, const uint8_t *g2_powers /* 65*192 */);
+void or_vsetup_free(or_vsetup *s);
+/* KZG commitment of a blob; 0, or 6 for a non-canonical blob. */
+int or_commit(const or_vsetup *s, uint8_t out[48], const uint8_t blob[131072]);
+/* Commitments of n blobs (n*48 bytes), computed in parallel; 0 or 6. */
+int or_commit_batch(const or_vsetup *s, uint8_t *out, const uint8_t *blobs, size_t n);
+/* Batch verification of n (commitment, cell index, cell, proof) items with
+ * random weights drawn from `seed`: 1 valid, 0 invalid, -1 malformed input. */
+int or_verify_cell_proofs(const or_vsetup *s, const uint8_t *commitments, const uint32_t *cell_indices,
+                          const uint8_t *cells, const uint8_t *proofs, size_t n, uint64_t seed);
+
 #ifdef __cplusplus
 }
 #endif

@@ -51,6 +51,23 @@ def lib():
         L.or_fp_mul_be.argtypes = [c, u8p, u8p]
         L.or_fr_inv_be.argtypes = [c, u8p]
         L.or_root_8192_be.argtypes = [c, ctypes.c_uint32]
+        L.or_g2_generator.argtypes = [c]
+        L.or_g2_raw_point.argtypes = [c, ctypes.c_uint64]
+        L.or_g2_on_curve.argtypes = [u8p]
+        L.or_g2_mul.argtypes = [c, u8p, u8p, ctypes.c_uint32]
+        L.or_g2_add.argtypes = [c, u8p, u8p]
+        L.or_g2_powers_from_tau.argtypes = [c, u8p, ctypes.c_uint32]
+        L.or_g2_cofactor.argtypes = [c]
+        L.or_final_exp_hard.argtypes = [c]
+        L.or_pairing.argtypes = [c, u8p, u8p]
+        L.or_pairing_check.argtypes = [u8p, u8p, ctypes.c_size_t]
+        L.or_vsetup_new.restype = ctypes.c_void_p
+        L.or_vsetup_new.argtypes = [u8p, u8p]
+        L.or_vsetup_free.argtypes = [ctypes.c_void_p]
+        L.or_commit.argtypes = [ctypes.c_void_p, c, u8p]
+        L.or_commit_batch.argtypes = [ctypes.c_void_p, c, u8p, ctypes.c_size_t]
+        L.or_verify_cell_proofs.argtypes = [ctypes.c_void_p, u8p, ctypes.POINTER(ctypes.c_uint32), u8p, u8p,
+                                            ctypes.c_size_t, ctypes.c_uint64]
         _lib = L
     return _lib
 
@@ -174,3 +191,119 @@ def root_8192(k: int) -> int:
     out = ctypes.create_string_buffer(32)
     lib().or_root_8192_be(out, k)
     return int.from_bytes(out.raw, "big")
+
+
+# ---- pairing and batch cell-proof verification ----
+
+def g2_generator() -> bytes:
+    out = ctypes.create_string_buffer(192)
+    lib().or_g2_generator(out)
+    return out.raw
+
+
+def g2_raw_point(seed: int) -> bytes:
+    out = ctypes.create_string_buffer(192)
+    lib().or_g2_raw_point(out, seed)
+    return out.raw
+
+
+def g2_on_curve(p: bytes) -> bool:
+    return bool(lib().or_g2_on_curve(p))
+
+
+def g2_mul(p: bytes, k: int) -> bytes:
+    kb = k.to_bytes(max(1, (k.bit_length() + 7) // 8), "big")
+    out = ctypes.create_string_buffer(192)
+    if lib().or_g2_mul(out, p, kb, len(kb)):
+        raise ValueError("bad G2 point")
+    return out.raw
+
+
+def g2_add(p: bytes, q: bytes) -> bytes:
+    out = ctypes.create_string_buffer(192)
+    if lib().or_g2_add(out, p, q):
+        raise ValueError("bad G2 point")
+    return out.raw
+
+
+def g2_powers_from_tau(tau: int, n: int = 65) -> bytes:
+    out = ctypes.create_string_buffer(192 * n)
+    lib().or_g2_powers_from_tau(out, (tau % R).to_bytes(32, "big"), n)
+    return out.raw
+
+
+def g2_cofactor() -> int:
+    out = ctypes.create_string_buffer(64)
+    lib().or_g2_cofactor(out)
+    return int.from_bytes(out.raw, "big")
+
+
+def final_exp_hard() -> int:
+    out = ctypes.create_string_buffer(256)
+    lib().or_final_exp_hard(out)
+    return int.from_bytes(out.raw, "big")
+
+
+def pairing(p: bytes, q: bytes) -> bytes:
+    out = ctypes.create_string_buffer(576)
+    if lib().or_pairing(out, p, q):
+        raise ValueError("bad point")
+    return out.raw
+
+
+def pairing_check(pairs) -> bool:
+    """True when the product of e(P, Q) over (P, Q) in pairs is 1."""
+    g1 = b"".join(p for p, _ in pairs)
+    g2 = b"".join(q for _, q in pairs)
+    rc = lib().or_pairing_check(g1, g2, len(pairs))
+    if rc < 0:
+        raise ValueError("bad point")
+    return rc == 1
+
+
+class VerifySetup:
+    """Monomial G1 setup plus G2 powers for commitments and cell-proof checks."""
+
+    def __init__(self, g1_monomial: bytes, g2_powers: bytes):
+        assert len(g1_monomial) == 4096 * 48 and len(g2_powers) >= 65 * 192
+        self._h = lib().or_vsetup_new(g1_monomial, g2_powers)
+        if not self._h:
+            raise ValueError("invalid setup point")
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().or_vsetup_free(self._h)
+            self._h = None
+
+    def commit(self, blob: bytes) -> bytes:
+        out = ctypes.create_string_buffer(48)
+        rc = lib().or_commit(self._h, out, blob)
+        if rc:
+            raise ValueError("non-canonical blob")
+        return out.raw
+
+    def commit_batch(self, blobs: bytes, n: int) -> bytes:
+        out = ctypes.create_string_buffer(48 * max(1, n))
+        if lib().or_commit_batch(self._h, out, blobs, n):
+            raise ValueError("non-canonical blob")
+        return out.raw[:48 * n]
+
+    def verify(self, commitments: bytes, cell_indices, cells: bytes, proofs: bytes, seed: int = 1) -> bool:
+        """Batch check of n items (commitment, cell index, cell, proof).
+        Returns True/False; raises ValueError on malformed input."""
+        n = len(cell_indices)
+        assert len(commitments) == 48 * n and len(cells) == 2048 * n and len(proofs) == 48 * n
+        idx = (ctypes.c_uint32 * max(1, n))(*cell_indices)
+        rc = lib().or_verify_cell_proofs(self._h, commitments, idx, cells, proofs, n, seed)
+        if rc < 0:
+            raise ValueError("malformed verification input")
+        return rc == 1
+
+    def verify_blobs(self, blobs: bytes, proofs: bytes, n: int, cells: bytes = None, seed: int = 1) -> bool:
+        """Every proof of n blobs against the blobs' commitments and cells."""
+        one = self.commit_batch(blobs, n)
+        coms = b"".join(one[48 * b:48 * (b + 1)] * CELLS_PER_BLOB for b in range(n))
+        if cells is None:
+            cells = b"".join(blob_to_cells(blobs[b * BYTES_PER_BLOB:(b + 1) * BYTES_PER_BLOB])[1]
+                             for b in range(n))
+        return self.verify(coms, list(range(CELLS_PER_BLOB)) * n, cells, proofs, seed)

@@ -74,6 +74,13 @@ def seeded_setup_bytes(oracle_mod):
 
 
 @pytest.fixture(scope="session")
+def seeded_verify_setup(oracle_mod, seeded_setup_bytes):
+    """Commitment and batch cell-proof verification for the seeded-tau setup
+    (G1 powers plus 65 G2 powers [tau^i]Q)."""
+    return oracle_mod.VerifySetup(seeded_setup_bytes, oracle_mod.g2_powers_from_tau(SEEDED_TAU, 65))
+
+
+@pytest.fixture(scope="session")
 def cells_of(vectors, oracle_mod):
     """Oracle cells of a compute vector (their sha256 is checked by test_oracle)."""
     cache = {}

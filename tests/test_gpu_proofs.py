@@ -79,8 +79,9 @@ def test_concurrent_calls(gpu_prover, vectors):
     assert not bad
 
 
-def test_seeded_setup_closed_form(gpu_seeded_prover, oracle_mod):
-    """A seeded-tau setup: every GPU proof equals [q_c(tau)]G1."""
+def test_seeded_setup_closed_form(gpu_seeded_prover, oracle_mod, seeded_verify_setup):
+    """A seeded-tau setup: every GPU proof equals [q_c(tau)]G1 and passes
+    the pairing check of batch cell-proof verification."""
     rng = random.Random(31)
     blobs = [random_blob(rng) for _ in range(3)]
     out = gpu_seeded_prover.compute_cell_kzg_proofs_batch(b"".join(blobs), 3)
@@ -88,11 +89,13 @@ def test_seeded_setup_closed_form(gpu_seeded_prover, oracle_mod):
         rc, want = oracle_mod.proofs_from_tau(blob, SEEDED_TAU)
         assert rc == 0
         assert out[PROOFS * i:PROOFS * (i + 1)] == want
+    assert seeded_verify_setup.verify_blobs(b"".join(blobs), out, 3)
 
 
-def test_seeded_setup_256_blobs(gpu_seeded_prover, oracle_mod):
-    """The 256-blob batch: spot-check blobs against the closed form and the
-    whole batch for linearity (proofs(a)+proofs(b) = proofs(a+b))."""
+def test_seeded_setup_256_blobs(gpu_seeded_prover, oracle_mod, seeded_verify_setup):
+    """The 256-blob batch: spot-check blobs against the closed form, the
+    whole batch for linearity (proofs(a)+proofs(b) = proofs(a+b)), and every
+    one of the 32768 proofs by the pairing check."""
     R = 0x73EDA753299D7D483339D80809A1D80553BDA402FFFE5BFEFFFFFFFF00000001
     rng = random.Random(32)
     base = [random_blob(rng) for _ in range(3)]
@@ -109,6 +112,7 @@ def test_seeded_setup_256_blobs(gpu_seeded_prover, oracle_mod):
         s1 = out[PROOFS * 1 + 48 * c:PROOFS * 1 + 48 * c + 48]
         s255 = out[PROOFS * 255 + 48 * c:PROOFS * 255 + 48 * c + 48]
         assert oracle_mod.g1_add(s0, s1) == s255
+    assert seeded_verify_setup.verify_blobs(b"".join(blobs), out, 256, seed=256)
 
 
 def test_device_entry_point_matches_host(gpu_prover, vectors):
