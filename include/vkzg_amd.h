@@ -44,10 +44,36 @@ typedef struct {
     uint32_t max_batch_size;   /* chunk size actually used */
     double setup_ms;           /* wall time of prover creation */
     uint32_t last_splits;      /* sub-streams the last call's batches were spread over */
+    uint32_t graphs_enabled;   /* 1 unless VKZG_AMD_GRAPHS=0 */
+    uint64_t graph_captures;   /* pipelines captured into HIP graphs by the synchronous calls */
+    uint64_t graph_replays;    /* calls served by replaying a captured graph */
+    uint32_t autotuned;        /* 1 after vkzg_amd_autotune */
+    double autotune_ms;        /* its wall time */
 } vkzg_amd_info;
 
 /* Fills `out` (never fails for a non-NULL prover). */
 vkzg_result vkzg_amd_prover_info(const vkzg_prover *p, vkzg_amd_info *out);
+
+/* ------------------------------------------------------------- tuning
+ * Launch shape of the proof pipeline per batch-size class (<= 1, <= 4,
+ * <= 16, <= 64 and more blobs).  Every tuning gives bit-identical proofs.
+ * No reference counterpart: the reference fixes one launch shape. */
+typedef struct {
+    uint32_t phase_a_groups;        /* window groups (work-queue tiles) per phase-A MSM, 1-5 */
+    uint32_t phase_a_blocks_per_cu; /* persistent phase-A grid in blocks per CU, 1-2 */
+    uint32_t normalize_chunk;       /* points per batched inversion, 1-256 */
+    uint32_t splits;                /* sub-streams a batch is spread over, 1-4 */
+} vkzg_amd_tuning;
+
+/* The tuning used for a batch of `num_blobs` blobs. */
+vkzg_result vkzg_amd_get_tuning(const vkzg_prover *p, size_t num_blobs, vkzg_amd_tuning *out);
+/* Sets the tuning of the size class of `num_blobs`; BADARGS when a field is out of range. */
+vkzg_result vkzg_amd_set_tuning(vkzg_prover *p, size_t num_blobs, const vkzg_amd_tuning *t);
+/* Times candidate tunings on synthetic blobs for every size class up to the
+ * prover's batch size and keeps the fastest; *ms_out (optional) receives the
+ * time it took.  $VKZG_AMD_AUTOTUNE=1 runs it at prover creation.  Do not run
+ * it while a server (below) is using the prover. */
+vkzg_result vkzg_amd_autotune(vkzg_prover *p, double *ms_out);
 
 /* ------------------------------------------------------------------ server
  * Continuous batching of proof and recovery requests on one prover.  A

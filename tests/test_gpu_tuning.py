@@ -1,0 +1,90 @@
+"""GPU tests of the launch-shape knobs: every tuning (phase-A work-queue tiles,
+persistent grid, normalisation chunk, sub-streams) and the autotuner's choice
+give bit-identical proofs, and captured HIP graphs replay to the same bytes as
+kernel-by-kernel launches."""
+import random
+
+import pytest
+
+import vkzg_amd
+
+pytestmark = pytest.mark.gpu
+
+PROOFS = 128 * 48
+NAMES = ["compute_cells_and_kzg_proofs_case_valid_%d" % i for i in (2, 3, 4)]
+
+
+@pytest.fixture
+def restore_tuning(gpu_prover):
+    saved = {n: gpu_prover.get_tuning(n) for n in (1, 3, 10, 40, 200)}
+    yield
+    for n, t in saved.items():
+        gpu_prover.set_tuning(n, **t)
+
+
+@pytest.mark.parametrize("groups", [1, 2, 4, 5])
+def test_phase_a_groups_bit_exact(gpu_prover, vectors, restore_tuning, groups):
+    blobs = b"".join(vectors.blob(n) for n in NAMES)
+    want = b"".join(vectors.proofs(n) for n in NAMES)
+    for bpc, chunk in ((1, 1), (2, 64)):
+        gpu_prover.set_tuning(3, phase_a_groups=groups, phase_a_blocks_per_cu=bpc, normalize_chunk=chunk)
+        assert gpu_prover.get_tuning(3)["phase_a_groups"] == groups
+        assert gpu_prover.compute_cell_kzg_proofs_batch(blobs, 3) == want
+
+
+def test_splits_and_odd_sizes_bit_exact(gpu_prover, vectors, restore_tuning):
+    names = [NAMES[i % 3] for i in range(37)]
+    blobs = b"".join(vectors.blob(n) for n in names)
+    want = b"".join(vectors.proofs(n) for n in names)
+    gpu_prover.set_tuning(37, phase_a_groups=5, splits=3, normalize_chunk=16)
+    assert gpu_prover.compute_cell_kzg_proofs_batch(blobs, 37) == want
+
+
+def test_tuning_rejects_out_of_range(gpu_prover, restore_tuning):
+    for bad in (dict(phase_a_groups=0), dict(phase_a_groups=6), dict(phase_a_blocks_per_cu=3),
+                dict(normalize_chunk=0), dict(normalize_chunk=257), dict(splits=5)):
+        with pytest.raises(vkzg_amd.VkzgError) as e:
+            gpu_prover.set_tuning(3, **bad)
+        assert e.value.code == vkzg_amd.ERR_BADARGS
+
+
+def test_autotune_keeps_results(gpu_prover_small_batch, vectors):
+    p = gpu_prover_small_batch
+    ms = p.autotune()
+    info = p.info()
+    assert info["autotuned"] == 1 and ms > 0
+    for n in (1, 4):
+        t = p.get_tuning(n)
+        assert 1 <= t["phase_a_groups"] <= 5 and 1 <= t["splits"] <= 4
+    names = [NAMES[i % 3] for i in range(6)]  # chunks of 4 + 2
+    blobs = b"".join(vectors.blob(n) for n in names)
+    assert p.compute_cell_kzg_proofs_batch(blobs, 6) == b"".join(vectors.proofs(n) for n in names)
+
+
+def test_graph_replays_match_eager(gpu_prover, vectors, cells_of):
+    info0 = gpu_prover.info()
+    if not info0["graphs_enabled"]:
+        pytest.skip("graphs disabled by VKZG_AMD_GRAPHS=0")
+    rng = random.Random(3)
+    names = [NAMES[rng.randrange(3)] for _ in range(5)]
+    blobs = b"".join(vectors.blob(n) for n in names)
+    want = b"".join(vectors.proofs(n) for n in names)
+    cells = b"".join(cells_of(n) for n in names)
+    present = bytearray(128 * 5)
+    for b in range(5):
+        for c in rng.sample(range(128), 64 + b * 10):
+            present[128 * b + c] = 1
+    # eager reference (stage timing forces kernel-by-kernel launches)
+    gpu_prover.set_stage_timing(True)
+    eager_p = gpu_prover.compute_cell_kzg_proofs_batch(blobs, 5)
+    eager_r = gpu_prover.recover_cells_batch(cells, bytes(present), 5)
+    gpu_prover.set_stage_timing(False)
+    assert eager_p == want and eager_r == cells
+    # graph capture, then several replays with other work in between
+    for rep in range(4):
+        assert gpu_prover.compute_cell_kzg_proofs_batch(blobs, 5) == want, rep
+        assert gpu_prover.recover_cells_batch(cells, bytes(present), 5) == cells, rep
+        assert gpu_prover.compute_cell_kzg_proofs_batch(blobs[:131072], 1) == want[:PROOFS], rep
+    info1 = gpu_prover.info()
+    assert info1["graph_captures"] + info1["graph_replays"] >= info0["graph_captures"] + info0["graph_replays"] + 12
+    assert info1["graph_replays"] >= info0["graph_replays"] + 9

@@ -13,7 +13,8 @@ the reference's Java binding (bindings/java/src/main/java/vkzg/Vkzg.java):
 import ctypes
 import threading
 
-from ._lib import LIB_PATH, RequestInfo, ServerOptions, ServerStats, VkzgAmdInfo, VkzgOptions, load
+from ._lib import (LIB_PATH, RequestInfo, ServerOptions, ServerStats, Tuning, VkzgAmdInfo, VkzgOptions,
+                   load)
 
 FIELD_ELEMENTS_PER_BLOB = 4096
 BYTES_PER_FIELD_ELEMENT = 32
@@ -111,6 +112,25 @@ class Prover:
         pm = ctypes.create_string_buffer(bytes(cell_present), max(1, len(cell_present)))
         _check(load().vkzg_recover_cells_batch(self._h, out, c, pm, blob_count))
         return out.raw[: len(cells)]
+
+    def get_tuning(self, blob_count):
+        t = Tuning()
+        _check(load().vkzg_amd_get_tuning(self._h, blob_count, ctypes.byref(t)))
+        return {f: getattr(t, f) for f, _ in Tuning._fields_}
+
+    def set_tuning(self, blob_count, **fields):
+        """Set fields of the tuning of blob_count's size class (others unchanged)."""
+        t = Tuning()
+        _check(load().vkzg_amd_get_tuning(self._h, blob_count, ctypes.byref(t)))
+        for k, v in fields.items():
+            setattr(t, k, v)
+        _check(load().vkzg_amd_set_tuning(self._h, blob_count, ctypes.byref(t)))
+
+    def autotune(self):
+        """Run the autotuner; returns its wall time in ms."""
+        ms = ctypes.c_double()
+        _check(load().vkzg_amd_autotune(self._h, ctypes.byref(ms)))
+        return ms.value
 
     def set_stage_timing(self, enable=True):
         _check(load().vkzg_amd_set_stage_timing(self._h, 1 if enable else 0))

@@ -38,6 +38,9 @@ EXT_SYMBOLS = [
     "vkzg_amd_server_poll",
     "vkzg_amd_server_wait",
     "vkzg_amd_server_stats",
+    "vkzg_amd_get_tuning",
+    "vkzg_amd_set_tuning",
+    "vkzg_amd_autotune",
 ]
 
 
@@ -54,6 +57,20 @@ class VkzgAmdInfo(ctypes.Structure):
         ("max_batch_size", ctypes.c_uint32),
         ("setup_ms", ctypes.c_double),
         ("last_splits", ctypes.c_uint32),
+        ("graphs_enabled", ctypes.c_uint32),
+        ("graph_captures", ctypes.c_uint64),
+        ("graph_replays", ctypes.c_uint64),
+        ("autotuned", ctypes.c_uint32),
+        ("autotune_ms", ctypes.c_double),
+    ]
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [
+        ("phase_a_groups", ctypes.c_uint32),
+        ("phase_a_blocks_per_cu", ctypes.c_uint32),
+        ("normalize_chunk", ctypes.c_uint32),
+        ("splits", ctypes.c_uint32),
     ]
 
 
@@ -134,6 +151,12 @@ def load():
     L.vkzg_amd_server_wait.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(RequestInfo)]
     L.vkzg_amd_server_stats.restype = ctypes.c_int
     L.vkzg_amd_server_stats.argtypes = [vp, ctypes.POINTER(ServerStats)]
+    L.vkzg_amd_get_tuning.restype = ctypes.c_int
+    L.vkzg_amd_get_tuning.argtypes = [vp, sz, ctypes.POINTER(Tuning)]
+    L.vkzg_amd_set_tuning.restype = ctypes.c_int
+    L.vkzg_amd_set_tuning.argtypes = [vp, sz, ctypes.POINTER(Tuning)]
+    L.vkzg_amd_autotune.restype = ctypes.c_int
+    L.vkzg_amd_autotune.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
     for name in ("vkzg_amd_debug_fp_mul", "vkzg_amd_debug_fr_mul"):
         f = getattr(L, name)
         f.restype = ctypes.c_int

@@ -23,7 +23,7 @@ constexpr int kColPitch = 4 * 128;        // LDS pitch of the 4 per-wave circula
 // circulant columns = 144 KiB (gfx950 has 160 KiB per CU).
 __global__ void __launch_bounds__(kScalarThreads)
     k_fk20_scalars(Fr *scal, const uint8_t *blobs, uint32_t *err, const Fr *roots,
-                   const Fr *roots_inv) {
+                   const Fr *roots_inv, uint32_t *queue) {
     __shared__ uint32_t poly[8 * kPolyPitch];
     __shared__ uint32_t cols[8 * kColPitch];
     const uint32_t blob = blockIdx.x;
@@ -43,8 +43,11 @@ __global__ void __launch_bounds__(kScalarThreads)
         if (!limbs_lt<8>(v.v, cfr::P)) bad = true;
         lds_store_fr(poly, kPolyPitch, i, fr_from_canonical(v));
     }
-    if (bad) atomicOr(&err[blob], 1u);
-    __syncthreads();
+    // Every blob's flag is written (0 or 1), so err needs no clearing first.
+    const int any_bad = __syncthreads_or(bad ? 1 : 0);
+    if (tid == 0) err[blob] = any_bad ? 1u : 0u;
+    // Clear the tile counter of the persistent phase-A launch that follows.
+    if (blob == 0 && tid == 0) queue[0] = 0u;
 
     // 2. Inverse NTT_4096 -> monomial coefficients, scaled by 1/4096.
     lds_ntt_dit<12>(poly, kPolyPitch, 1, roots_inv, tid, kScalarThreads);
@@ -98,9 +101,9 @@ __global__ void __launch_bounds__(kScalarThreads)
 }  // namespace
 
 hipError_t launch_fk20_scalars(Fr *scal, const uint8_t *blobs, uint32_t *err, const Fr *roots,
-                               const Fr *roots_inv, uint32_t nblobs, hipStream_t s) {
+                               const Fr *roots_inv, uint32_t *queue, uint32_t nblobs, hipStream_t s) {
     hipLaunchKernelGGL(k_fk20_scalars, dim3(nblobs), dim3(kScalarThreads), 0, s, scal, blobs, err,
-                       roots, roots_inv);
+                       roots, roots_inv, queue);
     return hipGetLastError();
 }
 

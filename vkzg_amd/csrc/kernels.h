@@ -11,7 +11,7 @@
 //   G1A table[128][64][W][H]  table[j][i][w][d-1] = d * 2^(c*w) * xext[j][i],
 //                             c = window bits, W = ceil(256/c), H = 2^(c-1)
 //   Fr scal[B][128][64]       phase-A scalars, CANONICAL (not Montgomery)
-//   G1X u[B][128]             phase-A outputs
+//   G1X u[B][128][G]          phase-A partial sums, one per window group g < G
 //   G1J lad[B][32][128]       2^(8d) * u[j]
 //   G1J fold[B][2][64][32]    L+ / L- ladders, normalised into G1A lad_aff with
 //                             the same layout
@@ -44,11 +44,14 @@ hipError_t launch_table_multiples(G1A *table, const G1A *pw, uint32_t c, uint32_
 hipError_t launch_phase_b_scalars(Fr *out, const Fr *roots, hipStream_t s);
 
 // ----------------------------------------------------------------- proofs
+// Also clears queue[0], the tile counter of the phase-A launch that follows.
 hipError_t launch_fk20_scalars(Fr *scal, const uint8_t *blobs, uint32_t *err, const Fr *roots,
-                               const Fr *roots_inv, uint32_t nblobs, hipStream_t s);
-hipError_t launch_phase_a(G1X *u, const Fr *scal, const G1A *table, uint32_t c, uint32_t W,
-                          uint32_t nblobs, hipStream_t s);
-hipError_t launch_ladder(G1J *lad, const G1X *u, uint32_t nblobs, hipStream_t s);
+                               const Fr *roots_inv, uint32_t *queue, uint32_t nblobs, hipStream_t s);
+// Persistent: `grid` blocks drain (msm, window group) tiles from queue[0].
+hipError_t launch_phase_a(G1X *part, const Fr *scal, const G1A *table, uint32_t c, uint32_t W,
+                          uint32_t nblobs, uint32_t groups, uint32_t grid, uint32_t *queue,
+                          hipStream_t s);
+hipError_t launch_ladder(G1J *lad, const G1X *part, uint32_t groups, uint32_t nblobs, hipStream_t s);
 hipError_t launch_fold(G1J *fold, const G1J *lad, uint32_t nblobs, hipStream_t s);
 hipError_t launch_phase_b(G1X *buckets, const G1A *lad_aff, const uint32_t *items,
                           const uint32_t *offsets, uint32_t nblobs, hipStream_t s);

@@ -4,15 +4,19 @@
 namespace vkzg_amd {
 
 namespace {
-// lad[blob][d][j] = 2^(8d) * u[blob][j], d < 32.
-__global__ void k_ladder(G1J *lad, const G1X *u, uint32_t nblobs) {
+// lad[blob][d][j] = 2^(8d) * u[blob][j], d < 32, where u[blob][j] is the
+// sum of phase A's `groups` window-group partials part[(blob*128 + j)*groups + g].
+__global__ void k_ladder(G1J *lad, const G1X *part, uint32_t groups, uint32_t nblobs) {
     // A long dependent chain: give these waves issue priority over the
     // throughput kernels of another sub-batch sharing the SIMD.
     __builtin_amdgcn_s_setprio(3);
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= nblobs * 128) return;
     const uint32_t blob = g / 128, j = g % 128;
-    G1J p = g1j_from_xyzz(u[g]);
+    G1X u = part[(size_t)g * groups];
+#pragma unroll 1
+    for (uint32_t k = 1; k < groups; k++) u = g1x_add(u, part[(size_t)g * groups + k]);
+    G1J p = g1j_from_xyzz(u);
     G1J *dst = lad + (size_t)blob * 32 * 128 + j;
 #pragma unroll 1
     for (int d = 0; d < 32; d++) {
@@ -40,9 +44,10 @@ __global__ void k_fold(G1J *fold, const G1J *lad, uint32_t nblobs) {
 
 }  // namespace
 
-hipError_t launch_ladder(G1J *lad, const G1X *u, uint32_t nblobs, hipStream_t s) {
-    hipLaunchKernelGGL(k_ladder, dim3(blocks_for(nblobs * 128ull, 64)), dim3(64), 0, s, lad, u,
-                       nblobs);
+hipError_t launch_ladder(G1J *lad, const G1X *part, uint32_t groups, uint32_t nblobs,
+                         hipStream_t s) {
+    hipLaunchKernelGGL(k_ladder, dim3(blocks_for(nblobs * 128ull, 64)), dim3(64), 0, s, lad, part,
+                       groups, nblobs);
     return hipGetLastError();
 }
 

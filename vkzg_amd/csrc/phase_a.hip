@@ -14,78 +14,110 @@ VKZG_DEV uint32_t window_bits(const Fr &s, uint32_t w, uint32_t c) {
     return lo & ((1u << c) - 1u);
 }
 
-constexpr int kMsmLanes = 16;  // lanes per phase-A MSM (64 terms / 16 = 4 each)
+constexpr int kMsmLanes = 16;  // lanes per tile (64 terms / 16 = 4 each)
+constexpr int kTilesPerItem = 256 / kMsmLanes;  // tiles one 256-thread block takes per queue item
 
-// One MSM recomputed with the checked point operations: reached only when the
-// fast path flagged a doubling or cancellation case for this MSM.
+// Windows [w0, w1) of the 64 terms of one MSM, recomputed with the checked
+// point operations: reached only when the fast path flagged a doubling or
+// cancellation case in this tile.
 __device__ __attribute__((noinline)) void phase_a_slow(G1X *out, const Fr *scal, const G1A *tab_j,
-                                                       uint32_t c, uint32_t W) {
+                                                       uint32_t c, uint32_t W, uint32_t w0,
+                                                       uint32_t w1) {
     const uint32_t H = 1u << (c - 1);
     G1X acc = g1x_identity();
     for (uint32_t i = 0; i < 64; i++) {
         const Fr s = scal[i];
         const G1A *tab = tab_j + (size_t)i * W * H;
         uint32_t carry = 0;
-        for (uint32_t w = 0; w < W; w++) {
+        for (uint32_t w = 0; w < w1; w++) {
             const uint32_t raw = window_bits(s, w, c) + carry;
             carry = raw > H ? 1u : 0u;
             const uint32_t mag = carry ? (2u * H - raw) : raw;
-            if (mag) acc = g1x_madd(acc, g1a_neg_if(tab[(size_t)w * H + (mag - 1)], carry != 0));
+            if (w >= w0 && mag) acc = g1x_madd(acc, g1a_neg_if(tab[(size_t)w * H + (mag - 1)], carry != 0));
         }
     }
     *out = acc;
 }
 
-// u[blob][j] = sum_{i<64} scal[blob][j][i] * xext[j][i], with signed c-bit
-// digits v in [-(2^(c-1)-1), 2^(c-1)] (the recoding of k_phase_a_sort.comp:
-// 31-40 with a c-bit window) and table[j][i][w][|v|-1] = |v| * 2^(cw) * xext[j][i].
+// Persistent phase A.  The work is a queue of tiles (msm, g): windows
+// [g W / G, (g+1) W / G) of MSM msm = blob * 128 + j, where
+//   u[msm] = sum_{i<64} scal[msm][i] * xext[j][i]
+// with signed c-bit digits v in [-(2^(c-1)-1), 2^(c-1)] (the recoding of
+// k_phase_a_sort.comp:31-40 with a c-bit window) and
+// table[j][i][w][|v|-1] = |v| * 2^(cw) * xext[j][i].  Tile (msm, g) writes its
+// partial sum to part[msm * G + g]; the ladder adds the G partials.  A block
+// takes 16 consecutive tiles per queue item (16 lanes per tile, 4 terms per
+// lane) from the counter q[0] until the queue is drained; every block exits
+// after one failed take, so the grid always drains.  q[0] is cleared by the
+// kernel that runs before this one.
 __global__ void __launch_bounds__(256, 2)
-    k_phase_a(G1X *u, const Fr *scal, const G1A *table, uint32_t c, uint32_t W, uint32_t nmsm) {
-    const uint32_t msm = blockIdx.x * (256 / kMsmLanes) + threadIdx.x / kMsmLanes;
+    k_phase_a(G1X *part, const Fr *scal, const G1A *table, uint32_t c, uint32_t W, uint32_t nmsm,
+              uint32_t groups, uint32_t *q) {
+    __shared__ uint32_t item_sh;
     const uint32_t t = threadIdx.x % kMsmLanes;
-    const bool live = msm < nmsm;
-    const uint32_t j = msm % 128;
     const uint32_t H = 1u << (c - 1);
-    G1X acc = g1x_identity();
-    bool exc = false;
-    if (live) {
+    const uint32_t ntiles = nmsm * groups;
+    const uint32_t nitems = (ntiles + kTilesPerItem - 1) / kTilesPerItem;
 #pragma unroll 1
-        for (uint32_t q = 0; q < 64 / kMsmLanes; q++) {
-            const uint32_t i = t * (64 / kMsmLanes) + q;
-            const Fr s = scal[(size_t)msm * 64 + i];
-            const G1A *tab = table + ((size_t)(j * 64 + i) * W) * H;
-            uint32_t carry = 0;
+    for (;;) {
+        if (threadIdx.x == 0) item_sh = atomicAdd(q, 1u);
+        __syncthreads();
+        const uint32_t item = item_sh;
+        __syncthreads();
+        if (item >= nitems) break;
+        const uint32_t tile = item * kTilesPerItem + threadIdx.x / kMsmLanes;
+        const bool live = tile < ntiles;
+        const uint32_t msm = live ? tile / groups : 0u;
+        const uint32_t g = live ? tile % groups : 0u;
+        const uint32_t j = msm % 128;
+        const uint32_t w0 = g * W / groups, w1 = (g + 1) * W / groups;
+        G1X acc = g1x_identity();
+        bool exc = false;
+        if (live) {
 #pragma unroll 1
-            for (uint32_t w = 0; w < W; w++) {
-                uint32_t raw = window_bits(s, w, c) + carry;
-                carry = raw > H ? 1u : 0u;
-                const bool neg = carry != 0;
-                const uint32_t mag = neg ? (2u * H - raw) : raw;
-                if (mag == 0) continue;
-                const G1A pt = g1a_neg_if(tab[(size_t)w * H + (mag - 1)], neg);
-                acc = g1x_madd_fast(acc, pt, exc);
+            for (uint32_t qq = 0; qq < 64 / kMsmLanes; qq++) {
+                const uint32_t i = t * (64 / kMsmLanes) + qq;
+                const Fr s = scal[(size_t)msm * 64 + i];
+                const G1A *tab = table + ((size_t)(j * 64 + i) * W) * H;
+                uint32_t carry = 0;
+#pragma unroll 1
+                for (uint32_t w = 0; w < w1; w++) {
+                    const uint32_t raw = window_bits(s, w, c) + carry;
+                    carry = raw > H ? 1u : 0u;
+                    const bool neg = carry != 0;
+                    const uint32_t mag = neg ? (2u * H - raw) : raw;
+                    if (w < w0 || mag == 0) continue;
+                    const G1A pt = g1a_neg_if(tab[(size_t)w * H + (mag - 1)], neg);
+                    acc = g1x_madd_fast(acc, pt, exc);
+                }
             }
         }
-    }
-    // Sum the 16 partial results of this MSM.
+        // Sum the 16 partial results of this tile.
 #pragma unroll 1
-    for (int m = 1; m < kMsmLanes; m <<= 1) acc = g1x_add_fast(acc, g1x_shfl_xor(acc, m), exc);
-    const uint64_t flagged = __ballot(exc);
-    const uint32_t group = (threadIdx.x & 63u) / kMsmLanes;
-    const bool redo = ((flagged >> (group * kMsmLanes)) & ((1ull << kMsmLanes) - 1)) != 0;
-    if (live && t == 0) {
-        if (redo) phase_a_slow(&u[msm], scal + (size_t)msm * 64, table + (size_t)j * 64 * W * H, c, W);
-        else u[msm] = acc;
+        for (int m = 1; m < kMsmLanes; m <<= 1) acc = g1x_add_fast(acc, g1x_shfl_xor(acc, m), exc);
+        const uint64_t flagged = __ballot(exc);
+        const uint32_t sub = (threadIdx.x & 63u) / kMsmLanes;
+        const bool redo = ((flagged >> (sub * kMsmLanes)) & ((1ull << kMsmLanes) - 1)) != 0;
+        if (live && t == 0) {
+            if (redo)
+                phase_a_slow(&part[tile], scal + (size_t)msm * 64, table + (size_t)j * 64 * W * H, c,
+                             W, w0, w1);
+            else
+                part[tile] = acc;
+        }
     }
 }
 
 }  // namespace
 
-hipError_t launch_phase_a(G1X *u, const Fr *scal, const G1A *table, uint32_t c, uint32_t W,
-                          uint32_t nblobs, hipStream_t s) {
-    const uint32_t nmsm = nblobs * 128;
-    hipLaunchKernelGGL(k_phase_a, dim3(blocks_for(nmsm, 256 / kMsmLanes)), dim3(256), 0, s, u, scal,
-                       table, c, W, nmsm);
+hipError_t launch_phase_a(G1X *part, const Fr *scal, const G1A *table, uint32_t c, uint32_t W,
+                          uint32_t nblobs, uint32_t groups, uint32_t grid, uint32_t *queue,
+                          hipStream_t s) {
+    const uint32_t nitems = (nblobs * 128u * groups + kTilesPerItem - 1) / kTilesPerItem;
+    if (grid > nitems) grid = nitems;
+    if (grid == 0) grid = 1;
+    hipLaunchKernelGGL(k_phase_a, dim3(grid), dim3(256), 0, s, part, scal, table, c, W, nblobs * 128u,
+                       groups, queue);
     return hipGetLastError();
 }
 

@@ -168,8 +168,10 @@ void WorkSet::allocate(uint32_t B) {
     blobs = arena.dalloc<uint8_t>((size_t)B * kBlobBytes);
     proofs = arena.dalloc<uint8_t>((size_t)B * kProofBytes);
     err = arena.dalloc<uint32_t>(B);
+    queue = arena.dalloc<uint32_t>(B);
+    ck(hipMemset(queue, 0, (size_t)B * sizeof(uint32_t)));
     scal = arena.dalloc<Fr>((size_t)B * 128 * 64);
-    u = arena.dalloc<G1X>((size_t)B * 128);
+    u = arena.dalloc<G1X>((size_t)B * 128 * kMaxGroups);
     lad = arena.dalloc<G1J>((size_t)B * 4096);
     fold = arena.dalloc<G1J>((size_t)B * 4096);
     lad_aff = arena.dalloc<G1A>((size_t)B * 4096);
@@ -189,37 +191,58 @@ void WorkSet::allocate(uint32_t B) {
 // latency-bound stages (ladder, inversions) with another's phase A.  Measured
 // at 256 blobs it loses (28.8 ms in one stream, 32.4 ms over two, 33.1 ms over
 // three): the ladder's workgroups wait for slots held by phase A, so the
-// dependent chains get longer, not hidden.  One stream is the default;
-// $VKZG_AMD_SPLITS (1-4) selects more.
-uint32_t choose_splits(const vkzg_prover *p, uint32_t nb) {
-    uint32_t s = p->forced_splits > 0 ? (uint32_t)p->forced_splits : 1u;
-    if (s > (uint32_t)kMaxSplits) s = kMaxSplits;
-    if (s > nb) s = nb;
-    return s < 1 ? 1 : s;
+// dependent chains get longer, not hidden.  The default tunings therefore use
+// one stream; the autotuner may pick two, and $VKZG_AMD_SPLITS (1-4) forces a
+// count.
+Tuning tuning_for(const vkzg_prover *p, uint32_t nb) {
+    Tuning t = p->tune[tune_class(nb)];
+    if (p->forced_splits > 0) t.splits = (uint32_t)p->forced_splits;
+    if (t.splits > (uint32_t)kMaxSplits) t.splits = kMaxSplits;
+    if (t.splits > nb) t.splits = nb;
+    if (t.splits < 1) t.splits = 1;
+    return t;
+}
+
+// Defaults before (or without) autotuning: small batches split each MSM into
+// more window groups and invert in smaller chunks, so that more lanes work
+// on the few MSMs there are.
+void default_tunings(vkzg_prover *p) {
+    static const uint32_t groups[kTuneClasses] = {4, 4, 2, 1, 1};
+    static const uint32_t chunk[kTuneClasses] = {8, 8, 16, 32, 32};
+    for (int c = 0; c < kTuneClasses; c++) {
+        p->tune[c] = Tuning{};
+        p->tune[c].groups = groups[c];
+        p->tune[c].norm_chunk = chunk[c];
+    }
+}
+
+bool tuning_valid(const Tuning &t) {
+    return t.groups >= 1 && t.groups <= kMaxGroups && t.blocks_per_cu >= 1 && t.blocks_per_cu <= 2 &&
+           t.norm_chunk >= 1 && t.norm_chunk <= 256 && t.splits >= 1 && t.splits <= (uint32_t)kMaxSplits;
 }
 
 namespace {
 
 // One sub-batch: blobs [off, off + nb) of working set ws.
 void proofs_range(vkzg_prover *p, WorkSet &ws, uint32_t off, uint32_t nb, uint8_t *d_proofs,
-                  const uint8_t *d_blobs, hipStream_t s, StageClock &clk) {
+                  const uint8_t *d_blobs, hipStream_t s, StageClock &clk, const Tuning &tn) {
     uint32_t *err = ws.err + off;
+    uint32_t *queue = ws.queue + off;
     Fr *scal = ws.scal + (size_t)off * 8192;
-    G1X *u = ws.u + (size_t)off * 128;
+    G1X *u = ws.u + (size_t)off * 128 * kMaxGroups;
     G1J *lad = ws.lad + (size_t)off * 4096, *fold = ws.fold + (size_t)off * 4096;
     G1A *lad_aff = ws.lad_aff + (size_t)off * 4096;
     Fp *prefix = ws.prefix + (size_t)off * 4096;
     G1X *buckets = ws.buckets + (size_t)off * 128 * 128, *cvals = ws.cvals + (size_t)off * 128;
-    ck(hipMemsetAsync(err, 0, nb * sizeof(uint32_t), s));
-    ck(launch_fk20_scalars(scal, d_blobs, err, p->roots, p->roots_inv, nb, s));
+    ck(launch_fk20_scalars(scal, d_blobs, err, p->roots, p->roots_inv, queue, nb, s));
     clk.mark("fk20_scalars");
-    ck(launch_phase_a(u, scal, p->table, p->c, p->W, nb, s));
+    ck(launch_phase_a(u, scal, p->table, p->c, p->W, nb, tn.groups, p->cus * tn.blocks_per_cu, queue, s));
     clk.mark("phase_a");
-    ck(launch_ladder(lad, u, nb, s));
+    ck(launch_ladder(lad, u, tn.groups, nb, s));
     clk.mark("ladder");
     ck(launch_fold(fold, lad, nb, s));
     clk.mark("fold");
-    ck(launch_jac_to_affine(lad_aff, fold, prefix, (uint64_t)nb * 4096, 32, s));
+    ck(launch_jac_to_affine(lad_aff, fold, prefix, (uint64_t)nb * 4096, tn.norm_chunk, s));
     clk.mark("normalize");
     ck(launch_phase_b(buckets, lad_aff, p->kb_items, p->kb_offsets, nb, s));
     clk.mark("phase_b");
@@ -287,25 +310,45 @@ void split_over_streams(StreamSet &ss, uint32_t nb, uint32_t splits, hipStream_t
 
 }  // namespace
 
+namespace {
+uint64_t tuning_word(const Tuning &tn) {
+    return (uint64_t)tn.splits | ((uint64_t)tn.groups << 8) | ((uint64_t)tn.blocks_per_cu << 16) |
+           ((uint64_t)tn.norm_chunk << 24);
+}
+}  // namespace
+
 void enqueue_proofs(vkzg_prover *p, WorkSet &ws, StreamSet &ss, uint8_t *d_proofs,
                     const uint8_t *d_blobs, uint32_t nb, hipStream_t s, StageClock &clk,
-                    uint32_t splits) {
+                    const Tuning &tn, GraphCache *graphs) {
     StageClock off_clock(p, s, false);
-    split_over_streams(ss, nb, splits, s, clk, [&](uint32_t k, uint32_t off, uint32_t n, hipStream_t q) {
-        proofs_range(p, ws, off, n, d_proofs + (size_t)off * kProofBytes,
-                     d_blobs + (size_t)off * kBlobBytes, q, k == 0 ? clk : off_clock);
-    });
+    auto body = [&](hipStream_t s0) {
+        split_over_streams(ss, nb, tn.splits, s0, clk, [&](uint32_t k, uint32_t off, uint32_t n, hipStream_t q) {
+            proofs_range(p, ws, off, n, d_proofs + (size_t)off * kProofBytes,
+                         d_blobs + (size_t)off * kBlobBytes, q, k == 0 ? clk : off_clock, tn);
+        });
+    };
+    if (!graphs) return body(s);
+    const GraphKey key{{1u, nb, tuning_word(tn), (uint64_t)(uintptr_t)d_proofs,
+                        (uint64_t)(uintptr_t)d_blobs, (uint64_t)(uintptr_t)&ws}};
+    enqueue_maybe_graph(p, *graphs, key, s, clk, body);
 }
 
 void enqueue_recovery(vkzg_prover *p, WorkSet &ws, StreamSet &ss, uint8_t *d_out,
                       const uint8_t *d_cells, const uint8_t *d_present, uint32_t nb, hipStream_t s,
-                      StageClock &clk, uint32_t splits) {
+                      StageClock &clk, const Tuning &tn, GraphCache *graphs) {
     StageClock off_clock(p, s, false);
-    split_over_streams(ss, nb, splits, s, clk, [&](uint32_t k, uint32_t off, uint32_t n, hipStream_t q) {
-        recovery_range(p, ws, off, n, d_out + (size_t)off * kCellsBytes,
-                       d_cells + (size_t)off * kCellsBytes, d_present + (size_t)off * 128, q,
-                       k == 0 ? clk : off_clock);
-    });
+    auto body = [&](hipStream_t s0) {
+        split_over_streams(ss, nb, tn.splits, s0, clk, [&](uint32_t k, uint32_t off, uint32_t n, hipStream_t q) {
+            recovery_range(p, ws, off, n, d_out + (size_t)off * kCellsBytes,
+                           d_cells + (size_t)off * kCellsBytes, d_present + (size_t)off * 128, q,
+                           k == 0 ? clk : off_clock);
+        });
+    };
+    if (!graphs) return body(s);
+    const GraphKey key{{2u | ((uint64_t)(uintptr_t)d_present << 2), nb, tn.splits,
+                        (uint64_t)(uintptr_t)d_out, (uint64_t)(uintptr_t)d_cells,
+                        (uint64_t)(uintptr_t)&ws}};
+    enqueue_maybe_graph(p, *graphs, key, s, clk, body);
 }
 
 bool chunk_missing_ok(const uint8_t *present, uint32_t nb) {
@@ -317,6 +360,72 @@ bool chunk_missing_ok(const uint8_t *present, uint32_t nb) {
     return true;
 }
 
+
+// Autotuner: for each batch-size class, time the proof pipeline on synthetic
+// blobs of the class's largest size under candidate tunings and keep the
+// fastest, one knob at a time (phase-A window groups, persistent grid,
+// normalisation chunk, sub-streams).  Uses the prover's own working set;
+// the caller holds p->mu (or owns p exclusively).
+void autotune(vkzg_prover *p) {
+    const auto t0 = std::chrono::steady_clock::now();
+    hipStream_t s = p->stream;
+    // Random canonical blobs: 32-byte big-endian elements below 2^254 < r.
+    const uint32_t nmax = p->max_batch;
+    uint64_t x = 0x9E3779B97F4A7C15ull;
+    for (size_t k = 0; k < (size_t)nmax * kBlobBytes; k += 8) {
+        x ^= x << 13;
+        x ^= x >> 7;
+        x ^= x << 17;
+        std::memcpy(p->h_in + k, &x, 8);
+    }
+    for (size_t e = 0; e < (size_t)nmax * 4096; e++) p->h_in[32 * e] &= 0x3f;
+    ck(hipMemcpyAsync(p->ws.blobs, p->h_in, (size_t)nmax * kBlobBytes, hipMemcpyHostToDevice, s));
+    ck(hipStreamSynchronize(s));
+    hipEvent_t e0, e1;
+    ck(hipEventCreate(&e0));
+    ck(hipEventCreate(&e1));
+    auto time_one = [&](uint32_t nb, const Tuning &tn) {
+        StageClock clk(p, s, false);
+        float best = 1e30f;
+        for (int rep = 0; rep < 3; rep++) {
+            ck(hipEventRecord(e0, s));
+            enqueue_proofs(p, p->ws, p->streams, p->ws.proofs, p->ws.blobs, nb, s, clk, tn, nullptr);
+            ck(hipEventRecord(e1, s));
+            ck(hipEventSynchronize(e1));
+            float ms = 0;
+            ck(hipEventElapsedTime(&ms, e0, e1));
+            if (rep > 0 && ms < best) best = ms;  // rep 0 warms up
+        }
+        return best;
+    };
+    for (int c = 0; c < kTuneClasses; c++) {
+        const uint32_t nb = std::min<uint32_t>(kTuneClassTop[c], nmax);
+        if (c > 0 && kTuneClassTop[c - 1] >= nmax) {
+            p->tune[c] = p->tune[c - 1];  // class beyond the batch size: never used alone
+            continue;
+        }
+        Tuning best = p->tune[c];
+        float best_ms = time_one(nb, best);
+        auto consider = [&](Tuning cand) {
+            if (!tuning_valid(cand)) return;
+            const float ms = time_one(nb, cand);
+            if (ms < best_ms * 0.98f) {
+                best_ms = ms;
+                best = cand;
+            }
+        };
+        for (uint32_t g : {1u, 2u, 4u, 5u}) { Tuning t = best; t.groups = g; consider(t); }
+        for (uint32_t b : {1u, 2u}) { Tuning t = best; t.blocks_per_cu = b; consider(t); }
+        for (uint32_t ch : {8u, 16u, 32u, 64u}) { Tuning t = best; t.norm_chunk = ch; consider(t); }
+        if (nb >= 8) { Tuning t = best; t.splits = 2; consider(t); }
+        p->tune[c] = best;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    p->autotuned = true;
+    p->autotune_ms =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
 
 vkzg_result create(vkzg_prover **out, const vkzg_options *opts, const uint8_t *setup) {
     const auto t0 = std::chrono::steady_clock::now();
@@ -337,10 +446,12 @@ vkzg_result create(vkzg_prover **out, const vkzg_options *opts, const uint8_t *s
         }
         p->cus = (uint32_t)prop.multiProcessorCount;
         if (const char *env = std::getenv("VKZG_AMD_SPLITS")) p->forced_splits = std::atoi(env);
+        if (const char *env = std::getenv("VKZG_AMD_GRAPHS")) p->use_graphs = std::atoi(env) != 0;
         ck(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
         p->streams.create();
         for (auto &e : p->ev) ck(hipEventCreate(&e));
         p->max_batch = opts && opts->max_batch_size ? opts->max_batch_size : kDefaultBatch;
+        default_tunings(p);
         build_tables(p, setup);
         p->ws.allocate(p->max_batch);
         const size_t B = p->max_batch;
@@ -349,6 +460,8 @@ vkzg_result create(vkzg_prover **out, const vkzg_options *opts, const uint8_t *s
         p->h_present = p->arena.halloc<uint8_t>(B * 128);
         p->h_err = p->arena.halloc<uint32_t>(B);
         ck(hipStreamSynchronize(p->stream));
+        if (const char *env = std::getenv("VKZG_AMD_AUTOTUNE"))
+            if (std::atoi(env) != 0) autotune(p);
     } catch (HipFail &) {
         delete p;
         return VKZG_ERR_GPU;
@@ -375,6 +488,7 @@ vkzg_prover::~vkzg_prover() {
     (void)hipSetDevice(device);
     if (stream) (void)hipStreamSynchronize(stream);
     (void)hipDeviceSynchronize();
+    graphs.clear();
     ws.arena.release();
     arena.release();
     for (auto &e : ev)
@@ -441,9 +555,10 @@ vkzg_result vkzg_compute_proofs_batch(vkzg_prover *p, uint8_t *proofs, const uin
             std::memcpy(p->h_in, blobs + done * kBlobBytes, nb * kBlobBytes);
             ck(hipMemcpyAsync(p->ws.blobs, p->h_in, nb * kBlobBytes, hipMemcpyHostToDevice, s));
             StageClock clk(p, s, true);
-            p->last_splits = choose_splits(p, nb);
-            enqueue_proofs(p, p->ws, p->streams, p->ws.proofs, p->ws.blobs, nb, s, clk,
-                           p->last_splits);
+            const Tuning tn = tuning_for(p, nb);
+            p->last_splits = tn.splits;
+            enqueue_proofs(p, p->ws, p->streams, p->ws.proofs, p->ws.blobs, nb, s, clk, tn,
+                           &p->graphs);
             ck(hipMemcpyAsync(p->h_err, p->ws.err, nb * 4, hipMemcpyDeviceToHost, s));
             ck(hipMemcpyAsync(p->h_out, p->ws.proofs, nb * kProofBytes, hipMemcpyDeviceToHost, s));
             ck(hipStreamSynchronize(s));
@@ -477,9 +592,10 @@ vkzg_result vkzg_recover_cells_batch(vkzg_prover *p, uint8_t *cells_out, const u
             ck(hipMemcpyAsync(p->ws.cells, p->h_in, nb * kCellsBytes, hipMemcpyHostToDevice, s));
             ck(hipMemcpyAsync(p->ws.present, p->h_present, nb * 128, hipMemcpyHostToDevice, s));
             StageClock clk(p, s, true);
-            p->last_splits = choose_splits(p, nb);
+            const Tuning tn = tuning_for(p, nb);
+            p->last_splits = tn.splits;
             enqueue_recovery(p, p->ws, p->streams, p->ws.cells_out, p->ws.cells, p->ws.present, nb,
-                             s, clk, p->last_splits);
+                             s, clk, tn, &p->graphs);
             ck(hipMemcpyAsync(p->h_out, p->ws.cells_out, nb * kCellsBytes, hipMemcpyDeviceToHost, s));
             ck(hipStreamSynchronize(s));
             clk.finish();
@@ -504,10 +620,11 @@ vkzg_result vkzg_amd_compute_proofs_device(vkzg_prover *p, void *d_proofs, const
         for (size_t done = 0; done < num_blobs;) {
             const uint32_t nb = (uint32_t)std::min<size_t>(p->max_batch, num_blobs - done);
             StageClock clk(p, s, true);
-            p->last_splits = choose_splits(p, nb);
+            const Tuning tn = tuning_for(p, nb);
+            p->last_splits = tn.splits;
             enqueue_proofs(p, p->ws, p->streams, (uint8_t *)d_proofs + done * kProofBytes,
-                           (const uint8_t *)d_blobs + done * kBlobBytes, nb, s, clk,
-                           p->last_splits);
+                           (const uint8_t *)d_blobs + done * kBlobBytes, nb, s, clk, tn,
+                           &p->graphs);
             ck(hipMemcpyAsync(p->h_err, p->ws.err, nb * 4, hipMemcpyDeviceToHost, s));
             ck(hipStreamSynchronize(s));
             clk.finish();
@@ -535,10 +652,11 @@ vkzg_result vkzg_amd_recover_cells_device(vkzg_prover *p, void *d_cells_out, con
             const uint32_t nb = (uint32_t)std::min<size_t>(p->max_batch, num_blobs - done);
             if (!chunk_missing_ok(h_present + done * 128, nb)) return VKZG_ERR_BADARGS;
             StageClock clk(p, s, true);
-            p->last_splits = choose_splits(p, nb);
+            const Tuning tn = tuning_for(p, nb);
+            p->last_splits = tn.splits;
             enqueue_recovery(p, p->ws, p->streams, (uint8_t *)d_cells_out + done * kCellsBytes,
                              (const uint8_t *)d_cells + done * kCellsBytes,
-                             (const uint8_t *)d_present + done * 128, nb, s, clk, p->last_splits);
+                             (const uint8_t *)d_present + done * 128, nb, s, clk, tn, &p->graphs);
             ck(hipStreamSynchronize(s));
             clk.finish();
             done += nb;
@@ -549,6 +667,42 @@ vkzg_result vkzg_amd_recover_cells_device(vkzg_prover *p, void *d_cells_out, con
     return VKZG_OK;
 }
 
+vkzg_result vkzg_amd_get_tuning(const vkzg_prover *p, size_t num_blobs, vkzg_amd_tuning *out) {
+    if (!p || !out || num_blobs == 0) return VKZG_ERR_BADARGS;
+    const Tuning t = p->tune[tune_class((uint32_t)std::min<size_t>(num_blobs, 0xffffffffu))];
+    out->phase_a_groups = t.groups;
+    out->phase_a_blocks_per_cu = t.blocks_per_cu;
+    out->normalize_chunk = t.norm_chunk;
+    out->splits = t.splits;
+    return VKZG_OK;
+}
+
+vkzg_result vkzg_amd_set_tuning(vkzg_prover *p, size_t num_blobs, const vkzg_amd_tuning *in) {
+    if (!p || !in || num_blobs == 0) return VKZG_ERR_BADARGS;
+    Tuning t;
+    t.groups = in->phase_a_groups;
+    t.blocks_per_cu = in->phase_a_blocks_per_cu;
+    t.norm_chunk = in->normalize_chunk;
+    t.splits = in->splits;
+    if (!tuning_valid(t)) return VKZG_ERR_BADARGS;
+    std::lock_guard<std::mutex> lock(p->mu);
+    p->tune[tune_class((uint32_t)std::min<size_t>(num_blobs, 0xffffffffu))] = t;
+    return VKZG_OK;
+}
+
+vkzg_result vkzg_amd_autotune(vkzg_prover *p, double *ms_out) {
+    if (!p) return VKZG_ERR_BADARGS;
+    std::lock_guard<std::mutex> lock(p->mu);
+    DeviceGuard guard(p->device);
+    try {
+        autotune(p);
+    } catch (HipFail &) {
+        return VKZG_ERR_GPU;
+    }
+    if (ms_out) *ms_out = p->autotune_ms;
+    return VKZG_OK;
+}
+
 vkzg_result vkzg_amd_prover_info(const vkzg_prover *p, vkzg_amd_info *out) {
     if (!p || !out) return VKZG_ERR_BADARGS;
     out->window_bits = p->c;
@@ -556,6 +710,11 @@ vkzg_result vkzg_amd_prover_info(const vkzg_prover *p, vkzg_amd_info *out) {
     out->table_bytes = p->table_bytes;
     out->working_bytes = WorkSet::bytes_per_blob() * p->max_batch;
     out->last_splits = p->last_splits;
+    out->graphs_enabled = p->use_graphs ? 1u : 0u;
+    out->graph_captures = p->graphs.captures;
+    out->graph_replays = p->graphs.replays;
+    out->autotuned = p->autotuned ? 1u : 0u;
+    out->autotune_ms = p->autotune_ms;
     out->max_batch_size = p->max_batch;
     out->setup_ms = p->setup_ms;
     return VKZG_OK;

@@ -24,24 +24,37 @@ namespace {
 // Per blob: zd[k] = z(omega_128^k), izc[k] = 1 / z(7^64 omega_128^k),
 // z(y) = prod over missing wire cells c of (y - omega_128^brev7(c))
 // (k_vanishing_poly.comp, src/vkzg.cpp:1043-1050).
-__global__ void __launch_bounds__(256)
+// On the coset every y satisfies y^128 = 7^8192, so the product over all 128
+// cells is y^128 - 1 = 7^8192 - 1, a constant K, and
+//   1 / z(y) = K^-1 * prod over present cells c of (y - omega_128^brev7(c)),
+// which needs no inversion.  Each value is split over two threads (cells
+// 0-63 and 64-127) whose partial products meet in LDS.
+__global__ void __launch_bounds__(512)
     k_rec_vanishing(Fr *zd, Fr *izc, const uint8_t *present, const Fr *roots) {
-    __builtin_amdgcn_s_setprio(3);  // dependent inversion chain: issue first
+    __builtin_amdgcn_s_setprio(3);  // short dependent chains: issue first
+    __shared__ Fr part[256];
     const uint32_t blob = blockIdx.x;
-    const uint32_t t = threadIdx.x;
+    const uint32_t t = threadIdx.x & 255u;
+    const uint32_t half = threadIdx.x >> 8;
     const uint32_t k = t & 127u;
     const bool coset = t >= 128;
     Fr x = roots[(64u * k) & 8191u];
     if (coset) x = fr_mul(x, fr_const(cfr::SEVEN_POW64_M));
     const uint8_t *pm = present + 128ull * blob;
+    // domain side: factors of missing cells; coset side: factors of present cells
+    const uint8_t want = coset ? 1 : 0;
     Fr prod = fr_one();
 #pragma unroll 1
-    for (uint32_t c = 0; c < 128; c++) {
-        if (pm[c]) continue;
+    for (uint32_t c = 64u * half; c < 64u * half + 64u; c++) {
+        if ((pm[c] != 0) != (want != 0)) continue;
         const Fr root = roots[(64u * brev_bits(c, 7)) & 8191u];
         prod = fr_mul(prod, fr_sub(x, root));
     }
-    if (coset) izc[blob * 128 + k] = fr_inv(prod);
+    if (half) part[t] = prod;
+    __syncthreads();
+    if (half) return;
+    prod = fr_mul(prod, part[t]);
+    if (coset) izc[blob * 128 + k] = fr_mul(prod, fr_const(cfr::INV_SEVEN_POW8192_MINUS_1_M));
     else zd[blob * 128 + k] = prod;
 }
 
@@ -132,7 +145,7 @@ __global__ void __launch_bounds__(kThreads)
 
 hipError_t launch_rec_vanishing(Fr *zd, Fr *izc, const uint8_t *present, const Fr *roots,
                                 uint32_t nblobs, hipStream_t s) {
-    hipLaunchKernelGGL(k_rec_vanishing, dim3(nblobs), dim3(256), 0, s, zd, izc, present, roots);
+    hipLaunchKernelGGL(k_rec_vanishing, dim3(nblobs), dim3(512), 0, s, zd, izc, present, roots);
     return hipGetLastError();
 }
 

@@ -19,6 +19,25 @@ constexpr size_t kBlobBytes = VKZG_BYTES_PER_BLOB;
 constexpr size_t kProofBytes = (size_t)VKZG_NUM_CELL_PROOFS * VKZG_BYTES_PER_PROOF;
 constexpr size_t kCellsBytes = (size_t)VKZG_NUM_CELL_PROOFS * VKZG_BYTES_PER_CELL;
 constexpr int kMaxSplits = 4;  // sub-streams one batch may be spread over
+constexpr uint32_t kMaxGroups = 5;  // phase-A window groups per MSM
+
+// Launch shape of the proof pipeline for one batch-size class.  Every choice
+// gives bit-identical proofs; only the time differs.
+struct Tuning {
+    uint32_t groups = 1;         // phase-A window groups (tiles) per MSM, 1..kMaxGroups
+    uint32_t blocks_per_cu = 2;  // persistent phase-A grid = CUs x this, 1..2
+    uint32_t norm_chunk = 32;    // points per batched inversion in the normalisation
+    uint32_t splits = 1;         // sub-streams, 1..kMaxSplits
+};
+
+// Batch-size classes: <= 1, <= 4, <= 16, <= 64 and more blobs.
+constexpr int kTuneClasses = 5;
+constexpr uint32_t kTuneClassTop[kTuneClasses] = {1, 4, 16, 64, 0xffffffffu};
+inline int tune_class(uint32_t nb) {
+    int c = 0;
+    while (c + 1 < kTuneClasses && nb > kTuneClassTop[c]) c++;
+    return c;
+}
 
 struct HipFail {};
 
@@ -77,6 +96,7 @@ struct WorkSet {
     // proofs
     uint8_t *blobs = nullptr, *proofs = nullptr;
     uint32_t *err = nullptr;
+    uint32_t *queue = nullptr;  // phase-A tile counter of the sub-batch at each blob offset
     Fr *scal = nullptr;
     G1X *u = nullptr, *buckets = nullptr, *cvals = nullptr;
     G1J *lad = nullptr, *fold = nullptr;
@@ -110,6 +130,75 @@ struct StreamSet {
     }
 };
 
+// Instantiated HIP graphs of whole pipelines, keyed by what the captured
+// launches depend on (kind, blob count, split count, buffer addresses).  A
+// pipeline is captured the first time its key is seen and replayed after
+// that, which turns 10-20 kernel launches into one graph launch.  The
+// captured pipelines hold kernels and event dependencies only: no memset or
+// memcpy nodes.
+struct GraphKey {
+    uint64_t v[6];
+    bool operator==(const GraphKey &o) const {
+        for (int i = 0; i < 6; i++)
+            if (v[i] != o.v[i]) return false;
+        return true;
+    }
+};
+
+struct GraphCache {
+    struct Entry {
+        GraphKey key;
+        hipGraphExec_t exec;
+        uint64_t last_use;
+    };
+    std::vector<Entry> entries;
+    uint64_t tick = 0;
+    uint64_t captures = 0, replays = 0;
+    static constexpr size_t kCapacity = 48;
+
+    // Enqueue `body(s)` on s through a cached graph.
+    template <typename Body>
+    void run(const GraphKey &key, hipStream_t s, Body body) {
+        hipGraphExec_t exec = nullptr;
+        for (auto &e : entries)
+            if (e.key == key) {
+                exec = e.exec;
+                e.last_use = ++tick;
+            }
+        if (!exec) {
+            ck(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            hipGraph_t g = nullptr;
+            try {
+                body(s);
+            } catch (HipFail &) {
+                (void)hipStreamEndCapture(s, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            ck(hipStreamEndCapture(s, &g));
+            const hipError_t ie = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            ck(ie);
+            if (entries.size() >= kCapacity) {
+                size_t old = 0;
+                for (size_t i = 1; i < entries.size(); i++)
+                    if (entries[i].last_use < entries[old].last_use) old = i;
+                (void)hipGraphExecDestroy(entries[old].exec);
+                entries.erase(entries.begin() + (long)old);
+            }
+            entries.push_back({key, exec, ++tick});
+            captures++;
+        } else {
+            replays++;
+        }
+        ck(hipGraphLaunch(exec, s));
+    }
+    void clear() {
+        for (auto &e : entries) (void)hipGraphExecDestroy(e.exec);
+        entries.clear();
+    }
+};
+
 }  // namespace vkzg_amd
 
 struct vkzg_prover {
@@ -123,6 +212,11 @@ struct vkzg_prover {
     double setup_ms = 0;
     uint32_t last_splits = 1;
     int forced_splits = 0;  // $VKZG_AMD_SPLITS, 0 = automatic
+    bool use_graphs = true;  // $VKZG_AMD_GRAPHS=0 launches kernel by kernel
+    vkzg_amd::Tuning tune[vkzg_amd::kTuneClasses];
+    bool autotuned = false;
+    double autotune_ms = 0;
+    vkzg_amd::GraphCache graphs;  // graphs of the synchronous entry points
 
     // Resident tables.
     vkzg_amd::DeviceArena arena;
@@ -177,19 +271,33 @@ struct StageClock {
     }
 };
 
-// Number of sub-streams a batch of nb blobs is spread over.
-uint32_t choose_splits(const vkzg_prover *p, uint32_t nb);
+// The tuning for a batch of nb blobs ($VKZG_AMD_SPLITS overrides its splits).
+Tuning tuning_for(const vkzg_prover *p, uint32_t nb);
+
+// Enqueue a pipeline through `cache` when graphs are on and stage timing is
+// off (timing records events between stages), else launch it directly.
+// A stream the caller is already capturing gets the launches themselves, so
+// they land in the caller's graph.
+template <typename Body>
+void enqueue_maybe_graph(vkzg_prover *p, GraphCache &cache, const GraphKey &key, hipStream_t s,
+                         const StageClock &clk, Body body) {
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    ck(hipStreamIsCapturing(s, &cs));
+    if (p->use_graphs && !clk.on && cs == hipStreamCaptureStatusNone) cache.run(key, s, body);
+    else body(s);
+}
 
 // Enqueue the whole proof / recovery pipeline for nb blobs on stream s, using
 // working set ws (capacity >= nb).  The batch is split over sub-streams when
 // that overlaps latency-bound stages with throughput-bound ones; the split is
 // forked from and joined back into s.  `clk` times the first sub-batch.
+// With `graphs` non-null the pipeline goes through that graph cache.
 void enqueue_proofs(vkzg_prover *p, WorkSet &ws, StreamSet &ss, uint8_t *d_proofs,
                     const uint8_t *d_blobs, uint32_t nb, hipStream_t s, StageClock &clk,
-                    uint32_t splits);
+                    const Tuning &tn, GraphCache *graphs);
 void enqueue_recovery(vkzg_prover *p, WorkSet &ws, StreamSet &ss, uint8_t *d_out,
                       const uint8_t *d_cells, const uint8_t *d_present, uint32_t nb, hipStream_t s,
-                      StageClock &clk, uint32_t splits);
+                      StageClock &clk, const Tuning &tn, GraphCache *graphs);
 
 bool chunk_missing_ok(const uint8_t *present, uint32_t nb);
 

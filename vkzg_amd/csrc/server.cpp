@@ -63,6 +63,7 @@ struct Piece {
 struct Slot {
     WorkSet ws;
     StreamSet ss;
+    GraphCache graphs;
     hipStream_t stream = nullptr;
     hipEvent_t done_ev = nullptr;
     uint8_t *h_in = nullptr, *h_out = nullptr, *h_present = nullptr;
@@ -141,17 +142,18 @@ bool vkzg_amd_server::launch(Slot &slot) {
         DeviceGuard guard(p->device);
         hipStream_t s = slot.stream;
         StageClock clk(p, s, false);
-        const uint32_t splits = choose_splits(p, slot.nb);
+        const Tuning tn = tuning_for(p, slot.nb);
         if (slot.kind == Kind::Proofs) {
             ck(hipMemcpyAsync(slot.ws.blobs, slot.h_in, slot.nb * in_bytes, hipMemcpyHostToDevice, s));
-            enqueue_proofs(p, slot.ws, slot.ss, slot.ws.proofs, slot.ws.blobs, slot.nb, s, clk, splits);
+            enqueue_proofs(p, slot.ws, slot.ss, slot.ws.proofs, slot.ws.blobs, slot.nb, s, clk, tn,
+                           &slot.graphs);
             ck(hipMemcpyAsync(slot.h_err, slot.ws.err, slot.nb * 4, hipMemcpyDeviceToHost, s));
             ck(hipMemcpyAsync(slot.h_out, slot.ws.proofs, slot.nb * out_bytes, hipMemcpyDeviceToHost, s));
         } else {
             ck(hipMemcpyAsync(slot.ws.cells, slot.h_in, slot.nb * in_bytes, hipMemcpyHostToDevice, s));
             ck(hipMemcpyAsync(slot.ws.present, slot.h_present, slot.nb * 128, hipMemcpyHostToDevice, s));
             enqueue_recovery(p, slot.ws, slot.ss, slot.ws.cells_out, slot.ws.cells, slot.ws.present,
-                             slot.nb, s, clk, splits);
+                             slot.nb, s, clk, tn, &slot.graphs);
             ck(hipMemcpyAsync(slot.h_out, slot.ws.cells_out, slot.nb * out_bytes, hipMemcpyDeviceToHost, s));
         }
         ck(hipEventRecord(slot.done_ev, s));
@@ -246,6 +248,7 @@ vkzg_amd_server::~vkzg_amd_server() {
     for (auto &s : slots) {
         if (s->stream) (void)hipStreamSynchronize(s->stream);
         if (s->done_ev) (void)hipEventDestroy(s->done_ev);
+        s->graphs.clear();
         s->ss.destroy();
         if (s->stream) (void)hipStreamDestroy(s->stream);
     }
