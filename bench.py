@@ -24,6 +24,7 @@ PEAK_INT = 256 * 2.4e9 * 64  # CU x clock x lanes: lane-ops/s (v_mad_u64_u32 iss
 FP_M = 300                    # limb-MACs per Fp multiplication (12-limb CIOS)
 FR_M = 136                    # limb-MACs per Fr multiplication (8-limb CIOS)
 W_PROOF = 1.734e9             # limb-MACs per blob of the reference's FK20 as written
+LAD_POS = 17                  # ladder positions 2^(8d) u, d < 17 (GLV halves of the phase-B scalars)
 METRIC = "proof blobs/s @256-blob batch; recovery blobs/s; p50/p99 serve latency"
 HERE = os.path.dirname(os.path.abspath(__file__))
 
@@ -40,8 +41,8 @@ def stage_work(info, nb, items_per_output):
     return {
         "fk20_scalars": nb * (12 * 2048 + 4096 + 64 * 448 + 8192 + 4096) * FR_M,
         "phase_a": nb * 128 * (64 * W * madd + 15 * add) * FP_M,
-        "ladder": nb * 128 * 248 * dbl_j * FP_M,
-        "fold": nb * 64 * 32 * 2 * add_j * FP_M,
+        "ladder": nb * 128 * (LAD_POS - 1) * 8 * dbl_j * FP_M,
+        "fold": nb * 64 * LAD_POS * 2 * add_j * FP_M,
         "phase_b": nb * 128 * items_per_output * madd * FP_M,
         "bucket_reduce": nb * 128 * (2 * 128 + 8) * add * FP_M,
     }
@@ -180,6 +181,7 @@ def main():
     ap.add_argument("--cpu-sample-blobs", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default tunings")
     ap.add_argument("--serve-seconds", type=float, default=4.0)
     ap.add_argument("--serve-rate", type=float, default=150.0, help="mix requests per second")
     ap.add_argument("--probe-rate", type=float, default=25.0, help="6-blob proof probes per second")
@@ -216,6 +218,11 @@ def main():
     d_proofs = hip.DeviceBuffer(nb * 6144)
     stream = hip.Stream()
 
+    tuning = None
+    if not a.no_autotune:
+        autotune_ms = prover.autotune()
+        tuning = {"autotune_ms": autotune_ms,
+                  "classes": {str(n): prover.get_tuning(n) for n in (1, 4, 16, 64, 256)}}
     for _ in range(a.warmup):
         prover.compute_proofs_device(d_proofs.ptr, d_blobs.ptr, nb, stream.handle)
     prover.set_stage_timing(True)
@@ -232,7 +239,7 @@ def main():
     value = nb * a.steps / elapsed
     stage_ms = {k: v / a.steps for k, v in stage_tot.items()}
 
-    items_per_output = 1048.0  # phase-B items per output (66 split-circulant scalars)
+    items_per_output = info["phase_b_items"] / 2.0  # each output uses one half's item list
     work = stage_work(info, nb, items_per_output)
     frac = {k: (work[k] / (stage_ms[k] * 1e-3)) / PEAK_INT for k in work if stage_ms.get(k)}
     dom = max(stage_ms, key=stage_ms.get)
@@ -252,7 +259,17 @@ def main():
         "traffic": traffic,
     }
 
-    secondary = {"stage_ms": stage_ms, "stage_frac_of_int_peak": frac,
+    # The same step replayed as one captured HIP graph (stage timing off).
+    prover.compute_proofs_device(d_proofs.ptr, d_blobs.ptr, nb, stream.handle)
+    stream.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        prover.compute_proofs_device(d_proofs.ptr, d_blobs.ptr, nb, stream.handle)
+    stream.synchronize()
+    graph_value = nb * a.steps / (time.perf_counter() - t0)
+
+    secondary = {"graph_replay_blobs_per_s": graph_value, "tuning": tuning,
+                 "stage_ms": stage_ms, "stage_frac_of_int_peak": frac,
                  "pipeline_frac_of_int_peak_vs_reference_work": value * W_PROOF / PEAK_INT,
                  "phase_a_window_bits": info["window_bits"],
                  "table_gib": info["table_bytes"] / 2**30, "setup_ms": info["setup_ms"]}
@@ -298,7 +315,9 @@ def main():
             masks.append(bytes(1 if k in keep else 0 for k in range(128)))
         umask = b"".join(masks)
         d_umask = hip.DeviceBuffer.from_bytes(umask)
-        prover.recover_cells_device(d_uout.ptr, d_ucells.ptr, d_umask.ptr, umask, maxn, stream.handle)
+        for n in calls:  # first use of each size captures its graph
+            prover.recover_cells_device(d_uout.ptr, d_ucells.ptr, d_umask.ptr, umask[:128 * n], n,
+                                        stream.handle)
         t0 = time.perf_counter()
         for n in calls:
             prover.recover_cells_device(d_uout.ptr, d_ucells.ptr, d_umask.ptr, umask[:128 * n], n,
@@ -313,6 +332,16 @@ def main():
         for _ in range(calls):
             prover.compute_cell_kzg_proofs_batch(one, 1)
         secondary["plain_per_call_1_blob_blobs_per_s"] = calls / (time.perf_counter() - t0)
+        # latency of one synchronous C-ABI call per batch size (host copies included)
+        per_call = {}
+        for n in (1, 4, 16, 64):
+            chunk = blobs[:131072 * n]
+            prover.compute_cell_kzg_proofs_batch(chunk, n)
+            t0 = time.perf_counter()
+            for _ in range(4):
+                prover.compute_cell_kzg_proofs_batch(chunk, n)
+            per_call[str(n)] = (time.perf_counter() - t0) / 4 * 1e3
+        secondary["per_call_ms"] = per_call
 
     if not a.no_secondary and a.serve_seconds > 0:
         secondary["serve"] = serve_bench(prover, random.Random(a.seed + 3), a.serve_seconds,

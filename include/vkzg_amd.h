@@ -49,6 +49,7 @@ typedef struct {
     uint64_t graph_replays;    /* calls served by replaying a captured graph */
     uint32_t autotuned;        /* 1 after vkzg_amd_autotune */
     double autotune_ms;        /* its wall time */
+    uint32_t phase_b_items;    /* bucket items of both phase-B halves (each output uses one half) */
 } vkzg_amd_info;
 
 /* Fills `out` (never fails for a non-NULL prover). */

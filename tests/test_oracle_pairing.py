@@ -112,3 +112,25 @@ def test_verifier_rejects_wrong_inputs(oracle_mod, seeded_verify_setup):
     # malformed input raises
     with pytest.raises(ValueError):
         seeded_verify_setup.verify(coms[:48], [128], cells[:2048], proofs[:48])
+
+
+def test_glv_endomorphism_constants(oracle_mod, mainnet_setup_bytes):
+    """The device's phase-B split uses phi(x, y) = (beta x, y) = [lambda] P with
+    lambda = z^2 - 1; check that pairing of beta and lambda on G1 points."""
+    import importlib.util
+    import os
+    here = os.path.dirname(os.path.abspath(__file__))
+    spec = importlib.util.spec_from_file_location(
+        "gen_constants", os.path.join(here, "..", "vkzg_amd", "csrc", "gen_constants.py"))
+    gen = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(gen)
+    beta = gen.BETA
+    lam = (Z * Z - 1) % R
+    assert lam == 0xAC45A4010001A40200000000FFFFFFFF and (lam * lam + lam + 1) % R == 0
+    assert pow(beta, 3, P) == 1 and beta != 1
+    for pt in (oracle_mod.g1_mul_gen(1), mainnet_setup_bytes[48 * 7:48 * 8], mainnet_setup_bytes[-48:]):
+        lp = oracle_mod.g1_mul(pt, lam)
+        x = int.from_bytes(bytes([pt[0] & 0x1F]) + pt[1:], "big")
+        lx = int.from_bytes(bytes([lp[0] & 0x1F]) + lp[1:], "big")
+        assert lx == beta * x % P
+        assert (lp[0] & 0x20) == (pt[0] & 0x20)  # same y

@@ -62,6 +62,7 @@ class VkzgAmdInfo(ctypes.Structure):
         ("graph_replays", ctypes.c_uint64),
         ("autotuned", ctypes.c_uint32),
         ("autotune_ms", ctypes.c_double),
+        ("phase_b_items", ctypes.c_uint32),
     ]
 
 

@@ -347,8 +347,10 @@ VKZG_DEV_CALL void g1a_compress_words(uint32_t *w, const G1A &p) {
     }
     const Fp x = fp_to_canonical(p.x);
     const Fp y = fp_to_canonical(p.y);
+    uint32_t xw[12];
+    fp_to_words(xw, x);
 #pragma unroll
-    for (int i = 0; i < 12; i++) w[11 - i] = x.v[i];
+    for (int i = 0; i < 12; i++) w[11 - i] = xw[i];
     uint32_t flags = 0x80000000u;
     if (fp_is_lex_largest(y)) flags |= 0x20000000u;
     w[0] |= flags;

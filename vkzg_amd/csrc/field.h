@@ -1,15 +1,16 @@
-// BLS12-381 Fp (12 x 32-bit limbs) and Fr (8 x 32-bit limbs) Montgomery
-// arithmetic for gfx950, held in VGPRs.
+// BLS12-381 Fp and Fr Montgomery arithmetic for gfx950, held in VGPRs.
 //
-// Multiplication is Montgomery by product scanning (mont_mul below): each
-// 32x32->64 limb product is one v_mad_u64_u32 whose carry-out feeds a
-// v_addc_co_u32 into a 96-bit column accumulator.  Both moduli have a clear
-// top bit in their top limb (p: 0x1a0111ea..., r: 0x73eda753...), so inputs
-// below the modulus give a product below twice the modulus in N words,
-// reduced by one conditional subtraction.
+// Fr: 8 x 32-bit limbs, radix 2^256, canonical results.  Multiplication is
+// Montgomery by product scanning (mont_mul below): each 32x32->64 limb
+// product is one v_mad_u64_u32 whose carry-out feeds a v_addc_co_u32 into a
+// 96-bit column accumulator; r's top limb has a clear top bit, so one
+// conditional subtraction finishes.
+//
+// Fp: 14 x 28-bit limbs, radix 2^392, values in [0, 2p) (see the Fp section).
 //
 // Semantics follow the reference's field library (src/shaders/field.glsl.inc
-// :69-297): little-endian limbs, Montgomery radix 2^(32N), canonical results.
+// :69-297): the same field elements in Montgomery form; byte encodings are
+// canonical big-endian.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -26,7 +27,7 @@ struct Fr {
     uint32_t v[8];
 };
 struct Fp {
-    uint32_t v[12];
+    uint32_t v[14];  // 28-bit limbs, see below
 };
 
 // ------------------------------------------------------------ limb helpers
@@ -282,59 +283,190 @@ VKZG_DEV void fr_to_be_words(uint32_t *w, const Fr &a) {
 }
 
 // ------------------------------------------------------------------ Fp
+//
+// Fp in registers: 14 limbs of 28 bits (one per 32-bit word), Montgomery
+// form with radix R = 2^392, value kept in [0, 2p) ("almost reduced") with
+// every limb below 2^28.
+//  - Limb products are below 2^56, so one Montgomery column (at most 28
+//    products plus the incoming carry) fits a single 64-bit accumulator: each
+//    product is one v_mad_u64_u32 and no carry is propagated inside a column.
+//  - 4p < R, so the Montgomery product of two values below 2p is below 2p
+//    and needs no final subtraction.
+//  - add/sub bring their result back to [0, 2p) with one conditional
+//    correction by 2p; comparisons and encodings reduce to [0, p) first
+//    (fp_canonical).
+
+constexpr int kFpLimbs = 14;
+constexpr uint32_t kFpMask = (1u << 28) - 1u;
 
 VKZG_DEV Fp fp_zero() {
     Fp r;
 #pragma unroll
-    for (int i = 0; i < 12; i++) r.v[i] = 0;
+    for (int i = 0; i < kFpLimbs; i++) r.v[i] = 0;
     return r;
 }
-VKZG_DEV Fp fp_one() {
+VKZG_DEV Fp fp_const(const uint32_t (&c)[kFpLimbs]) {
     Fp r;
 #pragma unroll
-    for (int i = 0; i < 12; i++) r.v[i] = cfp::ONE[i];
+    for (int i = 0; i < kFpLimbs; i++) r.v[i] = c[i];
     return r;
 }
-VKZG_DEV Fp fp_const(const uint32_t (&c)[12]) {
-    Fp r;
-#pragma unroll
-    for (int i = 0; i < 12; i++) r.v[i] = c[i];
-    return r;
-}
+VKZG_DEV Fp fp_one() { return fp_const(cfp::ONE); }
+
+// acc += x * y: the compiler emits one v_mad_u64_u32 (no carry-out needed,
+// the accumulator cannot overflow).
+VKZG_DEV void fp_mac(uint64_t &acc, uint32_t x, uint32_t y) { acc += (uint64_t)x * y; }
+
+// Montgomery multiplication by finely integrated product scanning: column k
+// of a*b and of m*p share the accumulator, m_k = low 28 bits * N0 clears the
+// low limb of columns k < 14, and the accumulator shifts down 28 bits per
+// column.  Inputs below 2p give a result below 2p (no final subtraction).
 VKZG_DEV Fp fp_mul(const Fp &a, const Fp &b) {
+    uint32_t m[kFpLimbs];
+    uint64_t acc = 0;
+#pragma unroll
+    for (int i = 0; i < kFpLimbs; i++) {
+#pragma unroll
+        for (int j = 0; j < i; j++) {
+            fp_mac(acc, a.v[j], b.v[i - j]);
+            fp_mac(acc, m[j], cfp::P[i - j]);
+        }
+        fp_mac(acc, a.v[i], b.v[0]);
+        m[i] = ((uint32_t)acc * cfp::N0) & kFpMask;
+        fp_mac(acc, m[i], cfp::P[0]);
+        acc >>= 28;
+    }
     Fp r;
-    mont_mul<12>(r.v, a.v, b.v, cfp::P, cfp::N0);
+#pragma unroll
+    for (int i = kFpLimbs; i < 2 * kFpLimbs; i++) {
+#pragma unroll
+        for (int j = i - kFpLimbs + 1; j < kFpLimbs; j++) {
+            fp_mac(acc, a.v[j], b.v[i - j]);
+            fp_mac(acc, m[j], cfp::P[i - j]);
+        }
+        r.v[i - kFpLimbs] = (uint32_t)acc & kFpMask;
+        acc >>= 28;
+    }
     return r;
 }
 VKZG_DEV Fp fp_sqr(const Fp &a) { return fp_mul(a, a); }
+
+// a + b in [0, 2p): s = a + b and t = s - 2p with their own signed carry
+// chains; t when non-negative.
 VKZG_DEV Fp fp_add(const Fp &a, const Fp &b) {
+    Fp s, t;
+    int32_t cs = 0, ct = 0;
+#pragma unroll
+    for (int i = 0; i < kFpLimbs; i++) {
+        const int32_t x = (int32_t)(a.v[i] + b.v[i]) + cs;
+        s.v[i] = (uint32_t)x & kFpMask;
+        cs = x >> 28;
+        const int32_t y = (int32_t)s.v[i] - (int32_t)cfp::P2[i] + ct;
+        t.v[i] = (uint32_t)y & kFpMask;
+        ct = y >> 28;
+    }
     Fp r;
-    add_mod<12>(r.v, a.v, b.v, cfp::P);
+#pragma unroll
+    for (int i = 0; i < kFpLimbs; i++) r.v[i] = ct < 0 ? s.v[i] : t.v[i];
     return r;
 }
 VKZG_DEV Fp fp_dbl(const Fp &a) { return fp_add(a, a); }
+
+// a - b in [0, 2p): d = a - b (two's complement over 392 bits) and d + 2p;
+// the latter when d is negative.
 VKZG_DEV Fp fp_sub(const Fp &a, const Fp &b) {
+    Fp d, e;
+    int32_t cd = 0, ce = 0;
+#pragma unroll
+    for (int i = 0; i < kFpLimbs; i++) {
+        const int32_t x = (int32_t)a.v[i] - (int32_t)b.v[i] + cd;
+        d.v[i] = (uint32_t)x & kFpMask;
+        cd = x >> 28;
+        const int32_t y = (int32_t)d.v[i] + (int32_t)cfp::P2[i] + ce;
+        e.v[i] = (uint32_t)y & kFpMask;
+        ce = y >> 28;
+    }
     Fp r;
-    sub_mod<12>(r.v, a.v, b.v, cfp::P);
+#pragma unroll
+    for (int i = 0; i < kFpLimbs; i++) r.v[i] = cd < 0 ? e.v[i] : d.v[i];
     return r;
 }
 VKZG_DEV Fp fp_neg(const Fp &a) { return fp_sub(fp_zero(), a); }
-VKZG_DEV bool fp_is_zero(const Fp &a) { return limbs_zero<12>(a.v); }
-VKZG_DEV bool fp_eq(const Fp &a, const Fp &b) { return limbs_eq<12>(a.v, b.v); }
+
+// [0, 2p) -> [0, p)
+VKZG_DEV Fp fp_canonical(const Fp &a) {
+    Fp t;
+    int32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < kFpLimbs; i++) {
+        const int32_t y = (int32_t)a.v[i] - (int32_t)cfp::P[i] + c;
+        t.v[i] = (uint32_t)y & kFpMask;
+        c = y >> 28;
+    }
+    Fp r;
+#pragma unroll
+    for (int i = 0; i < kFpLimbs; i++) r.v[i] = c < 0 ? a.v[i] : t.v[i];
+    return r;
+}
+// For a value in [0, 2p): zero iff its limbs are those of 0 or of p.
+VKZG_DEV bool fp_is_zero(const Fp &a) {
+    uint32_t z = 0, zp = 0;
+#pragma unroll
+    for (int i = 0; i < kFpLimbs; i++) {
+        z |= a.v[i];
+        zp |= a.v[i] ^ cfp::P[i];
+    }
+    return z == 0 || zp == 0;
+}
+VKZG_DEV bool fp_eq(const Fp &a, const Fp &b) {
+    return limbs_eq<kFpLimbs>(fp_canonical(a).v, fp_canonical(b).v);
+}
 VKZG_DEV Fp fp_cond_neg(const Fp &a, bool neg) {
     const Fp n = fp_neg(a);
     Fp r;
 #pragma unroll
-    for (int i = 0; i < 12; i++) r.v[i] = neg ? n.v[i] : a.v[i];
+    for (int i = 0; i < kFpLimbs; i++) r.v[i] = neg ? n.v[i] : a.v[i];
     return r;
 }
 
+// Montgomery -> canonical integer in [0, p).
 VKZG_DEV Fp fp_to_canonical(const Fp &a) {
     Fp one = fp_zero();
     one.v[0] = 1;
-    return fp_mul(a, one);
+    return fp_canonical(fp_mul(a, one));
 }
+// Integer below p (28-bit limbs) -> Montgomery form.
 VKZG_DEV Fp fp_from_canonical(const Fp &a) { return fp_mul(a, fp_const(cfp::R2)); }
+
+// 12 little-endian 32-bit words (a value below 2^384) <-> 28-bit limbs.
+VKZG_DEV Fp fp_from_words(const uint32_t (&w)[12]) {
+    Fp r;
+#pragma unroll
+    for (int i = 0; i < kFpLimbs; i++) {
+        const int bit = 28 * i, wi = bit >> 5, sh = bit & 31;
+        uint32_t x = w[wi] >> sh;
+        if (sh > 4 && wi + 1 < 12) x |= w[wi + 1] << (32 - sh);
+        r.v[i] = x & kFpMask;
+    }
+    return r;
+}
+VKZG_DEV void fp_to_words(uint32_t (&w)[12], const Fp &a) {
+#pragma unroll
+    for (int k = 0; k < 12; k++) {
+        const int bit = 32 * k, li = bit / 28, sh = bit % 28;
+        uint32_t x = a.v[li] >> sh;
+        if (li + 1 < kFpLimbs) x |= a.v[li + 1] << (28 - sh);
+        if (sh > 24 && li + 2 < kFpLimbs) x |= a.v[li + 2] << (56 - sh);
+        w[k] = x;
+    }
+}
+// a < p for a value in 28-bit limbs
+VKZG_DEV bool fp_limbs_lt_p(const Fp &a) {
+    int32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < kFpLimbs; i++) c = ((int32_t)a.v[i] - (int32_t)cfp::P[i] + c) >> 28;
+    return c < 0;
+}
 
 template <int NE>
 VKZG_DEV Fp fp_pow_const(const Fp &a, const uint32_t (&e)[NE]) {
@@ -354,14 +486,35 @@ VKZG_DEV Fp fp_inv(const Fp &a) { return fp_pow_const<12>(a, cfp::P_MINUS_2); }
 
 // canonical value > (p-1)/2 (the sign flag of the compressed encoding)
 VKZG_DEV bool fp_is_lex_largest(const Fp &canonical) {
-    // (p-1)/2 < x  <=>  borrow out of (HALF - x)
-    uint32_t br = 0;
+    // (p-1)/2 < x  <=>  HALF - x < 0
+    int32_t c = 0;
 #pragma unroll
-    for (int i = 0; i < 12; i++) {
-        const uint64_t s = (uint64_t)cfp::HALF[i] - canonical.v[i] - br;
-        br = (uint32_t)(s >> 32) & 1u;
+    for (int i = 0; i < kFpLimbs; i++) c = ((int32_t)cfp::HALF[i] - (int32_t)canonical.v[i] + c) >> 28;
+    return c < 0;
+}
+
+// Reference Montgomery product (operand scanning, plain C++) for the test
+// hooks: same radix and limbs, fully reduced.
+VKZG_DEV Fp fp_mul_reference(const Fp &a, const Fp &b) {
+    uint64_t t[kFpLimbs + 2];
+#pragma unroll
+    for (int j = 0; j < kFpLimbs + 2; j++) t[j] = 0;
+    for (int i = 0; i < kFpLimbs; i++) {
+        for (int j = 0; j < kFpLimbs; j++) t[j] += (uint64_t)a.v[j] * b.v[i];
+        const uint32_t m = ((uint32_t)t[0] * cfp::N0) & kFpMask;
+        for (int j = 0; j < kFpLimbs; j++) t[j] += (uint64_t)m * cfp::P[j];
+        // t[0] is now divisible by 2^28: shift the whole array one limb down
+        uint64_t carry = t[0] >> 28;
+        for (int j = 1; j <= kFpLimbs; j++) {
+            const uint64_t v = t[j] + carry;
+            t[j - 1] = v & kFpMask;
+            carry = v >> 28;
+        }
+        t[kFpLimbs] = carry;
     }
-    return br != 0;
+    Fp r;
+    for (int j = 0; j < kFpLimbs; j++) r.v[j] = (uint32_t)t[j];
+    return fp_canonical(r);
 }
 
 }  // namespace vkzg_amd

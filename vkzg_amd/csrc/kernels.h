@@ -12,9 +12,10 @@
 //                             c = window bits, W = ceil(256/c), H = 2^(c-1)
 //   Fr scal[B][128][64]       phase-A scalars, CANONICAL (not Montgomery)
 //   G1X u[B][128][G]          phase-A partial sums, one per window group g < G
-//   G1J lad[B][32][128]       2^(8d) * u[j]
-//   G1J fold[B][2][64][32]    L+ / L- ladders, normalised into G1A lad_aff with
-//                             the same layout
+//   G1J lad[B][17][128]       2^(8d) * u[j], d < kLadPos
+//   G1J fold[B][2][64][17]    L+ / L- ladders, normalised into G1A lad_aff with
+//                             the same layout; G1A lad_phi: phi(lad_aff), where
+//                             phi(x, y) = (beta x, y) = lambda * P (GLV)
 //   G1X buckets[B][128][128]  phase-B buckets, [output a][bucket k]
 //   G1X cvals[B][128]         C+ (a < 64) and C- (a >= 64)
 #pragma once
@@ -24,6 +25,11 @@
 #include <cstdint>
 
 namespace vkzg_amd {
+
+// Phase-B scalars are split as k = k1 + lambda k2 (k1, k2 < 2^128) with the
+// G1 endomorphism phi = [lambda], so each ladder needs 2^(8d) u for d < 17
+// only (17 signed 8-bit digits of a 128-bit half, the top one a carry).
+constexpr uint32_t kLadPos = 17;
 
 struct Fr;
 struct Fp;
@@ -53,7 +59,8 @@ hipError_t launch_phase_a(G1X *part, const Fr *scal, const G1A *table, uint32_t 
                           hipStream_t s);
 hipError_t launch_ladder(G1J *lad, const G1X *part, uint32_t groups, uint32_t nblobs, hipStream_t s);
 hipError_t launch_fold(G1J *fold, const G1J *lad, uint32_t nblobs, hipStream_t s);
-hipError_t launch_phase_b(G1X *buckets, const G1A *lad_aff, const uint32_t *items,
+hipError_t launch_glv_phi(G1A *phi, const G1A *pts, uint64_t count, hipStream_t s);
+hipError_t launch_phase_b(G1X *buckets, const G1A *lad_aff, const G1A *lad_phi, const uint32_t *items,
                           const uint32_t *offsets, uint32_t nblobs, hipStream_t s);
 hipError_t launch_bucket_reduce(G1X *cvals, const G1X *buckets, uint32_t nblobs, hipStream_t s);
 hipError_t launch_finalize(uint8_t *proofs, const G1X *cvals, uint32_t nblobs, hipStream_t s);

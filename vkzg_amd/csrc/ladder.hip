@@ -4,7 +4,7 @@
 namespace vkzg_amd {
 
 namespace {
-// lad[blob][d][j] = 2^(8d) * u[blob][j], d < 32, where u[blob][j] is the
+// lad[blob][d][j] = 2^(8d) * u[blob][j], d < kLadPos, where u[blob][j] is the
 // sum of phase A's `groups` window-group partials part[(blob*128 + j)*groups + g].
 __global__ void k_ladder(G1J *lad, const G1X *part, uint32_t groups, uint32_t nblobs) {
     // A long dependent chain: give these waves issue priority over the
@@ -17,11 +17,11 @@ __global__ void k_ladder(G1J *lad, const G1X *part, uint32_t groups, uint32_t nb
 #pragma unroll 1
     for (uint32_t k = 1; k < groups; k++) u = g1x_add(u, part[(size_t)g * groups + k]);
     G1J p = g1j_from_xyzz(u);
-    G1J *dst = lad + (size_t)blob * 32 * 128 + j;
+    G1J *dst = lad + (size_t)blob * kLadPos * 128 + j;
 #pragma unroll 1
-    for (int d = 0; d < 32; d++) {
+    for (uint32_t d = 0; d < kLadPos; d++) {
         dst[d * 128] = p;
-        if (d == 31) break;
+        if (d == kLadPos - 1) break;
 #pragma unroll 1
         for (int k = 0; k < 8; k++) p = g1j_dbl(p);
     }
@@ -30,16 +30,26 @@ __global__ void k_ladder(G1J *lad, const G1X *part, uint32_t groups, uint32_t nb
 // fold[blob][h][j][d] = L[j][d] + (h ? -1 : 1) * L[j+64][d]  (k_fold_ladder.comp)
 __global__ void k_fold(G1J *fold, const G1J *lad, uint32_t nblobs) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= nblobs * 64 * 32) return;
-    const uint32_t blob = g / (64 * 32);
-    const uint32_t r = g % (64 * 32);
-    const uint32_t j = r / 32, d = r % 32;
-    const G1J *L = lad + (size_t)blob * 32 * 128;
+    if (g >= nblobs * 64 * kLadPos) return;
+    const uint32_t blob = g / (64 * kLadPos);
+    const uint32_t r = g % (64 * kLadPos);
+    const uint32_t j = r / kLadPos, d = r % kLadPos;
+    const G1J *L = lad + (size_t)blob * kLadPos * 128;
     const G1J a = L[d * 128 + j];
     const G1J b = L[d * 128 + j + 64];
-    G1J *F = fold + (size_t)blob * 2 * 64 * 32;
-    F[(0 * 64 + j) * 32 + d] = g1j_add(a, b);
-    F[(1 * 64 + j) * 32 + d] = g1j_add(a, g1j_neg(b));
+    G1J *F = fold + (size_t)blob * 2 * 64 * kLadPos;
+    F[(0 * 64 + j) * kLadPos + d] = g1j_add(a, b);
+    F[(1 * 64 + j) * kLadPos + d] = g1j_add(a, g1j_neg(b));
+}
+
+// phi[k] = (beta x, y) of pts[k]: the endomorphism [lambda], lambda = z^2 - 1
+// (the identity (0, 0) maps to itself).
+__global__ void k_glv_phi(G1A *phi, const G1A *pts, uint64_t count) {
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    G1A q = pts[k];
+    q.x = fp_mul(q.x, fp_const(cfp::BETA_M));
+    phi[k] = q;
 }
 
 }  // namespace
@@ -52,8 +62,13 @@ hipError_t launch_ladder(G1J *lad, const G1X *part, uint32_t groups, uint32_t nb
 }
 
 hipError_t launch_fold(G1J *fold, const G1J *lad, uint32_t nblobs, hipStream_t s) {
-    hipLaunchKernelGGL(k_fold, dim3(blocks_for(nblobs * 2048ull, 64)), dim3(64), 0, s, fold, lad,
-                       nblobs);
+    hipLaunchKernelGGL(k_fold, dim3(blocks_for(nblobs * 64ull * kLadPos, 64)), dim3(64), 0, s, fold,
+                       lad, nblobs);
+    return hipGetLastError();
+}
+
+hipError_t launch_glv_phi(G1A *phi, const G1A *pts, uint64_t count, hipStream_t s) {
+    hipLaunchKernelGGL(k_glv_phi, dim3(blocks_for(count, 256)), dim3(256), 0, s, phi, pts, count);
     return hipGetLastError();
 }
 

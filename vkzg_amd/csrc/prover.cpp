@@ -18,8 +18,9 @@ extern "C" const uint8_t vkzg_amd_mainnet_g1_end[];
 
 using namespace vkzg_amd;
 
-static_assert(sizeof(Fr) == 32 && sizeof(Fp) == 48, "field layout");
-static_assert(sizeof(G1A) == 96 && sizeof(G1J) == 144 && sizeof(G1X) == 192, "point layout");
+// Fp is 14 x 28-bit limbs in 32-bit words (56 bytes) in memory as in registers.
+static_assert(sizeof(Fr) == 32 && sizeof(Fp) == 56, "field layout");
+static_assert(sizeof(G1A) == 112 && sizeof(G1J) == 168 && sizeof(G1X) == 224, "point layout");
 
 namespace vkzg_amd {
 
@@ -47,22 +48,52 @@ uint32_t choose_window(size_t free_bytes) {
 
 // Signed 8-bit digits of the 66 phase-B scalars, grouped by bucket |v|-1:
 // item = e | d << 8 | (v < 0) << 16, offsets[half][129] (absolute).
+// k = k1 + lambda k2 with k1 = k mod lambda and k2 = k div lambda, for the
+// G1 endomorphism phi(x, y) = (beta x, y) = [lambda] P, lambda = z^2 - 1 =
+// 0xac45a4010001a40200000000ffffffff.  k < r = lambda^2 + lambda + 1 keeps
+// both halves below 2^128.
+void glv_split(const uint32_t k[8], unsigned __int128 &k1, unsigned __int128 &k2) {
+    const unsigned __int128 lambda =
+        ((unsigned __int128)0xac45a4010001a402ull << 64) | (unsigned __int128)0x00000000ffffffffull;
+    unsigned __int128 rem = 0, q = 0;
+    for (int bit = 255; bit >= 0; bit--) {
+        const bool top = (rem >> 127) != 0;  // rem < lambda < 2^128 before the shift
+        rem = (rem << 1) | ((k[bit / 32] >> (bit % 32)) & 1u);
+        if (top || rem >= lambda) {
+            rem -= lambda;
+            if (bit >= 128) throw VKZG_ERR_SETUP;  // impossible for k < 2^255: k / lambda < 2^128
+            q |= (unsigned __int128)1 << bit;
+        }
+    }
+    k1 = rem;
+    k2 = q;
+}
+
+// Phase-B item lists: for each half and bucket k, the (e, d, sign) with
+// signed 8-bit digit of magnitude k+1 at position d of the scalar
+// K[half][e] (k_phase_b_split.comp:25-61, 8-bit signed digits), where d < 17
+// indexes the digits of k1 and 17 <= d < 34 those of k2.
 void build_phase_b_items(const Fr *kpm, std::vector<uint32_t> &items, std::vector<uint32_t> &offsets) {
     std::vector<uint32_t> lists[2][128];
     for (int half = 0; half < 2; half++) {
         for (uint32_t e = 0; e < 64; e++) {
-            const Fr &s = kpm[half * 64 + e];
-            int carry = 0;
-            for (uint32_t d = 0; d < 32; d++) {
-                int v = (int)((s.v[d / 4] >> (8 * (d % 4))) & 0xffu) + carry;
-                carry = 0;
-                if (v > 128) {
-                    v -= 256;
-                    carry = 1;
+            unsigned __int128 parts[2];
+            glv_split(kpm[half * 64 + e].v, parts[0], parts[1]);
+            for (int g = 0; g < 2; g++) {
+                int carry = 0;
+                for (uint32_t d = 0; d < kLadPos; d++) {
+                    const uint32_t byte = d < 16 ? (uint32_t)(parts[g] >> (8 * d)) & 0xffu : 0u;
+                    int v = (int)byte + carry;
+                    carry = 0;
+                    if (v > 128) {
+                        v -= 256;
+                        carry = 1;
+                    }
+                    if (v == 0) continue;
+                    const uint32_t mag = (uint32_t)(v < 0 ? -v : v);
+                    const uint32_t pos = d + (uint32_t)g * kLadPos;
+                    lists[half][mag - 1].push_back(e | (pos << 8) | ((v < 0 ? 1u : 0u) << 16));
                 }
-                if (v == 0) continue;
-                const uint32_t mag = (uint32_t)(v < 0 ? -v : v);
-                lists[half][mag - 1].push_back(e | (d << 8) | ((v < 0 ? 1u : 0u) << 16));
             }
         }
     }
@@ -113,6 +144,7 @@ void build_tables(vkzg_prover *p, const uint8_t *setup) {
     ck(hipStreamSynchronize(s));
     std::vector<uint32_t> items, offsets;
     build_phase_b_items(kpm.data(), items, offsets);
+    p->kb_count = (uint32_t)items.size();
     p->kb_items = p->arena.dalloc<uint32_t>(items.size());
     p->kb_offsets = p->arena.dalloc<uint32_t>(offsets.size());
     ck(hipMemcpyAsync(p->kb_items, items.data(), items.size() * 4, hipMemcpyHostToDevice, s));
@@ -157,8 +189,9 @@ void build_tables(vkzg_prover *p, const uint8_t *setup) {
 
 
 uint64_t WorkSet::bytes_per_blob() {
-    return kBlobBytes + kProofBytes + 4 + 128 * 64 * sizeof(Fr) + 128 * sizeof(G1X) +
-           2 * 4096 * sizeof(G1J) + 4096 * sizeof(G1A) + 4096 * sizeof(Fp) +
+    const uint64_t npos = 128ull * kLadPos;
+    return kBlobBytes + kProofBytes + 4 + 4 + 128 * 64 * sizeof(Fr) + 128 * kMaxGroups * sizeof(G1X) +
+           2 * npos * sizeof(G1J) + 2 * npos * sizeof(G1A) + npos * sizeof(Fp) +
            128 * 128 * sizeof(G1X) + 128 * sizeof(G1X) + 2 * kCellsBytes + 128 +
            2 * 128 * sizeof(Fr) + 2 * 8192 * sizeof(Fr);
 }
@@ -172,10 +205,11 @@ void WorkSet::allocate(uint32_t B) {
     ck(hipMemset(queue, 0, (size_t)B * sizeof(uint32_t)));
     scal = arena.dalloc<Fr>((size_t)B * 128 * 64);
     u = arena.dalloc<G1X>((size_t)B * 128 * kMaxGroups);
-    lad = arena.dalloc<G1J>((size_t)B * 4096);
-    fold = arena.dalloc<G1J>((size_t)B * 4096);
-    lad_aff = arena.dalloc<G1A>((size_t)B * 4096);
-    prefix = arena.dalloc<Fp>((size_t)B * 4096);
+    lad = arena.dalloc<G1J>((size_t)B * 128 * kLadPos);
+    fold = arena.dalloc<G1J>((size_t)B * 128 * kLadPos);
+    lad_aff = arena.dalloc<G1A>((size_t)B * 128 * kLadPos);
+    lad_phi = arena.dalloc<G1A>((size_t)B * 128 * kLadPos);
+    prefix = arena.dalloc<Fp>((size_t)B * 128 * kLadPos);
     buckets = arena.dalloc<G1X>((size_t)B * 128 * 128);
     cvals = arena.dalloc<G1X>((size_t)B * 128);
     cells = arena.dalloc<uint8_t>((size_t)B * kCellsBytes);
@@ -230,9 +264,10 @@ void proofs_range(vkzg_prover *p, WorkSet &ws, uint32_t off, uint32_t nb, uint8_
     uint32_t *queue = ws.queue + off;
     Fr *scal = ws.scal + (size_t)off * 8192;
     G1X *u = ws.u + (size_t)off * 128 * kMaxGroups;
-    G1J *lad = ws.lad + (size_t)off * 4096, *fold = ws.fold + (size_t)off * 4096;
-    G1A *lad_aff = ws.lad_aff + (size_t)off * 4096;
-    Fp *prefix = ws.prefix + (size_t)off * 4096;
+    const size_t npos = (size_t)128 * kLadPos;  // ladder points per blob
+    G1J *lad = ws.lad + off * npos, *fold = ws.fold + off * npos;
+    G1A *lad_aff = ws.lad_aff + off * npos, *lad_phi = ws.lad_phi + off * npos;
+    Fp *prefix = ws.prefix + off * npos;
     G1X *buckets = ws.buckets + (size_t)off * 128 * 128, *cvals = ws.cvals + (size_t)off * 128;
     ck(launch_fk20_scalars(scal, d_blobs, err, p->roots, p->roots_inv, queue, nb, s));
     clk.mark("fk20_scalars");
@@ -242,9 +277,10 @@ void proofs_range(vkzg_prover *p, WorkSet &ws, uint32_t off, uint32_t nb, uint8_
     clk.mark("ladder");
     ck(launch_fold(fold, lad, nb, s));
     clk.mark("fold");
-    ck(launch_jac_to_affine(lad_aff, fold, prefix, (uint64_t)nb * 4096, tn.norm_chunk, s));
+    ck(launch_jac_to_affine(lad_aff, fold, prefix, (uint64_t)nb * npos, tn.norm_chunk, s));
+    ck(launch_glv_phi(lad_phi, lad_aff, (uint64_t)nb * npos, s));
     clk.mark("normalize");
-    ck(launch_phase_b(buckets, lad_aff, p->kb_items, p->kb_offsets, nb, s));
+    ck(launch_phase_b(buckets, lad_aff, lad_phi, p->kb_items, p->kb_offsets, nb, s));
     clk.mark("phase_b");
     ck(launch_bucket_reduce(cvals, buckets, nb, s));
     clk.mark("bucket_reduce");
@@ -715,6 +751,7 @@ vkzg_result vkzg_amd_prover_info(const vkzg_prover *p, vkzg_amd_info *out) {
     out->graph_replays = p->graphs.replays;
     out->autotuned = p->autotuned ? 1u : 0u;
     out->autotune_ms = p->autotune_ms;
+    out->phase_b_items = p->kb_count;
     out->max_batch_size = p->max_batch;
     out->setup_ms = p->setup_ms;
     return VKZG_OK;

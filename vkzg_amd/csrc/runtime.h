@@ -100,7 +100,7 @@ struct WorkSet {
     Fr *scal = nullptr;
     G1X *u = nullptr, *buckets = nullptr, *cvals = nullptr;
     G1J *lad = nullptr, *fold = nullptr;
-    G1A *lad_aff = nullptr;
+    G1A *lad_aff = nullptr, *lad_phi = nullptr;
     Fp *prefix = nullptr;
     // recovery
     uint8_t *cells = nullptr, *present = nullptr, *cells_out = nullptr;
@@ -154,7 +154,7 @@ struct GraphCache {
     std::vector<Entry> entries;
     uint64_t tick = 0;
     uint64_t captures = 0, replays = 0;
-    static constexpr size_t kCapacity = 48;
+    static constexpr size_t kCapacity = 256;  // every batch size of both kinds up to 128
 
     // Enqueue `body(s)` on s through a cached graph.
     template <typename Body>
@@ -225,6 +225,7 @@ struct vkzg_prover {
     uint32_t c = 0, W = 0, H = 0;
     uint64_t table_bytes = 0;
     uint32_t *kb_items = nullptr, *kb_offsets = nullptr;
+    uint32_t kb_count = 0;
 
     // Working set of the synchronous entry points and its pinned staging.
     vkzg_amd::WorkSet ws;

@@ -51,11 +51,12 @@ __global__ void k_decompress(G1A *out, const uint8_t *in, uint32_t n, uint32_t *
         if (bad) atomicOr(err, 1u);
         return;
     }
-    Fp x;
+    uint32_t xw[12];
 #pragma unroll
-    for (int k = 0; k < 12; k++) x.v[k] = w[11 - k];
-    x.v[11] &= 0x1fffffffu;
-    if (!limbs_lt<12>(x.v, cfp::P)) bad = true;
+    for (int k = 0; k < 12; k++) xw[k] = w[11 - k];
+    xw[11] &= 0x1fffffffu;
+    const Fp x = fp_from_words(xw);
+    if (!fp_limbs_lt_p(x)) bad = true;
     const Fp xm = fp_from_canonical(x);
     const Fp rhs = fp_add(fp_mul(fp_sqr(xm), xm), fp_const(cfp::FOUR_M));
     Fp y = fp_pow_const<12>(rhs, cfp::SQRT_EXP);
@@ -96,19 +97,21 @@ __global__ void k_phase_b_scalars(Fr *out, const Fr *roots) {
 __global__ void k_debug_fp_mul(uint32_t *out, const uint32_t *a, const uint32_t *b, uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    Fp x, y;
+    uint32_t aw[12], bw[12];
 #pragma unroll
     for (int k = 0; k < 12; k++) {
-        x.v[k] = a[12 * i + k];
-        y.v[k] = b[12 * i + k];
+        aw[k] = a[12 * i + k];
+        bw[k] = b[12 * i + k];
     }
-    const Fp xm = fp_from_canonical(x), ym = fp_from_canonical(y);
-    const Fp r = fp_to_canonical(fp_mul(xm, ym));
-    Fp c;
-    mont_mul_cios<12>(c.v, xm.v, ym.v, cfp::P, cfp::N0);
-    const bool agree = limbs_eq<12>(c.v, fp_mul(xm, ym).v);
+    const Fp xm = fp_from_canonical(fp_from_words(aw)), ym = fp_from_canonical(fp_from_words(bw));
+    const Fp prod = fp_mul(xm, ym);
+    const Fp r = fp_to_canonical(prod);
+    // the scheduled product must equal the plain operand-scanning one modulo p
+    const bool agree = limbs_eq<kFpLimbs>(fp_canonical(prod).v, fp_mul_reference(xm, ym).v);
+    uint32_t rw[12];
+    fp_to_words(rw, r);
 #pragma unroll
-    for (int k = 0; k < 12; k++) out[12 * i + k] = agree ? r.v[k] : 0xffffffffu;
+    for (int k = 0; k < 12; k++) out[12 * i + k] = agree ? rw[k] : 0xffffffffu;
 }
 
 __global__ void k_debug_fr_mul(uint32_t *out, const uint32_t *a, const uint32_t *b, uint32_t n) {
