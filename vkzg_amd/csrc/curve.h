@@ -29,6 +29,11 @@ struct G1A {
 struct G1J {
     Fp x, y, z;
 };
+// A phase-A table entry: an affine point padded to exactly one 128-byte
+// line, so each random gather touches one line instead of two.
+struct __attribute__((aligned(128))) G1AT {
+    G1A p;
+};
 struct G1X {
     Fp x, y, zz, zzz;
 };

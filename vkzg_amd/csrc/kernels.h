@@ -8,7 +8,8 @@
 //   Fr shift_fwd[8192]        7^k / 8192;    shift_inv: 7^-k / 8192
 //   G1A srs[4096]             the G1 monomial setup, affine
 //   G1A xext[128][64]         x_ext_fft[j][i] (FK20 setup in the Fourier domain)
-//   G1A table[128][64][W][H]  table[j][i][w][d-1] = d * 2^(c*w) * xext[j][i],
+//   G1AT table[128][64][W][H] table[j][i][w][d-1] = d * 2^(c*w) * xext[j][i], one
+//                             128-byte line per entry,
 //                             c = window bits, W = ceil(256/c), H = 2^(c-1)
 //   Fr scal[B][128][64]       phase-A scalars, CANONICAL (not Montgomery)
 //   G1X u[B][128][G]          phase-A partial sums, one per window group g < G
@@ -34,6 +35,7 @@ constexpr uint32_t kLadPos = 17;
 struct Fr;
 struct Fp;
 struct G1A;
+struct G1AT;
 struct G1J;
 struct G1X;
 
@@ -44,7 +46,7 @@ hipError_t launch_xext_fft(G1J *out, const G1A *srs, const Fr *roots, hipStream_
 hipError_t launch_jac_to_affine(G1A *out, const G1J *in, Fp *prefix, uint64_t count, uint32_t chunk,
                                 hipStream_t s);
 hipError_t launch_table_chains(G1J *out, const G1A *xext, uint32_t c, uint32_t W, hipStream_t s);
-hipError_t launch_table_multiples(G1A *table, const G1A *pw, uint32_t c, uint32_t W,
+hipError_t launch_table_multiples(G1AT *table, const G1A *pw, uint32_t c, uint32_t W,
                                   uint64_t thread_begin, uint32_t thread_count, G1J *scratch,
                                   Fp *prefix, hipStream_t s);
 hipError_t launch_phase_b_scalars(Fr *out, const Fr *roots, hipStream_t s);
@@ -54,7 +56,7 @@ hipError_t launch_phase_b_scalars(Fr *out, const Fr *roots, hipStream_t s);
 hipError_t launch_fk20_scalars(Fr *scal, const uint8_t *blobs, uint32_t *err, const Fr *roots,
                                const Fr *roots_inv, uint32_t *queue, uint32_t nblobs, hipStream_t s);
 // Persistent: `grid` blocks drain (msm, window group) tiles from queue[0].
-hipError_t launch_phase_a(G1X *part, const Fr *scal, const G1A *table, uint32_t c, uint32_t W,
+hipError_t launch_phase_a(G1X *part, const Fr *scal, const G1AT *table, uint32_t c, uint32_t W,
                           uint32_t nblobs, uint32_t groups, uint32_t grid, uint32_t *queue,
                           hipStream_t s);
 hipError_t launch_ladder(G1J *lad, const G1X *part, uint32_t groups, uint32_t nblobs, hipStream_t s);

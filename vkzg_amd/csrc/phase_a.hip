@@ -20,20 +20,20 @@ constexpr int kTilesPerItem = 256 / kMsmLanes;  // tiles one 256-thread block ta
 // Windows [w0, w1) of the 64 terms of one MSM, recomputed with the checked
 // point operations: reached only when the fast path flagged a doubling or
 // cancellation case in this tile.
-__device__ __attribute__((noinline)) void phase_a_slow(G1X *out, const Fr *scal, const G1A *tab_j,
+__device__ __attribute__((noinline)) void phase_a_slow(G1X *out, const Fr *scal, const G1AT *tab_j,
                                                        uint32_t c, uint32_t W, uint32_t w0,
                                                        uint32_t w1) {
     const uint32_t H = 1u << (c - 1);
     G1X acc = g1x_identity();
     for (uint32_t i = 0; i < 64; i++) {
         const Fr s = scal[i];
-        const G1A *tab = tab_j + (size_t)i * W * H;
+        const G1AT *tab = tab_j + (size_t)i * W * H;
         uint32_t carry = 0;
         for (uint32_t w = 0; w < w1; w++) {
             const uint32_t raw = window_bits(s, w, c) + carry;
             carry = raw > H ? 1u : 0u;
             const uint32_t mag = carry ? (2u * H - raw) : raw;
-            if (w >= w0 && mag) acc = g1x_madd(acc, g1a_neg_if(tab[(size_t)w * H + (mag - 1)], carry != 0));
+            if (w >= w0 && mag) acc = g1x_madd(acc, g1a_neg_if(tab[(size_t)w * H + (mag - 1)].p, carry != 0));
         }
     }
     *out = acc;
@@ -51,7 +51,7 @@ __device__ __attribute__((noinline)) void phase_a_slow(G1X *out, const Fr *scal,
 // after one failed take, so the grid always drains.  q[0] is cleared by the
 // kernel that runs before this one.
 __global__ void __launch_bounds__(256, 2)
-    k_phase_a(G1X *part, const Fr *scal, const G1A *table, uint32_t c, uint32_t W, uint32_t nmsm,
+    k_phase_a(G1X *part, const Fr *scal, const G1AT *table, uint32_t c, uint32_t W, uint32_t nmsm,
               uint32_t groups, uint32_t *q) {
     __shared__ uint32_t item_sh;
     const uint32_t t = threadIdx.x % kMsmLanes;
@@ -78,7 +78,7 @@ __global__ void __launch_bounds__(256, 2)
             for (uint32_t qq = 0; qq < 64 / kMsmLanes; qq++) {
                 const uint32_t i = t * (64 / kMsmLanes) + qq;
                 const Fr s = scal[(size_t)msm * 64 + i];
-                const G1A *tab = table + ((size_t)(j * 64 + i) * W) * H;
+                const G1AT *tab = table + ((size_t)(j * 64 + i) * W) * H;
                 uint32_t carry = 0;
 #pragma unroll 1
                 for (uint32_t w = 0; w < w1; w++) {
@@ -87,7 +87,7 @@ __global__ void __launch_bounds__(256, 2)
                     const bool neg = carry != 0;
                     const uint32_t mag = neg ? (2u * H - raw) : raw;
                     if (w < w0 || mag == 0) continue;
-                    const G1A pt = g1a_neg_if(tab[(size_t)w * H + (mag - 1)], neg);
+                    const G1A pt = g1a_neg_if(tab[(size_t)w * H + (mag - 1)].p, neg);
                     acc = g1x_madd_fast(acc, pt, exc);
                 }
             }
@@ -110,7 +110,7 @@ __global__ void __launch_bounds__(256, 2)
 
 }  // namespace
 
-hipError_t launch_phase_a(G1X *part, const Fr *scal, const G1A *table, uint32_t c, uint32_t W,
+hipError_t launch_phase_a(G1X *part, const Fr *scal, const G1AT *table, uint32_t c, uint32_t W,
                           uint32_t nblobs, uint32_t groups, uint32_t grid, uint32_t *queue,
                           hipStream_t s) {
     const uint32_t nitems = (nblobs * 128u * groups + kTilesPerItem - 1) / kTilesPerItem;

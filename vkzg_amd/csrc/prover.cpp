@@ -20,7 +20,8 @@ using namespace vkzg_amd;
 
 // Fp is 14 x 28-bit limbs in 32-bit words (56 bytes) in memory as in registers.
 static_assert(sizeof(Fr) == 32 && sizeof(Fp) == 56, "field layout");
-static_assert(sizeof(G1A) == 112 && sizeof(G1J) == 168 && sizeof(G1X) == 224, "point layout");
+static_assert(sizeof(G1A) == 112 && sizeof(G1J) == 168 && sizeof(G1X) == 224 && sizeof(G1AT) == 128,
+              "point layout");
 
 namespace vkzg_amd {
 
@@ -30,7 +31,7 @@ constexpr uint32_t kDefaultBatch = 256;
 uint64_t table_bytes_for(uint32_t c) {
     const uint64_t W = (256 + c - 1) / c;
     const uint64_t H = 1ull << (c - 1);
-    return 8192ull * W * H * sizeof(G1A);
+    return 8192ull * W * H * sizeof(G1AT);
 }
 
 // Window bits of the resident phase-A table: $VKZG_AMD_WINDOW if set (6..14),
@@ -157,7 +158,7 @@ void build_tables(vkzg_prover *p, const uint8_t *setup) {
     p->W = (256 + p->c - 1) / p->c;
     p->H = 1u << (p->c - 1);
     p->table_bytes = table_bytes_for(p->c);
-    p->table = p->arena.dalloc<G1A>(p->table_bytes / sizeof(G1A));
+    p->table = p->arena.dalloc<G1AT>(p->table_bytes / sizeof(G1AT));
 
     G1J *pwj = p->arena.dalloc<G1J>((size_t)p->W * 8192);
     Fp *pw_prefix = p->arena.dalloc<Fp>((size_t)p->W * 8192);

@@ -221,7 +221,8 @@ struct vkzg_prover {
     // Resident tables.
     vkzg_amd::DeviceArena arena;
     vkzg_amd::Fr *roots = nullptr, *roots_inv = nullptr, *shift_fwd = nullptr, *shift_inv = nullptr;
-    vkzg_amd::G1A *srs = nullptr, *xext = nullptr, *table = nullptr;
+    vkzg_amd::G1A *srs = nullptr, *xext = nullptr;
+    vkzg_amd::G1AT *table = nullptr;
     uint32_t c = 0, W = 0, H = 0;
     uint64_t table_bytes = 0;
     uint32_t *kb_items = nullptr, *kb_offsets = nullptr;

@@ -76,7 +76,7 @@ __device__ __attribute__((noinline)) void table_run_slow(G1J *scratch, Fp *prefi
 // Thread g = (b*W + w)*nseg + seg writes table[b][w][d-1] for d in
 // [seg*S+1, seg*S+S]: a running sum of the affine base P = pw[w][b],
 // normalised with one inversion per thread.
-__global__ void k_table_multiples(G1A *table, const G1A *pw, uint32_t c, uint32_t W,
+__global__ void k_table_multiples(G1AT *table, const G1A *pw, uint32_t c, uint32_t W,
                                   uint64_t thread_begin, uint32_t thread_count, G1J *scratch,
                                   Fp *prefix) {
     const uint32_t lt = blockIdx.x * blockDim.x + threadIdx.x;
@@ -113,13 +113,13 @@ __global__ void k_table_multiples(G1A *table, const G1A *pw, uint32_t c, uint32_
     }
     if (exc) table_run_slow(scratch, prefix, &acc, P, d0, S, thread_count, lt);
     Fp inv = fp_inv(acc);
-    G1A *dst = table + bw * H + (d0 - 1);
+    G1AT *dst = table + bw * H + (d0 - 1);
 #pragma unroll 1
     for (int m = (int)S - 1; m >= 0; m--) {
         const size_t slot = (size_t)m * thread_count + lt;
         const G1J p = scratch[slot];
         if (fp_is_zero(p.z)) {
-            dst[m] = g1a_identity();
+            dst[m].p = g1a_identity();
             continue;
         }
         const Fp zi = fp_mul(inv, prefix[slot]);
@@ -128,7 +128,7 @@ __global__ void k_table_multiples(G1A *table, const G1A *pw, uint32_t c, uint32_
         G1A r;
         r.x = fp_mul(p.x, zi2);
         r.y = fp_mul(p.y, fp_mul(zi2, zi));
-        dst[m] = r;
+        dst[m].p = r;
     }
 }
 
@@ -147,7 +147,7 @@ hipError_t launch_table_chains(G1J *out, const G1A *xext, uint32_t c, uint32_t W
     return hipGetLastError();
 }
 
-hipError_t launch_table_multiples(G1A *table, const G1A *pw, uint32_t c, uint32_t W,
+hipError_t launch_table_multiples(G1AT *table, const G1A *pw, uint32_t c, uint32_t W,
                                   uint64_t thread_begin, uint32_t thread_count, G1J *scratch,
                                   Fp *prefix, hipStream_t s) {
     hipLaunchKernelGGL(k_table_multiples, dim3(blocks_for(thread_count, 64)), dim3(64), 0, s, table,
