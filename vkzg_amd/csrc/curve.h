@@ -260,30 +260,55 @@ VKZG_DEV_CALL G1X g1x_madd(const G1X &a, const G1A &b) {
     return r;
 }
 
-// Hot-loop variants.  They handle either operand being the identity, and
-// when the inputs are equal or opposite (the doubling / cancellation cases,
-// which random data never reaches) they set `exc` and return an unspecified
-// point; callers then recompute that result with the checked forms above
-// outside their loops.  Keeping the rare path out of the loop body is what
-// keeps the accumulator in VGPRs (a call inside the loop that takes the
-// point by address pins it to scratch memory).
-VKZG_DEV G1X g1x_madd_fast(const G1X &a, const G1A &b, bool &exc) {
-    if (g1a_is_identity(b)) return a;
-    if (fp_is_zero(a.zz)) return g1x_from_affine(b);
+// Hot-loop variants.  When the inputs are equal or opposite (the doubling /
+// cancellation cases, which random data never reaches) they set `exc` and
+// return an unspecified point; callers then recompute that result with the
+// checked forms above outside their loops.  Keeping the rare path out of the
+// loop body is what keeps the accumulator in VGPRs (a call inside the loop
+// that takes the point by address pins it to scratch memory).  P, R and
+// Q - X3 only feed products, so they are lazy (fp_sub_lazy), and P == 0 is
+// detected on PP = P^2.
+
+// Accumulator step a + (+-b) for an accumulator that is never the identity
+// (callers start it from their first point).  b is an affine point in
+// normalised limbs, negated when `neg`; `exc` is also set when b is the
+// identity (all-zero limbs: no point of the prime-order group has y = 0).
+VKZG_DEV G1X g1x_madd_acc(const G1X &a, const G1A &b, bool neg, bool &exc) {
+    exc |= limbs_zero<kFpLimbs>(b.y.v);
+    const Fp by = fp_neg_lazy_if(b.y, neg);
     const Fp U2 = fp_mul(b.x, a.zz);
-    const Fp S2 = fp_mul(b.y, a.zzz);
-    const Fp P = fp_sub(U2, a.x);
-    const Fp R = fp_sub(S2, a.y);
-    exc |= fp_is_zero(P);
+    const Fp S2 = fp_mul(by, a.zzz);
+    const Fp P = fp_sub_lazy(U2, a.x);
+    const Fp R = fp_sub_lazy(S2, a.y);
     const Fp PP = fp_sqr(P);
+    exc |= fp_is_zero(PP);
     const Fp PPP = fp_mul(P, PP);
     const Fp Q = fp_mul(a.x, PP);
     G1X r;
     r.x = fp_sub(fp_sub(fp_sqr(R), PPP), fp_dbl(Q));
-    r.y = fp_sub(fp_mul(R, fp_sub(Q, r.x)), fp_mul(a.y, PPP));
+    r.y = fp_sub(fp_mul(R, fp_sub_lazy(Q, r.x)), fp_mul(a.y, PPP));
     r.zz = fp_mul(a.zz, PP);
     r.zzz = fp_mul(a.zzz, PPP);
     return r;
+}
+
+// The first point of an accumulator: (+-b) in XYZZ; flags the identity.
+VKZG_DEV G1X g1x_start(const G1A &b, bool neg, bool &exc) {
+    exc |= limbs_zero<kFpLimbs>(b.y.v);
+    G1X r;
+    r.x = b.x;
+    r.y = fp_cond_neg(b.y, neg);
+    r.zz = fp_one();
+    r.zzz = fp_one();
+    return r;
+}
+
+// As g1x_madd_acc, but a may be the identity (used where accumulators can
+// start empty).
+VKZG_DEV G1X g1x_madd_fast(const G1X &a, const G1A &b, bool &exc) {
+    if (g1a_is_identity(b)) return a;
+    if (fp_is_zero(a.zz)) return g1x_from_affine(b);
+    return g1x_madd_acc(a, b, false, exc);
 }
 
 VKZG_DEV G1X g1x_add_fast(const G1X &a, const G1X &b, bool &exc) {
@@ -293,15 +318,15 @@ VKZG_DEV G1X g1x_add_fast(const G1X &a, const G1X &b, bool &exc) {
     const Fp U2 = fp_mul(b.x, a.zz);
     const Fp S1 = fp_mul(a.y, b.zzz);
     const Fp S2 = fp_mul(b.y, a.zzz);
-    const Fp P = fp_sub(U2, U1);
-    const Fp R = fp_sub(S2, S1);
-    exc |= fp_is_zero(P);
+    const Fp P = fp_sub_lazy(U2, U1);
+    const Fp R = fp_sub_lazy(S2, S1);
     const Fp PP = fp_sqr(P);
+    exc |= fp_is_zero(PP);
     const Fp PPP = fp_mul(P, PP);
     const Fp Q = fp_mul(U1, PP);
     G1X r;
     r.x = fp_sub(fp_sub(fp_sqr(R), PPP), fp_dbl(Q));
-    r.y = fp_sub(fp_mul(R, fp_sub(Q, r.x)), fp_mul(S1, PPP));
+    r.y = fp_sub(fp_mul(R, fp_sub_lazy(Q, r.x)), fp_mul(S1, PPP));
     r.zz = fp_mul(fp_mul(a.zz, b.zz), PP);
     r.zzz = fp_mul(fp_mul(a.zzz, b.zzz), PPP);
     return r;

@@ -72,7 +72,7 @@ __global__ void __launch_bounds__(256, 2)
         const uint32_t j = msm % 128;
         const uint32_t w0 = g * W / groups, w1 = (g + 1) * W / groups;
         G1X acc = g1x_identity();
-        bool exc = false;
+        bool exc = false, started = false;
         if (live) {
 #pragma unroll 1
             for (uint32_t qq = 0; qq < 64 / kMsmLanes; qq++) {
@@ -87,8 +87,13 @@ __global__ void __launch_bounds__(256, 2)
                     const bool neg = carry != 0;
                     const uint32_t mag = neg ? (2u * H - raw) : raw;
                     if (w < w0 || mag == 0) continue;
-                    const G1A pt = g1a_neg_if(tab[(size_t)w * H + (mag - 1)].p, neg);
-                    acc = g1x_madd_fast(acc, pt, exc);
+                    const G1A pt = tab[(size_t)w * H + (mag - 1)].p;
+                    if (started) {
+                        acc = g1x_madd_acc(acc, pt, neg, exc);
+                    } else {
+                        acc = g1x_start(pt, neg, exc);
+                        started = true;
+                    }
                 }
             }
         }

@@ -38,7 +38,7 @@ __global__ void __launch_bounds__(64, 2)
     const G1A *L = lad_aff + (size_t)(blob * 2 + half) * 64 * kLadPos;
     const G1A *Phi = lad_phi + (size_t)(blob * 2 + half) * 64 * kLadPos;
     G1X acc = g1x_identity();
-    bool exc = false;
+    bool exc = false, started = false;
 #pragma unroll 1
     for (uint32_t it = lo; it < hi; it++) {
         const uint32_t packed = items[it];
@@ -48,7 +48,12 @@ __global__ void __launch_bounds__(64, 2)
         if (half && e > aa) neg = !neg;
         const uint32_t src = ((aa - e) & 63u) * kLadPos;
         const G1A pt = d < kLadPos ? L[src + d] : Phi[src + d - kLadPos];
-        acc = g1x_madd_fast(acc, g1a_neg_if(pt, neg), exc);
+        if (started) {
+            acc = g1x_madd_acc(acc, pt, neg, exc);
+        } else {
+            acc = g1x_start(pt, neg, exc);
+            started = true;
+        }
     }
     if (exc) phase_b_slow(&buckets[(size_t)ba * 128 + k], L, Phi, items, lo, hi, half, aa);
     else buckets[(size_t)ba * 128 + k] = acc;
