@@ -342,6 +342,24 @@ def main():
                 prover.compute_cell_kzg_proofs_batch(chunk, n)
             per_call[str(n)] = (time.perf_counter() - t0) / 4 * 1e3
         secondary["per_call_ms"] = per_call
+        # compute_cells_and_kzg_proofs and recover_cells_and_kzg_proofs through
+        # the host C ABI (host copies included)
+        nc = min(nb, 64)
+        chunk = blobs[:131072 * nc]
+        prover.compute_cells_and_kzg_proofs_batch(chunk, nc)
+        t0 = time.perf_counter()
+        for _ in range(3):
+            cells_out, _ = prover.compute_cells_and_kzg_proofs_batch(chunk, nc)
+        secondary["cells_and_proofs_host_%d_blobs_per_s" % nc] = 3 * nc / (time.perf_counter() - t0)
+        keep = bytearray(128 * nc)
+        for b in range(nc):
+            for c in rng.sample(range(128), 64):
+                keep[128 * b + c] = 1
+        prover.recover_cells_and_kzg_proofs_batch(cells_out, bytes(keep), nc)
+        t0 = time.perf_counter()
+        for _ in range(3):
+            prover.recover_cells_and_kzg_proofs_batch(cells_out, bytes(keep), nc)
+        secondary["recover_cells_and_proofs_host_%d_blobs_per_s" % nc] = 3 * nc / (time.perf_counter() - t0)
 
     if not a.no_secondary and a.serve_seconds > 0:
         secondary["serve"] = serve_bench(prover, random.Random(a.seed + 3), a.serve_seconds,

@@ -36,6 +36,27 @@ vkzg_result vkzg_amd_recover_cells_device(vkzg_prover *p, void *d_cells_out, con
                                           const uint8_t *h_cell_present, size_t num_blobs,
                                           void *hip_stream);
 
+/* ---------------------------------------------------- cells and proofs
+ * The consensus-spec pair of functions around the reference's two entry
+ * points (the reference leaves both to the caller, include/vkzg.h:14-18):
+ *
+ * compute_cells_and_kzg_proofs for a batch: the 128 cells (2x Reed-Solomon
+ * extension, n x 262,144 bytes, canonical big-endian, cell order) and the 128
+ * proofs (n x 6,144 bytes, the bytes vkzg_compute_proofs_batch returns) of
+ * each blob.  Either output may be NULL, not both.  Errors as
+ * vkzg_compute_proofs_batch (INVALID_BLOB for a chunk with a non-canonical
+ * element, nothing written for that chunk). */
+vkzg_result vkzg_amd_compute_cells_and_proofs(vkzg_prover *p, uint8_t *cells, uint8_t *proofs,
+                                              const uint8_t *blobs, size_t num_blobs);
+
+/* recover_cells_and_kzg_proofs for a batch: vkzg_recover_cells_batch's cells
+ * plus the 128 proofs of each recovered blob, computed on the device from the
+ * recovered cells (no host round trip).  Same buffer contract and errors as
+ * vkzg_recover_cells_batch; proofs_out is n x 6,144 bytes. */
+vkzg_result vkzg_amd_recover_cells_and_proofs(vkzg_prover *p, uint8_t *cells_out, uint8_t *proofs_out,
+                                              const uint8_t *cells, const uint8_t *cell_present,
+                                              size_t num_blobs);
+
 typedef struct {
     uint32_t window_bits;      /* phase-A table window c */
     uint32_t windows;          /* W = ceil(256 / c) */

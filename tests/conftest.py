@@ -35,6 +35,9 @@ class Vectors:
         r = self.compute[name]
         return self.data[r["proofs_offset"]:r["proofs_offset"] + 6144]
 
+    def cells_sha256(self, name):
+        return self.compute[name]["cells_sha256"]
+
     def valid_compute(self):
         return [r["name"] for r in self.index["compute"] if r["valid"]]
 

@@ -113,6 +113,33 @@ class Prover:
         _check(load().vkzg_recover_cells_batch(self._h, out, c, pm, blob_count))
         return out.raw[: len(cells)]
 
+    def compute_cells_and_kzg_proofs_batch(self, blobs, blob_count):
+        """(cells, proofs) of each blob: the 2x extension and the 128 proofs."""
+        if len(blobs) != blob_count * BYTES_PER_BLOB:
+            raise ValueError("blobs.length must be blobCount * %d" % BYTES_PER_BLOB)
+        ncells = blob_count * CELL_PROOFS_PER_BLOB * BYTES_PER_CELL
+        nproofs = blob_count * CELL_PROOFS_PER_BLOB * BYTES_PER_PROOF
+        cells = ctypes.create_string_buffer(max(1, ncells))
+        proofs = ctypes.create_string_buffer(max(1, nproofs))
+        src = ctypes.create_string_buffer(bytes(blobs), max(1, len(blobs)))
+        _check(load().vkzg_amd_compute_cells_and_proofs(self._h, cells, proofs, src, blob_count))
+        return cells.raw[:ncells], proofs.raw[:nproofs]
+
+    def recover_cells_and_kzg_proofs_batch(self, cells, cell_present, blob_count):
+        """(all 128 cells, 128 proofs) of each blob from any >= 64 of its cells."""
+        per_blob = CELL_PROOFS_PER_BLOB * BYTES_PER_CELL
+        if len(cells) != blob_count * per_blob:
+            raise ValueError("cells.length must be blobCount * %d" % per_blob)
+        if len(cell_present) != blob_count * CELL_PROOFS_PER_BLOB:
+            raise ValueError("cellPresent.length must be blobCount * %d" % CELL_PROOFS_PER_BLOB)
+        nproofs = blob_count * CELL_PROOFS_PER_BLOB * BYTES_PER_PROOF
+        out = ctypes.create_string_buffer(max(1, len(cells)))
+        proofs = ctypes.create_string_buffer(max(1, nproofs))
+        c = ctypes.create_string_buffer(bytes(cells), max(1, len(cells)))
+        pm = ctypes.create_string_buffer(bytes(cell_present), max(1, len(cell_present)))
+        _check(load().vkzg_amd_recover_cells_and_proofs(self._h, out, proofs, c, pm, blob_count))
+        return out.raw[:len(cells)], proofs.raw[:nproofs]
+
     def get_tuning(self, blob_count):
         t = Tuning()
         _check(load().vkzg_amd_get_tuning(self._h, blob_count, ctypes.byref(t)))

@@ -38,6 +38,8 @@ EXT_SYMBOLS = [
     "vkzg_amd_server_poll",
     "vkzg_amd_server_wait",
     "vkzg_amd_server_stats",
+    "vkzg_amd_compute_cells_and_proofs",
+    "vkzg_amd_recover_cells_and_proofs",
     "vkzg_amd_get_tuning",
     "vkzg_amd_set_tuning",
     "vkzg_amd_autotune",
@@ -152,6 +154,10 @@ def load():
     L.vkzg_amd_server_wait.argtypes = [vp, ctypes.c_uint64, ctypes.POINTER(RequestInfo)]
     L.vkzg_amd_server_stats.restype = ctypes.c_int
     L.vkzg_amd_server_stats.argtypes = [vp, ctypes.POINTER(ServerStats)]
+    L.vkzg_amd_compute_cells_and_proofs.restype = ctypes.c_int
+    L.vkzg_amd_compute_cells_and_proofs.argtypes = [vp, vp, vp, vp, sz]
+    L.vkzg_amd_recover_cells_and_proofs.restype = ctypes.c_int
+    L.vkzg_amd_recover_cells_and_proofs.argtypes = [vp, vp, vp, vp, vp, sz]
     L.vkzg_amd_get_tuning.restype = ctypes.c_int
     L.vkzg_amd_get_tuning.argtypes = [vp, sz, ctypes.POINTER(Tuning)]
     L.vkzg_amd_set_tuning.restype = ctypes.c_int

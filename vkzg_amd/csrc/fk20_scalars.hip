@@ -22,13 +22,13 @@ constexpr int kColPitch = 4 * 128;        // LDS pitch of the 4 per-wave circula
 // One block per blob.  LDS: 8 x 4096 words of polynomial + 8 x 512 words of
 // circulant columns = 144 KiB (gfx950 has 160 KiB per CU).
 __global__ void __launch_bounds__(kScalarThreads)
-    k_fk20_scalars(Fr *scal, const uint8_t *blobs, uint32_t *err, const Fr *roots,
-                   const Fr *roots_inv, uint32_t *queue) {
+    k_fk20_scalars(Fr *scal, const uint8_t *blobs, uint64_t blob_stride, uint32_t *err,
+                   const Fr *roots, const Fr *roots_inv, uint32_t *queue, Fr *coeffs) {
     __shared__ uint32_t poly[8 * kPolyPitch];
     __shared__ uint32_t cols[8 * kColPitch];
     const uint32_t blob = blockIdx.x;
     const int tid = threadIdx.x;
-    const uint32_t *src = reinterpret_cast<const uint32_t *>(blobs + 131072ull * blob);
+    const uint32_t *src = reinterpret_cast<const uint32_t *>(blobs + blob_stride * blob);
 
     // 1. Parse, check canonical, enter Montgomery form (k_blob_to_fr.comp:23-37).
     //    blob[i] = p(omega_4096^brev12(i)), i.e. the bit-reversed evaluation
@@ -56,6 +56,14 @@ __global__ void __launch_bounds__(kScalarThreads)
     for (int i = tid; i < 4096; i += kScalarThreads)
         lds_store_fr(poly, kPolyPitch, i, fr_mul(lds_load_fr(poly, kPolyPitch, i), inv_n));
     __syncthreads();
+    // Optionally the monomial coefficients, zero-padded to 8192, for the cell
+    // extension (compute_cells_and_kzg_proofs).
+    if (coeffs) {
+        Fr *cdst = coeffs + (size_t)blob * 8192;
+#pragma unroll 1
+        for (int i = tid; i < 8192; i += kScalarThreads)
+            cdst[i] = i < 4096 ? lds_load_fr(poly, kPolyPitch, i) : fr_zero();
+    }
 
     // 3. 64 circulant columns, 4 at a time (one per wave): tc_i[0] = c[4095-i],
     //    tc_i[128-m] = c[4095-i-64m] (m = 1..62), zero elsewhere; NTT_128;
@@ -100,10 +108,11 @@ __global__ void __launch_bounds__(kScalarThreads)
 
 }  // namespace
 
-hipError_t launch_fk20_scalars(Fr *scal, const uint8_t *blobs, uint32_t *err, const Fr *roots,
-                               const Fr *roots_inv, uint32_t *queue, uint32_t nblobs, hipStream_t s) {
-    hipLaunchKernelGGL(k_fk20_scalars, dim3(nblobs), dim3(kScalarThreads), 0, s, scal, blobs, err,
-                       roots, roots_inv, queue);
+hipError_t launch_fk20_scalars(Fr *scal, const uint8_t *blobs, uint64_t blob_stride, uint32_t *err,
+                               const Fr *roots, const Fr *roots_inv, uint32_t *queue, Fr *coeffs,
+                               uint32_t nblobs, hipStream_t s) {
+    hipLaunchKernelGGL(k_fk20_scalars, dim3(nblobs), dim3(kScalarThreads), 0, s, scal, blobs,
+                       blob_stride, err, roots, roots_inv, queue, coeffs);
     return hipGetLastError();
 }
 

@@ -52,9 +52,13 @@ hipError_t launch_table_multiples(G1AT *table, const G1A *pw, uint32_t c, uint32
 hipError_t launch_phase_b_scalars(Fr *out, const Fr *roots, hipStream_t s);
 
 // ----------------------------------------------------------------- proofs
-// Also clears queue[0], the tile counter of the phase-A launch that follows.
-hipError_t launch_fk20_scalars(Fr *scal, const uint8_t *blobs, uint32_t *err, const Fr *roots,
-                               const Fr *roots_inv, uint32_t *queue, uint32_t nblobs, hipStream_t s);
+// Blob b is read at blobs + b * blob_stride.  Also clears queue[0], the tile
+// counter of the phase-A launch that follows, and, when coeffs is non-null,
+// writes the blob polynomial's monomial coefficients there (Fr[B][8192],
+// Montgomery, zero above 4096).
+hipError_t launch_fk20_scalars(Fr *scal, const uint8_t *blobs, uint64_t blob_stride, uint32_t *err,
+                               const Fr *roots, const Fr *roots_inv, uint32_t *queue, Fr *coeffs,
+                               uint32_t nblobs, hipStream_t s);
 // Persistent: `grid` blocks drain (msm, window group) tiles from queue[0].
 hipError_t launch_phase_a(G1X *part, const Fr *scal, const G1AT *table, uint32_t c, uint32_t W,
                           uint32_t nblobs, uint32_t groups, uint32_t grid, uint32_t *queue,

@@ -260,7 +260,8 @@ namespace {
 
 // One sub-batch: blobs [off, off + nb) of working set ws.
 void proofs_range(vkzg_prover *p, WorkSet &ws, uint32_t off, uint32_t nb, uint8_t *d_proofs,
-                  const uint8_t *d_blobs, hipStream_t s, StageClock &clk, const Tuning &tn) {
+                  const uint8_t *d_blobs, hipStream_t s, StageClock &clk, const Tuning &tn,
+                  size_t blob_stride = kBlobBytes, Fr *coeffs = nullptr) {
     uint32_t *err = ws.err + off;
     uint32_t *queue = ws.queue + off;
     Fr *scal = ws.scal + (size_t)off * 8192;
@@ -270,7 +271,8 @@ void proofs_range(vkzg_prover *p, WorkSet &ws, uint32_t off, uint32_t nb, uint8_
     G1A *lad_aff = ws.lad_aff + off * npos, *lad_phi = ws.lad_phi + off * npos;
     Fp *prefix = ws.prefix + off * npos;
     G1X *buckets = ws.buckets + (size_t)off * 128 * 128, *cvals = ws.cvals + (size_t)off * 128;
-    ck(launch_fk20_scalars(scal, d_blobs, err, p->roots, p->roots_inv, queue, nb, s));
+    ck(launch_fk20_scalars(scal, d_blobs, blob_stride, err, p->roots, p->roots_inv, queue, coeffs, nb,
+                           s));
     clk.mark("fk20_scalars");
     ck(launch_phase_a(u, scal, p->table, p->c, p->W, nb, tn.groups, p->cus * tn.blocks_per_cu, queue, s));
     clk.mark("phase_a");
@@ -386,6 +388,30 @@ void enqueue_recovery(vkzg_prover *p, WorkSet &ws, StreamSet &ss, uint8_t *d_out
                         (uint64_t)(uintptr_t)d_out, (uint64_t)(uintptr_t)d_cells,
                         (uint64_t)(uintptr_t)&ws}};
     enqueue_maybe_graph(p, *graphs, key, s, clk, body);
+}
+
+// Cells (the 2x extension, as compute_cells_and_kzg_proofs returns them) and
+// proofs of nb blobs read at blob_stride: the proof pipeline also writes the
+// blob coefficients to ws.bufA, and the forward evaluation passes of the
+// recovery turn them into canonical cell bytes.  d_cells may be null.
+void enqueue_cells_and_proofs(vkzg_prover *p, WorkSet &ws, uint8_t *d_cells, uint8_t *d_proofs,
+                              const uint8_t *d_blobs, size_t blob_stride, uint32_t nb, hipStream_t s,
+                              const Tuning &tn, GraphCache *graphs) {
+    StageClock off_clock(p, s, false);
+    auto body = [&](hipStream_t q) {
+        proofs_range(p, ws, 0, nb, d_proofs, d_blobs, q, off_clock, tn, blob_stride,
+                     d_cells ? ws.bufA : nullptr);
+        if (d_cells) {
+            ck(launch_rec_pass_a(ws.bufB, ws.bufA, 1, nullptr, false, p->roots, p->roots_inv, nb, q));
+            ck(launch_rec_pass_b(d_cells, ws.bufB, 2, nullptr, nullptr, false, p->roots, p->roots_inv,
+                                 nb, q));
+        }
+    };
+    if (!graphs) return body(s);
+    const GraphKey key{{3u | ((uint64_t)(uintptr_t)d_cells << 2), nb, tuning_word(tn) | ((uint64_t)blob_stride << 32),
+                        (uint64_t)(uintptr_t)d_proofs, (uint64_t)(uintptr_t)d_blobs,
+                        (uint64_t)(uintptr_t)&ws}};
+    enqueue_maybe_graph(p, *graphs, key, s, off_clock, body);
 }
 
 bool chunk_missing_ok(const uint8_t *present, uint32_t nb) {
@@ -637,6 +663,83 @@ vkzg_result vkzg_recover_cells_batch(vkzg_prover *p, uint8_t *cells_out, const u
             ck(hipStreamSynchronize(s));
             clk.finish();
             std::memcpy(cells_out + done * kCellsBytes, p->h_out, nb * kCellsBytes);
+            done += nb;
+        }
+    } catch (HipFail &) {
+        return VKZG_ERR_GPU;
+    }
+    return VKZG_OK;
+}
+
+vkzg_result vkzg_amd_compute_cells_and_proofs(vkzg_prover *p, uint8_t *cells, uint8_t *proofs,
+                                              const uint8_t *blobs, size_t num_blobs) {
+    if (!p || !blobs || (!cells && !proofs)) return VKZG_ERR_BADARGS;
+    if (num_blobs == 0) return VKZG_OK;
+    std::lock_guard<std::mutex> lock(p->mu);
+    DeviceGuard guard(p->device);
+    reset_stage_times(p);
+    try {
+        for (size_t done = 0; done < num_blobs;) {
+            const uint32_t nb = (uint32_t)std::min<size_t>(p->max_batch, num_blobs - done);
+            hipStream_t s = p->stream;
+            std::memcpy(p->h_in, blobs + done * kBlobBytes, nb * kBlobBytes);
+            ck(hipMemcpyAsync(p->ws.blobs, p->h_in, nb * kBlobBytes, hipMemcpyHostToDevice, s));
+            const Tuning tn = tuning_for(p, nb);
+            Tuning one = tn;
+            one.splits = 1;
+            enqueue_cells_and_proofs(p, p->ws, cells ? p->ws.cells_out : nullptr, p->ws.proofs,
+                                     p->ws.blobs, kBlobBytes, nb, s, one, &p->graphs);
+            ck(hipMemcpyAsync(p->h_err, p->ws.err, nb * 4, hipMemcpyDeviceToHost, s));
+            if (cells)
+                ck(hipMemcpyAsync(p->h_out, p->ws.cells_out, nb * kCellsBytes, hipMemcpyDeviceToHost, s));
+            if (proofs)  // h_in's blobs were consumed by the copy above, in stream order
+                ck(hipMemcpyAsync(p->h_in, p->ws.proofs, nb * kProofBytes, hipMemcpyDeviceToHost, s));
+            ck(hipStreamSynchronize(s));
+            for (uint32_t b = 0; b < nb; b++)
+                if (p->h_err[b]) return VKZG_ERR_INVALID_BLOB;
+            if (cells) std::memcpy(cells + done * kCellsBytes, p->h_out, nb * kCellsBytes);
+            if (proofs) std::memcpy(proofs + done * kProofBytes, p->h_in, nb * kProofBytes);
+            done += nb;
+        }
+    } catch (HipFail &) {
+        return VKZG_ERR_GPU;
+    }
+    return VKZG_OK;
+}
+
+vkzg_result vkzg_amd_recover_cells_and_proofs(vkzg_prover *p, uint8_t *cells_out, uint8_t *proofs_out,
+                                              const uint8_t *cells, const uint8_t *cell_present,
+                                              size_t num_blobs) {
+    if (!p || !cells_out || !proofs_out || !cells || !cell_present) return VKZG_ERR_BADARGS;
+    if (num_blobs == 0) return VKZG_OK;
+    std::lock_guard<std::mutex> lock(p->mu);
+    DeviceGuard guard(p->device);
+    reset_stage_times(p);
+    try {
+        for (size_t done = 0; done < num_blobs;) {
+            const uint32_t nb = (uint32_t)std::min<size_t>(p->max_batch, num_blobs - done);
+            const uint8_t *pm = cell_present + done * 128;
+            if (!chunk_missing_ok(pm, nb)) return VKZG_ERR_BADARGS;
+            hipStream_t s = p->stream;
+            std::memcpy(p->h_in, cells + done * kCellsBytes, nb * kCellsBytes);
+            std::memcpy(p->h_present, pm, nb * 128);
+            ck(hipMemcpyAsync(p->ws.cells, p->h_in, nb * kCellsBytes, hipMemcpyHostToDevice, s));
+            ck(hipMemcpyAsync(p->ws.present, p->h_present, nb * 128, hipMemcpyHostToDevice, s));
+            const Tuning tn = tuning_for(p, nb);
+            Tuning one = tn;
+            one.splits = 1;
+            StageClock clk(p, s, false);
+            enqueue_recovery(p, p->ws, p->streams, p->ws.cells_out, p->ws.cells, p->ws.present, nb, s,
+                             clk, one, &p->graphs);
+            // The first 64 recovered cells of a blob are the blob itself (cell c
+            // holds p(w^brev13(64c+t)) and brev13(i) = 2 brev12(i) for i < 4096).
+            enqueue_cells_and_proofs(p, p->ws, nullptr, p->ws.proofs, p->ws.cells_out, kCellsBytes, nb,
+                                     s, one, &p->graphs);
+            ck(hipMemcpyAsync(p->h_out, p->ws.cells_out, nb * kCellsBytes, hipMemcpyDeviceToHost, s));
+            ck(hipMemcpyAsync(p->h_in, p->ws.proofs, nb * kProofBytes, hipMemcpyDeviceToHost, s));
+            ck(hipStreamSynchronize(s));
+            std::memcpy(cells_out + done * kCellsBytes, p->h_out, nb * kCellsBytes);
+            std::memcpy(proofs_out + done * kProofBytes, p->h_in, nb * kProofBytes);
             done += nb;
         }
     } catch (HipFail &) {
