@@ -77,14 +77,16 @@ typedef struct {
 vkzg_result vkzg_amd_prover_info(const vkzg_prover *p, vkzg_amd_info *out);
 
 /* ------------------------------------------------------------- tuning
- * Launch shape of the proof pipeline per batch-size class (<= 1, <= 4,
- * <= 16, <= 64 and more blobs).  Every tuning gives bit-identical proofs.
+ * Launch shape of the proof and recovery pipelines per batch-size class
+ * (<= 1, <= 4, <= 16, <= 64 and more blobs).  Every tuning gives
+ * bit-identical results.
  * No reference counterpart: the reference fixes one launch shape. */
 typedef struct {
     uint32_t phase_a_groups;        /* window groups (work-queue tiles) per phase-A MSM, 1-5 */
     uint32_t phase_a_blocks_per_cu; /* persistent phase-A grid in blocks per CU, 1-2 */
     uint32_t normalize_chunk;       /* points per batched inversion, 1-256 */
     uint32_t splits;                /* sub-streams a batch is spread over, 1-4 */
+    uint32_t recovery_cols;         /* recovery NTT_8192 sub-transforms per pass block: 2, 4, 8 or 16 */
 } vkzg_amd_tuning;
 
 /* The tuning used for a batch of `num_blobs` blobs. */
@@ -156,6 +158,10 @@ int vkzg_amd_last_stage_times(vkzg_prover *p, float *ms, const char **names, int
  * mod r first), computed by the device Montgomery code. */
 vkzg_result vkzg_amd_debug_fp_mul(vkzg_prover *p, uint32_t *out, const uint32_t *a,
                                   const uint32_t *b, size_t n);
+/* n inverses mod p (0 -> 0) of canonical Fp vectors by the device's
+ * binary-GCD inversion, cross-checked in the kernel against the Fermat power
+ * (all-ones output on disagreement). */
+vkzg_result vkzg_amd_debug_fp_inv(vkzg_prover *p, uint32_t *out, const uint32_t *a, size_t n);
 vkzg_result vkzg_amd_debug_fr_mul(vkzg_prover *p, uint32_t *out, const uint32_t *a,
                                   const uint32_t *b, size_t n);
 

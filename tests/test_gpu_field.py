@@ -38,3 +38,14 @@ def test_fr_mul_random_edges_and_unreduced_inputs():
     got = vkzg_amd.debug_fr_mul([limbs(x, 8) for x in a], [limbs(y, 8) for y in b])
     for x, y, g in zip(a, b, got):
         assert value(g) == (x % R) * (y % R) % R, (hex(x), hex(y))
+
+
+def test_fp_inverse_binary_gcd():
+    """The device binary-GCD inverse against Python (and, inside the kernel,
+    against the Fermat power): random values, 0, 1, p - 1, powers of two."""
+    rng = random.Random(13)
+    edge = [0, 1, 2, 3, P - 1, P - 2, (P - 1) // 2, (P + 1) // 2] + [1 << k for k in range(0, 381, 7)]
+    a = edge + [rng.randrange(P) for _ in range(4000)]
+    got = vkzg_amd.debug_fp_inv([limbs(x, 12) for x in a])
+    for x, g in zip(a, got):
+        assert value(g) == (pow(x, -1, P) if x else 0), hex(x)

@@ -40,9 +40,25 @@ def test_splits_and_odd_sizes_bit_exact(gpu_prover, vectors, restore_tuning):
     assert gpu_prover.compute_cell_kzg_proofs_batch(blobs, 37) == want
 
 
+@pytest.mark.parametrize("cols", [2, 4, 8, 16])
+def test_recovery_cols_bit_exact(gpu_prover, vectors, cells_of, restore_tuning, cols):
+    rng = random.Random(cols)
+    names = [NAMES[i % 3] for i in range(3)]
+    cells = b"".join(cells_of(n) for n in names)
+    present = bytearray(128 * 3)
+    for b in range(3):
+        for c in rng.sample(range(128), 64 + 20 * b):
+            present[128 * b + c] = 1
+    gpu_prover.set_tuning(3, recovery_cols=cols)
+    assert gpu_prover.recover_cells_batch(cells, bytes(present), 3) == cells
+    out, proofs = gpu_prover.compute_cells_and_kzg_proofs_batch(b"".join(vectors.blob(n) for n in names), 3)
+    assert out == cells and proofs == b"".join(vectors.proofs(n) for n in names)
+
+
 def test_tuning_rejects_out_of_range(gpu_prover, restore_tuning):
     for bad in (dict(phase_a_groups=0), dict(phase_a_groups=6), dict(phase_a_blocks_per_cu=3),
-                dict(normalize_chunk=0), dict(normalize_chunk=257), dict(splits=5)):
+                dict(normalize_chunk=0), dict(normalize_chunk=257), dict(splits=5),
+                dict(recovery_cols=3), dict(recovery_cols=32)):
         with pytest.raises(vkzg_amd.VkzgError) as e:
             gpu_prover.set_tuning(3, **bad)
         assert e.value.code == vkzg_amd.ERR_BADARGS

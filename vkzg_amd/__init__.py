@@ -331,6 +331,16 @@ def debug_fr_mul(a_limbs, b_limbs, prover=None):
     return _debug_mul(a_limbs, b_limbs, 8, prover, load().vkzg_amd_debug_fr_mul)
 
 
+def debug_fp_inv(a_limbs, prover=None):
+    """Inverses mod p (0 -> 0) of canonical Fp values as 12-limb tuples (test hook);
+    an all-ones tuple marks a disagreement with the Fermat inverse."""
+    n = len(a_limbs)
+    A = (ctypes.c_uint32 * (n * 12))(*[x for v in a_limbs for x in v])
+    C = (ctypes.c_uint32 * (n * 12))()
+    _check(load().vkzg_amd_debug_fp_inv(prover.handle if prover else None, C, A, n))
+    return [tuple(C[i * 12:(i + 1) * 12]) for i in range(n)]
+
+
 def _debug_mul(a, b, limbs, prover, fn):
     n = len(a)
     A = (ctypes.c_uint32 * (n * limbs))(*[x for v in a for x in v])

@@ -29,6 +29,7 @@ EXT_SYMBOLS = [
     "vkzg_amd_set_stage_timing",
     "vkzg_amd_last_stage_times",
     "vkzg_amd_debug_fp_mul",
+    "vkzg_amd_debug_fp_inv",
     "vkzg_amd_debug_fr_mul",
     "vkzg_amd_server_options_default",
     "vkzg_amd_server_new",
@@ -74,6 +75,7 @@ class Tuning(ctypes.Structure):
         ("phase_a_blocks_per_cu", ctypes.c_uint32),
         ("normalize_chunk", ctypes.c_uint32),
         ("splits", ctypes.c_uint32),
+        ("recovery_cols", ctypes.c_uint32),
     ]
 
 
@@ -164,6 +166,8 @@ def load():
     L.vkzg_amd_set_tuning.argtypes = [vp, sz, ctypes.POINTER(Tuning)]
     L.vkzg_amd_autotune.restype = ctypes.c_int
     L.vkzg_amd_autotune.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
+    L.vkzg_amd_debug_fp_inv.restype = ctypes.c_int
+    L.vkzg_amd_debug_fp_inv.argtypes = [vp, vp, vp, sz]
     for name in ("vkzg_amd_debug_fp_mul", "vkzg_amd_debug_fr_mul"):
         f = getattr(L, name)
         f.restype = ctypes.c_int

@@ -18,6 +18,7 @@
 #include <cstdint>
 
 #include "constants.h"
+#include "inverse.h"
 
 #define VKZG_DEV __device__ __forceinline__
 
@@ -532,7 +533,25 @@ VKZG_DEV Fp fp_pow_const(const Fp &a, const uint32_t (&e)[NE]) {
     }
     return acc;
 }
-VKZG_DEV Fp fp_inv(const Fp &a) { return fp_pow_const<12>(a, cfp::P_MINUS_2); }
+VKZG_DEV Fp fp_inv_fermat(const Fp &a) { return fp_pow_const<12>(a, cfp::P_MINUS_2); }
+
+// Inverse of a Montgomery value xR (0 -> 0): the binary GCD of inverse.h gives
+// the plain inverse (xR)^-1 as a signed value in (-31p, 31p); adding 32p makes
+// it non-negative and a Montgomery product by R^3 turns it into x^-1 R.
+VKZG_DEV Fp fp_inv(const Fp &a) {
+    if (fp_is_zero(a)) return fp_zero();
+    int32_t v[kFpLimbs];
+    if (!bgcd::inverse(v, a.v)) return fp_inv_fermat(a);
+    Fp t;
+    int64_t c = 0;
+#pragma unroll
+    for (int i = 0; i < kFpLimbs; i++) {
+        c += (int64_t)v[i] + cfp::P32[i];
+        t.v[i] = (uint32_t)c & kFpMask;
+        c >>= 28;
+    }
+    return fp_mul(t, fp_const(cfp::R3_M));
+}
 
 // canonical value > (p-1)/2 (the sign flag of the compressed encoding)
 VKZG_DEV bool fp_is_lex_largest(const Fp &canonical) {

@@ -75,17 +75,21 @@ hipError_t launch_finalize(uint8_t *proofs, const G1X *cvals, uint32_t nblobs, h
 hipError_t launch_rec_vanishing(Fr *zd, Fr *izc, const uint8_t *present, const Fr *roots,
                                 uint32_t nblobs, hipStream_t s);
 // in_mode 0: `in` is cell bytes, multiplied by zd on load; 1: `in` is Fr.
+// Four-step NTT_8192 passes; `cols` (2, 4, 8 or 16) sub-transforms per block.
+// Pass B out_mode 0: multiply by mul_table[k], store Fr; 1: multiply by
+// izc[k % 128], store Fr; 2: store canonical big-endian bytes at the cell wire
+// position.
 hipError_t launch_rec_pass_a(Fr *out, const void *in, int in_mode, const Fr *zd, bool inverse,
-                             const Fr *roots, const Fr *roots_inv, uint32_t nblobs, hipStream_t s);
-// out_mode 0: multiply by mul_table[k], store Fr; 1: multiply by izc[k % 128],
-// store Fr; 2: store canonical big-endian bytes at the cell wire position.
+                             const Fr *roots, const Fr *roots_inv, uint32_t nblobs, uint32_t cols,
+                             hipStream_t s);
 hipError_t launch_rec_pass_b(void *out, const Fr *in, int out_mode, const Fr *mul_table,
                              const Fr *izc, bool inverse, const Fr *roots, const Fr *roots_inv,
-                             uint32_t nblobs, hipStream_t s);
+                             uint32_t nblobs, uint32_t cols, hipStream_t s);
 
 // ------------------------------------------------------------ test hooks
 hipError_t launch_debug_fp_mul(uint32_t *out, const uint32_t *a, const uint32_t *b, uint32_t n,
                                hipStream_t s);
+hipError_t launch_debug_fp_inv(uint32_t *out, const uint32_t *a, uint32_t n, hipStream_t s);
 hipError_t launch_debug_fr_mul(uint32_t *out, const uint32_t *a, const uint32_t *b, uint32_t n,
                                hipStream_t s);
 

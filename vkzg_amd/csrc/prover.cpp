@@ -253,7 +253,8 @@ void default_tunings(vkzg_prover *p) {
 
 bool tuning_valid(const Tuning &t) {
     return t.groups >= 1 && t.groups <= kMaxGroups && t.blocks_per_cu >= 1 && t.blocks_per_cu <= 2 &&
-           t.norm_chunk >= 1 && t.norm_chunk <= 256 && t.splits >= 1 && t.splits <= (uint32_t)kMaxSplits;
+           t.norm_chunk >= 1 && t.norm_chunk <= 256 && t.splits >= 1 && t.splits <= (uint32_t)kMaxSplits &&
+           (t.rec_cols == 2 || t.rec_cols == 4 || t.rec_cols == 8 || t.rec_cols == 16);
 }
 
 namespace {
@@ -293,31 +294,31 @@ void proofs_range(vkzg_prover *p, WorkSet &ws, uint32_t off, uint32_t nb, uint8_
 
 void recovery_range(vkzg_prover *p, WorkSet &ws, uint32_t off, uint32_t nb, uint8_t *d_out,
                     const uint8_t *d_cells, const uint8_t *d_present, hipStream_t s,
-                    StageClock &clk) {
+                    StageClock &clk, uint32_t cols) {
     Fr *zd = ws.zd + (size_t)off * 128, *izc = ws.izc + (size_t)off * 128;
     Fr *A = ws.bufA + (size_t)off * 8192, *B = ws.bufB + (size_t)off * 8192;
     const Fr *r = p->roots, *ri = p->roots_inv;
     ck(launch_rec_vanishing(zd, izc, d_present, r, nb, s));
     clk.mark("vanishing");
     // (E * Z) -> coefficients, times 7^k / 8192
-    ck(launch_rec_pass_a(A, d_cells, 0, zd, true, r, ri, nb, s));
+    ck(launch_rec_pass_a(A, d_cells, 0, zd, true, r, ri, nb, cols, s));
     clk.mark("inv_pass_a");
-    ck(launch_rec_pass_b(B, A, 0, p->shift_fwd, nullptr, true, r, ri, nb, s));
+    ck(launch_rec_pass_b(B, A, 0, p->shift_fwd, nullptr, true, r, ri, nb, cols, s));
     clk.mark("inv_pass_b");
     // coset evaluations, divided by Z on the coset
-    ck(launch_rec_pass_a(A, B, 1, nullptr, false, r, ri, nb, s));
+    ck(launch_rec_pass_a(A, B, 1, nullptr, false, r, ri, nb, cols, s));
     clk.mark("coset_pass_a");
-    ck(launch_rec_pass_b(B, A, 1, nullptr, izc, false, r, ri, nb, s));
+    ck(launch_rec_pass_b(B, A, 1, nullptr, izc, false, r, ri, nb, cols, s));
     clk.mark("coset_pass_b");
     // coset interpolation: coefficients of P, times 7^-k / 8192
-    ck(launch_rec_pass_a(A, B, 1, nullptr, true, r, ri, nb, s));
+    ck(launch_rec_pass_a(A, B, 1, nullptr, true, r, ri, nb, cols, s));
     clk.mark("coset_inv_pass_a");
-    ck(launch_rec_pass_b(B, A, 0, p->shift_inv, nullptr, true, r, ri, nb, s));
+    ck(launch_rec_pass_b(B, A, 0, p->shift_inv, nullptr, true, r, ri, nb, cols, s));
     clk.mark("coset_inv_pass_b");
     // evaluate P on the whole domain, write cells
-    ck(launch_rec_pass_a(A, B, 1, nullptr, false, r, ri, nb, s));
+    ck(launch_rec_pass_a(A, B, 1, nullptr, false, r, ri, nb, cols, s));
     clk.mark("eval_pass_a");
-    ck(launch_rec_pass_b(d_out, A, 2, nullptr, nullptr, false, r, ri, nb, s));
+    ck(launch_rec_pass_b(d_out, A, 2, nullptr, nullptr, false, r, ri, nb, cols, s));
     clk.mark("eval_pass_b");
 }
 
@@ -352,7 +353,7 @@ void split_over_streams(StreamSet &ss, uint32_t nb, uint32_t splits, hipStream_t
 namespace {
 uint64_t tuning_word(const Tuning &tn) {
     return (uint64_t)tn.splits | ((uint64_t)tn.groups << 8) | ((uint64_t)tn.blocks_per_cu << 16) |
-           ((uint64_t)tn.norm_chunk << 24);
+           ((uint64_t)tn.norm_chunk << 24) | ((uint64_t)tn.rec_cols << 40);
 }
 }  // namespace
 
@@ -380,11 +381,11 @@ void enqueue_recovery(vkzg_prover *p, WorkSet &ws, StreamSet &ss, uint8_t *d_out
         split_over_streams(ss, nb, tn.splits, s0, clk, [&](uint32_t k, uint32_t off, uint32_t n, hipStream_t q) {
             recovery_range(p, ws, off, n, d_out + (size_t)off * kCellsBytes,
                            d_cells + (size_t)off * kCellsBytes, d_present + (size_t)off * 128, q,
-                           k == 0 ? clk : off_clock);
+                           k == 0 ? clk : off_clock, tn.rec_cols);
         });
     };
     if (!graphs) return body(s);
-    const GraphKey key{{2u | ((uint64_t)(uintptr_t)d_present << 2), nb, tn.splits,
+    const GraphKey key{{2u | ((uint64_t)(uintptr_t)d_present << 2), nb, tuning_word(tn),
                         (uint64_t)(uintptr_t)d_out, (uint64_t)(uintptr_t)d_cells,
                         (uint64_t)(uintptr_t)&ws}};
     enqueue_maybe_graph(p, *graphs, key, s, clk, body);
@@ -402,9 +403,10 @@ void enqueue_cells_and_proofs(vkzg_prover *p, WorkSet &ws, uint8_t *d_cells, uin
         proofs_range(p, ws, 0, nb, d_proofs, d_blobs, q, off_clock, tn, blob_stride,
                      d_cells ? ws.bufA : nullptr);
         if (d_cells) {
-            ck(launch_rec_pass_a(ws.bufB, ws.bufA, 1, nullptr, false, p->roots, p->roots_inv, nb, q));
+            ck(launch_rec_pass_a(ws.bufB, ws.bufA, 1, nullptr, false, p->roots, p->roots_inv, nb,
+                                 tn.rec_cols, q));
             ck(launch_rec_pass_b(d_cells, ws.bufB, 2, nullptr, nullptr, false, p->roots, p->roots_inv,
-                                 nb, q));
+                                 nb, tn.rec_cols, q));
         }
     };
     if (!graphs) return body(s);
@@ -482,6 +484,52 @@ void autotune(vkzg_prover *p) {
         for (uint32_t ch : {8u, 16u, 32u, 64u}) { Tuning t = best; t.norm_chunk = ch; consider(t); }
         if (nb >= 8) { Tuning t = best; t.splits = 2; consider(t); }
         p->tune[c] = best;
+    }
+    // Recovery: NTT sub-transforms per pass block, on random cells with the
+    // first 64 of 128 present.
+    for (size_t k = 0; k < (size_t)nmax * kCellsBytes; k += 8) {
+        x ^= x << 13;
+        x ^= x >> 7;
+        x ^= x << 17;
+        std::memcpy(p->h_in + k, &x, 8);
+    }
+    for (uint32_t b = 0; b < nmax; b++)
+        for (uint32_t c = 0; c < 128; c++) p->h_present[128 * b + c] = c < 64 ? 1 : 0;
+    ck(hipMemcpyAsync(p->ws.cells, p->h_in, (size_t)nmax * kCellsBytes, hipMemcpyHostToDevice, s));
+    ck(hipMemcpyAsync(p->ws.present, p->h_present, (size_t)nmax * 128, hipMemcpyHostToDevice, s));
+    auto time_rec = [&](uint32_t nb, const Tuning &tn) {
+        StageClock clk(p, s, false);
+        float best = 1e30f;
+        for (int rep = 0; rep < 4; rep++) {
+            ck(hipEventRecord(e0, s));
+            enqueue_recovery(p, p->ws, p->streams, p->ws.cells_out, p->ws.cells, p->ws.present, nb, s,
+                             clk, tn, nullptr);
+            ck(hipEventRecord(e1, s));
+            ck(hipEventSynchronize(e1));
+            float ms = 0;
+            ck(hipEventElapsedTime(&ms, e0, e1));
+            if (rep > 0 && ms < best) best = ms;
+        }
+        return best;
+    };
+    for (int c = 0; c < kTuneClasses; c++) {
+        const uint32_t nb = std::min<uint32_t>(kTuneClassTop[c], nmax);
+        if (c > 0 && kTuneClassTop[c - 1] >= nmax) {
+            p->tune[c].rec_cols = p->tune[c - 1].rec_cols;
+            continue;
+        }
+        Tuning t = p->tune[c];
+        float best_ms = 1e30f;
+        uint32_t best_cols = t.rec_cols;
+        for (uint32_t cols : {2u, 4u, 8u, 16u}) {
+            t.rec_cols = cols;
+            const float ms = time_rec(nb, t);
+            if (ms < best_ms * 0.98f) {
+                best_ms = ms;
+                best_cols = cols;
+            }
+        }
+        p->tune[c].rec_cols = best_cols;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -814,6 +862,7 @@ vkzg_result vkzg_amd_get_tuning(const vkzg_prover *p, size_t num_blobs, vkzg_amd
     out->phase_a_blocks_per_cu = t.blocks_per_cu;
     out->normalize_chunk = t.norm_chunk;
     out->splits = t.splits;
+    out->recovery_cols = t.rec_cols;
     return VKZG_OK;
 }
 
@@ -824,6 +873,7 @@ vkzg_result vkzg_amd_set_tuning(vkzg_prover *p, size_t num_blobs, const vkzg_amd
     t.blocks_per_cu = in->phase_a_blocks_per_cu;
     t.norm_chunk = in->normalize_chunk;
     t.splits = in->splits;
+    t.rec_cols = in->recovery_cols;
     if (!tuning_valid(t)) return VKZG_ERR_BADARGS;
     std::lock_guard<std::mutex> lock(p->mu);
     p->tune[tune_class((uint32_t)std::min<size_t>(num_blobs, 0xffffffffu))] = t;
@@ -879,8 +929,10 @@ int vkzg_amd_last_stage_times(vkzg_prover *p, float *ms, const char **names, int
     return n;
 }
 
-static vkzg_result debug_mul(vkzg_prover *p, uint32_t *out, const uint32_t *a, const uint32_t *b,
-                             size_t n, int limbs) {
+// op 0: Fp product, 1: Fr product, 2: Fp inverse of a (b unused).
+static vkzg_result debug_op(vkzg_prover *p, uint32_t *out, const uint32_t *a, const uint32_t *b,
+                            size_t n, int op) {
+    const int limbs = op == 1 ? 8 : 12;
     if (!out || !a || !b) return VKZG_ERR_BADARGS;
     if (n == 0) return VKZG_OK;
     // A prover is optional: without one the current device and its null
@@ -901,9 +953,11 @@ static vkzg_result debug_mul(vkzg_prover *p, uint32_t *out, const uint32_t *a, c
         ck(hipMalloc(&dc, bytes));
         ck(hipMemcpyAsync(da, a, bytes, hipMemcpyHostToDevice, stream));
         ck(hipMemcpyAsync(db, b, bytes, hipMemcpyHostToDevice, stream));
-        if (limbs == 12)
+        if (op == 0)
             ck(launch_debug_fp_mul((uint32_t *)dc, (const uint32_t *)da, (const uint32_t *)db,
                                    (uint32_t)n, stream));
+        else if (op == 2)
+            ck(launch_debug_fp_inv((uint32_t *)dc, (const uint32_t *)da, (uint32_t)n, stream));
         else
             ck(launch_debug_fr_mul((uint32_t *)dc, (const uint32_t *)da, (const uint32_t *)db,
                                    (uint32_t)n, stream));
@@ -920,12 +974,16 @@ static vkzg_result debug_mul(vkzg_prover *p, uint32_t *out, const uint32_t *a, c
 
 vkzg_result vkzg_amd_debug_fp_mul(vkzg_prover *p, uint32_t *out, const uint32_t *a,
                                   const uint32_t *b, size_t n) {
-    return debug_mul(p, out, a, b, n, 12);
+    return debug_op(p, out, a, b, n, 0);
+}
+
+vkzg_result vkzg_amd_debug_fp_inv(vkzg_prover *p, uint32_t *out, const uint32_t *a, size_t n) {
+    return debug_op(p, out, a, a, n, 2);
 }
 
 vkzg_result vkzg_amd_debug_fr_mul(vkzg_prover *p, uint32_t *out, const uint32_t *a,
                                   const uint32_t *b, size_t n) {
-    return debug_mul(p, out, a, b, n, 8);
+    return debug_op(p, out, a, b, n, 1);
 }
 
 }  // extern "C"

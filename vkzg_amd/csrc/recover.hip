@@ -58,23 +58,27 @@ __global__ void __launch_bounds__(512)
     else zd[blob * 128 + k] = prod;
 }
 
-constexpr int kCols = 8;       // sub-transforms per block
-constexpr int kThreads = 256;
+// Threads of a pass block: one per butterfly of a stage, at most 256.
+template <int COLS, int N>
+constexpr int pass_threads() {
+    return COLS * (N / 2) < 256 ? COLS * (N / 2) : 256;
+}
 
 // Pass A: for each n1 < 64, DFT_128 over n2 of x[n1 + 64 n2], times
-// omega^(+-n1 k2), stored at y[n1 * 128 + k2].  Block = 8 consecutive n1.
-template <int IN_MODE, bool INVERSE>
-__global__ void __launch_bounds__(kThreads)
+// omega^(+-n1 k2), stored at y[n1 * 128 + k2].  Block = COLS consecutive n1.
+template <int IN_MODE, bool INVERSE, int COLS>
+__global__ void __launch_bounds__((pass_threads<COLS, 128>()))
     k_rec_pass_a(Fr *out, const void *in, const Fr *zd, const Fr *roots_dir) {
     constexpr int n = 128;
-    __shared__ uint32_t lds[8 * kCols * n];
-    constexpr int pitch = kCols * n;
+    constexpr int nthreads = pass_threads<COLS, n>();
+    __shared__ uint32_t lds[8 * COLS * n];
+    constexpr int pitch = COLS * n;
     const uint32_t blob = blockIdx.y;
-    const uint32_t n1_0 = blockIdx.x * kCols;
+    const uint32_t n1_0 = blockIdx.x * COLS;
     const int tid = threadIdx.x;
 #pragma unroll 1
-    for (int e = tid; e < kCols * n; e += kThreads) {
-        const int c = e % kCols, n2 = e / kCols;
+    for (int e = tid; e < COLS * n; e += nthreads) {
+        const int c = e % COLS, n2 = e / COLS;
         const uint32_t idx = n1_0 + c + 64u * n2;  // natural evaluation index
         Fr v;
         if (IN_MODE == 0) {
@@ -91,9 +95,9 @@ __global__ void __launch_bounds__(kThreads)
         lds_store_fr(lds, pitch, c * n + (int)brev_bits(n2, 7), v);
     }
     __syncthreads();
-    lds_ntt_dit<7>(lds, pitch, kCols, roots_dir, tid, kThreads);
+    lds_ntt_dit<7>(lds, pitch, COLS, roots_dir, tid, nthreads);
 #pragma unroll 1
-    for (int e = tid; e < kCols * n; e += kThreads) {
+    for (int e = tid; e < COLS * n; e += nthreads) {
         const int k2 = e % n, c = e / n;
         const uint32_t n1 = n1_0 + c;
         const Fr v = fr_mul(lds_load_fr(lds, pitch, c * n + k2), roots_dir[(n1 * k2) & 8191u]);
@@ -102,27 +106,28 @@ __global__ void __launch_bounds__(kThreads)
 }
 
 // Pass B: for each k2 < 128, DFT_64 over n1 of y[n1 * 128 + k2], giving
-// X[k2 + 128 k1].  Block = 8 consecutive k2.
-template <int OUT_MODE, bool INVERSE>
-__global__ void __launch_bounds__(kThreads)
+// X[k2 + 128 k1].  Block = COLS consecutive k2.
+template <int OUT_MODE, bool INVERSE, int COLS>
+__global__ void __launch_bounds__((pass_threads<COLS, 64>()))
     k_rec_pass_b(void *out, const Fr *in, const Fr *mul_table, const Fr *izc, const Fr *roots_dir) {
     constexpr int n = 64;
-    __shared__ uint32_t lds[8 * kCols * n];
-    constexpr int pitch = kCols * n;
+    constexpr int nthreads = pass_threads<COLS, n>();
+    __shared__ uint32_t lds[8 * COLS * n];
+    constexpr int pitch = COLS * n;
     const uint32_t blob = blockIdx.y;
-    const uint32_t k2_0 = blockIdx.x * kCols;
+    const uint32_t k2_0 = blockIdx.x * COLS;
     const int tid = threadIdx.x;
 #pragma unroll 1
-    for (int e = tid; e < kCols * n; e += kThreads) {
-        const int c = e % kCols, n1 = e / kCols;
+    for (int e = tid; e < COLS * n; e += nthreads) {
+        const int c = e % COLS, n1 = e / COLS;
         const Fr v = in[(size_t)blob * 8192 + n1 * 128 + k2_0 + c];
         lds_store_fr(lds, pitch, c * n + (int)brev_bits(n1, 6), v);
     }
     __syncthreads();
-    lds_ntt_dit<6>(lds, pitch, kCols, roots_dir, tid, kThreads);
+    lds_ntt_dit<6>(lds, pitch, COLS, roots_dir, tid, nthreads);
 #pragma unroll 1
-    for (int e = tid; e < kCols * n; e += kThreads) {
-        const int c = e % kCols, k1 = e / kCols;
+    for (int e = tid; e < COLS * n; e += nthreads) {
+        const int c = e % COLS, k1 = e / COLS;
         const uint32_t k = k2_0 + c + 128u * k1;  // natural output index
         Fr v = lds_load_fr(lds, pitch, c * n + k1);
         if (OUT_MODE == 0) {
@@ -141,6 +146,36 @@ __global__ void __launch_bounds__(kThreads)
     }
 }
 
+template <int IN_MODE, bool INV, int COLS>
+void launch_a(Fr *out, const void *in, const Fr *zd, const Fr *r, uint32_t nblobs, hipStream_t s) {
+    hipLaunchKernelGGL((k_rec_pass_a<IN_MODE, INV, COLS>), dim3(64 / COLS, nblobs),
+                       dim3(pass_threads<COLS, 128>()), 0, s, out, in, zd, r);
+}
+
+template <int OUT_MODE, bool INV, int COLS>
+void launch_b(void *out, const Fr *in, const Fr *mul_table, const Fr *izc, const Fr *r, uint32_t nblobs,
+              hipStream_t s) {
+    hipLaunchKernelGGL((k_rec_pass_b<OUT_MODE, INV, COLS>), dim3(128 / COLS, nblobs),
+                       dim3(pass_threads<COLS, 64>()), 0, s, out, in, mul_table, izc, r);
+}
+
+template <int COLS>
+void pass_a_cols(Fr *out, const void *in, int in_mode, const Fr *zd, bool inverse, const Fr *r,
+                 uint32_t nblobs, hipStream_t s) {
+    if (in_mode == 0 && inverse) launch_a<0, true, COLS>(out, in, zd, r, nblobs, s);
+    else if (in_mode == 0) launch_a<0, false, COLS>(out, in, zd, r, nblobs, s);
+    else if (inverse) launch_a<1, true, COLS>(out, in, zd, r, nblobs, s);
+    else launch_a<1, false, COLS>(out, in, zd, r, nblobs, s);
+}
+
+template <int COLS>
+void pass_b_cols(void *out, const Fr *in, int out_mode, const Fr *mul_table, const Fr *izc, const Fr *r,
+                 uint32_t nblobs, hipStream_t s) {
+    if (out_mode == 0) launch_b<0, true, COLS>(out, in, mul_table, izc, r, nblobs, s);
+    else if (out_mode == 1) launch_b<1, false, COLS>(out, in, mul_table, izc, r, nblobs, s);
+    else launch_b<2, false, COLS>(out, in, mul_table, izc, r, nblobs, s);
+}
+
 }  // namespace
 
 hipError_t launch_rec_vanishing(Fr *zd, Fr *izc, const uint8_t *present, const Fr *roots,
@@ -150,31 +185,28 @@ hipError_t launch_rec_vanishing(Fr *zd, Fr *izc, const uint8_t *present, const F
 }
 
 hipError_t launch_rec_pass_a(Fr *out, const void *in, int in_mode, const Fr *zd, bool inverse,
-                             const Fr *roots, const Fr *roots_inv, uint32_t nblobs, hipStream_t s) {
-    const dim3 grid(64 / kCols, nblobs), block(kThreads);
+                             const Fr *roots, const Fr *roots_inv, uint32_t nblobs, uint32_t cols,
+                             hipStream_t s) {
     const Fr *r = inverse ? roots_inv : roots;
-    if (in_mode == 0 && inverse)
-        hipLaunchKernelGGL((k_rec_pass_a<0, true>), grid, block, 0, s, out, in, zd, r);
-    else if (in_mode == 0)
-        hipLaunchKernelGGL((k_rec_pass_a<0, false>), grid, block, 0, s, out, in, zd, r);
-    else if (inverse)
-        hipLaunchKernelGGL((k_rec_pass_a<1, true>), grid, block, 0, s, out, in, zd, r);
-    else
-        hipLaunchKernelGGL((k_rec_pass_a<1, false>), grid, block, 0, s, out, in, zd, r);
+    switch (cols) {
+        case 4: pass_a_cols<4>(out, in, in_mode, zd, inverse, r, nblobs, s); break;
+        case 16: pass_a_cols<16>(out, in, in_mode, zd, inverse, r, nblobs, s); break;
+        case 2: pass_a_cols<2>(out, in, in_mode, zd, inverse, r, nblobs, s); break;
+        default: pass_a_cols<8>(out, in, in_mode, zd, inverse, r, nblobs, s); break;
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_rec_pass_b(void *out, const Fr *in, int out_mode, const Fr *mul_table,
                              const Fr *izc, bool inverse, const Fr *roots, const Fr *roots_inv,
-                             uint32_t nblobs, hipStream_t s) {
-    const dim3 grid(128 / kCols, nblobs), block(kThreads);
+                             uint32_t nblobs, uint32_t cols, hipStream_t s) {
     const Fr *r = inverse ? roots_inv : roots;
-    if (out_mode == 0)
-        hipLaunchKernelGGL((k_rec_pass_b<0, true>), grid, block, 0, s, out, in, mul_table, izc, r);
-    else if (out_mode == 1)
-        hipLaunchKernelGGL((k_rec_pass_b<1, false>), grid, block, 0, s, out, in, mul_table, izc, r);
-    else
-        hipLaunchKernelGGL((k_rec_pass_b<2, false>), grid, block, 0, s, out, in, mul_table, izc, r);
+    switch (cols) {
+        case 4: pass_b_cols<4>(out, in, out_mode, mul_table, izc, r, nblobs, s); break;
+        case 16: pass_b_cols<16>(out, in, out_mode, mul_table, izc, r, nblobs, s); break;
+        case 2: pass_b_cols<2>(out, in, out_mode, mul_table, izc, r, nblobs, s); break;
+        default: pass_b_cols<8>(out, in, out_mode, mul_table, izc, r, nblobs, s); break;
+    }
     return hipGetLastError();
 }
 

@@ -80,7 +80,7 @@ VKZG_DEV void g1x_pair_to_affine(G1A &ra, G1A &rb, const G1X &a, const G1X &b) {
 
 // out[a] = C+[a] + C-[a], out[a+64] = C+[a] - C-[a] (k_combine_split.comp),
 // compressed and written at brev7(k) (k_compress_proofs.comp).
-__global__ void k_finalize(uint8_t *proofs, const G1X *cvals, uint32_t nblobs) {
+__global__ void __launch_bounds__(64) k_finalize(uint8_t *proofs, const G1X *cvals, uint32_t nblobs) {
     __builtin_amdgcn_s_setprio(3);  // dependent inversion chain: issue first
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= nblobs * 64) return;

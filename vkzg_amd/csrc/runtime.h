@@ -28,6 +28,7 @@ struct Tuning {
     uint32_t blocks_per_cu = 2;  // persistent phase-A grid = CUs x this, 1..2
     uint32_t norm_chunk = 32;    // points per batched inversion in the normalisation
     uint32_t splits = 1;         // sub-streams, 1..kMaxSplits
+    uint32_t rec_cols = 8;       // NTT_8192 sub-transforms per pass block: 2, 4, 8 or 16
 };
 
 // Batch-size classes: <= 1, <= 4, <= 16, <= 64 and more blobs.

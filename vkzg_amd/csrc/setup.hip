@@ -114,6 +114,23 @@ __global__ void k_debug_fp_mul(uint32_t *out, const uint32_t *a, const uint32_t 
     for (int k = 0; k < 12; k++) out[12 * i + k] = agree ? rw[k] : 0xffffffffu;
 }
 
+// fp_inv on canonical inputs (b is ignored); a nonzero result must also
+// agree with the Fermat inverse, else all ones.
+__global__ void k_debug_fp_inv(uint32_t *out, const uint32_t *a, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t aw[12];
+#pragma unroll
+    for (int k = 0; k < 12; k++) aw[k] = a[12 * i + k];
+    const Fp xm = fp_from_canonical(fp_from_words(aw));
+    const Fp inv = fp_inv(xm);
+    const bool agree = fp_eq(inv, fp_inv_fermat(xm));
+    uint32_t rw[12];
+    fp_to_words(rw, fp_to_canonical(inv));
+#pragma unroll
+    for (int k = 0; k < 12; k++) out[12 * i + k] = agree ? rw[k] : 0xffffffffu;
+}
+
 __global__ void k_debug_fr_mul(uint32_t *out, const uint32_t *a, const uint32_t *b, uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -153,6 +170,11 @@ hipError_t launch_phase_b_scalars(Fr *out, const Fr *roots, hipStream_t s) {
 hipError_t launch_debug_fp_mul(uint32_t *out, const uint32_t *a, const uint32_t *b, uint32_t n,
                                hipStream_t s) {
     hipLaunchKernelGGL(k_debug_fp_mul, dim3(blocks_for(n, 64)), dim3(64), 0, s, out, a, b, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_debug_fp_inv(uint32_t *out, const uint32_t *a, uint32_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_debug_fp_inv, dim3(blocks_for(n, 64)), dim3(64), 0, s, out, a, n);
     return hipGetLastError();
 }
 
