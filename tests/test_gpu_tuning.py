@@ -22,21 +22,22 @@ def restore_tuning(gpu_prover):
         gpu_prover.set_tuning(n, **t)
 
 
-@pytest.mark.parametrize("groups", [1, 2, 4, 5])
+@pytest.mark.parametrize("groups", [1, 2, 4, 5, 10])
 def test_phase_a_groups_bit_exact(gpu_prover, vectors, restore_tuning, groups):
     blobs = b"".join(vectors.blob(n) for n in NAMES)
     want = b"".join(vectors.proofs(n) for n in NAMES)
-    for bpc, chunk in ((1, 1), (2, 64)):
-        gpu_prover.set_tuning(3, phase_a_groups=groups, phase_a_blocks_per_cu=bpc, normalize_chunk=chunk)
+    for bpc, chunk, splits in ((1, 1, 1), (2, 64, 2), (2, 8, 4)):
+        gpu_prover.set_tuning(3, phase_a_groups=groups, phase_a_blocks_per_cu=bpc, normalize_chunk=chunk,
+                              ladder_splits=splits)
         assert gpu_prover.get_tuning(3)["phase_a_groups"] == groups
-        assert gpu_prover.compute_cell_kzg_proofs_batch(blobs, 3) == want
+        assert gpu_prover.compute_cell_kzg_proofs_batch(blobs, 3) == want, (bpc, chunk, splits)
 
 
 def test_splits_and_odd_sizes_bit_exact(gpu_prover, vectors, restore_tuning):
     names = [NAMES[i % 3] for i in range(37)]
     blobs = b"".join(vectors.blob(n) for n in names)
     want = b"".join(vectors.proofs(n) for n in names)
-    gpu_prover.set_tuning(37, phase_a_groups=5, splits=3, normalize_chunk=16)
+    gpu_prover.set_tuning(37, phase_a_groups=5, splits=3, normalize_chunk=16, ladder_splits=2)
     assert gpu_prover.compute_cell_kzg_proofs_batch(blobs, 37) == want
 
 
@@ -56,9 +57,10 @@ def test_recovery_cols_bit_exact(gpu_prover, vectors, cells_of, restore_tuning, 
 
 
 def test_tuning_rejects_out_of_range(gpu_prover, restore_tuning):
-    for bad in (dict(phase_a_groups=0), dict(phase_a_groups=6), dict(phase_a_blocks_per_cu=3),
+    for bad in (dict(phase_a_groups=0), dict(phase_a_blocks_per_cu=3),
                 dict(normalize_chunk=0), dict(normalize_chunk=257), dict(splits=5),
-                dict(recovery_cols=3), dict(recovery_cols=32)):
+                dict(recovery_cols=3), dict(recovery_cols=32), dict(ladder_splits=3),
+                dict(phase_a_groups=11)):
         with pytest.raises(vkzg_amd.VkzgError) as e:
             gpu_prover.set_tuning(3, **bad)
         assert e.value.code == vkzg_amd.ERR_BADARGS
@@ -71,7 +73,7 @@ def test_autotune_keeps_results(gpu_prover_small_batch, vectors):
     assert info["autotuned"] == 1 and ms > 0
     for n in (1, 4):
         t = p.get_tuning(n)
-        assert 1 <= t["phase_a_groups"] <= 5 and 1 <= t["splits"] <= 4
+        assert 1 <= t["phase_a_groups"] <= 10 and 1 <= t["splits"] <= 4 and t["ladder_splits"] in (1, 2, 4)
     names = [NAMES[i % 3] for i in range(6)]  # chunks of 4 + 2
     blobs = b"".join(vectors.blob(n) for n in names)
     assert p.compute_cell_kzg_proofs_batch(blobs, 6) == b"".join(vectors.proofs(n) for n in names)

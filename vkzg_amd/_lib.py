@@ -76,6 +76,7 @@ class Tuning(ctypes.Structure):
         ("normalize_chunk", ctypes.c_uint32),
         ("splits", ctypes.c_uint32),
         ("recovery_cols", ctypes.c_uint32),
+        ("ladder_splits", ctypes.c_uint32),
     ]
 
 

@@ -23,7 +23,8 @@ constexpr int kColPitch = 4 * 128;        // LDS pitch of the 4 per-wave circula
 // circulant columns = 144 KiB (gfx950 has 160 KiB per CU).
 __global__ void __launch_bounds__(kScalarThreads)
     k_fk20_scalars(Fr *scal, const uint8_t *blobs, uint64_t blob_stride, uint32_t *err,
-                   const Fr *roots, const Fr *roots_inv, uint32_t *queue, Fr *coeffs) {
+                   const Fr *roots, const Fr *roots_inv, uint32_t *queue, Fr *coeffs,
+                   uint32_t splits) {
     __shared__ uint32_t poly[8 * kPolyPitch];
     __shared__ uint32_t cols[8 * kColPitch];
     const uint32_t blob = blockIdx.x;
@@ -71,7 +72,17 @@ __global__ void __launch_bounds__(kScalarThreads)
     const int wave = tid >> 6;
     const int lane = tid & 63;
     uint32_t *col = cols + wave * 128;
-    Fr *out = scal + (size_t)blob * 128 * 64;
+    Fr *out = scal + (size_t)blob * splits * 128 * 64;
+    // Scalar sets of a split ladder: segment m starts at 2^(8 start_m) u.
+    Fr mult[kMaxLadSplits];
+    mult[0] = fr_one();
+    if (splits == 2) {
+        mult[1] = fr_const(cfr::TWO_POW_64_M);
+    } else if (splits == 4) {
+        mult[1] = fr_const(cfr::TWO_POW_32_M);
+        mult[2] = fr_const(cfr::TWO_POW_64_M);
+        mult[3] = fr_const(cfr::TWO_POW_96_M);
+    }
 #pragma unroll 1
     for (int round = 0; round < 16; round++) {
         const int i = round * 4 + wave;
@@ -100,7 +111,11 @@ __global__ void __launch_bounds__(kScalarThreads)
 #pragma unroll
         for (int h = 0; h < 2; h++) {
             const int j = lane + 64 * h;
-            out[j * 64 + i] = fr_to_canonical(lds_load_fr(col, kColPitch, j));
+            const Fr v = lds_load_fr(col, kColPitch, j);
+            out[j * 64 + i] = fr_to_canonical(v);
+#pragma unroll 1
+            for (uint32_t m = 1; m < splits; m++)
+                out[(size_t)m * 8192 + j * 64 + i] = fr_to_canonical(fr_mul(v, mult[m]));
         }
         __syncthreads();
     }
@@ -110,9 +125,9 @@ __global__ void __launch_bounds__(kScalarThreads)
 
 hipError_t launch_fk20_scalars(Fr *scal, const uint8_t *blobs, uint64_t blob_stride, uint32_t *err,
                                const Fr *roots, const Fr *roots_inv, uint32_t *queue, Fr *coeffs,
-                               uint32_t nblobs, hipStream_t s) {
+                               uint32_t splits, uint32_t nblobs, hipStream_t s) {
     hipLaunchKernelGGL(k_fk20_scalars, dim3(nblobs), dim3(kScalarThreads), 0, s, scal, blobs,
-                       blob_stride, err, roots, roots_inv, queue, coeffs);
+                       blob_stride, err, roots, roots_inv, queue, coeffs, splits);
     return hipGetLastError();
 }
 

@@ -31,6 +31,9 @@ namespace vkzg_amd {
 // G1 endomorphism phi = [lambda], so each ladder needs 2^(8d) u for d < 17
 // only (17 signed 8-bit digits of a 128-bit half, the top one a carry).
 constexpr uint32_t kLadPos = 17;
+// The ladder may be split into 1, 2 or 4 segments starting at positions
+// kLadStart[M][m]; phase A then also computes 2^(8 start) u directly.
+constexpr uint32_t kMaxLadSplits = 4;
 
 struct Fr;
 struct Fp;
@@ -56,14 +59,17 @@ hipError_t launch_phase_b_scalars(Fr *out, const Fr *roots, hipStream_t s);
 // counter of the phase-A launch that follows, and, when coeffs is non-null,
 // writes the blob polynomial's monomial coefficients there (Fr[B][8192],
 // Montgomery, zero above 4096).
+// With `splits` > 1 it writes `splits` scalar sets per blob,
+// scal[blob][m][j][i] = scal[blob][0][j][i] * 2^(8 start_m) mod r.
 hipError_t launch_fk20_scalars(Fr *scal, const uint8_t *blobs, uint64_t blob_stride, uint32_t *err,
                                const Fr *roots, const Fr *roots_inv, uint32_t *queue, Fr *coeffs,
-                               uint32_t nblobs, hipStream_t s);
+                               uint32_t splits, uint32_t nblobs, hipStream_t s);
 // Persistent: `grid` blocks drain (msm, window group) tiles from queue[0].
 hipError_t launch_phase_a(G1X *part, const Fr *scal, const G1AT *table, uint32_t c, uint32_t W,
-                          uint32_t nblobs, uint32_t groups, uint32_t grid, uint32_t *queue,
+                          uint32_t nmsm, uint32_t groups, uint32_t grid, uint32_t *queue,
                           hipStream_t s);
-hipError_t launch_ladder(G1J *lad, const G1X *part, uint32_t groups, uint32_t nblobs, hipStream_t s);
+hipError_t launch_ladder(G1J *lad, const G1X *part, uint32_t groups, uint32_t splits, uint32_t nblobs,
+                         hipStream_t s);
 hipError_t launch_fold(G1J *fold, const G1J *lad, uint32_t nblobs, hipStream_t s);
 hipError_t launch_glv_phi(G1A *phi, const G1A *pts, uint64_t count, hipStream_t s);
 hipError_t launch_phase_b(G1X *buckets, const G1A *lad_aff, const G1A *lad_phi, const uint32_t *items,

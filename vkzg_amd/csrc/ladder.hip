@@ -4,24 +4,28 @@
 namespace vkzg_amd {
 
 namespace {
-// lad[blob][d][j] = 2^(8d) * u[blob][j], d < kLadPos, where u[blob][j] is the
-// sum of phase A's `groups` window-group partials part[(blob*128 + j)*groups + g].
-__global__ void k_ladder(G1J *lad, const G1X *part, uint32_t groups, uint32_t nblobs) {
+// lad[blob][d][j] = 2^(8d) * u[blob][j], d < kLadPos.  With `splits` = M
+// segments, thread (blob, m, j) starts from phase A's 2^(8 start_m) u (the
+// sum of its `groups` window-group partials part[((blob*M + m)*128 + j)*G + g])
+// and writes positions [start_m, start_{m+1}): chains of 128 / M doublings.
+__global__ void k_ladder(G1J *lad, const G1X *part, uint32_t groups, uint32_t splits, uint32_t nblobs) {
     // A long dependent chain: give these waves issue priority over the
     // throughput kernels of another sub-batch sharing the SIMD.
     __builtin_amdgcn_s_setprio(3);
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= nblobs * 128) return;
-    const uint32_t blob = g / 128, j = g % 128;
+    if (g >= nblobs * 128 * splits) return;
+    const uint32_t j = g % 128, m = (g / 128) % splits, blob = g / (128 * splits);
     G1X u = part[(size_t)g * groups];
 #pragma unroll 1
     for (uint32_t k = 1; k < groups; k++) u = g1x_add(u, part[(size_t)g * groups + k]);
     G1J p = g1j_from_xyzz(u);
+    const uint32_t seg = 16 / splits;  // positions per segment; the last one also takes d = 16
+    const uint32_t d0 = m * seg, d1 = m + 1 == splits ? kLadPos : d0 + seg;
     G1J *dst = lad + (size_t)blob * kLadPos * 128 + j;
 #pragma unroll 1
-    for (uint32_t d = 0; d < kLadPos; d++) {
+    for (uint32_t d = d0; d < d1; d++) {
         dst[d * 128] = p;
-        if (d == kLadPos - 1) break;
+        if (d + 1 == d1) break;
 #pragma unroll 1
         for (int k = 0; k < 8; k++) p = g1j_dbl(p);
     }
@@ -54,10 +58,10 @@ __global__ void k_glv_phi(G1A *phi, const G1A *pts, uint64_t count) {
 
 }  // namespace
 
-hipError_t launch_ladder(G1J *lad, const G1X *part, uint32_t groups, uint32_t nblobs,
+hipError_t launch_ladder(G1J *lad, const G1X *part, uint32_t groups, uint32_t splits, uint32_t nblobs,
                          hipStream_t s) {
-    hipLaunchKernelGGL(k_ladder, dim3(blocks_for(nblobs * 128ull, 64)), dim3(64), 0, s, lad, part,
-                       groups, nblobs);
+    hipLaunchKernelGGL(k_ladder, dim3(blocks_for(nblobs * 128ull * splits, 64)), dim3(64), 0, s, lad,
+                       part, groups, splits, nblobs);
     return hipGetLastError();
 }
 

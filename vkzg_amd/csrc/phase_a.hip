@@ -40,7 +40,7 @@ __device__ __attribute__((noinline)) void phase_a_slow(G1X *out, const Fr *scal,
 }
 
 // Persistent phase A.  The work is a queue of tiles (msm, g): windows
-// [g W / G, (g+1) W / G) of MSM msm = blob * 128 + j, where
+// [g W / G, (g+1) W / G) of MSM msm = (blob * splits + m) * 128 + j, where
 //   u[msm] = sum_{i<64} scal[msm][i] * xext[j][i]
 // with signed c-bit digits v in [-(2^(c-1)-1), 2^(c-1)] (the recoding of
 // k_phase_a_sort.comp:31-40 with a c-bit window) and
@@ -116,13 +116,13 @@ __global__ void __launch_bounds__(256, 2)
 }  // namespace
 
 hipError_t launch_phase_a(G1X *part, const Fr *scal, const G1AT *table, uint32_t c, uint32_t W,
-                          uint32_t nblobs, uint32_t groups, uint32_t grid, uint32_t *queue,
+                          uint32_t nmsm, uint32_t groups, uint32_t grid, uint32_t *queue,
                           hipStream_t s) {
-    const uint32_t nitems = (nblobs * 128u * groups + kTilesPerItem - 1) / kTilesPerItem;
+    const uint32_t nitems = (nmsm * groups + kTilesPerItem - 1) / kTilesPerItem;
     if (grid > nitems) grid = nitems;
     if (grid == 0) grid = 1;
-    hipLaunchKernelGGL(k_phase_a, dim3(grid), dim3(256), 0, s, part, scal, table, c, W, nblobs * 128u,
-                       groups, queue);
+    hipLaunchKernelGGL(k_phase_a, dim3(grid), dim3(256), 0, s, part, scal, table, c, W, nmsm, groups,
+                       queue);
     return hipGetLastError();
 }
 

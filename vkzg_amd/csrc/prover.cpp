@@ -191,7 +191,8 @@ void build_tables(vkzg_prover *p, const uint8_t *setup) {
 
 uint64_t WorkSet::bytes_per_blob() {
     const uint64_t npos = 128ull * kLadPos;
-    return kBlobBytes + kProofBytes + 4 + 4 + 128 * 64 * sizeof(Fr) + 128 * kMaxGroups * sizeof(G1X) +
+    return kBlobBytes + kProofBytes + 4 + 4 + kMaxLadSplits * 128 * 64 * sizeof(Fr) +
+           kMaxLadSplits * 128 * kMaxGroups * sizeof(G1X) +
            2 * npos * sizeof(G1J) + 2 * npos * sizeof(G1A) + npos * sizeof(Fp) +
            128 * 128 * sizeof(G1X) + 128 * sizeof(G1X) + 2 * kCellsBytes + 128 +
            2 * 128 * sizeof(Fr) + 2 * 8192 * sizeof(Fr);
@@ -204,8 +205,8 @@ void WorkSet::allocate(uint32_t B) {
     err = arena.dalloc<uint32_t>(B);
     queue = arena.dalloc<uint32_t>(B);
     ck(hipMemset(queue, 0, (size_t)B * sizeof(uint32_t)));
-    scal = arena.dalloc<Fr>((size_t)B * 128 * 64);
-    u = arena.dalloc<G1X>((size_t)B * 128 * kMaxGroups);
+    scal = arena.dalloc<Fr>((size_t)B * 128 * 64 * kMaxLadSplits);
+    u = arena.dalloc<G1X>((size_t)B * 128 * kMaxGroups * kMaxLadSplits);
     lad = arena.dalloc<G1J>((size_t)B * 128 * kLadPos);
     fold = arena.dalloc<G1J>((size_t)B * 128 * kLadPos);
     lad_aff = arena.dalloc<G1A>((size_t)B * 128 * kLadPos);
@@ -254,7 +255,8 @@ void default_tunings(vkzg_prover *p) {
 bool tuning_valid(const Tuning &t) {
     return t.groups >= 1 && t.groups <= kMaxGroups && t.blocks_per_cu >= 1 && t.blocks_per_cu <= 2 &&
            t.norm_chunk >= 1 && t.norm_chunk <= 256 && t.splits >= 1 && t.splits <= (uint32_t)kMaxSplits &&
-           (t.rec_cols == 2 || t.rec_cols == 4 || t.rec_cols == 8 || t.rec_cols == 16);
+           (t.rec_cols == 2 || t.rec_cols == 4 || t.rec_cols == 8 || t.rec_cols == 16) &&
+           (t.ladder_splits == 1 || t.ladder_splits == 2 || t.ladder_splits == 4);
 }
 
 namespace {
@@ -265,19 +267,20 @@ void proofs_range(vkzg_prover *p, WorkSet &ws, uint32_t off, uint32_t nb, uint8_
                   size_t blob_stride = kBlobBytes, Fr *coeffs = nullptr) {
     uint32_t *err = ws.err + off;
     uint32_t *queue = ws.queue + off;
-    Fr *scal = ws.scal + (size_t)off * 8192;
-    G1X *u = ws.u + (size_t)off * 128 * kMaxGroups;
+    Fr *scal = ws.scal + (size_t)off * 8192 * kMaxLadSplits;
+    G1X *u = ws.u + (size_t)off * 128 * kMaxGroups * kMaxLadSplits;
     const size_t npos = (size_t)128 * kLadPos;  // ladder points per blob
     G1J *lad = ws.lad + off * npos, *fold = ws.fold + off * npos;
     G1A *lad_aff = ws.lad_aff + off * npos, *lad_phi = ws.lad_phi + off * npos;
     Fp *prefix = ws.prefix + off * npos;
     G1X *buckets = ws.buckets + (size_t)off * 128 * 128, *cvals = ws.cvals + (size_t)off * 128;
-    ck(launch_fk20_scalars(scal, d_blobs, blob_stride, err, p->roots, p->roots_inv, queue, coeffs, nb,
-                           s));
+    ck(launch_fk20_scalars(scal, d_blobs, blob_stride, err, p->roots, p->roots_inv, queue, coeffs,
+                           tn.ladder_splits, nb, s));
     clk.mark("fk20_scalars");
-    ck(launch_phase_a(u, scal, p->table, p->c, p->W, nb, tn.groups, p->cus * tn.blocks_per_cu, queue, s));
+    ck(launch_phase_a(u, scal, p->table, p->c, p->W, nb * 128 * tn.ladder_splits, tn.groups,
+                      p->cus * tn.blocks_per_cu, queue, s));
     clk.mark("phase_a");
-    ck(launch_ladder(lad, u, tn.groups, nb, s));
+    ck(launch_ladder(lad, u, tn.groups, tn.ladder_splits, nb, s));
     clk.mark("ladder");
     ck(launch_fold(fold, lad, nb, s));
     clk.mark("fold");
@@ -353,7 +356,8 @@ void split_over_streams(StreamSet &ss, uint32_t nb, uint32_t splits, hipStream_t
 namespace {
 uint64_t tuning_word(const Tuning &tn) {
     return (uint64_t)tn.splits | ((uint64_t)tn.groups << 8) | ((uint64_t)tn.blocks_per_cu << 16) |
-           ((uint64_t)tn.norm_chunk << 24) | ((uint64_t)tn.rec_cols << 40);
+           ((uint64_t)tn.norm_chunk << 24) | ((uint64_t)tn.rec_cols << 40) |
+           ((uint64_t)tn.ladder_splits << 48);
 }
 }  // namespace
 
@@ -479,7 +483,9 @@ void autotune(vkzg_prover *p) {
                 best = cand;
             }
         };
-        for (uint32_t g : {1u, 2u, 4u, 5u}) { Tuning t = best; t.groups = g; consider(t); }
+        for (uint32_t g : {1u, 2u, 4u, 5u, 10u}) { Tuning t = best; t.groups = g; consider(t); }
+        if (nb <= 64)
+            for (uint32_t m : {2u, 4u}) { Tuning t = best; t.ladder_splits = m; consider(t); }
         for (uint32_t b : {1u, 2u}) { Tuning t = best; t.blocks_per_cu = b; consider(t); }
         for (uint32_t ch : {8u, 16u, 32u, 64u}) { Tuning t = best; t.norm_chunk = ch; consider(t); }
         if (nb >= 8) { Tuning t = best; t.splits = 2; consider(t); }
@@ -863,6 +869,7 @@ vkzg_result vkzg_amd_get_tuning(const vkzg_prover *p, size_t num_blobs, vkzg_amd
     out->normalize_chunk = t.norm_chunk;
     out->splits = t.splits;
     out->recovery_cols = t.rec_cols;
+    out->ladder_splits = t.ladder_splits;
     return VKZG_OK;
 }
 
@@ -874,6 +881,7 @@ vkzg_result vkzg_amd_set_tuning(vkzg_prover *p, size_t num_blobs, const vkzg_amd
     t.norm_chunk = in->normalize_chunk;
     t.splits = in->splits;
     t.rec_cols = in->recovery_cols;
+    t.ladder_splits = in->ladder_splits;
     if (!tuning_valid(t)) return VKZG_ERR_BADARGS;
     std::lock_guard<std::mutex> lock(p->mu);
     p->tune[tune_class((uint32_t)std::min<size_t>(num_blobs, 0xffffffffu))] = t;

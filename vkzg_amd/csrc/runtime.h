@@ -19,7 +19,7 @@ constexpr size_t kBlobBytes = VKZG_BYTES_PER_BLOB;
 constexpr size_t kProofBytes = (size_t)VKZG_NUM_CELL_PROOFS * VKZG_BYTES_PER_PROOF;
 constexpr size_t kCellsBytes = (size_t)VKZG_NUM_CELL_PROOFS * VKZG_BYTES_PER_CELL;
 constexpr int kMaxSplits = 4;  // sub-streams one batch may be spread over
-constexpr uint32_t kMaxGroups = 5;  // phase-A window groups per MSM
+constexpr uint32_t kMaxGroups = 10;  // phase-A window groups per MSM
 
 // Launch shape of the proof pipeline for one batch-size class.  Every choice
 // gives bit-identical proofs; only the time differs.
@@ -29,6 +29,7 @@ struct Tuning {
     uint32_t norm_chunk = 32;    // points per batched inversion in the normalisation
     uint32_t splits = 1;         // sub-streams, 1..kMaxSplits
     uint32_t rec_cols = 8;       // NTT_8192 sub-transforms per pass block: 2, 4, 8 or 16
+    uint32_t ladder_splits = 1;  // ladder segments (phase A also computes their starts): 1, 2 or 4
 };
 
 // Batch-size classes: <= 1, <= 4, <= 16, <= 64 and more blobs.
