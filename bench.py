@@ -147,23 +147,42 @@ def serve_bench(prover, rng, seconds, rate, probe_rate, slots):
     lat = sorted((x["done_ns"] - x["submit_ns"]) * 1e-6 for x in infos)
     probe = sorted((x["done_ns"] - x["submit_ns"]) * 1e-6 for x, r in zip(infos, reqs) if r[1] == "probe")
     pct = lambda v, q: v[min(len(v) - 1, int(q * len(v)))]
-    # plain per-call form: the same requests, one synchronous call each
-    t0 = time.perf_counter()
-    for i, r in enumerate(reqs):
-        out = out_bufs[i % n_out]
-        if r[1] == "r":
-            rc = vkzg_amd.load().vkzg_recover_cells_batch(prover.handle, out, cell_pool, r[3], r[2])
-        else:
-            rc = vkzg_amd.load().vkzg_compute_proofs_batch(prover.handle, out, blob_pool, r[2])
-        assert rc == 0
-    plain_wall = time.perf_counter() - t0
+    # Plain per-call forms: the same requests, one synchronous call each.
+    def per_call():
+        t0 = time.perf_counter()
+        for i, r in enumerate(reqs):
+            out = out_bufs[i % n_out]
+            if r[1] == "r":
+                rc = vkzg_amd.load().vkzg_recover_cells_batch(prover.handle, out, cell_pool, r[3], r[2])
+            else:
+                rc = vkzg_amd.load().vkzg_compute_proofs_batch(prover.handle, out, blob_pool, r[2])
+            assert rc == 0
+        return time.perf_counter() - t0
+    # (1) the step-before-batching form: one fixed launch shape for every
+    # size and kernels launched one by one (no per-size tuning, no graphs)
+    saved = {n: prover.get_tuning(n) for n in (1, 4, 16, 64, 256)}
+    fixed = dict(phase_a_groups=1, phase_a_blocks_per_cu=2, normalize_chunk=32, splits=1,
+                 recovery_cols=8, ladder_splits=1, reduce_lanes=8)
+    for n in saved:
+        prover.set_tuning(n, **fixed)
+    prover.set_graphs(False)
+    plain_wall = per_call()
+    prover.set_graphs(True)
+    for n, t in saved.items():
+        prover.set_tuning(n, **t)
+    # (2) the same per-request calls with this build's tuned, graph-replayed pipeline
+    tuned_wall = per_call()
     return {
         "offered_requests_per_s": rate + probe_rate, "requests": len(reqs), "blobs": blobs,
         "served_blobs_per_s": blobs / wall, "p50_ms": pct(lat, 0.50), "p99_ms": pct(lat, 0.99),
         "probe_6blob_proof_p50_ms": pct(probe, 0.50), "probe_6blob_proof_p99_ms": pct(probe, 0.99),
         "saturated_served_blobs_per_s": blobs / sat_wall,
         "plain_per_call_blobs_per_s": blobs / plain_wall,
+        "plain_per_call_form": "each request one synchronous C-ABI call, one fixed launch shape, "
+                               "kernels launched one by one (no per-size tuning, no graphs)",
         "served_over_plain": (blobs / sat_wall) / (blobs / plain_wall),
+        "tuned_per_call_blobs_per_s": blobs / tuned_wall,
+        "served_over_tuned_per_call": (blobs / sat_wall) / (blobs / tuned_wall),
         "server": st,
         "mix": "70%% proofs k~U[1,48], 30%% recovery k~U[1,64] with 64-127 of 128 cells present, "
                "plus 6-blob proof probes at %g/s; Poisson arrivals at %g requests/s"

@@ -89,12 +89,17 @@ typedef struct {
     uint32_t recovery_cols;         /* recovery NTT_8192 sub-transforms per pass block: 2, 4, 8 or 16 */
     uint32_t ladder_splits;         /* doubling-ladder segments, 1, 2 or 4; phase A computes the start
                                        of each (more phase-A work, shorter dependent chains) */
+    uint32_t reduce_lanes;          /* bucket-reduction lanes per output: 8, 16 or 32 */
 } vkzg_amd_tuning;
 
 /* The tuning used for a batch of `num_blobs` blobs. */
 vkzg_result vkzg_amd_get_tuning(const vkzg_prover *p, size_t num_blobs, vkzg_amd_tuning *out);
 /* Sets the tuning of the size class of `num_blobs`; BADARGS when a field is out of range. */
 vkzg_result vkzg_amd_set_tuning(vkzg_prover *p, size_t num_blobs, const vkzg_amd_tuning *t);
+/* Replays of captured HIP graphs on (enable != 0, the default unless
+ * VKZG_AMD_GRAPHS=0) or off (every call launches its kernels directly). */
+vkzg_result vkzg_amd_set_graphs(vkzg_prover *p, int enable);
+
 /* Times candidate tunings on synthetic blobs for every size class up to the
  * prover's batch size and keeps the fastest; *ms_out (optional) receives the
  * time it took.  $VKZG_AMD_AUTOTUNE=1 runs it at prover creation.  Do not run

@@ -153,6 +153,9 @@ class Prover:
             setattr(t, k, v)
         _check(load().vkzg_amd_set_tuning(self._h, blob_count, ctypes.byref(t)))
 
+    def set_graphs(self, enable=True):
+        _check(load().vkzg_amd_set_graphs(self._h, 1 if enable else 0))
+
     def autotune(self):
         """Run the autotuner; returns its wall time in ms."""
         ms = ctypes.c_double()

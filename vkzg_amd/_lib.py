@@ -44,6 +44,7 @@ EXT_SYMBOLS = [
     "vkzg_amd_get_tuning",
     "vkzg_amd_set_tuning",
     "vkzg_amd_autotune",
+    "vkzg_amd_set_graphs",
 ]
 
 
@@ -77,6 +78,7 @@ class Tuning(ctypes.Structure):
         ("splits", ctypes.c_uint32),
         ("recovery_cols", ctypes.c_uint32),
         ("ladder_splits", ctypes.c_uint32),
+        ("reduce_lanes", ctypes.c_uint32),
     ]
 
 
@@ -165,6 +167,8 @@ def load():
     L.vkzg_amd_get_tuning.argtypes = [vp, sz, ctypes.POINTER(Tuning)]
     L.vkzg_amd_set_tuning.restype = ctypes.c_int
     L.vkzg_amd_set_tuning.argtypes = [vp, sz, ctypes.POINTER(Tuning)]
+    L.vkzg_amd_set_graphs.restype = ctypes.c_int
+    L.vkzg_amd_set_graphs.argtypes = [vp, ctypes.c_int]
     L.vkzg_amd_autotune.restype = ctypes.c_int
     L.vkzg_amd_autotune.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
     L.vkzg_amd_debug_fp_inv.restype = ctypes.c_int

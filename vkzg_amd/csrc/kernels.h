@@ -74,7 +74,9 @@ hipError_t launch_fold(G1J *fold, const G1J *lad, uint32_t nblobs, hipStream_t s
 hipError_t launch_glv_phi(G1A *phi, const G1A *pts, uint64_t count, hipStream_t s);
 hipError_t launch_phase_b(G1X *buckets, const G1A *lad_aff, const G1A *lad_phi, const uint32_t *items,
                           const uint32_t *offsets, uint32_t nblobs, hipStream_t s);
-hipError_t launch_bucket_reduce(G1X *cvals, const G1X *buckets, uint32_t nblobs, hipStream_t s);
+// `lanes` (8, 16 or 32) lanes per output.
+hipError_t launch_bucket_reduce(G1X *cvals, const G1X *buckets, uint32_t nblobs, uint32_t lanes,
+                                hipStream_t s);
 hipError_t launch_finalize(uint8_t *proofs, const G1X *cvals, uint32_t nblobs, hipStream_t s);
 
 // --------------------------------------------------------------- recovery

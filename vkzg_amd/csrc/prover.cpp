@@ -256,7 +256,8 @@ bool tuning_valid(const Tuning &t) {
     return t.groups >= 1 && t.groups <= kMaxGroups && t.blocks_per_cu >= 1 && t.blocks_per_cu <= 2 &&
            t.norm_chunk >= 1 && t.norm_chunk <= 256 && t.splits >= 1 && t.splits <= (uint32_t)kMaxSplits &&
            (t.rec_cols == 2 || t.rec_cols == 4 || t.rec_cols == 8 || t.rec_cols == 16) &&
-           (t.ladder_splits == 1 || t.ladder_splits == 2 || t.ladder_splits == 4);
+           (t.ladder_splits == 1 || t.ladder_splits == 2 || t.ladder_splits == 4) &&
+           (t.reduce_lanes == 8 || t.reduce_lanes == 16 || t.reduce_lanes == 32);
 }
 
 namespace {
@@ -289,7 +290,7 @@ void proofs_range(vkzg_prover *p, WorkSet &ws, uint32_t off, uint32_t nb, uint8_
     clk.mark("normalize");
     ck(launch_phase_b(buckets, lad_aff, lad_phi, p->kb_items, p->kb_offsets, nb, s));
     clk.mark("phase_b");
-    ck(launch_bucket_reduce(cvals, buckets, nb, s));
+    ck(launch_bucket_reduce(cvals, buckets, nb, tn.reduce_lanes, s));
     clk.mark("bucket_reduce");
     ck(launch_finalize(d_proofs, cvals, nb, s));
     clk.mark("finalize");
@@ -357,7 +358,7 @@ namespace {
 uint64_t tuning_word(const Tuning &tn) {
     return (uint64_t)tn.splits | ((uint64_t)tn.groups << 8) | ((uint64_t)tn.blocks_per_cu << 16) |
            ((uint64_t)tn.norm_chunk << 24) | ((uint64_t)tn.rec_cols << 40) |
-           ((uint64_t)tn.ladder_splits << 48);
+           ((uint64_t)tn.ladder_splits << 48) | ((uint64_t)tn.reduce_lanes << 52);
 }
 }  // namespace
 
@@ -486,6 +487,7 @@ void autotune(vkzg_prover *p) {
         for (uint32_t g : {1u, 2u, 4u, 5u, 10u}) { Tuning t = best; t.groups = g; consider(t); }
         if (nb <= 64)
             for (uint32_t m : {2u, 4u}) { Tuning t = best; t.ladder_splits = m; consider(t); }
+        for (uint32_t r : {16u, 32u}) { Tuning t = best; t.reduce_lanes = r; consider(t); }
         for (uint32_t b : {1u, 2u}) { Tuning t = best; t.blocks_per_cu = b; consider(t); }
         for (uint32_t ch : {8u, 16u, 32u, 64u}) { Tuning t = best; t.norm_chunk = ch; consider(t); }
         if (nb >= 8) { Tuning t = best; t.splits = 2; consider(t); }
@@ -870,6 +872,7 @@ vkzg_result vkzg_amd_get_tuning(const vkzg_prover *p, size_t num_blobs, vkzg_amd
     out->splits = t.splits;
     out->recovery_cols = t.rec_cols;
     out->ladder_splits = t.ladder_splits;
+    out->reduce_lanes = t.reduce_lanes;
     return VKZG_OK;
 }
 
@@ -882,9 +885,17 @@ vkzg_result vkzg_amd_set_tuning(vkzg_prover *p, size_t num_blobs, const vkzg_amd
     t.splits = in->splits;
     t.rec_cols = in->recovery_cols;
     t.ladder_splits = in->ladder_splits;
+    t.reduce_lanes = in->reduce_lanes;
     if (!tuning_valid(t)) return VKZG_ERR_BADARGS;
     std::lock_guard<std::mutex> lock(p->mu);
     p->tune[tune_class((uint32_t)std::min<size_t>(num_blobs, 0xffffffffu))] = t;
+    return VKZG_OK;
+}
+
+vkzg_result vkzg_amd_set_graphs(vkzg_prover *p, int enable) {
+    if (!p) return VKZG_ERR_BADARGS;
+    std::lock_guard<std::mutex> lock(p->mu);
+    p->use_graphs = enable != 0;
     return VKZG_OK;
 }
 

@@ -4,8 +4,6 @@
 namespace vkzg_amd {
 
 namespace {
-constexpr int kReduceLanes = 8;
-
 // One output's reduction recomputed with the checked point operations.
 __device__ __attribute__((noinline)) void reduce_slow(G1X *out, const G1X *B) {
     G1X run = g1x_identity(), sum = g1x_identity();
@@ -17,13 +15,16 @@ __device__ __attribute__((noinline)) void reduce_slow(G1X *out, const G1X *B) {
 }
 
 // cvals[blob][a] = sum_{k=1..128} k * buckets[blob][a][k-1]
-// (field.glsl.inc:478-517).  Lane l of a team owns buckets [16l, 16l+16):
-// rs_l = sum_i (i+1) B[16l+i] by running sums, s_l = sum_i B[16l+i]; the
-// output is sum_l (rs_l + 16 l * s_l).
+// (field.glsl.inc:478-517).  A team of LANES lanes shares an output; lane l
+// owns buckets [per l, per l + per) with per = 128 / LANES:
+// rs_l = sum_i (i+1) B[per l + i] by running sums, s_l = sum_i B[per l + i];
+// the output is sum_l (rs_l + per l * s_l).  More lanes shorten each lane's
+// dependent chain and add total work (the per l * s_l ladders).
+template <int LANES>
 __global__ void __launch_bounds__(256, 2) k_bucket_reduce(G1X *cvals, const G1X *buckets, uint32_t nout) {
-    const uint32_t o = blockIdx.x * (256 / kReduceLanes) + threadIdx.x / kReduceLanes;
-    const uint32_t l = threadIdx.x % kReduceLanes;
-    constexpr uint32_t per = 128 / kReduceLanes;
+    const uint32_t o = blockIdx.x * (256 / LANES) + threadIdx.x / LANES;
+    const uint32_t l = threadIdx.x % LANES;
+    constexpr uint32_t per = 128 / LANES;
     G1X s = g1x_identity(), rs = g1x_identity();
     bool exc = false;
     if (o < nout) {
@@ -34,7 +35,7 @@ __global__ void __launch_bounds__(256, 2) k_bucket_reduce(G1X *cvals, const G1X 
             rs = g1x_add_fast(rs, s, exc);
         }
     }
-    // t = (16 l) * s by double-and-add over the bits of 16 l (t never equals
+    // t = (per l) * s by double-and-add over the bits of per l (t never equals
     // +-s when it is added: it is an even multiple of s there).
     const uint32_t mult = per * l;
     G1X t = g1x_identity();
@@ -47,10 +48,11 @@ __global__ void __launch_bounds__(256, 2) k_bucket_reduce(G1X *cvals, const G1X 
     }
     G1X v = g1x_add_fast(rs, t, exc);
 #pragma unroll 1
-    for (int m = 1; m < kReduceLanes; m <<= 1) v = g1x_add_fast(v, g1x_shfl_xor(v, m), exc);
+    for (int m = 1; m < LANES; m <<= 1) v = g1x_add_fast(v, g1x_shfl_xor(v, m), exc);
     const uint64_t flagged = __ballot(exc);
-    const uint32_t team = (threadIdx.x & 63u) / kReduceLanes;
-    const bool redo = ((flagged >> (team * kReduceLanes)) & ((1ull << kReduceLanes) - 1)) != 0;
+    const uint32_t team = (threadIdx.x & 63u) / LANES;
+    const uint64_t team_mask = LANES == 64 ? ~0ull : ((1ull << LANES) - 1);
+    const bool redo = ((flagged >> (team * LANES)) & team_mask) != 0;
     if (o < nout && l == 0 && redo) {
         reduce_slow(&cvals[o], buckets + (size_t)o * 128);
         return;
@@ -102,10 +104,23 @@ __global__ void __launch_bounds__(64) k_finalize(uint8_t *proofs, const G1X *cva
 
 }  // namespace
 
-hipError_t launch_bucket_reduce(G1X *cvals, const G1X *buckets, uint32_t nblobs, hipStream_t s) {
+hipError_t launch_bucket_reduce(G1X *cvals, const G1X *buckets, uint32_t nblobs, uint32_t lanes,
+                                hipStream_t s) {
     const uint32_t nout = nblobs * 128;
-    hipLaunchKernelGGL(k_bucket_reduce, dim3(blocks_for(nout, 256 / kReduceLanes)), dim3(256), 0, s,
-                       cvals, buckets, nout);
+    switch (lanes) {
+        case 16:
+            hipLaunchKernelGGL(k_bucket_reduce<16>, dim3(blocks_for(nout, 256 / 16)), dim3(256), 0, s,
+                               cvals, buckets, nout);
+            break;
+        case 32:
+            hipLaunchKernelGGL(k_bucket_reduce<32>, dim3(blocks_for(nout, 256 / 32)), dim3(256), 0, s,
+                               cvals, buckets, nout);
+            break;
+        default:
+            hipLaunchKernelGGL(k_bucket_reduce<8>, dim3(blocks_for(nout, 256 / 8)), dim3(256), 0, s,
+                               cvals, buckets, nout);
+            break;
+    }
     return hipGetLastError();
 }
 
