@@ -27,44 +27,85 @@ __device__ __attribute__((noinline)) void phase_b_slow(G1X *out, const G1A *L, c
 // +-2^(8d) L[aa-e] for d < 17 and +-2^(8(d-17)) phi(L[aa-e]) above (the two
 // GLV halves of the scalar).  The item list depends only on (half, k), so it
 // is uniform across a wave.
-__global__ void __launch_bounds__(64, 2)
-    k_phase_b(G1X *buckets, const G1A *lad_aff, const G1A *lad_phi, const uint32_t *items,
-              const uint32_t *offsets) {
-    const uint32_t k = blockIdx.x;                       // bucket, 0..127
-    const uint32_t ba = blockIdx.y * 64 + threadIdx.x;   // blob * 128 + a
+//
+// Persistent: `grid` blocks of 4 waves take work items from the counter q[0]
+// (cleared by k_glv_phi, the kernel before) until the queue is drained; every
+// block leaves after one failed take.  Item i is bucket k = i % 128 for the
+// four 64-lane groups of (blob, a) starting at 4 (i / 128); the four waves of
+// a block share k, so they run the same item list.  A flagged bucket is
+// recomputed by k_phase_b_fix, which keeps this kernel free of calls.
+__global__ void __launch_bounds__(256, 2)
+    k_phase_b(G1X *buckets, uint8_t *flags, const G1A *lad_aff, const G1A *lad_phi, const uint32_t *items,
+              const uint32_t *offsets, uint32_t ngroups, uint32_t *q) {
+    __shared__ uint32_t item_sh;
+    const uint32_t nitems = 128u * ((ngroups + 3) / 4);
+#pragma unroll 1
+    for (;;) {
+        if (threadIdx.x == 0) item_sh = atomicAdd(q, 1u);
+        __syncthreads();
+        const uint32_t item = item_sh;
+        __syncthreads();
+        if (item >= nitems) break;
+        const uint32_t k = item % 128;  // bucket
+        const uint32_t group = (item / 128) * 4 + threadIdx.x / 64;
+        if (group >= ngroups) continue;
+        const uint32_t ba = group * 64 + threadIdx.x % 64;  // blob * 128 + a
+        const uint32_t blob = ba / 128, a = ba % 128;
+        const uint32_t half = a >> 6, aa = a & 63u;
+        const uint32_t lo = offsets[half * 129 + k], hi = offsets[half * 129 + k + 1];
+        const G1A *L = lad_aff + (size_t)(blob * 2 + half) * 64 * kLadPos;
+        const G1A *Phi = lad_phi + (size_t)(blob * 2 + half) * 64 * kLadPos;
+        G1X acc = g1x_identity();
+        bool exc = false, started = false;
+#pragma unroll 1
+        for (uint32_t it = lo; it < hi; it++) {
+            const uint32_t packed = items[it];
+            const uint32_t e = packed & 0xffu;
+            const uint32_t d = (packed >> 8) & 0xffu;
+            bool neg = ((packed >> 16) & 1u) != 0;
+            if (half && e > aa) neg = !neg;
+            const uint32_t src = ((aa - e) & 63u) * kLadPos;
+            const G1A pt = d < kLadPos ? L[src + d] : Phi[src + d - kLadPos];
+            if (started) {
+                acc = g1x_madd_acc(acc, pt, neg, exc);
+            } else {
+                acc = g1x_start(pt, neg, exc);
+                started = true;
+            }
+        }
+        buckets[(size_t)ba * 128 + k] = acc;
+        flags[(size_t)ba * 128 + k] = exc ? 1 : 0;
+    }
+}
+
+// Recompute, with the checked point operations, the buckets k_phase_b flagged
+// (a doubling or cancellation inside the bucket; random data never has one).
+__global__ void k_phase_b_fix(G1X *buckets, const uint8_t *flags, const G1A *lad_aff, const G1A *lad_phi,
+                              const uint32_t *items, const uint32_t *offsets, uint32_t nbuckets) {
+    const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nbuckets || !flags[idx]) return;
+    const uint32_t ba = idx / 128, k = idx % 128;
     const uint32_t blob = ba / 128, a = ba % 128;
     const uint32_t half = a >> 6, aa = a & 63u;
     const uint32_t lo = offsets[half * 129 + k], hi = offsets[half * 129 + k + 1];
     const G1A *L = lad_aff + (size_t)(blob * 2 + half) * 64 * kLadPos;
     const G1A *Phi = lad_phi + (size_t)(blob * 2 + half) * 64 * kLadPos;
-    G1X acc = g1x_identity();
-    bool exc = false, started = false;
-#pragma unroll 1
-    for (uint32_t it = lo; it < hi; it++) {
-        const uint32_t packed = items[it];
-        const uint32_t e = packed & 0xffu;
-        const uint32_t d = (packed >> 8) & 0xffu;
-        bool neg = ((packed >> 16) & 1u) != 0;
-        if (half && e > aa) neg = !neg;
-        const uint32_t src = ((aa - e) & 63u) * kLadPos;
-        const G1A pt = d < kLadPos ? L[src + d] : Phi[src + d - kLadPos];
-        if (started) {
-            acc = g1x_madd_acc(acc, pt, neg, exc);
-        } else {
-            acc = g1x_start(pt, neg, exc);
-            started = true;
-        }
-    }
-    if (exc) phase_b_slow(&buckets[(size_t)ba * 128 + k], L, Phi, items, lo, hi, half, aa);
-    else buckets[(size_t)ba * 128 + k] = acc;
+    phase_b_slow(&buckets[idx], L, Phi, items, lo, hi, half, aa);
 }
 
 }  // namespace
 
-hipError_t launch_phase_b(G1X *buckets, const G1A *lad_aff, const G1A *lad_phi, const uint32_t *items,
-                          const uint32_t *offsets, uint32_t nblobs, hipStream_t s) {
-    hipLaunchKernelGGL(k_phase_b, dim3(128, nblobs * 2), dim3(64), 0, s, buckets, lad_aff, lad_phi,
-                       items, offsets);
+hipError_t launch_phase_b(G1X *buckets, uint8_t *flags, const G1A *lad_aff, const G1A *lad_phi,
+                          const uint32_t *items, const uint32_t *offsets, uint32_t nblobs, uint32_t grid,
+                          uint32_t *queue, hipStream_t s) {
+    const uint32_t ngroups = nblobs * 2;  // 64-lane groups of (blob, a)
+    const uint32_t nitems = 128u * ((ngroups + 3) / 4);
+    if (grid > nitems) grid = nitems;
+    hipLaunchKernelGGL(k_phase_b, dim3(grid), dim3(256), 0, s, buckets, flags, lad_aff, lad_phi, items,
+                       offsets, ngroups, queue);
+    const uint32_t nbuckets = nblobs * 128 * 128;
+    hipLaunchKernelGGL(k_phase_b_fix, dim3(blocks_for(nbuckets, 256)), dim3(256), 0, s, buckets, flags,
+                       lad_aff, lad_phi, items, offsets, nbuckets);
     return hipGetLastError();
 }
 

@@ -99,9 +99,11 @@ struct WorkSet {
     // proofs
     uint8_t *blobs = nullptr, *proofs = nullptr;
     uint32_t *err = nullptr;
-    uint32_t *queue = nullptr;  // phase-A tile counter of the sub-batch at each blob offset
+    uint32_t *queue = nullptr;    // phase-A tile counter of the sub-batch at each blob offset
+    uint32_t *queue_b = nullptr;  // phase-B work counter, likewise
     Fr *scal = nullptr;
     G1X *u = nullptr, *buckets = nullptr, *cvals = nullptr;
+    uint8_t *bucket_flags = nullptr;  // phase-B buckets to recompute with checked operations
     G1J *lad = nullptr, *fold = nullptr;
     G1A *lad_aff = nullptr, *lad_phi = nullptr;
     Fp *prefix = nullptr;
