@@ -28,28 +28,25 @@ __device__ __attribute__((noinline)) void phase_b_slow(G1X *out, const G1A *L, c
 // GLV halves of the scalar).  The item list depends only on (half, k), so it
 // is uniform across a wave.
 //
-// Persistent: `grid` blocks of 4 waves take work items from the counter q[0]
-// (cleared by k_glv_phi, the kernel before) until the queue is drained; every
-// block leaves after one failed take.  Item i is bucket k = i % 128 for the
-// four 64-lane groups of (blob, a) starting at 4 (i / 128); the four waves of
-// a block share k, so they run the same item list.  A flagged bucket is
-// recomputed by k_phase_b_fix, which keeps this kernel free of calls.
+// Persistent: every wave of the `grid` blocks takes work items from the
+// counter q[0] (cleared by k_glv_phi, the kernel before) until the queue is
+// drained, and leaves after one failed take; waves share no barrier.  Item i
+// is bucket k = i % 128 of the 64-lane group i / 128 of (blob, a).  A flagged
+// bucket is recomputed by k_phase_b_fix, which keeps this kernel free of
+// calls.
 __global__ void __launch_bounds__(256, 2)
     k_phase_b(G1X *buckets, uint8_t *flags, const G1A *lad_aff, const G1A *lad_phi, const uint32_t *items,
               const uint32_t *offsets, uint32_t ngroups, uint32_t *q) {
-    __shared__ uint32_t item_sh;
-    const uint32_t nitems = 128u * ((ngroups + 3) / 4);
+    const uint32_t lane = threadIdx.x % 64;
+    const uint32_t nitems = 128u * ngroups;
 #pragma unroll 1
     for (;;) {
-        if (threadIdx.x == 0) item_sh = atomicAdd(q, 1u);
-        __syncthreads();
-        const uint32_t item = item_sh;
-        __syncthreads();
+        uint32_t item = 0;
+        if (lane == 0) item = atomicAdd(q, 1u);
+        item = __shfl(item, 0);
         if (item >= nitems) break;
         const uint32_t k = item % 128;  // bucket
-        const uint32_t group = (item / 128) * 4 + threadIdx.x / 64;
-        if (group >= ngroups) continue;
-        const uint32_t ba = group * 64 + threadIdx.x % 64;  // blob * 128 + a
+        const uint32_t ba = (item / 128) * 64 + lane;  // blob * 128 + a
         const uint32_t blob = ba / 128, a = ba % 128;
         const uint32_t half = a >> 6, aa = a & 63u;
         const uint32_t lo = offsets[half * 129 + k], hi = offsets[half * 129 + k + 1];
@@ -99,8 +96,8 @@ hipError_t launch_phase_b(G1X *buckets, uint8_t *flags, const G1A *lad_aff, cons
                           const uint32_t *items, const uint32_t *offsets, uint32_t nblobs, uint32_t grid,
                           uint32_t *queue, hipStream_t s) {
     const uint32_t ngroups = nblobs * 2;  // 64-lane groups of (blob, a)
-    const uint32_t nitems = 128u * ((ngroups + 3) / 4);
-    if (grid > nitems) grid = nitems;
+    const uint32_t nitems = 128u * ngroups;
+    if (grid > (nitems + 3) / 4) grid = (nitems + 3) / 4;
     hipLaunchKernelGGL(k_phase_b, dim3(grid), dim3(256), 0, s, buckets, flags, lad_aff, lad_phi, items,
                        offsets, ngroups, queue);
     const uint32_t nbuckets = nblobs * 128 * 128;
