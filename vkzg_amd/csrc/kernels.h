@@ -71,16 +71,19 @@ hipError_t launch_phase_a(G1X *part, const Fr *scal, const G1AT *table, uint32_t
 hipError_t launch_ladder(G1J *lad, const G1X *part, uint32_t groups, uint32_t splits, uint32_t nblobs,
                          hipStream_t s);
 hipError_t launch_fold(G1J *fold, const G1J *lad, uint32_t nblobs, hipStream_t s);
-// Also clears queue[0], the work counter of the phase-B launch that follows.
-hipError_t launch_glv_phi(G1A *phi, const G1A *pts, uint64_t count, uint32_t *queue, hipStream_t s);
+// Also clears the work counters of the phase-B and bucket-reduction launches
+// that follow (queue_b[0], queue_r[0]).
+hipError_t launch_glv_phi(G1A *phi, const G1A *pts, uint64_t count, uint32_t *queue_b, uint32_t *queue_r,
+                          hipStream_t s);
 // Persistent (`grid` blocks drain queue[0]), then the recompute of flagged
 // buckets; flags[B][128][128] is scratch.
 hipError_t launch_phase_b(G1X *buckets, uint8_t *flags, const G1A *lad_aff, const G1A *lad_phi,
                           const uint32_t *items, const uint32_t *offsets, uint32_t nblobs, uint32_t grid,
                           uint32_t *queue, hipStream_t s);
-// `lanes` (8, 16 or 32) lanes per output.
-hipError_t launch_bucket_reduce(G1X *cvals, const G1X *buckets, uint32_t nblobs, uint32_t lanes,
-                                hipStream_t s);
+// `lanes` (8, 16 or 32) lanes per output; persistent (`grid` blocks drain
+// queue[0]), then the recompute of flagged outputs; flags[B][128] is scratch.
+hipError_t launch_bucket_reduce(G1X *cvals, uint8_t *flags, const G1X *buckets, uint32_t nblobs,
+                                uint32_t lanes, uint32_t grid, uint32_t *queue, hipStream_t s);
 hipError_t launch_finalize(uint8_t *proofs, const G1X *cvals, uint32_t nblobs, hipStream_t s);
 
 // --------------------------------------------------------------- recovery

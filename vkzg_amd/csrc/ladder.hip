@@ -48,9 +48,12 @@ __global__ void k_fold(G1J *fold, const G1J *lad, uint32_t nblobs) {
 
 // phi[k] = (beta x, y) of pts[k]: the endomorphism [lambda], lambda = z^2 - 1
 // (the identity (0, 0) maps to itself).
-__global__ void k_glv_phi(G1A *phi, const G1A *pts, uint64_t count, uint32_t *queue) {
+__global__ void k_glv_phi(G1A *phi, const G1A *pts, uint64_t count, uint32_t *queue_b, uint32_t *queue_r) {
     const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k == 0) queue[0] = 0u;  // phase B's work counter
+    if (k == 0) {  // work counters of phase B and the bucket reduction
+        queue_b[0] = 0u;
+        queue_r[0] = 0u;
+    }
     if (k >= count) return;
     G1A q = pts[k];
     q.x = fp_mul(q.x, fp_const(cfp::BETA_M));
@@ -72,8 +75,10 @@ hipError_t launch_fold(G1J *fold, const G1J *lad, uint32_t nblobs, hipStream_t s
     return hipGetLastError();
 }
 
-hipError_t launch_glv_phi(G1A *phi, const G1A *pts, uint64_t count, uint32_t *queue, hipStream_t s) {
-    hipLaunchKernelGGL(k_glv_phi, dim3(blocks_for(count, 256)), dim3(256), 0, s, phi, pts, count, queue);
+hipError_t launch_glv_phi(G1A *phi, const G1A *pts, uint64_t count, uint32_t *queue_b, uint32_t *queue_r,
+                          hipStream_t s) {
+    hipLaunchKernelGGL(k_glv_phi, dim3(blocks_for(count, 256)), dim3(256), 0, s, phi, pts, count, queue_b,
+                       queue_r);
     return hipGetLastError();
 }
 

@@ -191,7 +191,7 @@ void build_tables(vkzg_prover *p, const uint8_t *setup) {
 
 uint64_t WorkSet::bytes_per_blob() {
     const uint64_t npos = 128ull * kLadPos;
-    return kBlobBytes + kProofBytes + 4 + 4 + 4 + kMaxLadSplits * 128 * 64 * sizeof(Fr) +
+    return kBlobBytes + kProofBytes + 4 * 4 + 128 + kMaxLadSplits * 128 * 64 * sizeof(Fr) +
            kMaxLadSplits * 128 * kMaxGroups * sizeof(G1X) +
            2 * npos * sizeof(G1J) + 2 * npos * sizeof(G1A) + npos * sizeof(Fp) +
            128 * 128 * (sizeof(G1X) + 1) + 128 * sizeof(G1X) + 2 * kCellsBytes + 128 +
@@ -207,6 +207,8 @@ void WorkSet::allocate(uint32_t B) {
     ck(hipMemset(queue, 0, (size_t)B * sizeof(uint32_t)));
     queue_b = arena.dalloc<uint32_t>(B);
     ck(hipMemset(queue_b, 0, (size_t)B * sizeof(uint32_t)));
+    queue_r = arena.dalloc<uint32_t>(B);
+    ck(hipMemset(queue_r, 0, (size_t)B * sizeof(uint32_t)));
     scal = arena.dalloc<Fr>((size_t)B * 128 * 64 * kMaxLadSplits);
     u = arena.dalloc<G1X>((size_t)B * 128 * kMaxGroups * kMaxLadSplits);
     lad = arena.dalloc<G1J>((size_t)B * 128 * kLadPos);
@@ -216,6 +218,7 @@ void WorkSet::allocate(uint32_t B) {
     prefix = arena.dalloc<Fp>((size_t)B * 128 * kLadPos);
     buckets = arena.dalloc<G1X>((size_t)B * 128 * 128);
     bucket_flags = arena.dalloc<uint8_t>((size_t)B * 128 * 128);
+    cval_flags = arena.dalloc<uint8_t>((size_t)B * 128);
     cvals = arena.dalloc<G1X>((size_t)B * 128);
     cells = arena.dalloc<uint8_t>((size_t)B * kCellsBytes);
     cells_out = arena.dalloc<uint8_t>((size_t)B * kCellsBytes);
@@ -289,12 +292,13 @@ void proofs_range(vkzg_prover *p, WorkSet &ws, uint32_t off, uint32_t nb, uint8_
     ck(launch_fold(fold, lad, nb, s));
     clk.mark("fold");
     ck(launch_jac_to_affine(lad_aff, fold, prefix, (uint64_t)nb * npos, tn.norm_chunk, s));
-    ck(launch_glv_phi(lad_phi, lad_aff, (uint64_t)nb * npos, ws.queue_b + off, s));
+    ck(launch_glv_phi(lad_phi, lad_aff, (uint64_t)nb * npos, ws.queue_b + off, ws.queue_r + off, s));
     clk.mark("normalize");
     ck(launch_phase_b(buckets, ws.bucket_flags + off * 128 * 128, lad_aff, lad_phi, p->kb_items,
                       p->kb_offsets, nb, p->cus * 2, ws.queue_b + off, s));
     clk.mark("phase_b");
-    ck(launch_bucket_reduce(cvals, buckets, nb, tn.reduce_lanes, s));
+    ck(launch_bucket_reduce(cvals, ws.cval_flags + off * 128, buckets, nb, tn.reduce_lanes, p->cus * 2,
+                            ws.queue_r + off, s));
     clk.mark("bucket_reduce");
     ck(launch_finalize(d_proofs, cvals, nb, s));
     clk.mark("finalize");

@@ -20,44 +20,66 @@ __device__ __attribute__((noinline)) void reduce_slow(G1X *out, const G1X *B) {
 // rs_l = sum_i (i+1) B[per l + i] by running sums, s_l = sum_i B[per l + i];
 // the output is sum_l (rs_l + per l * s_l).  More lanes shorten each lane's
 // dependent chain and add total work (the per l * s_l ladders).
+//
+// Persistent: every wave takes items of 64 / LANES consecutive outputs from
+// the counter q[0] (cleared by k_glv_phi) until the queue is drained and
+// leaves after one failed take.  Flagged outputs (a doubling or cancellation
+// met by the fast additions) are recomputed by k_reduce_fix.
 template <int LANES>
-__global__ void __launch_bounds__(256, 2) k_bucket_reduce(G1X *cvals, const G1X *buckets, uint32_t nout) {
-    const uint32_t o = blockIdx.x * (256 / LANES) + threadIdx.x / LANES;
-    const uint32_t l = threadIdx.x % LANES;
+__global__ void __launch_bounds__(256, 2)
+    k_bucket_reduce(G1X *cvals, uint8_t *flags, const G1X *buckets, uint32_t nout, uint32_t *q) {
     constexpr uint32_t per = 128 / LANES;
-    G1X s = g1x_identity(), rs = g1x_identity();
-    bool exc = false;
-    if (o < nout) {
-        const G1X *B = buckets + (size_t)o * 128 + l * per;
+    constexpr uint32_t teams = 64 / LANES;
+    const uint32_t lane = threadIdx.x % 64;
+    const uint32_t l = lane % LANES;
+    const uint32_t nitems = (nout + teams - 1) / teams;
 #pragma unroll 1
-        for (int i = per - 1; i >= 0; i--) {
-            s = g1x_add_fast(s, B[i], exc);
-            rs = g1x_add_fast(rs, s, exc);
+    for (;;) {
+        uint32_t item = 0;
+        if (lane == 0) item = atomicAdd(q, 1u);
+        item = __shfl(item, 0);
+        if (item >= nitems) break;
+        const uint32_t o = item * teams + lane / LANES;
+        G1X s = g1x_identity(), rs = g1x_identity();
+        bool exc = false;
+        if (o < nout) {
+            const G1X *B = buckets + (size_t)o * 128 + l * per;
+#pragma unroll 1
+            for (int i = per - 1; i >= 0; i--) {
+                s = g1x_add_fast(s, B[i], exc);
+                rs = g1x_add_fast(rs, s, exc);
+            }
+        }
+        // t = (per l) * s by double-and-add over the bits of per l (t never
+        // equals +-s when it is added: it is an even multiple of s there).
+        const uint32_t mult = per * l;
+        G1X t = g1x_identity();
+        if (mult) {
+#pragma unroll 1
+            for (int bit = 31 - __builtin_clz(mult); bit >= 0; bit--) {
+                t = g1x_dbl(t);
+                if ((mult >> bit) & 1u) t = g1x_add_fast(t, s, exc);
+            }
+        }
+        G1X v = g1x_add_fast(rs, t, exc);
+#pragma unroll 1
+        for (int m = 1; m < LANES; m <<= 1) v = g1x_add_fast(v, g1x_shfl_xor(v, m), exc);
+        const uint64_t flagged = __ballot(exc);
+        const uint32_t team = lane / LANES;
+        const uint64_t team_mask = LANES == 64 ? ~0ull : ((1ull << LANES) - 1);
+        const bool redo = ((flagged >> (team * LANES)) & team_mask) != 0;
+        if (o < nout && l == 0) {
+            cvals[o] = v;
+            flags[o] = redo ? 1 : 0;
         }
     }
-    // t = (per l) * s by double-and-add over the bits of per l (t never equals
-    // +-s when it is added: it is an even multiple of s there).
-    const uint32_t mult = per * l;
-    G1X t = g1x_identity();
-    if (mult) {
-#pragma unroll 1
-        for (int bit = 31 - __builtin_clz(mult); bit >= 0; bit--) {
-            t = g1x_dbl(t);
-            if ((mult >> bit) & 1u) t = g1x_add_fast(t, s, exc);
-        }
-    }
-    G1X v = g1x_add_fast(rs, t, exc);
-#pragma unroll 1
-    for (int m = 1; m < LANES; m <<= 1) v = g1x_add_fast(v, g1x_shfl_xor(v, m), exc);
-    const uint64_t flagged = __ballot(exc);
-    const uint32_t team = (threadIdx.x & 63u) / LANES;
-    const uint64_t team_mask = LANES == 64 ? ~0ull : ((1ull << LANES) - 1);
-    const bool redo = ((flagged >> (team * LANES)) & team_mask) != 0;
-    if (o < nout && l == 0 && redo) {
-        reduce_slow(&cvals[o], buckets + (size_t)o * 128);
-        return;
-    }
-    if (o < nout && l == 0) cvals[o] = v;
+}
+
+// Recompute the flagged outputs of k_bucket_reduce with checked additions.
+__global__ void k_reduce_fix(G1X *cvals, const uint8_t *flags, const G1X *buckets, uint32_t nout) {
+    const uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= nout || !flags[o]) return;
+    reduce_slow(&cvals[o], buckets + (size_t)o * 128);
 }
 
 // XYZZ -> affine for two points with one inversion.
@@ -104,23 +126,28 @@ __global__ void __launch_bounds__(64) k_finalize(uint8_t *proofs, const G1X *cva
 
 }  // namespace
 
-hipError_t launch_bucket_reduce(G1X *cvals, const G1X *buckets, uint32_t nblobs, uint32_t lanes,
-                                hipStream_t s) {
+hipError_t launch_bucket_reduce(G1X *cvals, uint8_t *flags, const G1X *buckets, uint32_t nblobs,
+                                uint32_t lanes, uint32_t grid, uint32_t *queue, hipStream_t s) {
     const uint32_t nout = nblobs * 128;
+    const uint32_t teams = 64 / (lanes == 16 || lanes == 32 ? lanes : 8);
+    const uint32_t waves = (nout + teams - 1) / teams;
+    if (grid > (waves + 3) / 4) grid = (waves + 3) / 4;
     switch (lanes) {
         case 16:
-            hipLaunchKernelGGL(k_bucket_reduce<16>, dim3(blocks_for(nout, 256 / 16)), dim3(256), 0, s,
-                               cvals, buckets, nout);
+            hipLaunchKernelGGL(k_bucket_reduce<16>, dim3(grid), dim3(256), 0, s, cvals, flags, buckets, nout,
+                               queue);
             break;
         case 32:
-            hipLaunchKernelGGL(k_bucket_reduce<32>, dim3(blocks_for(nout, 256 / 32)), dim3(256), 0, s,
-                               cvals, buckets, nout);
+            hipLaunchKernelGGL(k_bucket_reduce<32>, dim3(grid), dim3(256), 0, s, cvals, flags, buckets, nout,
+                               queue);
             break;
         default:
-            hipLaunchKernelGGL(k_bucket_reduce<8>, dim3(blocks_for(nout, 256 / 8)), dim3(256), 0, s,
-                               cvals, buckets, nout);
+            hipLaunchKernelGGL(k_bucket_reduce<8>, dim3(grid), dim3(256), 0, s, cvals, flags, buckets, nout,
+                               queue);
             break;
     }
+    hipLaunchKernelGGL(k_reduce_fix, dim3(blocks_for(nout, 256)), dim3(256), 0, s, cvals, flags, buckets,
+                       nout);
     return hipGetLastError();
 }
 

@@ -101,9 +101,11 @@ struct WorkSet {
     uint32_t *err = nullptr;
     uint32_t *queue = nullptr;    // phase-A tile counter of the sub-batch at each blob offset
     uint32_t *queue_b = nullptr;  // phase-B work counter, likewise
+    uint32_t *queue_r = nullptr;  // bucket-reduction work counter, likewise
     Fr *scal = nullptr;
     G1X *u = nullptr, *buckets = nullptr, *cvals = nullptr;
     uint8_t *bucket_flags = nullptr;  // phase-B buckets to recompute with checked operations
+    uint8_t *cval_flags = nullptr;    // reduced outputs to recompute likewise
     G1J *lad = nullptr, *fold = nullptr;
     G1A *lad_aff = nullptr, *lad_phi = nullptr;
     Fp *prefix = nullptr;
