@@ -57,6 +57,39 @@ vkzg_result vkzg_amd_recover_cells_and_proofs(vkzg_prover *p, uint8_t *cells_out
                                               const uint8_t *cells, const uint8_t *cell_present,
                                               size_t num_blobs);
 
+/* -------------------------------------------- verification and commitments
+ * The consensus-spec functions on either side of the reference's two entry
+ * points (the reference has no verifier and no commitment function,
+ * include/vkzg.h:14-18), on the device. */
+
+/* Loads the G2 side of the setup for verification: the 65 G2 monomial points
+ * [tau^i]G2, i = 0..64, 96-byte compressed each (the trusted setup's G2
+ * list), matching the prover's G1 setup.  The Miller-loop lines of
+ * [tau^64]G2 and [tau^0]G2 are precomputed on the device and kept resident.
+ * BADARGS for a wrong length; SETUP for a malformed point or a point at
+ * infinity at index 0 or 64. */
+#define VKZG_AMD_G2_SETUP_BYTES (65 * 96)
+vkzg_result vkzg_amd_load_g2_setup(vkzg_prover *p, const uint8_t *g2_monomial, size_t len);
+
+/* verify_cell_kzg_proof_batch (consensus specs, polynomial-commitments-
+ * sampling): *ok = 1 when every cell k (2048 bytes) with index
+ * cell_indices[k] is consistent with commitments[k] (48 bytes) and
+ * proofs[k] (48 bytes), checked all at once with Fiat-Shamir weights (the
+ * specs' challenge: SHA-256 over every input) and one product of two
+ * pairings; *ok = 0 when not.  num_cells == 0 gives *ok = 1.
+ * BADARGS: NULL arguments, a cell index >= 128, a malformed or non-G1
+ * commitment or proof, or a cell element >= r (as the specs' assertions);
+ * SETUP: no G2 setup loaded. */
+vkzg_result vkzg_amd_verify_cell_kzg_proof_batch(vkzg_prover *p, int *ok, const uint8_t *commitments,
+                                                 const uint64_t *cell_indices, const uint8_t *cells,
+                                                 const uint8_t *proofs, size_t num_cells);
+
+/* blob_to_kzg_commitment for a batch: commitments (n x 48 bytes) of blobs
+ * (n x 131,072 bytes) against the prover's G1 setup.  INVALID_BLOB for a
+ * chunk with a non-canonical element (nothing written for it). */
+vkzg_result vkzg_amd_blob_to_commitments(vkzg_prover *p, uint8_t *commitments, const uint8_t *blobs,
+                                         size_t num_blobs);
+
 typedef struct {
     uint32_t window_bits;      /* phase-A table window c */
     uint32_t windows;          /* W = ceil(256 / c) */
@@ -171,6 +204,29 @@ vkzg_result vkzg_amd_debug_fp_mul(vkzg_prover *p, uint32_t *out, const uint32_t 
 vkzg_result vkzg_amd_debug_fp_inv(vkzg_prover *p, uint32_t *out, const uint32_t *a, size_t n);
 vkzg_result vkzg_amd_debug_fr_mul(vkzg_prover *p, uint32_t *out, const uint32_t *a,
                                   const uint32_t *b, size_t n);
+/* n device pairings e(P_i, Q_i)^3 (P_i 48-byte compressed G1, Q_i 96-byte
+ * compressed G2), each as 12 Fp values of 48 bytes, big-endian, in the order
+ * (w^0 real, w^0 imaginary, w^1 real, ...) of Fp12 = Fp2[w]/(w^6 - (1 + u)). */
+vkzg_result vkzg_amd_debug_pairing(vkzg_prover *p, uint8_t *out, const uint8_t *g1, const uint8_t *g2,
+                                   size_t n);
+/* sum_i [s_i] P_i by the device MSM (points 48-byte compressed, scalars
+ * 32-byte big-endian below r), compressed. */
+vkzg_result vkzg_amd_debug_g1_msm(vkzg_prover *p, uint8_t out[48], const uint8_t *points,
+                                  const uint8_t *scalars, size_t n);
+/* vkzg_amd_verify_cell_kzg_proof_batch that also returns the two G1 inputs
+ * of its pairing check, compressed: terms[0..48) = sum r^k proof_k,
+ * terms[48..96) = the right-hand G1 point RL (num_cells >= 1). */
+vkzg_result vkzg_amd_debug_verify_terms(vkzg_prover *p, int *ok, uint8_t terms[96], const uint8_t *commitments,
+                                        const uint64_t *cell_indices, const uint8_t *cells,
+                                        const uint8_t *proofs, size_t num_cells);
+/* Host-side helpers of verification (no device needed): the Fiat-Shamir
+ * challenge r (32 bytes big-endian) of a verification call, and SHA-256 with
+ * (portable = 0, when the CPU has the SHA extensions) or without them. */
+vkzg_result vkzg_amd_verify_challenge(uint8_t r_out[32], const uint8_t *commitments,
+                                      const uint64_t *cell_indices, const uint8_t *cells,
+                                      const uint8_t *proofs, size_t num_cells, int portable_sha);
+vkzg_result vkzg_amd_sha256(uint8_t out[32], const uint8_t *data, size_t len, int portable);
+int vkzg_amd_sha256_hw(void);
 
 #ifdef __cplusplus
 } /* extern "C" */

@@ -1711,3 +1711,159 @@ out:
     if (bad == 1) return -1;
     return bad ? 0 : 1;
 }
+
+
+/* ---- spec-weighted verification, G2 compression, MSM (test-side checks of
+ * the device verifier) ---- */
+
+/* As or_verify_cell_proofs, with the weights r^k of the consensus specs'
+ * verify_cell_kzg_proof_batch for a given challenge r (32 bytes big-endian,
+ * below r) instead of seeded ones.  The commitment sum is taken over cells
+ * (sum r^k C_k), which is the specs' sum over deduplicated commitments with
+ * summed weights.  out_a / out_b (may be NULL) receive, compressed,
+ * A = sum r^k pi_k and RL = sum r^k (C_k + z_k pi_k) - [sum r^k I_k](tau).
+ * Returns 1 (valid), 0 (invalid), -1 (bad encoding, non-canonical element or
+ * cell index >= 128). */
+int or_verify_cell_proofs_r(const or_vsetup *s, const uint8_t *commitments, const uint32_t *cell_indices,
+                            const uint8_t *cells, const uint8_t *proofs, size_t n, const uint8_t r_be[32],
+                            uint8_t out_a[48], uint8_t out_b[48]) {
+    fr r;
+    if (fr_from_be_canonical(&r, r_be)) return -1;
+    size_t m = n ? n : 1;
+    g1a *pi = (g1a *)malloc(sizeof(g1a) * m);
+    g1a *cm = (g1a *)malloc(sizeof(g1a) * m);
+    fr *w = (fr *)malloc(sizeof(fr) * m);
+    fr *wz = (fr *)malloc(sizeof(fr) * m);
+    fr (*ik)[64] = (fr (*)[64])malloc(sizeof(fr) * 64 * m);
+    int bad = 0;
+    fr pw = FR_ONE;
+    for (size_t k = 0; k < n; k++) {
+        w[k] = pw;
+        fr_mul(&pw, &pw, &r);
+        if (cell_indices[k] >= 128) bad = 1;
+    }
+    if (!bad) {
+#pragma omp parallel for schedule(static) reduction(| : bad)
+        for (int k = 0; k < (int)n; k++) {
+            if (g1a_decompress(&pi[k], proofs + 48 * (size_t)k) ||
+                g1a_decompress(&cm[k], commitments + 48 * (size_t)k)) { bad |= 1; continue; }
+            const uint32_t hc = brev(cell_indices[k], 7);
+            const fr z = ROOTS[(64u * hc) & 8191u];
+            fr_mul(&wz[k], &w[k], &z);
+            fr y[64];
+            for (uint32_t j = 0; j < 64; j++)
+                if (fr_from_be_canonical(&y[brev(j, 6)], cells + 2048 * (size_t)k + 32 * j)) bad |= 1;
+            fr_ntt_inplace(y, 64, 1);
+            fr_scale_inv_n(y, 64);
+            const fr hinv = ROOTS[(8192u - hc) & 8191u];
+            fr hp = w[k];
+            for (int i = 0; i < 64; i++) {
+                fr_mul(&ik[k][i], &y[i], &hp);
+                fr_mul(&hp, &hp, &hinv);
+            }
+        }
+    }
+    int rc = -1;
+    if (!bad) {
+        fr agg[64];
+        memset(agg, 0, sizeof(agg));
+        for (size_t k = 0; k < n; k++)
+            for (int i = 0; i < 64; i++) fr_add(&agg[i], &agg[i], &ik[k][i]);
+        g1j A, B, t;
+        msm_pippenger(&A, pi, w, n);
+        msm_pippenger(&B, pi, wz, n);
+        msm_pippenger(&t, cm, w, n);
+        g1_add(&B, &B, &t);
+        msm_pippenger(&t, s->srs, agg, 64);
+        g1_neg(&t, &t);
+        g1_add(&B, &B, &t);
+        if (out_a) g1_compress(out_a, &A);
+        if (out_b) g1_compress(out_b, &B);
+        g1_neg(&B, &B);
+        g1a ps[2];
+        g2a qs[2] = {s->g2_tau64, s->g2_gen};
+        g1_to_affine(&ps[0], &A);
+        g1_to_affine(&ps[1], &B);
+        rc = pairing_product_is_one(ps, qs, 2) ? 1 : 0;
+    }
+    free(pi);
+    free(cm);
+    free(w);
+    free(wz);
+    free(ik);
+    return rc;
+}
+
+/* ZCash compressed G2 (96 bytes: x.c1 then x.c0, big-endian, flags in the
+ * first byte; y is "larger" when y.c1 > (p-1)/2, or y.c1 == 0 and
+ * y.c0 > (p-1)/2) from / to this oracle's 192-byte encoding. */
+static int fp2_is_lex_largest(const fp2 *y) {
+    if (!fp_is_zero(&y->c1)) return fp_is_lex_largest(&y->c1);
+    return fp_is_lex_largest(&y->c0);
+}
+
+int or_g2_compress(uint8_t out[96], const uint8_t in[192]) {
+    g2a q;
+    if (g2_from_bytes(&q, in)) return -1;
+    memset(out, 0, 96);
+    if (q.inf) { out[0] = 0xc0; return 0; }
+    uint64_t plain[6];
+    fp_to_plain(plain, &q.x.c1);
+    limbs_to_be(out, plain, 6);
+    fp_to_plain(plain, &q.x.c0);
+    limbs_to_be(out + 48, plain, 6);
+    out[0] |= 0x80;
+    if (fp2_is_lex_largest(&q.y)) out[0] |= 0x20;
+    return 0;
+}
+
+int or_g2_decompress(uint8_t out[192], const uint8_t in[96]) {
+    const uint8_t b0 = in[0];
+    if (!(b0 & 0x80)) return -1;
+    g2a q;
+    memset(&q, 0, sizeof(q));
+    if (b0 & 0x40) {
+        if (b0 & 0x3f) return -1;
+        for (int k = 1; k < 96; k++) if (in[k]) return -1;
+        q.inf = 1;
+        g2_to_bytes(out, &q);
+        return 0;
+    }
+    uint8_t buf[48];
+    uint64_t plain[6];
+    memcpy(buf, in, 48);
+    buf[0] &= 0x1f;
+    be_to_limbs(plain, buf, 6);
+    if (cmp_n(plain, P_MOD, 6) >= 0) return -1;
+    fp_from_plain(&q.x.c1, plain);
+    be_to_limbs(plain, in + 48, 6);
+    if (cmp_n(plain, P_MOD, 6) >= 0) return -1;
+    fp_from_plain(&q.x.c0, plain);
+    fp2 rhs, b;
+    fp2_sqr(&rhs, &q.x);
+    fp2_mul(&rhs, &rhs, &q.x);
+    g2_b(&b);
+    fp2_add(&rhs, &rhs, &b);
+    if (fp2_sqrt(&q.y, &rhs)) return -1;
+    if (fp2_is_lex_largest(&q.y) != !!(b0 & 0x20)) fp2_neg(&q.y, &q.y);
+    g2_to_bytes(out, &q);
+    return 0;
+}
+
+/* sum [s_i] P_i (points compressed, scalars 32 bytes big-endian below r). */
+int or_g1_msm(uint8_t out[48], const uint8_t *points, const uint8_t *scalars, size_t n) {
+    size_t m = n ? n : 1;
+    g1a *ps = (g1a *)malloc(sizeof(g1a) * m);
+    fr *ks = (fr *)malloc(sizeof(fr) * m);
+    int bad = 0;
+    for (size_t i = 0; i < n && !bad; i++)
+        bad = g1a_decompress(&ps[i], points + 48 * i) || fr_from_be_canonical(&ks[i], scalars + 32 * i);
+    if (!bad) {
+        g1j r;
+        msm_pippenger(&r, ps, ks, n);
+        g1_compress(out, &r);
+    }
+    free(ps);
+    free(ks);
+    return bad ? -1 : 0;
+}

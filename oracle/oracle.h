@@ -122,6 +122,17 @@ int or_commit_batch(const or_vsetup *s, uint8_t *out, const uint8_t *blobs, size
 int or_verify_cell_proofs(const or_vsetup *s, const uint8_t *commitments, const uint32_t *cell_indices,
                           const uint8_t *cells, const uint8_t *proofs, size_t n, uint64_t seed);
 
+/* verify_cell_kzg_proof_batch with the weights r^k of a given challenge r;
+ * out_a / out_b receive the two G1 inputs of the pairing check (compressed). */
+int or_verify_cell_proofs_r(const or_vsetup *s, const uint8_t *commitments, const uint32_t *cell_indices,
+                            const uint8_t *cells, const uint8_t *proofs, size_t n, const uint8_t r_be[32],
+                            uint8_t out_a[48], uint8_t out_b[48]);
+/* ZCash-compressed G2 <-> the 192-byte encoding above. */
+int or_g2_compress(uint8_t out[96], const uint8_t in[192]);
+int or_g2_decompress(uint8_t out[192], const uint8_t in[96]);
+/* sum [s_i] P_i, compressed. */
+int or_g1_msm(uint8_t out[48], const uint8_t *points, const uint8_t *scalars, size_t n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -68,6 +68,11 @@ def lib():
         L.or_commit_batch.argtypes = [ctypes.c_void_p, c, u8p, ctypes.c_size_t]
         L.or_verify_cell_proofs.argtypes = [ctypes.c_void_p, u8p, ctypes.POINTER(ctypes.c_uint32), u8p, u8p,
                                             ctypes.c_size_t, ctypes.c_uint64]
+        L.or_verify_cell_proofs_r.argtypes = [ctypes.c_void_p, u8p, ctypes.POINTER(ctypes.c_uint32), u8p, u8p,
+                                              ctypes.c_size_t, u8p, c, c]
+        L.or_g2_compress.argtypes = [c, u8p]
+        L.or_g2_decompress.argtypes = [c, u8p]
+        L.or_g1_msm.argtypes = [c, u8p, u8p, ctypes.c_size_t]
         _lib = L
     return _lib
 
@@ -299,6 +304,19 @@ class VerifySetup:
             raise ValueError("malformed verification input")
         return rc == 1
 
+    def verify_r(self, commitments: bytes, cell_indices, cells: bytes, proofs: bytes, r: bytes):
+        """The specs' check with weights r^k for a given challenge r (32 bytes big-endian):
+        (valid, A, RL) with A = sum r^k pi_k and RL the other G1 input, compressed."""
+        n = len(cell_indices)
+        assert len(commitments) == 48 * n and len(cells) == 2048 * n and len(proofs) == 48 * n
+        idx = (ctypes.c_uint32 * max(1, n))(*cell_indices)
+        a = ctypes.create_string_buffer(48)
+        b = ctypes.create_string_buffer(48)
+        rc = lib().or_verify_cell_proofs_r(self._h, commitments, idx, cells, proofs, n, r, a, b)
+        if rc < 0:
+            raise ValueError("malformed verification input")
+        return rc == 1, a.raw, b.raw
+
     def verify_blobs(self, blobs: bytes, proofs: bytes, n: int, cells: bytes = None, seed: int = 1) -> bool:
         """Every proof of n blobs against the blobs' commitments and cells."""
         one = self.commit_batch(blobs, n)
@@ -307,3 +325,48 @@ class VerifySetup:
             cells = b"".join(blob_to_cells(blobs[b * BYTES_PER_BLOB:(b + 1) * BYTES_PER_BLOB])[1]
                              for b in range(n))
         return self.verify(coms, list(range(CELLS_PER_BLOB)) * n, cells, proofs, seed)
+
+
+def g2_compress(q192: bytes) -> bytes:
+    """ZCash 96-byte compression of a point in this oracle's 192-byte G2 encoding."""
+    out = ctypes.create_string_buffer(96)
+    if lib().or_g2_compress(out, q192):
+        raise ValueError("bad point")
+    return out.raw
+
+
+def g2_decompress(q96: bytes) -> bytes:
+    out = ctypes.create_string_buffer(192)
+    if lib().or_g2_decompress(out, q96):
+        raise ValueError("bad encoding")
+    return out.raw
+
+
+def g1_msm(points: bytes, scalars: bytes) -> bytes:
+    n = len(points) // 48
+    out = ctypes.create_string_buffer(48)
+    if lib().or_g1_msm(out, points, scalars, n):
+        raise ValueError("bad point or scalar")
+    return out.raw
+
+
+def verify_challenge(commitments: bytes, cell_indices, cells: bytes, proofs: bytes) -> bytes:
+    """compute_verify_cell_kzg_proof_batch_challenge of the consensus specs
+    (polynomial-commitments-sampling), with the deduplication of
+    verify_cell_kzg_proof_batch: 32 bytes big-endian, below r."""
+    import hashlib
+    n = len(cell_indices)
+    coms = [commitments[48 * k:48 * (k + 1)] for k in range(n)]
+    uniq = []
+    for c in coms:
+        if c not in uniq:
+            uniq.append(c)
+    h = b"RCKZGCBATCH__V1_"
+    h += (4096).to_bytes(8, "big") + (64).to_bytes(8, "big")
+    h += len(uniq).to_bytes(8, "big") + n.to_bytes(8, "big")
+    h += b"".join(uniq)
+    for k in range(n):
+        h += uniq.index(coms[k]).to_bytes(8, "big") + int(cell_indices[k]).to_bytes(8, "big")
+        h += cells[2048 * k:2048 * (k + 1)] + proofs[48 * k:48 * (k + 1)]
+    r = int.from_bytes(hashlib.sha256(h).digest(), "big") % R
+    return r.to_bytes(32, "big")

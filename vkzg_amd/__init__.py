@@ -140,6 +140,60 @@ class Prover:
         _check(load().vkzg_amd_recover_cells_and_proofs(self._h, out, proofs, c, pm, blob_count))
         return out.raw[:len(cells)], proofs.raw[:nproofs]
 
+    def load_g2_setup(self, g2_monomial):
+        """The 65 compressed G2 setup points [tau^i]G2 (65 x 96 bytes) for verification."""
+        g2 = bytes(g2_monomial)
+        _check(load().vkzg_amd_load_g2_setup(self._h, g2, len(g2)))
+
+    def verify_cell_kzg_proof_batch(self, commitments, cell_indices, cells, proofs):
+        """verify_cell_kzg_proof_batch: commitments, cells and proofs as flat bytes
+        (48, 2048 and 48 bytes per cell), cell_indices a sequence of ints."""
+        n = len(cell_indices)
+        if len(commitments) != 48 * n or len(proofs) != 48 * n or len(cells) != BYTES_PER_CELL * n:
+            raise ValueError("commitments, cells and proofs must have one entry per cell index")
+        ok = ctypes.c_int(0)
+        idx = (ctypes.c_uint64 * max(1, n))(*cell_indices)
+        _check(load().vkzg_amd_verify_cell_kzg_proof_batch(self._h, ctypes.byref(ok), bytes(commitments), idx,
+                                                           bytes(cells), bytes(proofs), n))
+        return bool(ok.value)
+
+    def debug_verify_terms(self, commitments, cell_indices, cells, proofs):
+        """(verdict, A, RL): the verification and the two G1 inputs of its pairing check."""
+        n = len(cell_indices)
+        ok = ctypes.c_int(0)
+        terms = ctypes.create_string_buffer(96)
+        idx = (ctypes.c_uint64 * max(1, n))(*cell_indices)
+        _check(load().vkzg_amd_debug_verify_terms(self._h, ctypes.byref(ok), terms, bytes(commitments), idx,
+                                                  bytes(cells), bytes(proofs), n))
+        return bool(ok.value), terms.raw[:48], terms.raw[48:]
+
+    def blob_to_kzg_commitments(self, blobs, blob_count):
+        """Commitments (blob_count x 48 bytes) of blob_count blobs."""
+        if len(blobs) != blob_count * BYTES_PER_BLOB:
+            raise ValueError("blobs.length must be blobCount * %d" % BYTES_PER_BLOB)
+        out = ctypes.create_string_buffer(max(1, 48 * blob_count))
+        src = ctypes.create_string_buffer(bytes(blobs), max(1, len(blobs)))
+        _check(load().vkzg_amd_blob_to_commitments(self._h, out, src, blob_count))
+        return out.raw[: 48 * blob_count]
+
+    def debug_pairing(self, g1_points, g2_points):
+        """Device e(P_i, Q_i)^3 for compressed P_i (48 B) and Q_i (96 B): 576 bytes each."""
+        n = len(g1_points) // 48
+        if len(g1_points) != 48 * n or len(g2_points) != 96 * n:
+            raise ValueError("need as many 48-byte G1 as 96-byte G2 points")
+        out = ctypes.create_string_buffer(max(1, 576 * n))
+        _check(load().vkzg_amd_debug_pairing(self._h, out, bytes(g1_points), bytes(g2_points), n))
+        return out.raw[: 576 * n]
+
+    def debug_g1_msm(self, points, scalars):
+        """sum [s_i] P_i on the device: 48-byte points, 32-byte big-endian scalars below r."""
+        n = len(points) // 48
+        if len(points) != 48 * n or len(scalars) != 32 * n:
+            raise ValueError("need as many 48-byte points as 32-byte scalars")
+        out = ctypes.create_string_buffer(48)
+        _check(load().vkzg_amd_debug_g1_msm(self._h, out, bytes(points), bytes(scalars), n))
+        return out.raw
+
     def get_tuning(self, blob_count):
         t = Tuning()
         _check(load().vkzg_amd_get_tuning(self._h, blob_count, ctypes.byref(t)))
@@ -357,3 +411,24 @@ __all__ = [
     "Prover", "Server", "VkzgError", "init", "deinit", "compute_cell_kzg_proofs_batch",
     "recover_cells_batch", "device_name", "gpu_core_count", "error_string", "LIB_PATH",
 ]
+
+
+def verify_challenge(commitments, cell_indices, cells, proofs, portable_sha=False):
+    """The Fiat-Shamir challenge r of a verification call (32 bytes big-endian), host only."""
+    n = len(cell_indices)
+    out = ctypes.create_string_buffer(32)
+    idx = (ctypes.c_uint64 * max(1, n))(*cell_indices)
+    _check(load().vkzg_amd_verify_challenge(out, bytes(commitments), idx, bytes(cells), bytes(proofs), n,
+                                            1 if portable_sha else 0))
+    return out.raw
+
+
+def sha256(data, portable=False):
+    """SHA-256 of the library's host implementation (SHA extensions unless portable)."""
+    out = ctypes.create_string_buffer(32)
+    _check(load().vkzg_amd_sha256(out, bytes(data), len(data), 1 if portable else 0))
+    return out.raw
+
+
+def sha256_hw():
+    return bool(load().vkzg_amd_sha256_hw())

@@ -45,6 +45,15 @@ EXT_SYMBOLS = [
     "vkzg_amd_set_tuning",
     "vkzg_amd_autotune",
     "vkzg_amd_set_graphs",
+    "vkzg_amd_load_g2_setup",
+    "vkzg_amd_verify_cell_kzg_proof_batch",
+    "vkzg_amd_blob_to_commitments",
+    "vkzg_amd_debug_pairing",
+    "vkzg_amd_debug_g1_msm",
+    "vkzg_amd_debug_verify_terms",
+    "vkzg_amd_verify_challenge",
+    "vkzg_amd_sha256",
+    "vkzg_amd_sha256_hw",
 ]
 
 
@@ -173,6 +182,25 @@ def load():
     L.vkzg_amd_autotune.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
     L.vkzg_amd_debug_fp_inv.restype = ctypes.c_int
     L.vkzg_amd_debug_fp_inv.argtypes = [vp, vp, vp, sz]
+    L.vkzg_amd_load_g2_setup.restype = ctypes.c_int
+    L.vkzg_amd_load_g2_setup.argtypes = [vp, u8p, sz]
+    L.vkzg_amd_verify_cell_kzg_proof_batch.restype = ctypes.c_int
+    L.vkzg_amd_verify_cell_kzg_proof_batch.argtypes = [vp, ctypes.POINTER(ctypes.c_int), u8p, u64p, u8p, u8p,
+                                                       sz]
+    L.vkzg_amd_blob_to_commitments.restype = ctypes.c_int
+    L.vkzg_amd_blob_to_commitments.argtypes = [vp, vp, vp, sz]
+    L.vkzg_amd_debug_pairing.restype = ctypes.c_int
+    L.vkzg_amd_debug_pairing.argtypes = [vp, vp, u8p, u8p, sz]
+    L.vkzg_amd_debug_g1_msm.restype = ctypes.c_int
+    L.vkzg_amd_debug_g1_msm.argtypes = [vp, vp, u8p, u8p, sz]
+    L.vkzg_amd_debug_verify_terms.restype = ctypes.c_int
+    L.vkzg_amd_debug_verify_terms.argtypes = [vp, ctypes.POINTER(ctypes.c_int), vp, u8p, u64p, u8p, u8p, sz]
+    L.vkzg_amd_verify_challenge.restype = ctypes.c_int
+    L.vkzg_amd_verify_challenge.argtypes = [vp, u8p, u64p, u8p, u8p, sz, ctypes.c_int]
+    L.vkzg_amd_sha256.restype = ctypes.c_int
+    L.vkzg_amd_sha256.argtypes = [vp, u8p, sz, ctypes.c_int]
+    L.vkzg_amd_sha256_hw.restype = ctypes.c_int
+    L.vkzg_amd_sha256_hw.argtypes = []
     for name in ("vkzg_amd_debug_fp_mul", "vkzg_amd_debug_fr_mul"):
         f = getattr(L, name)
         f.restype = ctypes.c_int

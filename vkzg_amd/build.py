@@ -19,7 +19,9 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("VKZG_AMD_ARCH", "gfx950")
 
 SOURCES = ["setup.hip", "setup_fft.hip", "setup_tables.hip", "fk20_scalars.hip", "phase_a.hip",
-           "ladder.hip", "phase_b.hip", "reduce.hip", "recover.hip", "prover.cpp", "server.cpp"]
+           "ladder.hip", "phase_b.hip", "reduce.hip", "recover.hip", "msm.hip", "pairing.hip",
+           "verify.hip", "prover.cpp", "server.cpp", "verify.cpp", "sha256.cc"]
+HOST_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-Wall"]  # plain host C++ (.cc): g++
 HIP_FLAGS = [
     "--offload-arch=" + ARCH,
     "-O3",
@@ -44,6 +46,9 @@ def run(cmd):
 
 def compile_one(src):
     obj = os.path.join(OUT_DIR, os.path.basename(src) + ".o")
+    if src.endswith(".cc"):
+        run(["g++"] + HOST_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
+        return obj
     cmd = [HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
     if src.endswith(".cpp"):
         cmd.insert(1, "-x")

@@ -388,4 +388,38 @@ VKZG_DEV_CALL void g1a_compress_words(uint32_t *w, const G1A &p) {
     for (int i = 0; i < 12; i++) w[i] = __builtin_bswap32(w[i]);
 }
 
+// 48-byte ZCash compressed encoding (src: its 12 words as stored, 4-byte
+// aligned) -> affine.  The identity decodes to (0, 0).  False for a
+// malformed encoding: no compression flag, stray bits after the infinity
+// flag, x >= p or an x with no point on the curve.  No subgroup check.
+VKZG_DEV bool g1a_decompress(G1A &out, const uint32_t *src) {
+    uint32_t w[12];
+#pragma unroll
+    for (int k = 0; k < 12; k++) w[k] = __builtin_bswap32(src[k]);
+    const uint32_t b0 = w[0] >> 24;
+    bool ok = (b0 & 0x80u) != 0;
+    if (b0 & 0x40u) {
+        uint32_t rest = w[0] & 0x3fffffffu;
+#pragma unroll
+        for (int k = 1; k < 12; k++) rest |= w[k];
+        out = g1a_identity();
+        return ok && rest == 0;
+    }
+    uint32_t xw[12];
+#pragma unroll
+    for (int k = 0; k < 12; k++) xw[k] = w[11 - k];
+    xw[11] &= 0x1fffffffu;
+    const Fp x = fp_from_words(xw);
+    if (!fp_limbs_lt_p(x)) ok = false;
+    const Fp xm = fp_from_canonical(x);
+    const Fp rhs = fp_add(fp_mul(fp_sqr(xm), xm), fp_const(cfp::FOUR_M));
+    Fp y = fp_pow_const<12>(rhs, cfp::SQRT_EXP);
+    if (!fp_eq(fp_sqr(y), rhs)) ok = false;
+    const bool want_large = (b0 & 0x20u) != 0;
+    if (fp_is_lex_largest(fp_to_canonical(y)) != want_large) y = fp_neg(y);
+    out.x = xm;
+    out.y = y;
+    return ok;
+}
+
 }  // namespace vkzg_amd

@@ -615,6 +615,7 @@ vkzg_prover::~vkzg_prover() {
     (void)hipSetDevice(device);
     if (stream) (void)hipStreamSynchronize(stream);
     (void)hipDeviceSynchronize();
+    vkzg_amd::destroy_verify_state(vfy);
     graphs.clear();
     ws.arena.release();
     arena.release();

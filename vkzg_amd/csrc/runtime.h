@@ -206,6 +206,10 @@ struct GraphCache {
     }
 };
 
+// Verification state (verify.cpp): G2 setup lines and per-call buffers.
+struct VerifyState;
+void destroy_verify_state(VerifyState *v);
+
 }  // namespace vkzg_amd
 
 struct vkzg_prover {
@@ -239,6 +243,9 @@ struct vkzg_prover {
     vkzg_amd::WorkSet ws;
     uint8_t *h_in = nullptr, *h_out = nullptr, *h_present = nullptr;
     uint32_t *h_err = nullptr;
+
+    // Batch verification and commitments (created on first use).
+    vkzg_amd::VerifyState *vfy = nullptr;
 
     // Per-stage timing of the last call (events on the stream of the first
     // sub-batch), accumulated over its chunks.

@@ -36,38 +36,10 @@ __global__ void k_roots(Fr *roots, Fr *roots_inv, Fr *shift_fwd, Fr *shift_inv) 
 __global__ void k_decompress(G1A *out, const uint8_t *in, uint32_t n, uint32_t *err) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const uint32_t *src = reinterpret_cast<const uint32_t *>(in + 48ull * i);
-    uint32_t w[12];
-#pragma unroll
-    for (int k = 0; k < 12; k++) w[k] = __builtin_bswap32(src[k]);
-    const uint32_t b0 = w[0] >> 24;
-    bool bad = (b0 & 0x80u) == 0;
-    if (b0 & 0x40u) {
-        uint32_t rest = w[0] & 0x3fffffffu;
-#pragma unroll
-        for (int k = 1; k < 12; k++) rest |= w[k];
-        if (rest) bad = true;
-        out[i] = g1a_identity();
-        if (bad) atomicOr(err, 1u);
-        return;
-    }
-    uint32_t xw[12];
-#pragma unroll
-    for (int k = 0; k < 12; k++) xw[k] = w[11 - k];
-    xw[11] &= 0x1fffffffu;
-    const Fp x = fp_from_words(xw);
-    if (!fp_limbs_lt_p(x)) bad = true;
-    const Fp xm = fp_from_canonical(x);
-    const Fp rhs = fp_add(fp_mul(fp_sqr(xm), xm), fp_const(cfp::FOUR_M));
-    Fp y = fp_pow_const<12>(rhs, cfp::SQRT_EXP);
-    if (!fp_eq(fp_sqr(y), rhs)) bad = true;
-    const bool want_large = (b0 & 0x20u) != 0;
-    if (fp_is_lex_largest(fp_to_canonical(y)) != want_large) y = fp_neg(y);
     G1A r;
-    r.x = xm;
-    r.y = y;
+    const bool ok = g1a_decompress(r, reinterpret_cast<const uint32_t *>(in + 48ull * i));
     out[i] = r;
-    if (bad) atomicOr(err, 1u);
+    if (!ok) atomicOr(err, 1u);
 }
 
 // Split-circulant phase-B scalars, canonical: out[e] = K+[e], out[64+e] = K-[e]
