@@ -6,8 +6,9 @@ batch with the blobs already resident in HBM; one step = one 256-blob call
 through vkzg_amd_compute_proofs_device.  Prints ONE JSON line.
 
 Also measured (in "secondary"): recoverCellsBatch on 64 blobs with exactly 64
-of 128 cells present, the plain one-call-per-blob proof form, and the
-per-stage HIP-event times of the timed steps.  The CPU baseline is the
+of 128 cells present, the plain one-call-per-blob proof form, the per-stage
+HIP-event times of the timed steps, batch cell-proof verification and
+commitments on a seeded setup, and the request server.  The CPU baseline is the
 oracle (a C port of the reference's algorithm) on a bounded sample.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
@@ -66,6 +67,63 @@ def cpu_baseline(sample_blobs, seed):
                       "reference's FK20: bucket phase A on the 2^(8d) position table, G1 FFT "
                       "phase B), %.1f s; setup build excluded; the reference itself needs "
                       "Vulkan/lavapipe, absent here" % (sample_blobs, dt)}
+
+
+def verify_bench(seed, nb=64, reps=5):
+    """verify_cell_kzg_proof_batch and blob_to_kzg_commitment through the host C
+    ABI (host copies and the SHA-256 challenge included) on the seeded setup of
+    tests/golden/seeded_setup.bin, whose G2 side makes every verdict real: the
+    proofs and commitments are the device's own, and every timed call must
+    return valid."""
+    import ctypes
+    import vkzg_amd
+    from vkzg_amd import load
+    data = open(os.path.join(HERE, "tests", "golden", "seeded_setup.bin"), "rb").read()
+    vp = vkzg_amd.Prover(max_batch_size=nb, setup_g1_monomial=data[:4096 * 48])
+    vp.load_g2_setup(data[4096 * 48:])
+    blobs = seeded_blobs(nb, seed)
+    cells, proofs = vp.compute_cells_and_kzg_proofs_batch(blobs, nb)
+    coms = vp.blob_to_kzg_commitments(blobs, nb)
+    L = load()
+    out = {}
+    t0 = time.perf_counter()
+    for _ in range(3):
+        assert vp.blob_to_kzg_commitments(blobs, nb) == coms
+    out["commitments_%d_blobs_per_s" % nb] = 3 * nb / (time.perf_counter() - t0)
+
+    def timed(ncells, picks):
+        c = ctypes.create_string_buffer(b"".join(coms[48 * b:48 * (b + 1)] for b, _ in picks), 48 * ncells)
+        idx = (ctypes.c_uint64 * ncells)(*[i for _, i in picks])
+        ce = ctypes.create_string_buffer(b"".join(cells[(128 * b + i) * 2048:(128 * b + i + 1) * 2048]
+                                                  for b, i in picks), 2048 * ncells)
+        pr = ctypes.create_string_buffer(b"".join(proofs[(128 * b + i) * 48:(128 * b + i + 1) * 48]
+                                                  for b, i in picks), 48 * ncells)
+        ok = ctypes.c_int(0)
+
+        def call():
+            rc = L.vkzg_amd_verify_cell_kzg_proof_batch(vp.handle, ctypes.byref(ok), c, idx, ce, pr, ncells)
+            assert rc == 0 and ok.value == 1
+        call()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            call()
+        dt = (time.perf_counter() - t0) / reps
+        t0 = time.perf_counter()
+        vkzg_amd.verify_challenge(c.raw, list(idx), ce.raw, pr.raw)
+        return dt, time.perf_counter() - t0
+
+    all_picks = [(b, i) for b in range(nb) for i in range(128)]
+    dt, hash_dt = timed(len(all_picks), all_picks)
+    out["verify_%d_cells_per_s" % len(all_picks)] = len(all_picks) / dt
+    out["verify_%d_cells_ms" % len(all_picks)] = dt * 1e3
+    out["verify_%d_cells_challenge_ms" % len(all_picks)] = hash_dt * 1e3
+    for ncells in (128, 1):
+        dt, hash_dt = timed(ncells, all_picks[:ncells])
+        out["verify_%d_cells_ms" % ncells] = dt * 1e3
+    out["verify_setup"] = ("seeded tau (tests/golden/seeded_setup.bin), %d blobs' cells, proofs and "
+                           "commitments computed on the device; every timed call returned valid" % nb)
+    vp.close()
+    return out
 
 
 def serve_bench(prover, rng, seconds, rate, probe_rate, slots):
@@ -201,6 +259,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default tunings")
+    ap.add_argument("--no-verify", action="store_true", help="skip the verification/commitment leg")
     ap.add_argument("--serve-seconds", type=float, default=4.0)
     ap.add_argument("--serve-rate", type=float, default=150.0, help="mix requests per second")
     ap.add_argument("--probe-rate", type=float, default=25.0, help="6-blob proof probes per second")
@@ -379,6 +438,9 @@ def main():
         for _ in range(3):
             prover.recover_cells_and_kzg_proofs_batch(cells_out, bytes(keep), nc)
         secondary["recover_cells_and_proofs_host_%d_blobs_per_s" % nc] = 3 * nc / (time.perf_counter() - t0)
+
+    if not a.no_secondary and not a.no_verify:
+        secondary["verify"] = verify_bench(a.seed + 4)
 
     if not a.no_secondary and a.serve_seconds > 0:
         secondary["serve"] = serve_bench(prover, random.Random(a.seed + 3), a.serve_seconds,

@@ -143,3 +143,14 @@ def test_oracle_spec_weighted_verify(oracle_mod, seeded_verify_setup, rng):
     bad_cells[2048 + 31] ^= 1
     assert not seeded_verify_setup.verify_r(coms, idx, bytes(bad_cells), sel_proofs, r)[0]
     assert not seeded_verify_setup.verify(coms, idx, bytes(bad_cells), sel_proofs)
+
+
+def test_seeded_setup_fixture(seeded_setup_bytes, oracle_mod):
+    """tests/golden/seeded_setup.bin (read by bench.py) is the oracle's seeded setup."""
+    from tests.conftest import GOLDEN, SEEDED_TAU
+    data = open(os.path.join(GOLDEN, "seeded_setup.bin"), "rb").read()
+    assert len(data) == 4096 * 48 + 65 * 96
+    assert data[:4096 * 48] == seeded_setup_bytes
+    raw = oracle_mod.g2_powers_from_tau(SEEDED_TAU, 65)
+    for i in (0, 1, 63, 64):
+        assert oracle_mod.g2_decompress(data[4096 * 48 + 96 * i:4096 * 48 + 96 * (i + 1)]) == raw[192 * i:192 * (i + 1)]

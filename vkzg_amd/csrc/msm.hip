@@ -11,9 +11,10 @@
 //   count    histogram of (msm, window, |digit|) keys     (atomics)
 //   scan     bucket offsets (one block)
 //   scatter  point references sorted by key               (atomics)
-//   buckets  one lane per bucket sums its points          (XYZZ, checked)
-//   windows  one block per (msm, window): sum_b (b+1) B_b by running sums
-//            over 128 segments, then a tree in LDS
+//   buckets  one lane per bucket sums its points          (XYZZ, checked);
+//            buckets of more than 24 points, one block each
+//   windows  blocks per (msm, window): sum_b (b+1) B_b by running sums over
+//            segments of at most 4 buckets, then trees in LDS
 //   combine  one lane per MSM: Horner over windows (c doublings each) and the
 //            conversion to affine.
 // The point order inside a bucket depends on atomics, but the sum is the same
@@ -28,6 +29,9 @@ namespace vkzg_amd {
 namespace {
 
 constexpr uint32_t kWinThreads = 128;
+constexpr uint32_t kWinSeg = 4;  // buckets per thread of the window sums (more parts beyond)
+constexpr uint32_t kHotLen = 24; // longer buckets are summed by a block (k_msm_hot)
+constexpr uint32_t kHotBlocks = 512;
 constexpr uint32_t kPtMask = (1u << 30) - 1;
 
 // Signed digit w of a 128-bit scalar (4 words) with window c; carry in/out.
@@ -128,45 +132,83 @@ __global__ void k_msm_scatter(uint32_t *list, uint32_t *cur, const uint32_t *hs,
     }
 }
 
+// The point a sorted reference names: phi(P) when bit 30 is set, negated when bit 31 is.
+VKZG_DEV G1A msm_point(const G1A *pts, uint32_t ref) {
+    G1A p = pts[ref & kPtMask];
+    if (ref & (1u << 30)) p.x = fp_mul(p.x, fp_const(cfp::BETA_M));
+    return g1a_neg_if(p, (ref >> 31) != 0);
+}
+
+// One lane per bucket.  A bucket with more than kHotLen points (the few
+// digits of a narrow top window collect most points of every scalar) goes to
+// the hot list instead, for k_msm_hot.
 __global__ void k_msm_buckets(G1X *buckets, const uint32_t *off, const uint32_t *list, const G1A *pts,
-                              uint32_t K) {
+                              uint32_t K, uint32_t *hot, uint32_t *hot_n) {
     const uint32_t key = blockIdx.x * blockDim.x + threadIdx.x;
     if (key >= K) return;
-    G1X acc = g1x_identity();
     const uint32_t lo = off[key], hi = off[key + 1];
-#pragma unroll 1
-    for (uint32_t q = lo; q < hi; q++) {
-        const uint32_t ref = list[q];
-        G1A p = pts[ref & kPtMask];
-        if (ref & (1u << 30)) p.x = fp_mul(p.x, fp_const(cfp::BETA_M));
-        p = g1a_neg_if(p, (ref >> 31) != 0);
-        acc = g1x_madd(acc, p);
+    if (hi - lo > kHotLen) {
+        hot[atomicAdd(hot_n, 1u)] = key;
+        return;
     }
+    G1X acc = g1x_identity();
+#pragma unroll 1
+    for (uint32_t q = lo; q < hi; q++) acc = g1x_madd(acc, msm_point(pts, list[q]));
     buckets[key] = acc;
 }
 
-// [k] p for k < 2^16 (double-and-add, checked operations)
-VKZG_DEV G1X g1x_mul_small(const G1X &p, uint32_t k) {
-    G1X acc = g1x_identity();
+// Hot buckets, one block each (blocks stride over the list): 64 lanes sum
+// interleaved shares of the bucket's points, then a tree in LDS.
+__global__ void __launch_bounds__(64) k_msm_hot(G1X *buckets, const uint32_t *off, const uint32_t *list,
+                                                const G1A *pts, const uint32_t *hot, const uint32_t *hot_n) {
+    __shared__ G1X sh[64];
+    const uint32_t t = threadIdx.x;
+    const uint32_t n = *hot_n;
 #pragma unroll 1
-    for (int b = 15; b >= 0; b--) {
+    for (uint32_t q = blockIdx.x; q < n; q += gridDim.x) {
+        const uint32_t key = hot[q];
+        const uint32_t lo = off[key], hi = off[key + 1];
+        G1X acc = g1x_identity();
+#pragma unroll 1
+        for (uint32_t i = lo + t; i < hi; i += 64) acc = g1x_madd(acc, msm_point(pts, list[i]));
+        sh[t] = acc;
+        __syncthreads();
+        for (uint32_t stride = 32; stride > 0; stride >>= 1) {
+            if (t < stride) sh[t] = g1x_add(sh[t], sh[t + stride]);
+            __syncthreads();
+        }
+        if (t == 0) buckets[key] = sh[0];
+        __syncthreads();
+    }
+}
+
+// [k] p for 0 < k < 2^16 (double-and-add from the top set bit, checked operations)
+VKZG_DEV G1X g1x_mul_small(const G1X &p, uint32_t k) {
+    G1X acc = p;
+#pragma unroll 1
+    for (int b = 30 - __builtin_clz(k); b >= 0; b--) {
         acc = g1x_dbl(acc);
         if ((k >> b) & 1u) acc = g1x_add(acc, p);
     }
     return acc;
 }
 
-// Block (msm, window): sum_{b < bk} (b + 1) B_b.  Thread t owns buckets
-// [t seg, (t + 1) seg): running sums from the top give R = sum B_b and
-// T = sum (b - lo + 1) B_b, and its share is T + lo R.
-__global__ void __launch_bounds__(kWinThreads) k_msm_windows(G1X *wsum, const G1X *buckets, uint32_t bk) {
+// Block (msm, window, part): sum over the part's buckets b of (b + 1) B_b.
+// Thread t owns buckets [lo, lo + seg): running sums from the top give
+// R = sum B_b and T = sum (b - lo + 1) B_b, and its share is T + lo R; a
+// tree in LDS adds the shares.  A window's parts (enough of them for at most
+// kWinSeg buckets per thread) are added by k_msm_parts.
+__global__ void __launch_bounds__(kWinThreads) k_msm_windows(G1X *wpart, const G1X *buckets, uint32_t bk,
+                                                             uint32_t parts) {
     __shared__ G1X sh[kWinThreads];
     const uint32_t t = threadIdx.x;
-    const G1X *B = buckets + (size_t)blockIdx.x * bk;
-    const uint32_t seg = bk > kWinThreads ? bk / kWinThreads : 1;
-    const uint32_t lo = t * seg;
+    const uint32_t win = blockIdx.x / parts, part = blockIdx.x % parts;
+    const G1X *B = buckets + (size_t)win * bk;
+    const uint32_t per_part = bk / parts;
+    const uint32_t seg = per_part > kWinThreads ? per_part / kWinThreads : 1;
+    const uint32_t lo = part * per_part + t * seg;
     G1X share = g1x_identity();
-    if (lo < bk) {
+    if (t * seg < per_part) {
         G1X R = g1x_identity(), T = g1x_identity();
 #pragma unroll 1
         for (uint32_t b = lo + seg; b-- > lo;) {
@@ -181,21 +223,49 @@ __global__ void __launch_bounds__(kWinThreads) k_msm_windows(G1X *wsum, const G1
         if (t < stride) sh[t] = g1x_add(sh[t], sh[t + stride]);
         __syncthreads();
     }
+    if (t == 0) wpart[blockIdx.x] = sh[0];
+}
+
+// wsum[win] = sum of the window's parts (a tree over at most 64 parts).
+__global__ void __launch_bounds__(64) k_msm_parts(G1X *wsum, const G1X *wpart, uint32_t parts) {
+    __shared__ G1X sh[64];
+    const uint32_t t = threadIdx.x;
+    sh[t] = t < parts ? wpart[(size_t)blockIdx.x * parts + t] : g1x_identity();
+    __syncthreads();
+    for (uint32_t stride = 32; stride > 0; stride >>= 1) {
+        if (t < stride) sh[t] = g1x_add(sh[t], sh[t + stride]);
+        __syncthreads();
+    }
     if (t == 0) wsum[blockIdx.x] = sh[0];
 }
 
+// Horner over the windows with the checked addition (the rare path of k_msm_combine).
+VKZG_DEV_COLD void combine_checked(G1X *acc, const G1X *ws, uint32_t W, uint32_t c) {
+    G1X a = ws[W - 1];
+    for (uint32_t w = W - 1; w-- > 0;) {
+        for (uint32_t i = 0; i < c; i++) a = g1x_dbl(a);
+        a = g1x_add(a, ws[w]);
+    }
+    *acc = a;
+}
+
 // out[m] = sum_w 2^(c w) wsum[m W + w], affine ((0, 0) for the identity).
+// The doubling chain is this launch's critical path, so the loop uses the
+// call-free addition; an equal or opposite pair (flagged) redoes it checked.
 __global__ void __launch_bounds__(64) k_msm_combine(G1A *out, const G1X *wsum, uint32_t nmsm, uint32_t W,
                                                     uint32_t c) {
     const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= nmsm) return;
-    G1X acc = wsum[(size_t)m * W + W - 1];
+    const G1X *ws = wsum + (size_t)m * W;
+    G1X acc = ws[W - 1];
+    bool exc = false;
 #pragma unroll 1
     for (uint32_t w = W - 1; w-- > 0;) {
 #pragma unroll 1
         for (uint32_t i = 0; i < c; i++) acc = g1x_dbl(acc);
-        acc = g1x_add(acc, wsum[(size_t)m * W + w]);
+        acc = g1x_add_fast(acc, ws[w], exc);
     }
+    if (exc) combine_checked(&acc, ws, W, c);
     G1A r = g1a_identity();
     if (!g1x_is_identity(acc)) {
         // x = X / ZZ = X ZZZ / (ZZ ZZZ), y = Y / ZZZ = Y ZZ / (ZZ ZZZ): one inversion
@@ -220,6 +290,7 @@ MsmShape msm_shape(uint32_t ne, uint32_t nmsm) {
     sh.W = (129 + c - 1) / c;
     sh.bk = 1u << (c - 1);
     sh.K = nmsm * sh.W * sh.bk;
+    sh.parts = sh.bk > kWinThreads * kWinSeg ? sh.bk / (kWinThreads * kWinSeg) : 1;
     return sh;
 }
 
@@ -227,11 +298,11 @@ size_t msm_scratch_bytes(const MsmShape &sh) {
     const size_t nh = 2ull * sh.ne;
     size_t b = 0;
     b += 16 * nh;                        // half scalars
-    b += 4ull * (3 * (size_t)sh.K + 1);  // cnt, cur, off
+    b += 4ull * (4 * (size_t)sh.K + 3);  // cnt (+ hot count), cur, off, hot list
     b += 4 * nh * sh.W;                  // sorted references
     b += sizeof(G1X) * (size_t)sh.K;     // buckets
-    b += sizeof(G1X) * (size_t)sh.nmsm * sh.W;
-    return b + 6 * 256;
+    b += sizeof(G1X) * (size_t)sh.nmsm * sh.W * (sh.parts + 1);
+    return b + 16 * 256;  // each of the 9 regions is rounded up to 256 bytes
 }
 
 hipError_t launch_msm(const MsmShape &sh, void *scratch, const G1A *pts, const uint32_t *pidx, const Fr *sc,
@@ -245,21 +316,28 @@ hipError_t launch_msm(const MsmShape &sh, void *scratch, const G1A *pts, const u
         return r;
     };
     uint32_t *hs = reinterpret_cast<uint32_t *>(take(16ull * nh));
-    uint32_t *cnt = reinterpret_cast<uint32_t *>(take(4ull * sh.K));
+    uint32_t *cnt = reinterpret_cast<uint32_t *>(take(4ull * (sh.K + 1)));  // cnt[K]: hot-bucket count
+    uint32_t *hot = reinterpret_cast<uint32_t *>(take(4ull * sh.K));
     uint32_t *cur = reinterpret_cast<uint32_t *>(take(4ull * sh.K));
     uint32_t *off = reinterpret_cast<uint32_t *>(take(4ull * (sh.K + 1)));
     uint32_t *list = reinterpret_cast<uint32_t *>(take(4ull * nh * sh.W));
     G1X *buckets = reinterpret_cast<G1X *>(take(sizeof(G1X) * (size_t)sh.K));
     G1X *wsum = reinterpret_cast<G1X *>(take(sizeof(G1X) * (size_t)sh.nmsm * sh.W));
+    G1X *wpart = reinterpret_cast<G1X *>(take(sizeof(G1X) * (size_t)sh.nmsm * sh.W * sh.parts));
+    if ((size_t)(p - static_cast<uint8_t *>(scratch)) > msm_scratch_bytes(sh)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_msm_split, dim3(blocks_for(sh.ne, 256)), dim3(256), 0, s, hs, sc, sh.ne);
-    hipLaunchKernelGGL(k_msm_zero, dim3(blocks_for(sh.K, 256)), dim3(256), 0, s, cnt, sh.K);
+    hipLaunchKernelGGL(k_msm_zero, dim3(blocks_for(sh.K + 1, 256)), dim3(256), 0, s, cnt, sh.K + 1);
     hipLaunchKernelGGL(k_msm_count, dim3(blocks_for(nh, 256)), dim3(256), 0, s, cnt, hs, mid, nh, sh.c, sh.W);
     hipLaunchKernelGGL(k_msm_scan, dim3(1), dim3(1024), 0, s, off, cur, cnt, sh.K);
     hipLaunchKernelGGL(k_msm_scatter, dim3(blocks_for(nh, 256)), dim3(256), 0, s, list, cur, hs, mid, pidx, nh,
                        sh.c, sh.W);
     hipLaunchKernelGGL(k_msm_buckets, dim3(blocks_for(sh.K, 64)), dim3(64), 0, s, buckets, off, list, pts,
-                       sh.K);
-    hipLaunchKernelGGL(k_msm_windows, dim3(sh.nmsm * sh.W), dim3(kWinThreads), 0, s, wsum, buckets, sh.bk);
+                       sh.K, hot, cnt + sh.K);
+    hipLaunchKernelGGL(k_msm_hot, dim3(sh.K < kHotBlocks ? sh.K : kHotBlocks), dim3(64), 0, s, buckets, off, list,
+                       pts, hot, cnt + sh.K);
+    hipLaunchKernelGGL(k_msm_windows, dim3(sh.nmsm * sh.W * sh.parts), dim3(kWinThreads), 0, s, wpart, buckets,
+                       sh.bk, sh.parts);
+    hipLaunchKernelGGL(k_msm_parts, dim3(sh.nmsm * sh.W), dim3(64), 0, s, wsum, wpart, sh.parts);
     hipLaunchKernelGGL(k_msm_combine, dim3(blocks_for(sh.nmsm, 64)), dim3(64), 0, s, out, wsum, sh.nmsm, sh.W,
                        sh.c);
     return hipGetLastError();

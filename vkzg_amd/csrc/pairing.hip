@@ -222,17 +222,19 @@ __device__ void fp12_frob2(Fp *dst, const Fp *src) {
 // the result (12 Fp, Montgomery).
 __global__ void __launch_bounds__(kPairThreads) k_pair_final(uint32_t *status, Fp *out12, const Fp *L) {
     __shared__ Fp f[12], g[12], h[12], A[12], B[12], C[12], D[12];
+    __shared__ Fp Ls[kPairSteps * 12];  // the step products, staged in LDS (45 KiB)
     __shared__ Fp12Scratch sc;
     __shared__ uint32_t ok;
     const uint32_t t = threadIdx.x;
+    for (uint32_t i = t; i < kPairSteps * 12; i += blockDim.x) Ls[i] = L[i];
     fp12_set_one(f, t);
     __syncthreads();
     uint32_t s = 0;
 #pragma unroll 1
     for (int i = 62; i >= 0; i--) {
         fp12_mul_lds(f, f, f, sc);
-        fp12_mul_lds(f, f, L + 12 * s++, sc);
-        if ((kZAbs >> i) & 1u) fp12_mul_lds(f, f, L + 12 * s++, sc);
+        fp12_mul_lds(f, f, Ls + 12 * s++, sc);
+        if ((kZAbs >> i) & 1u) fp12_mul_lds(f, f, Ls + 12 * s++, sc);
     }
     // ---- easy part: f^((p^6 - 1)(p^2 + 1))
     if (t < 12) g[t] = ((t >> 1) & 1u) ? fp_neg(f[t]) : f[t];  // conj(f)

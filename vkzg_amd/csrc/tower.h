@@ -109,8 +109,10 @@ struct Fp12Scratch {
 // dst = a * b for Fp12 values held in LDS (12 Fp each).  Every thread of the
 // block calls it (it contains barriers); blockDim.x >= 144.  dst may alias a
 // or b.  Product a_{m1} b_{m2} lands on w^(m1+m2); a power m + 6 is
-// multiplied by xi: (x + y u) xi = (x - y) + (x + y) u.
-__device__ inline void fp12_mul_lds(Fp *dst, const Fp *a, const Fp *b, Fp12Scratch &s) {
+// multiplied by xi: (x + y u) xi = (x - y) + (x + y) u.  Inlined, so that
+// every operand access is an LDS access (a generic pointer would make them
+// flat loads with several times the latency).
+__device__ __forceinline__ void fp12_mul_lds(Fp *dst, const Fp *a, const Fp *b, Fp12Scratch &s) {
     const uint32_t t = threadIdx.x;
     if (t < 144) s.prod[t] = fp_mul(a[t / 12], b[t % 12]);
     __syncthreads();
@@ -130,12 +132,13 @@ __device__ inline void fp12_mul_lds(Fp *dst, const Fp *a, const Fp *b, Fp12Scrat
         s.part[t] = v;
     }
     __syncthreads();
-    if (t < 12) {
-        Fp acc = s.part[6 * t];
-#pragma unroll 1
-        for (int k = 1; k < 6; k++) acc = fp_add(acc, s.part[6 * t + k]);
-        dst[t] = acc;
+    // six partials per output: pairs first (36 lanes), then three (12 lanes)
+    if (t < 36) {
+        const uint32_t o = t / 3, q = t % 3;
+        s.prod[t] = fp_add(s.part[6 * o + 2 * q], s.part[6 * o + 2 * q + 1]);
     }
+    __syncthreads();
+    if (t < 12) dst[t] = fp_add(fp_add(s.prod[3 * t], s.prod[3 * t + 1]), s.prod[3 * t + 2]);
     __syncthreads();
 }
 
