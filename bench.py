@@ -346,7 +346,15 @@ def main():
     stream.synchronize()
     graph_value = nb * a.steps / (time.perf_counter() - t0)
 
-    secondary = {"graph_replay_blobs_per_s": graph_value, "tuning": tuning,
+    # ceiling of this build's Fp multiplication: the chip's rate of dependent
+    # products at 2 waves per SIMD (the hot kernels' occupancy) and at 8
+    fp_rates = {str(w): prover.debug_fp_mul_rate(w)[0] for w in (1, 2, 4, 8)}
+    one_wave_cycles = prover.debug_fp_mul_rate(1)[1]
+    phase_a_mults = nb * 128 * (64 * info["windows"] * 10 + 15 * 14)
+    fp_ceiling = {"fp_mul_per_s_by_waves_per_simd": fp_rates, "one_wave_cycles_per_fp_mul": one_wave_cycles,
+                  "phase_a_fp_mul_per_s": phase_a_mults / (stage_ms["phase_a"] * 1e-3),
+                  "phase_a_over_2_wave_ceiling": phase_a_mults / (stage_ms["phase_a"] * 1e-3) / fp_rates["2"]}
+    secondary = {"graph_replay_blobs_per_s": graph_value, "tuning": tuning, "fp_mul_ceiling": fp_ceiling,
                  "stage_ms": stage_ms, "stage_frac_of_int_peak": frac,
                  "pipeline_frac_of_int_peak_vs_reference_work": value * W_PROOF / PEAK_INT,
                  "phase_a_window_bits": info["window_bits"],

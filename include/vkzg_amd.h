@@ -204,6 +204,12 @@ vkzg_result vkzg_amd_debug_fp_mul(vkzg_prover *p, uint32_t *out, const uint32_t 
 vkzg_result vkzg_amd_debug_fp_inv(vkzg_prover *p, uint32_t *out, const uint32_t *a, size_t n);
 vkzg_result vkzg_amd_debug_fr_mul(vkzg_prover *p, uint32_t *out, const uint32_t *a,
                                   const uint32_t *b, size_t n);
+/* Issue-rate measurement of the device Fp multiplication: every lane of
+ * CUs x 4 x waves_per_simd one-wave blocks (1-8 waves per SIMD) runs `iters`
+ * dependent products; *muls_per_s is the chip's rate, *one_wave_cycles the
+ * cycles per product of a single wave alone (its dependency latency). */
+vkzg_result vkzg_amd_debug_fp_mul_rate(vkzg_prover *p, uint32_t waves_per_simd, uint32_t iters,
+                                       double *muls_per_s, double *one_wave_cycles);
 /* n device pairings e(P_i, Q_i)^3 (P_i 48-byte compressed G1, Q_i 96-byte
  * compressed G2), each as 12 Fp values of 48 bytes, big-endian, in the order
  * (w^0 real, w^0 imaginary, w^1 real, ...) of Fp12 = Fp2[w]/(w^6 - (1 + u)). */

@@ -49,3 +49,13 @@ def test_fp_inverse_binary_gcd():
     got = vkzg_amd.debug_fp_inv([limbs(x, 12) for x in a])
     for x, g in zip(a, got):
         assert value(g) == (pow(x, -1, P) if x else 0), hex(x)
+
+
+def test_fp_mul_rate_hook(gpu_prover):
+    """The issue-rate hook runs and reports a plausible rate: more waves per
+    SIMD never lower the chip's product rate much, and one wave alone needs at
+    least the ~590 VALU instructions of a product times the 4-cycle issue."""
+    r1, cyc = gpu_prover.debug_fp_mul_rate(1, iters=200)
+    r8, _ = gpu_prover.debug_fp_mul_rate(8, iters=200)
+    assert 1e9 < r1 <= r8 * 1.2
+    assert 2000 < cyc < 100000

@@ -157,6 +157,13 @@ class Prover:
                                                            bytes(cells), bytes(proofs), n))
         return bool(ok.value)
 
+    def debug_fp_mul_rate(self, waves_per_simd, iters=2000):
+        """(chip-wide Fp products per second at that occupancy, cycles per product of one wave)."""
+        rate, cyc = ctypes.c_double(), ctypes.c_double()
+        _check(load().vkzg_amd_debug_fp_mul_rate(self._h, waves_per_simd, iters, ctypes.byref(rate),
+                                                 ctypes.byref(cyc)))
+        return rate.value, cyc.value
+
     def debug_verify_terms(self, commitments, cell_indices, cells, proofs):
         """(verdict, A, RL): the verification and the two G1 inputs of its pairing check."""
         n = len(cell_indices)

@@ -1005,6 +1005,44 @@ vkzg_result vkzg_amd_debug_fp_mul(vkzg_prover *p, uint32_t *out, const uint32_t 
     return debug_op(p, out, a, b, n, 0);
 }
 
+vkzg_result vkzg_amd_debug_fp_mul_rate(vkzg_prover *p, uint32_t waves_per_simd, uint32_t iters,
+                                       double *muls_per_s, double *one_wave_cycles) {
+    if (!p || !muls_per_s || !one_wave_cycles || waves_per_simd == 0 || waves_per_simd > 8 || iters == 0)
+        return VKZG_ERR_BADARGS;
+    std::lock_guard<std::mutex> lock(p->mu);
+    DeviceGuard guard(p->device);
+    void *buf = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    vkzg_result rc = VKZG_OK;
+    try {
+        ck(hipMalloc(&buf, 1024 * sizeof(Fp)));
+        ck(hipEventCreate(&e0));
+        ck(hipEventCreate(&e1));
+        hipStream_t s = p->stream;
+        int clock_khz = 0;
+        ck(hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeClockRate, p->device));
+        const uint32_t blocks = p->cus * 4 * waves_per_simd;  // one wave per block, 4 SIMDs per CU
+        float ms[2] = {0, 0};
+        for (int pass = 0; pass < 2; pass++) {
+            const uint32_t nb = pass == 0 ? blocks : 1;
+            ck(launch_debug_fp_mul_chain(static_cast<Fp *>(buf), nb, iters, s));  // warm-up
+            ck(hipEventRecord(e0, s));
+            ck(launch_debug_fp_mul_chain(static_cast<Fp *>(buf), nb, iters, s));
+            ck(hipEventRecord(e1, s));
+            ck(hipEventSynchronize(e1));
+            ck(hipEventElapsedTime(&ms[pass], e0, e1));
+        }
+        *muls_per_s = (double)blocks * 64.0 * iters / (ms[0] * 1e-3);
+        *one_wave_cycles = (double)ms[1] * 1e-3 * clock_khz * 1e3 / iters;
+    } catch (HipFail &) {
+        rc = VKZG_ERR_GPU;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(buf);
+    return rc;
+}
+
 vkzg_result vkzg_amd_debug_fp_inv(vkzg_prover *p, uint32_t *out, const uint32_t *a, size_t n) {
     return debug_op(p, out, a, a, n, 2);
 }

@@ -121,7 +121,25 @@ __global__ void k_debug_fr_mul(uint32_t *out, const uint32_t *a, const uint32_t 
     for (int k = 0; k < 8; k++) out[8 * i + k] = agree ? r.v[k] : 0xffffffffu;
 }
 
+// n dependent Montgomery products per lane on lane-varying operands (the
+// issue-rate measurement of vkzg_amd_debug_fp_mul_rate); each lane stores its
+// result into a 1024-entry buffer so the chain is kept.
+__global__ void k_debug_fp_mul_chain(Fp *out, uint32_t n) {
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    Fp a = fp_one(), b = fp_const(cfp::BETA_M);
+    a.v[0] += gid & 0xffffu;
+    b.v[1] ^= threadIdx.x & 7u;
+#pragma unroll 1
+    for (uint32_t i = 0; i < n; i++) a = fp_mul(a, b);
+    out[gid & 1023u] = a;
+}
+
 }  // namespace
+
+hipError_t launch_debug_fp_mul_chain(Fp *out, uint32_t blocks, uint32_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_debug_fp_mul_chain, dim3(blocks), dim3(64), 0, s, out, n);
+    return hipGetLastError();
+}
 
 hipError_t launch_roots(Fr *roots, Fr *roots_inv, Fr *shift_fwd, Fr *shift_inv, hipStream_t s) {
     hipLaunchKernelGGL(k_roots, dim3(8192 / 256), dim3(256), 0, s, roots, roots_inv, shift_fwd,
