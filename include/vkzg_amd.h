@@ -90,6 +90,14 @@ vkzg_result vkzg_amd_verify_cell_kzg_proof_batch(vkzg_prover *p, int *ok, const 
 vkzg_result vkzg_amd_blob_to_commitments(vkzg_prover *p, uint8_t *commitments, const uint8_t *blobs,
                                          size_t num_blobs);
 
+/* Writes the reference's tables file, version 1 (the file its loader reads
+ * from $VKZG_TABLES_PATH, src/precomputed_tables.cpp:16-58), for this
+ * prover's setup: the 2^(8d) position table of x_ext_fft, the root tables,
+ * the phase-B item lists of the reference's 8-bit digit recoding and
+ * Montgomery 1/4096.  This build never reads such a file; it derives its own
+ * tables on the device.  IO when the file cannot be written. */
+vkzg_result vkzg_amd_write_v1_tables(vkzg_prover *p, const char *path);
+
 typedef struct {
     uint32_t window_bits;      /* phase-A table window c */
     uint32_t windows;          /* W = ceil(256 / c) */

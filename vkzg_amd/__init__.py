@@ -157,6 +157,10 @@ class Prover:
                                                            bytes(cells), bytes(proofs), n))
         return bool(ok.value)
 
+    def write_v1_tables(self, path):
+        """The reference's tables file (version 1) for this prover's setup."""
+        _check(load().vkzg_amd_write_v1_tables(self._h, str(path).encode()))
+
     def debug_fp_mul_rate(self, waves_per_simd, iters=2000):
         """(chip-wide Fp products per second at that occupancy, cycles per product of one wave)."""
         rate, cyc = ctypes.c_double(), ctypes.c_double()

@@ -52,6 +52,7 @@ EXT_SYMBOLS = [
     "vkzg_amd_debug_g1_msm",
     "vkzg_amd_debug_verify_terms",
     "vkzg_amd_debug_fp_mul_rate",
+    "vkzg_amd_write_v1_tables",
     "vkzg_amd_verify_challenge",
     "vkzg_amd_sha256",
     "vkzg_amd_sha256_hw",
@@ -197,6 +198,8 @@ def load():
     L.vkzg_amd_debug_fp_mul_rate.restype = ctypes.c_int
     L.vkzg_amd_debug_fp_mul_rate.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32,
                                              ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    L.vkzg_amd_write_v1_tables.restype = ctypes.c_int
+    L.vkzg_amd_write_v1_tables.argtypes = [vp, ctypes.c_char_p]
     L.vkzg_amd_debug_verify_terms.restype = ctypes.c_int
     L.vkzg_amd_debug_verify_terms.argtypes = [vp, ctypes.POINTER(ctypes.c_int), vp, u8p, u64p, u8p, u8p, sz]
     L.vkzg_amd_verify_challenge.restype = ctypes.c_int

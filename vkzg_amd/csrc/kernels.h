@@ -49,6 +49,8 @@ hipError_t launch_xext_fft(G1J *out, const G1A *srs, const Fr *roots, hipStream_
 hipError_t launch_jac_to_affine(G1A *out, const G1J *in, Fp *prefix, uint64_t count, uint32_t chunk,
                                 hipStream_t s);
 hipError_t launch_table_chains(G1J *out, const G1A *xext, uint32_t c, uint32_t W, hipStream_t s);
+// The reference's v1 position table (8192 x 32 entries of 24 words) from pw = 2^(8d) xext.
+hipError_t launch_v1_positions(uint32_t *out, const G1A *pw, hipStream_t s);
 hipError_t launch_table_multiples(G1AT *table, const G1A *pw, uint32_t c, uint32_t W,
                                   uint64_t thread_begin, uint32_t thread_count, G1J *scratch,
                                   Fp *prefix, hipStream_t s);

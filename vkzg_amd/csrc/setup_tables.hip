@@ -147,6 +147,33 @@ hipError_t launch_table_chains(G1J *out, const G1A *xext, uint32_t c, uint32_t W
     return hipGetLastError();
 }
 
+namespace {
+// Reference tables-file layout of the position table: entry (b = j*64 + i, d)
+// at words ((b * 32 + d) * 24), x then y, each 12 little-endian words of the
+// reference's Montgomery form x 2^384 mod p; the identity is (0, 0).
+// pw[d * 8192 + b] = 2^(8d) xext[b] (k_table_chains with c = 8, W = 32).
+__global__ void k_v1_positions(uint32_t *out, const G1A *pw) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= 8192u * 32u) return;
+    const uint32_t b = g / 32, d = g % 32;
+    const G1A pt = pw[(size_t)d * 8192 + b];
+    const Fp ref_r = fp_const(cfp::REF_R);
+    uint32_t w[12];
+    uint32_t *dst = out + (size_t)g * 24;
+    fp_to_words(w, fp_canonical(fp_mul(pt.x, ref_r)));
+#pragma unroll
+    for (int k = 0; k < 12; k++) dst[k] = w[k];
+    fp_to_words(w, fp_canonical(fp_mul(pt.y, ref_r)));
+#pragma unroll
+    for (int k = 0; k < 12; k++) dst[12 + k] = w[k];
+}
+}  // namespace
+
+hipError_t launch_v1_positions(uint32_t *out, const G1A *pw, hipStream_t s) {
+    hipLaunchKernelGGL(k_v1_positions, dim3(8192u * 32u / 64u), dim3(64), 0, s, out, pw);
+    return hipGetLastError();
+}
+
 hipError_t launch_table_multiples(G1AT *table, const G1A *pw, uint32_t c, uint32_t W,
                                   uint64_t thread_begin, uint32_t thread_count, G1J *scratch,
                                   Fp *prefix, hipStream_t s) {
