@@ -98,3 +98,34 @@ def test_java_style_harness_builds_and_checks_symbols(tmp_path):
     r = subprocess.run([str(exe), _lib.LIB_PATH, "--symbols-only"], capture_output=True, text=True)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "8 symbols ok" in r.stdout
+
+
+def test_verification_entry_points_reject_bad_arguments_without_a_gpu():
+    """The additive verification, commitment and tables entry points check
+    their arguments before any device call (NULL prover or buffers, an
+    out-of-range cell index, a wrong G2 setup length, rate-hook bounds)."""
+    L = _lib.load()
+    ok = ctypes.c_int(7)
+    idx = (ctypes.c_uint64 * 2)(1, 2)
+    buf = ctypes.create_string_buffer(4096)
+    BAD = 1
+    assert L.vkzg_amd_verify_cell_kzg_proof_batch(None, ctypes.byref(ok), buf, idx, buf, buf, 2) == BAD
+    fake = ctypes.c_void_p(1)  # never dereferenced: the argument checks come first
+    assert L.vkzg_amd_verify_cell_kzg_proof_batch(fake, None, buf, idx, buf, buf, 2) == BAD
+    assert L.vkzg_amd_verify_cell_kzg_proof_batch(fake, ctypes.byref(ok), None, idx, buf, buf, 2) == BAD
+    assert ok.value == 0  # *ok is cleared before the other checks
+    big = (ctypes.c_uint64 * 2)(1, 128)
+    assert L.vkzg_amd_verify_cell_kzg_proof_batch(fake, ctypes.byref(ok), buf, big, buf, buf, 2) == BAD
+    assert L.vkzg_amd_load_g2_setup(None, buf, 65 * 96) == BAD
+    assert L.vkzg_amd_load_g2_setup(fake, buf, 64 * 96) == BAD
+    assert L.vkzg_amd_load_g2_setup(fake, None, 65 * 96) == BAD
+    assert L.vkzg_amd_blob_to_commitments(None, buf, buf, 1) == BAD
+    assert L.vkzg_amd_blob_to_commitments(fake, None, buf, 1) == BAD
+    assert L.vkzg_amd_write_v1_tables(None, b"/tmp/x") == BAD
+    assert L.vkzg_amd_write_v1_tables(fake, None) == BAD
+    r, c = ctypes.c_double(), ctypes.c_double()
+    assert L.vkzg_amd_debug_fp_mul_rate(fake, 0, 10, ctypes.byref(r), ctypes.byref(c)) == BAD
+    assert L.vkzg_amd_debug_fp_mul_rate(fake, 9, 10, ctypes.byref(r), ctypes.byref(c)) == BAD
+    assert L.vkzg_amd_debug_g1_msm(fake, buf, buf, buf, 0) == BAD
+    assert L.vkzg_amd_sha256(None, buf, 1, 0) == BAD
+    assert L.vkzg_amd_verify_challenge(None, buf, idx, buf, buf, 2, 0) == BAD
