@@ -36,7 +36,7 @@ namespace vkzg_amd {
 
 namespace {
 
-constexpr uint32_t kPairThreads = 256;
+constexpr uint32_t kPairThreads = kFp12Threads;
 
 VKZG_DEV Fp frob2_const(uint32_t m) {
     switch (m) {
@@ -155,7 +155,6 @@ __global__ void k_g2_lines(Fp2 *lines, const G2A *qs, uint32_t nq, uint32_t *err
 __global__ void __launch_bounds__(kPairThreads) k_pair_lines(Fp *L, const Fp2 *lines1, const Fp2 *lines2,
                                                              const G1A *p1, const G1A *p2, uint32_t neg2) {
     __shared__ Fp l1[12], l2[12], res[12];
-    __shared__ Fp12Scratch sc;
     const uint32_t t = threadIdx.x, s = blockIdx.x;
     if (t < 12) {
         l1[t] = fp_zero();
@@ -179,7 +178,7 @@ __global__ void __launch_bounds__(kPairThreads) k_pair_lines(Fp *L, const Fp2 *l
         }
     }
     __syncthreads();
-    fp12_mul_lds(res, l1, l2, sc);
+    fp12_mul_lds(res, l1, l2);
     if (t < 12) L[(size_t)s * 12 + t] = res[t];
 }
 
@@ -187,24 +186,29 @@ __device__ void fp12_copy(Fp *dst, const Fp *src) {
     if (threadIdx.x < 12) dst[threadIdx.x] = src[threadIdx.x];
     __syncthreads();
 }
+// Fp12 products leave coefficients below 12p (tower.h); the helpers that
+// negate or add them reduce to [0, 2p) first.
 __device__ void fp12_conj_inplace(Fp *a) {
     const uint32_t t = threadIdx.x;
-    if (t < 12 && ((t >> 1) & 1u)) a[t] = fp_neg(a[t]);
+    if (t < 12) {
+        const Fp v = fp_reduce16(a[t]);
+        a[t] = ((t >> 1) & 1u) ? fp_neg(v) : v;
+    }
     __syncthreads();
 }
 // dst = src^|z| (dst != src)
-__device__ void fp12_pow_zabs(Fp *dst, const Fp *src, Fp12Scratch &sc) {
+__device__ void fp12_pow_zabs(Fp *dst, const Fp *src) {
     fp12_copy(dst, src);
 #pragma unroll 1
     for (int i = 62; i >= 0; i--) {
-        fp12_mul_lds(dst, dst, dst, sc);
-        if ((kZAbs >> i) & 1u) fp12_mul_lds(dst, dst, src, sc);
+        fp12_mul_lds(dst, dst, dst);
+        if ((kZAbs >> i) & 1u) fp12_mul_lds(dst, dst, src);
     }
 }
 __device__ void fp12_frob1(Fp *dst, const Fp *src) {
     const uint32_t t = threadIdx.x;
     if (t < 6) {
-        const Fp2 a = {src[2 * t], src[2 * t + 1]};
+        const Fp2 a = {fp_reduce16(src[2 * t]), fp_reduce16(src[2 * t + 1])};
         const Fp2 r = fp2_mul(fp2_conj(a), frob1_const(t));
         dst[2 * t] = r.c0;
         dst[2 * t + 1] = r.c1;
@@ -223,7 +227,6 @@ __device__ void fp12_frob2(Fp *dst, const Fp *src) {
 __global__ void __launch_bounds__(kPairThreads) k_pair_final(uint32_t *status, Fp *out12, const Fp *L) {
     __shared__ Fp f[12], g[12], h[12], A[12], B[12], C[12], D[12];
     __shared__ Fp Ls[kPairSteps * 12];  // the step products, staged in LDS (45 KiB)
-    __shared__ Fp12Scratch sc;
     __shared__ uint32_t ok;
     const uint32_t t = threadIdx.x;
     for (uint32_t i = t; i < kPairSteps * 12; i += blockDim.x) Ls[i] = L[i];
@@ -232,14 +235,15 @@ __global__ void __launch_bounds__(kPairThreads) k_pair_final(uint32_t *status, F
     uint32_t s = 0;
 #pragma unroll 1
     for (int i = 62; i >= 0; i--) {
-        fp12_mul_lds(f, f, f, sc);
-        fp12_mul_lds(f, f, Ls + 12 * s++, sc);
-        if ((kZAbs >> i) & 1u) fp12_mul_lds(f, f, Ls + 12 * s++, sc);
+        fp12_mul_lds(f, f, f);
+        fp12_mul_lds(f, f, Ls + 12 * s++);
+        if ((kZAbs >> i) & 1u) fp12_mul_lds(f, f, Ls + 12 * s++);
     }
     // ---- easy part: f^((p^6 - 1)(p^2 + 1))
-    if (t < 12) g[t] = ((t >> 1) & 1u) ? fp_neg(f[t]) : f[t];  // conj(f)
-    __syncthreads();
-    fp12_mul_lds(h, f, g, sc);  // f conj(f) = n0 + n1 w^2 + n2 w^4 (in Fp6)
+    fp12_copy(g, f);
+    fp12_conj_inplace(g);   // conj(f)
+    fp12_mul_lds(h, f, g);  // f conj(f) = n0 + n1 w^2 + n2 w^4 (in Fp6)
+    fp12_reduce(h);
     if (t == 0) {
         const Fp2 n0 = {h[0], h[1]}, n1 = {h[4], h[5]}, n2 = {h[8], h[9]};
         // inverse in Fp2[v]/(v^3 - xi), v = w^2
@@ -258,31 +262,32 @@ __global__ void __launch_bounds__(kPairThreads) k_pair_final(uint32_t *status, F
         h[9] = i2.c1;
     }
     __syncthreads();
-    fp12_mul_lds(h, g, h, sc);  // 1/f = conj(f) / (f conj(f))
-    fp12_mul_lds(g, g, h, sc);  // f^(p^6 - 1)
+    fp12_mul_lds(h, g, h);  // 1/f = conj(f) / (f conj(f))
+    fp12_mul_lds(g, g, h);  // f^(p^6 - 1)
     fp12_frob2(h, g);
-    fp12_mul_lds(f, h, g, sc);  // F = f^((p^6 - 1)(p^2 + 1)), in the cyclotomic subgroup
+    fp12_mul_lds(f, h, g);  // F = f^((p^6 - 1)(p^2 + 1)), in the cyclotomic subgroup
     // ---- hard part: F^(3 (p^4 - p^2 + 1)/r); conj is the inverse here
-    fp12_pow_zabs(A, f, sc);
-    fp12_mul_lds(A, A, f, sc);
+    fp12_pow_zabs(A, f);
+    fp12_mul_lds(A, A, f);
     fp12_conj_inplace(A);  // A = F^(z - 1)
-    fp12_pow_zabs(B, A, sc);
-    fp12_mul_lds(B, B, A, sc);
+    fp12_pow_zabs(B, A);
+    fp12_mul_lds(B, B, A);
     fp12_conj_inplace(B);  // B = A^(z - 1)
-    fp12_pow_zabs(C, B, sc);
+    fp12_pow_zabs(C, B);
     fp12_conj_inplace(C);
     fp12_frob1(g, B);
-    fp12_mul_lds(C, C, g, sc);  // C = B^(z + p)
-    fp12_pow_zabs(D, C, sc);
-    fp12_pow_zabs(A, D, sc);    // C^(z^2)
+    fp12_mul_lds(C, C, g);  // C = B^(z + p)
+    fp12_pow_zabs(D, C);
+    fp12_pow_zabs(A, D);    // C^(z^2)
     fp12_frob2(g, C);
-    fp12_mul_lds(A, A, g, sc);
+    fp12_mul_lds(A, A, g);
     fp12_copy(g, C);
     fp12_conj_inplace(g);
-    fp12_mul_lds(A, A, g, sc);  // C^(z^2 + p^2 - 1)
-    fp12_mul_lds(g, f, f, sc);
-    fp12_mul_lds(g, g, f, sc);  // F^3
-    fp12_mul_lds(A, A, g, sc);
+    fp12_mul_lds(A, A, g);  // C^(z^2 + p^2 - 1)
+    fp12_mul_lds(g, f, f);
+    fp12_mul_lds(g, g, f);  // F^3
+    fp12_mul_lds(A, A, g);
+    fp12_reduce(A);
     if (t == 0) ok = 1;
     __syncthreads();
     if (t < 12) {

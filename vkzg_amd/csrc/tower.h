@@ -10,10 +10,11 @@
 //
 // Fp2 operations run on one lane.  Fp12 products run on a whole workgroup:
 // the 144 Fp products a_i * b_j go to 144 lanes at once (one Montgomery
-// multiplication of latency instead of 54-144 in sequence), then 72 lanes
-// combine them into Fp2 partial sums and 12 lanes add those up.  That keeps
-// the pairing's long dependent chain (about 500 Fp12 products) at roughly a
-// millisecond on one CU, where a one-lane tower would take over ten.
+// multiplication of latency instead of 54-144 in sequence), then cross-lane
+// exchanges inside each wave (DPP quad broadcasts and an xor tree) combine
+// them into the 12 result coefficients.  That keeps the pairing's long
+// dependent chain (about 480 Fp12 products) near a millisecond on one CU,
+// where a one-lane tower would take over ten.
 #pragma once
 
 #include "curve.h"
@@ -100,45 +101,103 @@ VKZG_DEV bool fp2_is_lex_largest(const Fp2 &y) {
 }
 
 // ------------------------------------------------------------ Fp12 in LDS
-// Work buffers of one workgroup: 144 products and 72 partial sums.
-struct Fp12Scratch {
-    Fp prod[144];
-    Fp part[72];
-};
+
+// Lane k of this lane's quad (DPP quad_perm [k,k,k,k]), word by word.
+template <int K>
+VKZG_DEV Fp fp_quad_bcast(const Fp &v) {
+    Fp r;
+#pragma unroll
+    for (int i = 0; i < kFpLimbs; i++) r.v[i] = (uint32_t)__builtin_amdgcn_mov_dpp((int)v.v[i], K * 0x55, 0xf, 0xf, false);
+    return r;
+}
+VKZG_DEV Fp fp_shfl_xor(const Fp &v, int mask) {
+    Fp r;
+#pragma unroll
+    for (int i = 0; i < kFpLimbs; i++) r.v[i] = (uint32_t)__shfl_xor((int)v.v[i], mask);
+    return r;
+}
+
+constexpr uint32_t kFp12Threads = 192;  // lanes an Fp12 product uses (blockDim.x >= this)
+
+// Limb-wise sum without carries (limbs of values below 2p stay below 2^31
+// for up to 8 terms).
+VKZG_DEV Fp fp_add_raw(const Fp &a, const Fp &b) {
+    Fp r;
+#pragma unroll
+    for (int i = 0; i < kFpLimbs; i++) r.v[i] = a.v[i] + b.v[i];
+    return r;
+}
+// Carries of a raw sum propagated: 28-bit limbs, same value.
+VKZG_DEV Fp fp_normalize(const Fp &a) {
+    Fp r;
+    uint32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < kFpLimbs - 1; i++) {
+        const uint32_t x = a.v[i] + c;
+        r.v[i] = x & kFpMask;
+        c = x >> 28;
+    }
+    r.v[kFpLimbs - 1] = a.v[kFpLimbs - 1] + c;
+    return r;
+}
+// A normalised value below 16p -> [0, 2p) (conditional subtractions of 8p, 4p, 2p).
+VKZG_DEV Fp fp_reduce16(const Fp &a) {
+    Fp r = a;
+#pragma unroll
+    for (int k = 3; k >= 1; k--) {
+        Fp t;
+        int32_t c = 0;
+#pragma unroll
+        for (int i = 0; i < kFpLimbs; i++) {
+            // subtract (2^k p) limb by limb: 2^k * P[i] fits 31 bits
+            const int64_t y = (int64_t)r.v[i] - ((int64_t)cfp::P[i] << k) + c;
+            t.v[i] = (uint32_t)y & kFpMask;
+            c = (int32_t)(y >> 28);
+        }
+        if (c >= 0) r = t;
+    }
+    return r;
+}
 
 // dst = a * b for Fp12 values held in LDS (12 Fp each).  Every thread of the
-// block calls it (it contains barriers); blockDim.x >= 144.  dst may alias a
-// or b.  Product a_{m1} b_{m2} lands on w^(m1+m2); a power m + 6 is
-// multiplied by xi: (x + y u) xi = (x - y) + (x + y) u.  Inlined, so that
-// every operand access is an LDS access (a generic pointer would make them
-// flat loads with several times the latency).
-__device__ __forceinline__ void fp12_mul_lds(Fp *dst, const Fp *a, const Fp *b, Fp12Scratch &s) {
+// block calls it (it contains barriers); blockDim.x >= 192.  dst may alias a
+// or b.  Lane t works on output power M = t / 32 and on term m1 = (t % 32) / 4
+// (terms 6 and 7 of a group are empty): the quad's four lanes multiply
+// a_{m1} and b_{m2}, m2 = M - m1 mod 6, component by component, DPP quad
+// broadcasts give every lane the four products, lane 0 of the quad forms the
+// real and lane 1 the imaginary part of a_{m1} b_{m2} (times xi when
+// m1 + m2 >= 6: (x + y u) xi = (x - y) + (x + y) u), and an xor tree over the
+// group's eight quads adds the six terms without carries, normalised once.
+// The result is below 12p in 28-bit limbs: a valid Montgomery input (any
+// input below 45p gives a product below 2p), but callers that add, negate or
+// compare it reduce it first (fp12_reduce).  Operands and result stay in
+// LDS; the partial sums never leave registers.
+__device__ __forceinline__ void fp12_mul_lds(Fp *dst, const Fp *a, const Fp *b) {
     const uint32_t t = threadIdx.x;
-    if (t < 144) s.prod[t] = fp_mul(a[t / 12], b[t % 12]);
+    const uint32_t M = (t / 32) % 6, m1 = (t % 32) / 4, q = t & 3u;
+    const bool term = t < kFp12Threads && m1 < 6;
+    const uint32_t m1c = term ? m1 : 0;
+    const uint32_t m2 = (M + 6 - m1c) % 6;
+    const Fp x_in = a[2 * m1c + (q >> 1)];
+    const Fp y_in = b[2 * m2 + (q & 1u)];
+    __syncthreads();  // dst may alias a or b
+    const Fp prod = fp_mul(x_in, y_in);
+    const Fp p00 = fp_quad_bcast<0>(prod), p01 = fp_quad_bcast<1>(prod);
+    const Fp p10 = fp_quad_bcast<2>(prod), p11 = fp_quad_bcast<3>(prod);
+    const Fp x = fp_sub(p00, p11), y = fp_add(p01, p10);
+    const bool wrap = m1c > M;
+    Fp v = (q == 0) ? (wrap ? fp_sub(x, y) : x) : (wrap ? fp_add(x, y) : y);
+    if (!term || q > 1) v = fp_zero();
+    v = fp_add_raw(v, fp_shfl_xor(v, 4));
+    v = fp_add_raw(v, fp_shfl_xor(v, 8));
+    v = fp_add_raw(v, fp_shfl_xor(v, 16));
+    if (t < kFp12Threads && (t % 32) < 2) dst[2 * M + (t % 32)] = fp_normalize(v);
     __syncthreads();
-    if (t < 72) {
-        // output o = 2m + e, term m1 (m2 = m - m1 mod 6)
-        const uint32_t o = t / 6, m1 = t % 6, m = o >> 1, e = o & 1u;
-        const uint32_t m2 = (m + 6 - m1) % 6;
-        const Fp &p00 = s.prod[(2 * m1) * 12 + 2 * m2];
-        const Fp &p01 = s.prod[(2 * m1) * 12 + 2 * m2 + 1];
-        const Fp &p10 = s.prod[(2 * m1 + 1) * 12 + 2 * m2];
-        const Fp &p11 = s.prod[(2 * m1 + 1) * 12 + 2 * m2 + 1];
-        const Fp x = fp_sub(p00, p11);  // real part of a_{m1} b_{m2}
-        const Fp y = fp_add(p01, p10);  // imaginary part
-        Fp v;
-        if (m1 <= m) v = e ? y : x;
-        else v = e ? fp_add(x, y) : fp_sub(x, y);
-        s.part[t] = v;
-    }
-    __syncthreads();
-    // six partials per output: pairs first (36 lanes), then three (12 lanes)
-    if (t < 36) {
-        const uint32_t o = t / 3, q = t % 3;
-        s.prod[t] = fp_add(s.part[6 * o + 2 * q], s.part[6 * o + 2 * q + 1]);
-    }
-    __syncthreads();
-    if (t < 12) dst[t] = fp_add(fp_add(s.prod[3 * t], s.prod[3 * t + 1]), s.prod[3 * t + 2]);
+}
+
+// The 12 coefficients of an Fp12 product (below 12p) brought to [0, 2p).
+__device__ __forceinline__ void fp12_reduce(Fp *a) {
+    if (threadIdx.x < 12) a[threadIdx.x] = fp_reduce16(a[threadIdx.x]);
     __syncthreads();
 }
 
