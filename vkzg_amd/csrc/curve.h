@@ -413,7 +413,7 @@ VKZG_DEV bool g1a_decompress(G1A &out, const uint32_t *src) {
     if (!fp_limbs_lt_p(x)) ok = false;
     const Fp xm = fp_from_canonical(x);
     const Fp rhs = fp_add(fp_mul(fp_sqr(xm), xm), fp_const(cfp::FOUR_M));
-    Fp y = fp_pow_const<12>(rhs, cfp::SQRT_EXP);
+    Fp y = fp_pow_const_window<12>(rhs, cfp::SQRT_EXP);
     if (!fp_eq(fp_sqr(y), rhs)) ok = false;
     const bool want_large = (b0 & 0x20u) != 0;
     if (fp_is_lex_largest(fp_to_canonical(y)) != want_large) y = fp_neg(y);

@@ -535,6 +535,44 @@ VKZG_DEV Fp fp_pow_const(const Fp &a, const uint32_t (&e)[NE]) {
 }
 VKZG_DEV Fp fp_inv_fermat(const Fp &a) { return fp_pow_const<12>(a, cfp::P_MINUS_2); }
 
+// a^e for a constant exponent by a sliding window of 4 bits over the odd
+// powers a, a^3, ..., a^15: the same squarings as fp_pow_const and about a
+// fifth of its multiplications.
+template <int NE>
+VKZG_DEV Fp fp_pow_const_window(const Fp &a, const uint32_t (&e)[NE]) {
+    Fp odd[8];
+    odd[0] = a;
+    const Fp a2 = fp_sqr(a);
+#pragma unroll
+    for (int k = 1; k < 8; k++) odd[k] = fp_mul(odd[k - 1], a2);
+    Fp acc = fp_one();
+    int i = NE * 32 - 1;
+    while (i >= 0 && !((e[i / 32] >> (i % 32)) & 1u)) i--;
+    bool started = false;
+#pragma unroll 1
+    while (i >= 0) {
+        if (!((e[i / 32] >> (i % 32)) & 1u)) {
+            acc = fp_sqr(acc);
+            i--;
+            continue;
+        }
+        // longest window of at most 4 bits ending in a set bit
+        int j = i - 3 < 0 ? 0 : i - 3;
+        while (!((e[j / 32] >> (j % 32)) & 1u)) j++;
+        uint32_t w = 0;
+        for (int k = i; k >= j; k--) w = (w << 1) | ((e[k / 32] >> (k % 32)) & 1u);
+        if (started) {
+            for (int k = i; k >= j; k--) acc = fp_sqr(acc);
+            acc = fp_mul(acc, odd[w >> 1]);
+        } else {
+            acc = odd[w >> 1];
+            started = true;
+        }
+        i = j - 1;
+    }
+    return acc;
+}
+
 // Inverse of a Montgomery value xR (0 -> 0): the binary GCD of inverse.h gives
 // the plain inverse (xR)^-1 as a signed value in (-31p, 31p); adding 32p makes
 // it non-negative and a Montgomery product by R^3 turns it into x^-1 R.

@@ -32,7 +32,7 @@ __global__ void k_ladder(G1J *lad, const G1X *part, uint32_t groups, uint32_t sp
 }
 
 // fold[blob][h][j][d] = L[j][d] + (h ? -1 : 1) * L[j+64][d]  (k_fold_ladder.comp)
-__global__ void k_fold(G1J *fold, const G1J *lad, uint32_t nblobs) {
+__global__ void __launch_bounds__(64) k_fold(G1J *fold, const G1J *lad, uint32_t nblobs) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= nblobs * 64 * kLadPos) return;
     const uint32_t blob = g / (64 * kLadPos);

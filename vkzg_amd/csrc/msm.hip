@@ -139,10 +139,18 @@ VKZG_DEV G1A msm_point(const G1A *pts, uint32_t ref) {
     return g1a_neg_if(p, (ref >> 31) != 0);
 }
 
+// A bucket summed with the checked addition (a doubling or cancellation
+// inside it, flagged by the call-free loop of k_msm_buckets).
+VKZG_DEV_COLD void bucket_checked(G1X *acc, const G1A *pts, const uint32_t *list, uint32_t lo, uint32_t hi) {
+    G1X a = g1x_identity();
+    for (uint32_t q = lo; q < hi; q++) a = g1x_madd(a, msm_point(pts, list[q]));
+    *acc = a;
+}
+
 // One lane per bucket.  A bucket with more than kHotLen points (the few
 // digits of a narrow top window collect most points of every scalar) goes to
 // the hot list instead, for k_msm_hot.
-__global__ void k_msm_buckets(G1X *buckets, const uint32_t *off, const uint32_t *list, const G1A *pts,
+__global__ void __launch_bounds__(64) k_msm_buckets(G1X *buckets, const uint32_t *off, const uint32_t *list, const G1A *pts,
                               uint32_t K, uint32_t *hot, uint32_t *hot_n) {
     const uint32_t key = blockIdx.x * blockDim.x + threadIdx.x;
     if (key >= K) return;
@@ -152,8 +160,10 @@ __global__ void k_msm_buckets(G1X *buckets, const uint32_t *off, const uint32_t 
         return;
     }
     G1X acc = g1x_identity();
+    bool exc = false;
 #pragma unroll 1
-    for (uint32_t q = lo; q < hi; q++) acc = g1x_madd(acc, msm_point(pts, list[q]));
+    for (uint32_t q = lo; q < hi; q++) acc = g1x_madd_fast(acc, msm_point(pts, list[q]), exc);
+    if (exc) bucket_checked(&acc, pts, list, lo, hi);
     buckets[key] = acc;
 }
 

@@ -62,7 +62,7 @@ VKZG_DEV Fp2 frob1_const(uint32_t m) {
 // 96-byte compressed G2 (x.c1 then x.c0, big-endian; flags in the top bits
 // of the first byte as for G1) -> affine on E'.  inf[i] = 1 for the point at
 // infinity; err |= 1 for a malformed encoding or an x with no point.
-__global__ void k_g2_decompress(G2A *out, uint32_t *inf, const uint8_t *in, uint32_t n, uint32_t *err) {
+__global__ void __launch_bounds__(64) k_g2_decompress(G2A *out, uint32_t *inf, const uint8_t *in, uint32_t n, uint32_t *err) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t *src = reinterpret_cast<const uint32_t *>(in + 96ull * i);
@@ -109,7 +109,7 @@ __global__ void k_g2_decompress(G2A *out, uint32_t *inf, const uint8_t *in, uint
 // Affine Miller-loop lines of nq points: lines[q][s] = (lam xT - yT, -lam)
 // for the 68 steps s in loop order.  One lane per point; err |= 2 on a zero
 // denominator (impossible for a point of order r).
-__global__ void k_g2_lines(Fp2 *lines, const G2A *qs, uint32_t nq, uint32_t *err) {
+__global__ void __launch_bounds__(64) k_g2_lines(Fp2 *lines, const G2A *qs, uint32_t nq, uint32_t *err) {
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nq) return;
     const G2A Q = qs[q];

@@ -77,7 +77,7 @@ __global__ void __launch_bounds__(256, 2)
 
 // Recompute, with the checked point operations, the buckets k_phase_b flagged
 // (a doubling or cancellation inside the bucket; random data never has one).
-__global__ void k_phase_b_fix(G1X *buckets, const uint8_t *flags, const G1A *lad_aff, const G1A *lad_phi,
+__global__ void __launch_bounds__(256) k_phase_b_fix(G1X *buckets, const uint8_t *flags, const G1A *lad_aff, const G1A *lad_phi,
                               const uint32_t *items, const uint32_t *offsets, uint32_t nbuckets) {
     const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= nbuckets || !flags[idx]) return;

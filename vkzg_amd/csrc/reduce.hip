@@ -76,7 +76,7 @@ __global__ void __launch_bounds__(256, 2)
 }
 
 // Recompute the flagged outputs of k_bucket_reduce with checked additions.
-__global__ void k_reduce_fix(G1X *cvals, const uint8_t *flags, const G1X *buckets, uint32_t nout) {
+__global__ void __launch_bounds__(256) k_reduce_fix(G1X *cvals, const uint8_t *flags, const G1X *buckets, uint32_t nout) {
     const uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
     if (o >= nout || !flags[o]) return;
     reduce_slow(&cvals[o], buckets + (size_t)o * 128);

@@ -33,7 +33,7 @@ __global__ void k_roots(Fr *roots, Fr *roots_inv, Fr *shift_fwd, Fr *shift_inv) 
 }
 
 // 48-byte compressed G1 -> affine.  Sets err[0] on a malformed encoding.
-__global__ void k_decompress(G1A *out, const uint8_t *in, uint32_t n, uint32_t *err) {
+__global__ void __launch_bounds__(64) k_decompress(G1A *out, const uint8_t *in, uint32_t n, uint32_t *err) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     G1A r;

@@ -76,7 +76,7 @@ __device__ __attribute__((noinline)) void table_run_slow(G1J *scratch, Fp *prefi
 // Thread g = (b*W + w)*nseg + seg writes table[b][w][d-1] for d in
 // [seg*S+1, seg*S+S]: a running sum of the affine base P = pw[w][b],
 // normalised with one inversion per thread.
-__global__ void k_table_multiples(G1AT *table, const G1A *pw, uint32_t c, uint32_t W,
+__global__ void __launch_bounds__(64) k_table_multiples(G1AT *table, const G1A *pw, uint32_t c, uint32_t W,
                                   uint64_t thread_begin, uint32_t thread_count, G1J *scratch,
                                   Fp *prefix) {
     const uint32_t lt = blockIdx.x * blockDim.x + threadIdx.x;

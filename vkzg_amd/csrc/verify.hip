@@ -48,7 +48,7 @@ VKZG_DEV bool g1_in_subgroup(const G1A &p) {
 }
 
 // pts[i] = decompress(bytes[i]) for i < n; pts[n + i] = srs[i] for i < nsrs.
-__global__ void k_vfy_points(G1A *pts, const uint8_t *bytes, uint32_t n, const G1A *srs, uint32_t nsrs,
+__global__ void __launch_bounds__(64) k_vfy_points(G1A *pts, const uint8_t *bytes, uint32_t n, const G1A *srs, uint32_t nsrs,
                              uint32_t *err) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) {
