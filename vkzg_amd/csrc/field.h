@@ -600,6 +600,28 @@ VKZG_DEV bool fp_is_lex_largest(const Fp &canonical) {
     return c < 0;
 }
 
+// The value lane k of this lane's quad holds (DPP quad_perm [k,k,k,k]), word
+// by word; the quad's four lanes must all be active.
+template <int K>
+VKZG_DEV Fp fp_quad_bcast(const Fp &v) {
+    Fp r;
+#pragma unroll
+    for (int i = 0; i < kFpLimbs; i++)
+        r.v[i] = (uint32_t)__builtin_amdgcn_mov_dpp((int)v.v[i], K * 0x55, 0xf, 0xf, false);
+    return r;
+}
+
+// The value lane k of this lane's pair (lanes 2i, 2i+1) holds (DPP quad_perm
+// [0,0,2,2] or [1,1,3,3]); both lanes of the pair must be active.
+template <int K>
+VKZG_DEV Fp fp_pair_bcast(const Fp &v) {
+    Fp r;
+#pragma unroll
+    for (int i = 0; i < kFpLimbs; i++)
+        r.v[i] = (uint32_t)__builtin_amdgcn_mov_dpp((int)v.v[i], K ? 0xF5 : 0xA0, 0xf, 0xf, false);
+    return r;
+}
+
 // Reference Montgomery product (operand scanning, plain C++) for the test
 // hooks: same radix and limbs, fully reduced.
 VKZG_DEV Fp fp_mul_reference(const Fp &a, const Fp &b) {

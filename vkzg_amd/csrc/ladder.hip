@@ -4,15 +4,41 @@
 namespace vkzg_amd {
 
 namespace {
+// g1j_dbl (dbl-2009-l) shared by the two lanes of a pair, which both hold p.
+// Its seven products fall into four dependent rounds of at most two -- {X^2,
+// Y^2}, {B^2, (X+B)^2}, {E^2} (computed by both lanes) and {E(D - X3), YZ} --
+// and DPP exchanges hand each round's results to both lanes: a doubling waits
+// on three squarings and one product instead of five and two.  The formulas
+// and operand order are g1j_dbl's, so the coordinates are the same values.
+VKZG_DEV G1J g1j_dbl_pair(const G1J &p, uint32_t q) {
+    if (fp_is_zero(p.z)) return p;  // uniform in the pair
+    const Fp r1 = fp_sqr(q ? p.y : p.x);
+    const Fp A = fp_pair_bcast<0>(r1), B = fp_pair_bcast<1>(r1);
+    const Fp r2 = fp_sqr(q ? fp_add(p.x, B) : B);
+    const Fp C = fp_pair_bcast<0>(r2), T = fp_pair_bcast<1>(r2);
+    const Fp E = fp_add(fp_dbl(A), A);
+    const Fp F = fp_sqr(E);
+    const Fp D = fp_dbl(fp_sub(fp_sub(T, A), C));
+    G1J r;
+    r.x = fp_sub(F, fp_dbl(D));
+    const Fp C8 = fp_dbl(fp_dbl(fp_dbl(C)));
+    const Fp r4 = fp_mul(q ? p.y : E, q ? p.z : fp_sub(D, r.x));
+    r.y = fp_sub(fp_pair_bcast<0>(r4), C8);
+    r.z = fp_dbl(fp_pair_bcast<1>(r4));
+    return r;
+}
+
 // lad[blob][d][j] = 2^(8d) * u[blob][j], d < kLadPos.  With `splits` = M
-// segments, thread (blob, m, j) starts from phase A's 2^(8 start_m) u (the
+// segments, pair (blob, m, j) starts from phase A's 2^(8 start_m) u (the
 // sum of its `groups` window-group partials part[((blob*M + m)*128 + j)*G + g])
 // and writes positions [start_m, start_{m+1}): chains of 128 / M doublings.
-__global__ void k_ladder(G1J *lad, const G1X *part, uint32_t groups, uint32_t splits, uint32_t nblobs) {
+__global__ __launch_bounds__(256) void k_ladder(G1J *lad, const G1X *part, uint32_t groups, uint32_t splits, uint32_t nblobs) {
     // A long dependent chain: give these waves issue priority over the
     // throughput kernels of another sub-batch sharing the SIMD.
     __builtin_amdgcn_s_setprio(3);
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    // two lanes per ladder (g1j_dbl_pair); pairs exit together
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t g = tid >> 1, q = tid & 1u;
     if (g >= nblobs * 128 * splits) return;
     const uint32_t j = g % 128, m = (g / 128) % splits, blob = g / (128 * splits);
     G1X u = part[(size_t)g * groups];
@@ -24,10 +50,15 @@ __global__ void k_ladder(G1J *lad, const G1X *part, uint32_t groups, uint32_t sp
     G1J *dst = lad + (size_t)blob * kLadPos * 128 + j;
 #pragma unroll 1
     for (uint32_t d = d0; d < d1; d++) {
-        dst[d * 128] = p;
+        if (q == 0) {
+            dst[d * 128].x = p.x;
+            dst[d * 128].z = p.z;
+        } else {
+            dst[d * 128].y = p.y;
+        }
         if (d + 1 == d1) break;
 #pragma unroll 1
-        for (int k = 0; k < 8; k++) p = g1j_dbl(p);
+        for (int k = 0; k < 8; k++) p = g1j_dbl_pair(p, q);
     }
 }
 
@@ -64,7 +95,11 @@ __global__ void k_glv_phi(G1A *phi, const G1A *pts, uint64_t count, uint32_t *qu
 
 hipError_t launch_ladder(G1J *lad, const G1X *part, uint32_t groups, uint32_t splits, uint32_t nblobs,
                          hipStream_t s) {
-    hipLaunchKernelGGL(k_ladder, dim3(blocks_for(nblobs * 128ull * splits, 64)), dim3(64), 0, s, lad,
+    // Workgroups of four waves: the dispatcher spreads a workgroup's waves over
+    // a CU's four SIMDs, where one-wave workgroups doubled up on some SIMDs
+    // while others idled (1.6 ms instead of 0.9 ms for 256 blobs; each wave's
+    // chain is issue-bound, so a shared SIMD halves its rate).
+    hipLaunchKernelGGL(k_ladder, dim3(blocks_for(nblobs * 128ull * splits * 2, 256)), dim3(256), 0, s, lad,
                        part, groups, splits, nblobs);
     return hipGetLastError();
 }

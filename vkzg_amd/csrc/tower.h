@@ -102,14 +102,6 @@ VKZG_DEV bool fp2_is_lex_largest(const Fp2 &y) {
 
 // ------------------------------------------------------------ Fp12 in LDS
 
-// Lane k of this lane's quad (DPP quad_perm [k,k,k,k]), word by word.
-template <int K>
-VKZG_DEV Fp fp_quad_bcast(const Fp &v) {
-    Fp r;
-#pragma unroll
-    for (int i = 0; i < kFpLimbs; i++) r.v[i] = (uint32_t)__builtin_amdgcn_mov_dpp((int)v.v[i], K * 0x55, 0xf, 0xf, false);
-    return r;
-}
 VKZG_DEV Fp fp_shfl_xor(const Fp &v, int mask) {
     Fp r;
 #pragma unroll
