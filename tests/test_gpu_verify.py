@@ -132,6 +132,21 @@ def test_commitments_match_oracle(gpu_seeded_prover, seeded_verify_setup, rng):
     assert e.value.code == vkzg_amd.ERR_INVALID_BLOB
 
 
+def test_commitments_large_batch(gpu_seeded_prover, seeded_verify_setup, rng):
+    """24 blobs in one call: over 2^18 buckets, the MSM's large-launch regime
+    (16 buckets per window thread, hot buckets summed by 16-lane teams).  Each
+    commitment equals the one-blob call's (the small regime) and, for three of
+    them, the oracle's."""
+    from tests.conftest import random_blob
+    blobs = [random_blob(rng) for _ in range(23)] + [bytes(131072)]
+    got = gpu_seeded_prover.blob_to_kzg_commitments(b"".join(blobs), len(blobs))
+    for i, b in enumerate(blobs):
+        assert got[48 * i:48 * (i + 1)] == gpu_seeded_prover.blob_to_kzg_commitments(b, 1), i
+    for i in (0, 11, 22):
+        assert got[48 * i:48 * (i + 1)] == seeded_verify_setup.commit_batch(blobs[i], 1), i
+    assert got[48 * 23:] == G1_INF
+
+
 def test_verify_valid_and_terms(verifier, seeded_verify_setup, oracle_mod, rng):
     from tests.conftest import random_blob
     blobs = b"".join(random_blob(rng) for _ in range(3))

@@ -14,7 +14,7 @@ struct MsmShape {
     uint32_t W = 0;     // windows of a 128-bit half scalar, ceil(129 / c)
     uint32_t bk = 0;    // buckets per window, 2^(c-1)
     uint32_t K = 0;     // buckets in all, nmsm * W * bk
-    uint32_t parts = 1; // window-sum blocks per window (bk / 512 for bk > 512)
+    uint32_t parts = 1; // window-sum blocks per window (bk / (128 * buckets per thread))
 };
 
 MsmShape msm_shape(uint32_t ne, uint32_t nmsm);

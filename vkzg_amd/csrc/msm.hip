@@ -9,13 +9,16 @@
 // 128-bit scalars.  Then, for windows of c bits and signed digits in
 // [-2^(c-1), 2^(c-1)]:
 //   count    histogram of (msm, window, |digit|) keys     (atomics)
-//   scan     bucket offsets (one block)
+//   scan     bucket offsets (one block; three passes above 16k buckets)
 //   scatter  point references sorted by key               (atomics)
 //   buckets  one lane per bucket sums its points          (XYZZ, checked);
-//            buckets of more than 24 points, one block each
+//            buckets of more than 24 points by a team of 64 lanes (large
+//            launches: 16)
 //   windows  blocks per (msm, window): sum_b (b+1) B_b by running sums over
-//            segments of at most 4 buckets, then trees in LDS
-//   combine  one lane per MSM: Horner over windows (c doublings each) and the
+//            segments of at most 4 (large launches: 16) buckets, then trees
+//            in LDS
+//   combine  four lanes per MSM: Horner over windows (c doublings each),
+//            each doubling and addition shared by the quad, and the
 //            conversion to affine.
 // The point order inside a bucket depends on atomics, but the sum is the same
 // group element, so results are deterministic after normalisation.
@@ -29,9 +32,18 @@ namespace vkzg_amd {
 namespace {
 
 constexpr uint32_t kWinThreads = 128;
-constexpr uint32_t kWinSeg = 4;  // buckets per thread of the window sums (more parts beyond)
-constexpr uint32_t kHotLen = 24; // longer buckets are summed by a block (k_msm_hot)
-constexpr uint32_t kHotBlocks = 512;
+constexpr uint32_t kHotLen = 24; // longer buckets are summed by a team of lanes (k_msm_hot)
+// Launch regimes.  Small sums (verification) are latency-bound: 4 buckets per
+// window thread, and a hot bucket (often hundreds of points: a narrow top
+// window) summed by 64 lanes.  From kBigK buckets on (commitments of many
+// blobs) the launches are throughput-bound: 16 buckets per window thread
+// spread its fixed costs (the offset multiple and the tree), and hot buckets
+// (thousands, of some 30 points) are summed by teams of 16 lanes.
+constexpr uint32_t kBigK = 1u << 18;
+constexpr uint32_t kWinSegSmall = 4, kWinSegBig = 16;
+constexpr uint32_t kHotBlocks = 2048;
+constexpr uint32_t kScanOneBlock = 16384;  // larger scans use k_msm_scan_tiles
+constexpr uint32_t kScanTile = 4096;       // buckets per block of k_msm_scan_tiles
 constexpr uint32_t kPtMask = (1u << 30) - 1;
 
 // Signed digit w of a 128-bit scalar (4 words) with window c; carry in/out.
@@ -113,6 +125,65 @@ __global__ void __launch_bounds__(1024) k_msm_scan(uint32_t *off, uint32_t *cur,
     if (t == 1023) off[K] = part[1023];
 }
 
+// Three-pass form for large K.  Pass 1: each block scans its tile of
+// kScanTile counts (4 per thread) into tile-local exclusive offsets and
+// writes the tile total.
+__global__ void __launch_bounds__(1024) k_msm_scan_tiles(uint32_t *off, uint32_t *tot, const uint32_t *cnt,
+                                                         uint32_t K) {
+    __shared__ uint32_t part[1024];
+    const uint32_t t = threadIdx.x;
+    const uint32_t lo = blockIdx.x * kScanTile + 4 * t;
+    uint32_t v[4], sum = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        v[i] = lo + i < K ? cnt[lo + i] : 0;
+        sum += v[i];
+    }
+    part[t] = sum;
+    __syncthreads();
+    for (uint32_t d = 1; d < 1024; d <<= 1) {
+        const uint32_t x = t >= d ? part[t - d] : 0;
+        __syncthreads();
+        part[t] += x;
+        __syncthreads();
+    }
+    uint32_t run = part[t] - sum;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        if (lo + i < K) off[lo + i] = run;
+        run += v[i];
+    }
+    if (t == 1023) tot[blockIdx.x] = part[1023];
+}
+
+// Pass 2 (one block): exclusive scan of the ntiles <= 1024 tile totals in
+// place; off[K] = the grand total.
+__global__ void __launch_bounds__(1024) k_msm_scan_totals(uint32_t *off, uint32_t *tot, uint32_t ntiles,
+                                                          uint32_t K) {
+    __shared__ uint32_t part[1024];
+    const uint32_t t = threadIdx.x;
+    const uint32_t mine = t < ntiles ? tot[t] : 0;
+    part[t] = mine;
+    __syncthreads();
+    for (uint32_t d = 1; d < 1024; d <<= 1) {
+        const uint32_t x = t >= d ? part[t - d] : 0;
+        __syncthreads();
+        part[t] += x;
+        __syncthreads();
+    }
+    if (t < ntiles) tot[t] = part[t] - mine;
+    if (t == 1023) off[K] = part[1023];
+}
+
+// Pass 3: add each tile's offset; cur = off.
+__global__ void k_msm_scan_add(uint32_t *off, uint32_t *cur, const uint32_t *tot, uint32_t K) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    const uint32_t o = off[k] + tot[k / kScanTile];
+    off[k] = o;
+    cur[k] = o;
+}
+
 __global__ void k_msm_scatter(uint32_t *list, uint32_t *cur, const uint32_t *hs, const uint8_t *mid,
                               const uint32_t *pidx, uint32_t nh, uint32_t c, uint32_t W) {
     const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
@@ -167,27 +238,34 @@ __global__ void __launch_bounds__(64) k_msm_buckets(G1X *buckets, const uint32_t
     buckets[key] = acc;
 }
 
-// Hot buckets, one block each (blocks stride over the list): 64 lanes sum
-// interleaved shares of the bucket's points, then a tree in LDS.
+// Hot buckets, TEAM lanes each (64 / TEAM per block; blocks stride over the
+// list): the team's lanes sum interleaved shares of the bucket's points, then
+// a tree in LDS.  The loop bound is uniform over the block (barriers).
+template <uint32_t TEAM>
 __global__ void __launch_bounds__(64) k_msm_hot(G1X *buckets, const uint32_t *off, const uint32_t *list,
                                                 const G1A *pts, const uint32_t *hot, const uint32_t *hot_n) {
+    constexpr uint32_t kTeams = 64 / TEAM;
     __shared__ G1X sh[64];
-    const uint32_t t = threadIdx.x;
+    const uint32_t t = threadIdx.x, l = t % TEAM;
     const uint32_t n = *hot_n;
 #pragma unroll 1
-    for (uint32_t q = blockIdx.x; q < n; q += gridDim.x) {
-        const uint32_t key = hot[q];
-        const uint32_t lo = off[key], hi = off[key + 1];
+    for (uint32_t q0 = blockIdx.x * kTeams; q0 < n; q0 += gridDim.x * kTeams) {
+        const uint32_t q = q0 + t / TEAM;
+        uint32_t key = 0;
         G1X acc = g1x_identity();
+        if (q < n) {
+            key = hot[q];
+            const uint32_t lo = off[key], hi = off[key + 1];
 #pragma unroll 1
-        for (uint32_t i = lo + t; i < hi; i += 64) acc = g1x_madd(acc, msm_point(pts, list[i]));
+            for (uint32_t i = lo + l; i < hi; i += TEAM) acc = g1x_madd(acc, msm_point(pts, list[i]));
+        }
         sh[t] = acc;
         __syncthreads();
-        for (uint32_t stride = 32; stride > 0; stride >>= 1) {
-            if (t < stride) sh[t] = g1x_add(sh[t], sh[t + stride]);
+        for (uint32_t stride = TEAM / 2; stride > 0; stride >>= 1) {
+            if (l < stride) sh[t] = g1x_add(sh[t], sh[t + stride]);
             __syncthreads();
         }
-        if (t == 0) buckets[key] = sh[0];
+        if (q < n && l == 0) buckets[key] = sh[t];
         __syncthreads();
     }
 }
@@ -207,7 +285,7 @@ VKZG_DEV G1X g1x_mul_small(const G1X &p, uint32_t k) {
 // Thread t owns buckets [lo, lo + seg): running sums from the top give
 // R = sum B_b and T = sum (b - lo + 1) B_b, and its share is T + lo R; a
 // tree in LDS adds the shares.  A window's parts (enough of them for at most
-// kWinSeg buckets per thread) are added by k_msm_parts.
+// kWinSegSmall or kWinSegBig buckets per thread) are added by k_msm_parts.
 __global__ void __launch_bounds__(kWinThreads) k_msm_windows(G1X *wpart, const G1X *buckets, uint32_t bk,
                                                              uint32_t parts) {
     __shared__ G1X sh[kWinThreads];
@@ -259,12 +337,60 @@ VKZG_DEV_COLD void combine_checked(G1X *acc, const G1X *ws, uint32_t W, uint32_t
     *acc = a;
 }
 
-// out[m] = sum_w 2^(c w) wsum[m W + w], affine ((0, 0) for the identity).
-// The doubling chain is this launch's critical path, so the loop uses the
-// call-free addition; an equal or opposite pair (flagged) redoes it checked.
+// The four lanes of a quad hold the same point(s) and share one operation:
+// its products are split into dependent rounds of at most four, and DPP quad
+// broadcasts give every lane each round's products.  Same formulas and
+// operand order as g1x_dbl (3 rounds instead of 9 products) and g1x_add_fast
+// (4 rounds instead of 14).
+VKZG_DEV G1X g1x_dbl_quad(const G1X &p, uint32_t q) {
+    if (fp_is_zero(p.zz)) return p;  // uniform in the quad
+    const Fp U = fp_dbl(p.y);
+    const Fp r1 = fp_sqr((q & 1u) ? p.x : U);
+    const Fp V = fp_quad_bcast<0>(r1), X2 = fp_quad_bcast<1>(r1);
+    const Fp M = fp_add(fp_dbl(X2), X2);
+    const Fp r2 = fp_mul(q == 0 ? U : (q == 1 ? p.x : (q == 2 ? M : V)), q == 2 ? M : (q == 3 ? p.zz : V));
+    const Fp W = fp_quad_bcast<0>(r2), S = fp_quad_bcast<1>(r2);
+    G1X r;
+    r.x = fp_sub(fp_quad_bcast<2>(r2), fp_dbl(S));  // M^2 - 2S
+    r.zz = fp_quad_bcast<3>(r2);                    // V ZZ
+    const Fp r3 = fp_mul(q == 0 ? M : W, q == 0 ? fp_sub(S, r.x) : (q == 1 ? p.y : p.zzz));
+    r.y = fp_sub(fp_quad_bcast<0>(r3), fp_quad_bcast<1>(r3));  // M (S - X3) - W Y
+    r.zzz = fp_quad_bcast<2>(r3);                               // W ZZZ
+    return r;
+}
+
+VKZG_DEV G1X g1x_add_quad(const G1X &a, const G1X &b, uint32_t q, bool &exc) {
+    if (fp_is_zero(a.zz)) return b;  // uniform in the quad
+    if (fp_is_zero(b.zz)) return a;
+    const Fp r1 = fp_mul(q == 0 ? a.x : (q == 1 ? b.x : (q == 2 ? a.y : b.y)),
+                         q == 0 ? b.zz : (q == 1 ? a.zz : (q == 2 ? b.zzz : a.zzz)));
+    const Fp U1 = fp_quad_bcast<0>(r1), S1 = fp_quad_bcast<2>(r1);
+    const Fp P = fp_sub_lazy(fp_quad_bcast<1>(r1), U1);  // U2 - U1
+    const Fp R = fp_sub_lazy(fp_quad_bcast<3>(r1), S1);  // S2 - S1
+    const Fp r2 = fp_mul(q == 0 ? P : (q == 1 ? R : (q == 2 ? a.zz : a.zzz)),
+                         q == 0 ? P : (q == 1 ? R : (q == 2 ? b.zz : b.zzz)));
+    const Fp PP = fp_quad_bcast<0>(r2), RR = fp_quad_bcast<1>(r2);
+    const Fp ZZab = fp_quad_bcast<2>(r2), ZZZab = fp_quad_bcast<3>(r2);
+    exc |= fp_is_zero(PP);
+    const Fp r3 = fp_mul(q == 0 ? P : (q == 1 ? U1 : ZZab), PP);
+    const Fp PPP = fp_quad_bcast<0>(r3), Q = fp_quad_bcast<1>(r3);
+    G1X r;
+    r.zz = fp_quad_bcast<2>(r3);
+    r.x = fp_sub(fp_sub(RR, PPP), fp_dbl(Q));
+    const Fp r4 = fp_mul(q == 0 ? R : (q == 1 ? S1 : ZZZab), q == 0 ? fp_sub_lazy(Q, r.x) : PPP);
+    r.y = fp_sub(fp_quad_bcast<0>(r4), fp_quad_bcast<1>(r4));
+    r.zzz = fp_quad_bcast<2>(r4);
+    return r;
+}
+
+// out[m] = sum_w 2^(c w) wsum[m W + w], affine ((0, 0) for the identity),
+// by the quad m = tid / 4.  The doubling chain is this launch's critical
+// path, so the loop uses the call-free addition; an equal or opposite pair
+// (flagged) redoes it checked.  Quads exit together.
 __global__ void __launch_bounds__(64) k_msm_combine(G1A *out, const G1X *wsum, uint32_t nmsm, uint32_t W,
                                                     uint32_t c) {
-    const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t m = tid >> 2, q = tid & 3u;
     if (m >= nmsm) return;
     const G1X *ws = wsum + (size_t)m * W;
     G1X acc = ws[W - 1];
@@ -272,8 +398,8 @@ __global__ void __launch_bounds__(64) k_msm_combine(G1A *out, const G1X *wsum, u
 #pragma unroll 1
     for (uint32_t w = W - 1; w-- > 0;) {
 #pragma unroll 1
-        for (uint32_t i = 0; i < c; i++) acc = g1x_dbl(acc);
-        acc = g1x_add_fast(acc, ws[w], exc);
+        for (uint32_t i = 0; i < c; i++) acc = g1x_dbl_quad(acc, q);
+        acc = g1x_add_quad(acc, ws[w], q, exc);
     }
     if (exc) combine_checked(&acc, ws, W, c);
     G1A r = g1a_identity();
@@ -283,7 +409,7 @@ __global__ void __launch_bounds__(64) k_msm_combine(G1A *out, const G1X *wsum, u
         r.x = fp_mul(fp_mul(acc.x, acc.zzz), inv);
         r.y = fp_mul(fp_mul(acc.y, acc.zz), inv);
     }
-    out[m] = r;
+    if (q == 0) out[m] = r;
 }
 
 }  // namespace
@@ -300,7 +426,9 @@ MsmShape msm_shape(uint32_t ne, uint32_t nmsm) {
     sh.W = (129 + c - 1) / c;
     sh.bk = 1u << (c - 1);
     sh.K = nmsm * sh.W * sh.bk;
-    sh.parts = sh.bk > kWinThreads * kWinSeg ? sh.bk / (kWinThreads * kWinSeg) : 1;
+    const bool big = sh.K >= kBigK;
+    const uint32_t seg = big ? kWinSegBig : kWinSegSmall;
+    sh.parts = sh.bk > kWinThreads * seg ? sh.bk / (kWinThreads * seg) : 1;
     return sh;
 }
 
@@ -309,10 +437,11 @@ size_t msm_scratch_bytes(const MsmShape &sh) {
     size_t b = 0;
     b += 16 * nh;                        // half scalars
     b += 4ull * (4 * (size_t)sh.K + 3);  // cnt (+ hot count), cur, off, hot list
+    b += 4ull * (sh.K / kScanTile + 1);  // scan tile totals
     b += 4 * nh * sh.W;                  // sorted references
     b += sizeof(G1X) * (size_t)sh.K;     // buckets
     b += sizeof(G1X) * (size_t)sh.nmsm * sh.W * (sh.parts + 1);
-    return b + 16 * 256;  // each of the 9 regions is rounded up to 256 bytes
+    return b + 16 * 256;  // each of the 10 regions is rounded up to 256 bytes
 }
 
 hipError_t launch_msm(const MsmShape &sh, void *scratch, const G1A *pts, const uint32_t *pidx, const Fr *sc,
@@ -330,25 +459,37 @@ hipError_t launch_msm(const MsmShape &sh, void *scratch, const G1A *pts, const u
     uint32_t *hot = reinterpret_cast<uint32_t *>(take(4ull * sh.K));
     uint32_t *cur = reinterpret_cast<uint32_t *>(take(4ull * sh.K));
     uint32_t *off = reinterpret_cast<uint32_t *>(take(4ull * (sh.K + 1)));
+    const uint32_t ntiles = (sh.K + kScanTile - 1) / kScanTile;
+    uint32_t *tot = reinterpret_cast<uint32_t *>(take(4ull * (sh.K / kScanTile + 1)));
     uint32_t *list = reinterpret_cast<uint32_t *>(take(4ull * nh * sh.W));
     G1X *buckets = reinterpret_cast<G1X *>(take(sizeof(G1X) * (size_t)sh.K));
     G1X *wsum = reinterpret_cast<G1X *>(take(sizeof(G1X) * (size_t)sh.nmsm * sh.W));
     G1X *wpart = reinterpret_cast<G1X *>(take(sizeof(G1X) * (size_t)sh.nmsm * sh.W * sh.parts));
     if ((size_t)(p - static_cast<uint8_t *>(scratch)) > msm_scratch_bytes(sh)) return hipErrorInvalidValue;
+    if (sh.K > kScanOneBlock && ntiles > 1024) return hipErrorInvalidValue;  // beyond 4M buckets
     hipLaunchKernelGGL(k_msm_split, dim3(blocks_for(sh.ne, 256)), dim3(256), 0, s, hs, sc, sh.ne);
     hipLaunchKernelGGL(k_msm_zero, dim3(blocks_for(sh.K + 1, 256)), dim3(256), 0, s, cnt, sh.K + 1);
     hipLaunchKernelGGL(k_msm_count, dim3(blocks_for(nh, 256)), dim3(256), 0, s, cnt, hs, mid, nh, sh.c, sh.W);
-    hipLaunchKernelGGL(k_msm_scan, dim3(1), dim3(1024), 0, s, off, cur, cnt, sh.K);
+    if (sh.K <= kScanOneBlock) {
+        hipLaunchKernelGGL(k_msm_scan, dim3(1), dim3(1024), 0, s, off, cur, cnt, sh.K);
+    } else {
+        hipLaunchKernelGGL(k_msm_scan_tiles, dim3(ntiles), dim3(1024), 0, s, off, tot, cnt, sh.K);
+        hipLaunchKernelGGL(k_msm_scan_totals, dim3(1), dim3(1024), 0, s, off, tot, ntiles, sh.K);
+        hipLaunchKernelGGL(k_msm_scan_add, dim3(blocks_for(sh.K, 256)), dim3(256), 0, s, off, cur, tot, sh.K);
+    }
     hipLaunchKernelGGL(k_msm_scatter, dim3(blocks_for(nh, 256)), dim3(256), 0, s, list, cur, hs, mid, pidx, nh,
                        sh.c, sh.W);
     hipLaunchKernelGGL(k_msm_buckets, dim3(blocks_for(sh.K, 64)), dim3(64), 0, s, buckets, off, list, pts,
                        sh.K, hot, cnt + sh.K);
-    hipLaunchKernelGGL(k_msm_hot, dim3(sh.K < kHotBlocks ? sh.K : kHotBlocks), dim3(64), 0, s, buckets, off, list,
-                       pts, hot, cnt + sh.K);
+    const dim3 hot_grid(sh.K < kHotBlocks ? sh.K : kHotBlocks);
+    if (sh.K >= kBigK)
+        hipLaunchKernelGGL(k_msm_hot<16>, hot_grid, dim3(64), 0, s, buckets, off, list, pts, hot, cnt + sh.K);
+    else
+        hipLaunchKernelGGL(k_msm_hot<64>, hot_grid, dim3(64), 0, s, buckets, off, list, pts, hot, cnt + sh.K);
     hipLaunchKernelGGL(k_msm_windows, dim3(sh.nmsm * sh.W * sh.parts), dim3(kWinThreads), 0, s, wpart, buckets,
                        sh.bk, sh.parts);
     hipLaunchKernelGGL(k_msm_parts, dim3(sh.nmsm * sh.W), dim3(64), 0, s, wsum, wpart, sh.parts);
-    hipLaunchKernelGGL(k_msm_combine, dim3(blocks_for(sh.nmsm, 64)), dim3(64), 0, s, out, wsum, sh.nmsm, sh.W,
+    hipLaunchKernelGGL(k_msm_combine, dim3(blocks_for(sh.nmsm * 4ull, 64)), dim3(64), 0, s, out, wsum, sh.nmsm, sh.W,
                        sh.c);
     return hipGetLastError();
 }
