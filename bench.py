@@ -325,7 +325,7 @@ def main():
     traffic = None
     if os.path.exists(a.traffic_file):
         t = json.load(open(a.traffic_file))
-        if t.get("kernel") == dom and t.get("blobs") == nb:
+        if t.get("kernel") == dom and t.get("blobs") == nb and t.get("window_bits") == info["window_bits"]:
             traffic = t["fetch_bytes"] + t["write_bytes"]
     roofline = {
         "bound": "int-valu",

@@ -35,14 +35,16 @@ uint64_t table_bytes_for(uint32_t c) {
 }
 
 // Window bits of the resident phase-A table: $VKZG_AMD_WINDOW if set (6..14),
-// else the largest c <= 13 whose table fits in a third of the free memory.
+// else the largest c <= 14 whose table fits in 60% of the free memory.  On a
+// 288 GB MI355X that is c = 14 (W = 19, a 152 GiB table; c = 13 is W = 20 in
+// 80 GiB); a second prover in the same process gets a smaller window.
 uint32_t choose_window(size_t free_bytes) {
     if (const char *env = std::getenv("VKZG_AMD_WINDOW")) {
         const int c = std::atoi(env);
         if (c >= 6 && c <= 14) return (uint32_t)c;
     }
-    for (uint32_t c = 13; c > 8; c--) {
-        if (table_bytes_for(c) <= free_bytes / 3) return c;
+    for (uint32_t c = 14; c > 8; c--) {
+        if (table_bytes_for(c) <= free_bytes / 5 * 3) return c;
     }
     return 8;
 }
