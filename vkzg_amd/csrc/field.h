@@ -395,6 +395,20 @@ VKZG_DEV Fp fp_sub_lazy(const Fp &a, const Fp &b) {
     for (int i = 0; i < kFpLimbs; i++) r.v[i] = a.v[i] + cfp::P4S[i] - b.v[i];
     return r;
 }
+// a + b and 3a limb by limb, for normalised inputs below 2p: limbs below
+// 3 * 2^28 and values below 6p, the lazy form above.
+VKZG_DEV Fp fp_add_lazy(const Fp &a, const Fp &b) {
+    Fp r;
+#pragma unroll
+    for (int i = 0; i < kFpLimbs; i++) r.v[i] = a.v[i] + b.v[i];
+    return r;
+}
+VKZG_DEV Fp fp_triple_lazy(const Fp &a) {
+    Fp r;
+#pragma unroll
+    for (int i = 0; i < kFpLimbs; i++) r.v[i] = 3 * a.v[i];
+    return r;
+}
 VKZG_DEV Fp fp_neg_lazy_if(const Fp &a, bool neg) {
     Fp r;
 #pragma unroll

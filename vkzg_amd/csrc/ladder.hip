@@ -9,20 +9,22 @@ namespace {
 // Y^2}, {B^2, (X+B)^2}, {E^2} (computed by both lanes) and {E(D - X3), YZ} --
 // and DPP exchanges hand each round's results to both lanes: a doubling waits
 // on three squarings and one product instead of five and two.  The formulas
-// and operand order are g1j_dbl's, so the coordinates are the same values.
+// and operand order are g1j_dbl's; the sums that only feed products (X + B,
+// E = 3A, D - X3) stay lazy, so coordinates are the same field elements (the
+// proofs, canonical, are unchanged).
 VKZG_DEV G1J g1j_dbl_pair(const G1J &p, uint32_t q) {
     if (fp_is_zero(p.z)) return p;  // uniform in the pair
     const Fp r1 = fp_sqr(q ? p.y : p.x);
     const Fp A = fp_pair_bcast<0>(r1), B = fp_pair_bcast<1>(r1);
-    const Fp r2 = fp_sqr(q ? fp_add(p.x, B) : B);
+    const Fp r2 = fp_sqr(q ? fp_add_lazy(p.x, B) : B);
     const Fp C = fp_pair_bcast<0>(r2), T = fp_pair_bcast<1>(r2);
-    const Fp E = fp_add(fp_dbl(A), A);
+    const Fp E = fp_triple_lazy(A);  // feeds products only
     const Fp F = fp_sqr(E);
     const Fp D = fp_dbl(fp_sub(fp_sub(T, A), C));
     G1J r;
     r.x = fp_sub(F, fp_dbl(D));
     const Fp C8 = fp_dbl(fp_dbl(fp_dbl(C)));
-    const Fp r4 = fp_mul(q ? p.y : E, q ? p.z : fp_sub(D, r.x));
+    const Fp r4 = fp_mul(q ? p.y : E, q ? p.z : fp_sub_lazy(D, r.x));
     r.y = fp_sub(fp_pair_bcast<0>(r4), C8);
     r.z = fp_dbl(fp_pair_bcast<1>(r4));
     return r;

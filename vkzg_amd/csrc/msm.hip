@@ -341,19 +341,19 @@ VKZG_DEV_COLD void combine_checked(G1X *acc, const G1X *ws, uint32_t W, uint32_t
 // its products are split into dependent rounds of at most four, and DPP quad
 // broadcasts give every lane each round's products.  Same formulas and
 // operand order as g1x_dbl (3 rounds instead of 9 products) and g1x_add_fast
-// (4 rounds instead of 14).
+// (4 rounds instead of 14); sums that only feed products stay lazy.
 VKZG_DEV G1X g1x_dbl_quad(const G1X &p, uint32_t q) {
     if (fp_is_zero(p.zz)) return p;  // uniform in the quad
-    const Fp U = fp_dbl(p.y);
+    const Fp U = fp_add_lazy(p.y, p.y);  // U and M feed products only
     const Fp r1 = fp_sqr((q & 1u) ? p.x : U);
     const Fp V = fp_quad_bcast<0>(r1), X2 = fp_quad_bcast<1>(r1);
-    const Fp M = fp_add(fp_dbl(X2), X2);
+    const Fp M = fp_triple_lazy(X2);
     const Fp r2 = fp_mul(q == 0 ? U : (q == 1 ? p.x : (q == 2 ? M : V)), q == 2 ? M : (q == 3 ? p.zz : V));
     const Fp W = fp_quad_bcast<0>(r2), S = fp_quad_bcast<1>(r2);
     G1X r;
     r.x = fp_sub(fp_quad_bcast<2>(r2), fp_dbl(S));  // M^2 - 2S
     r.zz = fp_quad_bcast<3>(r2);                    // V ZZ
-    const Fp r3 = fp_mul(q == 0 ? M : W, q == 0 ? fp_sub(S, r.x) : (q == 1 ? p.y : p.zzz));
+    const Fp r3 = fp_mul(q == 0 ? M : W, q == 0 ? fp_sub_lazy(S, r.x) : (q == 1 ? p.y : p.zzz));
     r.y = fp_sub(fp_quad_bcast<0>(r3), fp_quad_bcast<1>(r3));  // M (S - X3) - W Y
     r.zzz = fp_quad_bcast<2>(r3);                               // W ZZZ
     return r;
