@@ -125,7 +125,8 @@ vkzg_result vkzg_amd_prover_info(const vkzg_prover *p, vkzg_amd_info *out);
 typedef struct {
     uint32_t phase_a_groups;        /* window groups (work-queue tiles) per phase-A MSM, 1-10 */
     uint32_t phase_a_blocks_per_cu; /* persistent phase-A grid in blocks per CU, 1-2 */
-    uint32_t normalize_chunk;       /* points per batched inversion, 1-256 */
+    uint32_t normalize_chunk;       /* points per batched inversion, 1-256; 0 = the smallest chunk
+                                       whose lanes fit in one wave per SIMD (the default) */
     uint32_t splits;                /* sub-streams a batch is spread over, 1-4 */
     uint32_t recovery_cols;         /* recovery NTT_8192 sub-transforms per pass block: 2, 4, 8 or 16 */
     uint32_t ladder_splits;         /* doubling-ladder segments, 1, 2 or 4; phase A computes the start

@@ -248,21 +248,31 @@ Tuning tuning_for(const vkzg_prover *p, uint32_t nb) {
 }
 
 // Defaults before (or without) autotuning: small batches split each MSM into
-// more window groups and invert in smaller chunks, so that more lanes work
-// on the few MSMs there are.
+// more window groups, so that more lanes work on the few MSMs there are; the
+// normalisation chunk is fitted to the chip (norm_chunk 0).
 void default_tunings(vkzg_prover *p) {
     static const uint32_t groups[kTuneClasses] = {4, 4, 2, 1, 1};
-    static const uint32_t chunk[kTuneClasses] = {8, 8, 16, 32, 32};
     for (int c = 0; c < kTuneClasses; c++) {
         p->tune[c] = Tuning{};
         p->tune[c].groups = groups[c];
-        p->tune[c].norm_chunk = chunk[c];
+        p->tune[c].norm_chunk = 0;
     }
+}
+
+// Points per lane of the normalisation for norm_chunk 0: the smallest chunk
+// whose lanes fit in one wave per SIMD.  Each lane's work is one inversion
+// plus 7 products per point, issued at one wave per SIMD; a second wave on a
+// SIMD doubles the stage, so 557,056 points (256 blobs) take chunks of 9 on
+// 1024 SIMDs, not 8.
+uint32_t fit_norm_chunk(uint64_t count, uint32_t cus) {
+    const uint64_t lanes = (uint64_t)cus * 4 * 64;
+    const uint64_t c = (count + lanes - 1) / lanes;
+    return (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(c, 256));
 }
 
 bool tuning_valid(const Tuning &t) {
     return t.groups >= 1 && t.groups <= kMaxGroups && t.blocks_per_cu >= 1 && t.blocks_per_cu <= 2 &&
-           t.norm_chunk >= 1 && t.norm_chunk <= 256 && t.splits >= 1 && t.splits <= (uint32_t)kMaxSplits &&
+           t.norm_chunk <= 256 && t.splits >= 1 && t.splits <= (uint32_t)kMaxSplits &&
            (t.rec_cols == 2 || t.rec_cols == 4 || t.rec_cols == 8 || t.rec_cols == 16) &&
            (t.ladder_splits == 1 || t.ladder_splits == 2 || t.ladder_splits == 4) &&
            (t.reduce_lanes == 8 || t.reduce_lanes == 16 || t.reduce_lanes == 32);
@@ -293,7 +303,8 @@ void proofs_range(vkzg_prover *p, WorkSet &ws, uint32_t off, uint32_t nb, uint8_
     clk.mark("ladder");
     ck(launch_fold(fold, lad, nb, s));
     clk.mark("fold");
-    ck(launch_jac_to_affine(lad_aff, fold, prefix, (uint64_t)nb * npos, tn.norm_chunk, s));
+    const uint32_t chunk = tn.norm_chunk ? tn.norm_chunk : fit_norm_chunk((uint64_t)nb * npos, p->cus);
+    ck(launch_jac_to_affine(lad_aff, fold, prefix, (uint64_t)nb * npos, chunk, s));
     ck(launch_glv_phi(lad_phi, lad_aff, (uint64_t)nb * npos, ws.queue_b + off, ws.queue_r + off, s));
     clk.mark("normalize");
     ck(launch_phase_b(buckets, ws.bucket_flags + off * 128 * 128, lad_aff, lad_phi, p->kb_items,
@@ -499,7 +510,7 @@ void autotune(vkzg_prover *p) {
             for (uint32_t m : {2u, 4u}) { Tuning t = best; t.ladder_splits = m; consider(t); }
         for (uint32_t r : {16u, 32u}) { Tuning t = best; t.reduce_lanes = r; consider(t); }
         for (uint32_t b : {1u, 2u}) { Tuning t = best; t.blocks_per_cu = b; consider(t); }
-        for (uint32_t ch : {8u, 16u, 32u, 64u}) { Tuning t = best; t.norm_chunk = ch; consider(t); }
+        for (uint32_t ch : {0u, 8u, 16u, 32u, 64u}) { Tuning t = best; t.norm_chunk = ch; consider(t); }
         if (nb >= 8) { Tuning t = best; t.splits = 2; consider(t); }
         p->tune[c] = best;
     }

@@ -26,7 +26,7 @@ constexpr uint32_t kMaxGroups = 10;  // phase-A window groups per MSM
 struct Tuning {
     uint32_t groups = 1;         // phase-A window groups (tiles) per MSM, 1..kMaxGroups
     uint32_t blocks_per_cu = 2;  // persistent phase-A grid = CUs x this, 1..2
-    uint32_t norm_chunk = 32;    // points per batched inversion in the normalisation
+    uint32_t norm_chunk = 0;     // points per batched inversion in the normalisation (0: fitted to the chip)
     uint32_t splits = 1;         // sub-streams, 1..kMaxSplits
     uint32_t rec_cols = 8;       // NTT_8192 sub-transforms per pass block: 2, 4, 8 or 16
     uint32_t ladder_splits = 1;  // ladder segments (phase A also computes their starts): 1, 2 or 4

@@ -8,7 +8,7 @@ namespace {
 // Batched Jacobian -> affine (Montgomery's trick, k_normalize.comp:29-56).
 // Thread t converts the points t, t+T, t+2T, ... (T = thread count) so that
 // neighbouring lanes touch neighbouring points.
-__global__ void __launch_bounds__(64) k_jac_to_affine(G1A *out, const G1J *in, Fp *prefix, uint64_t count, uint32_t chunk,
+__global__ void __launch_bounds__(256) k_jac_to_affine(G1A *out, const G1J *in, Fp *prefix, uint64_t count, uint32_t chunk,
                                 uint64_t nthreads) {
     __builtin_amdgcn_s_setprio(3);  // dependent inversion chain: issue first
     const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -137,7 +137,9 @@ __global__ void __launch_bounds__(64) k_table_multiples(G1AT *table, const G1A *
 hipError_t launch_jac_to_affine(G1A *out, const G1J *in, Fp *prefix, uint64_t count, uint32_t chunk,
                                 hipStream_t s) {
     const uint64_t nthreads = (count + chunk - 1) / chunk;
-    hipLaunchKernelGGL(k_jac_to_affine, dim3(blocks_for(nthreads, 64)), dim3(64), 0, s, out, in,
+    // four-wave workgroups: spread over a CU's SIMDs (one-wave workgroups
+    // can stack on some SIMDs while others idle)
+    hipLaunchKernelGGL(k_jac_to_affine, dim3(blocks_for(nthreads, 256)), dim3(256), 0, s, out, in,
                        prefix, count, chunk, nthreads);
     return hipGetLastError();
 }
