@@ -15,12 +15,13 @@
 namespace vkzg_amd {
 
 namespace {
-constexpr int kScalarThreads = 256;
-constexpr int kPolyPitch = 4096;          // LDS pitch of the polynomial (SoA limbs)
-constexpr int kColPitch = 4 * 128;        // LDS pitch of the 4 per-wave circulant columns
+constexpr int kScalarThreads = 512;
+constexpr int kScalarWaves = kScalarThreads / 64;
+constexpr int kPolyPitch = 4096;                 // LDS pitch of the polynomial (SoA limbs)
+constexpr int kColPitch = kScalarWaves * 128;    // LDS pitch of the per-wave circulant columns
 
-// One block per blob.  LDS: 8 x 4096 words of polynomial + 8 x 512 words of
-// circulant columns = 144 KiB (gfx950 has 160 KiB per CU).
+// One block per blob, 8 waves (2 per SIMD).  LDS: 8 x 4096 words of
+// polynomial + 8 x 1024 words of circulant columns = 160 KiB, all of a CU.
 __global__ void __launch_bounds__(kScalarThreads)
     k_fk20_scalars(Fr *scal, const uint8_t *blobs, uint64_t blob_stride, uint32_t *err,
                    const Fr *roots, const Fr *roots_inv, uint32_t *queue, Fr *coeffs,
@@ -44,9 +45,13 @@ __global__ void __launch_bounds__(kScalarThreads)
         if (!limbs_lt<8>(v.v, cfr::P)) bad = true;
         lds_store_fr(poly, kPolyPitch, i, fr_from_canonical(v));
     }
-    // Every blob's flag is written (0 or 1), so err needs no clearing first.
-    const int any_bad = __syncthreads_or(bad ? 1 : 0);
-    if (tid == 0) err[blob] = any_bad ? 1u : 0u;
+    // Every blob's flag is cleared here and set by any lane that saw a
+    // non-canonical element, so err needs no clearing before the launch (a
+    // barrier orders the clear before the sets; __syncthreads_or would take
+    // LDS beyond the 160 KiB the arrays use).
+    if (tid == 0) err[blob] = 0u;
+    __syncthreads();
+    if (bad) atomicOr(&err[blob], 1u);
     // Clear the tile counter of the persistent phase-A launch that follows.
     if (blob == 0 && tid == 0) queue[0] = 0u;
 
@@ -66,7 +71,7 @@ __global__ void __launch_bounds__(kScalarThreads)
             cdst[i] = i < 4096 ? lds_load_fr(poly, kPolyPitch, i) : fr_zero();
     }
 
-    // 3. 64 circulant columns, 4 at a time (one per wave): tc_i[0] = c[4095-i],
+    // 3. 64 circulant columns, 8 at a time (one per wave): tc_i[0] = c[4095-i],
     //    tc_i[128-m] = c[4095-i-64m] (m = 1..62), zero elsewhere; NTT_128;
     //    scal[j][i] = canonical(result[j]) (k_build_circulant.comp:30-72).
     const int wave = tid >> 6;
@@ -84,8 +89,8 @@ __global__ void __launch_bounds__(kScalarThreads)
         mult[3] = fr_const(cfr::TWO_POW_96_M);
     }
 #pragma unroll 1
-    for (int round = 0; round < 16; round++) {
-        const int i = round * 4 + wave;
+    for (int round = 0; round < 64 / kScalarWaves; round++) {
+        const int i = round * kScalarWaves + wave;
 #pragma unroll
         for (int h = 0; h < 2; h++) {
             const int t = lane + 64 * h;
@@ -95,7 +100,7 @@ __global__ void __launch_bounds__(kScalarThreads)
             lds_store_fr(col, kColPitch, (int)brev_bits(t, 7), v);
         }
         __syncthreads();
-        // NTT_128 of this wave's column; all four waves step together.
+        // NTT_128 of this wave's column; all waves step together.
 #pragma unroll 1
         for (int s = 1; s <= 7; s++) {
             const int half = 1 << (s - 1);
