@@ -392,8 +392,17 @@ def main():
         prover.set_stage_timing(True)
         prover.recover_cells_device(d_out.ptr, d_cells.ptr, d_present.ptr, present, nr,
                                     stream.handle)
-        secondary["recovery_stage_ms"] = prover.last_stage_times()
+        rec_stages = prover.last_stage_times()
+        secondary["recovery_stage_ms"] = rec_stages
         prover.set_stage_timing(False)
+        # Each NTT pass reads 256 KiB and writes 256 KiB per blob (8192 values
+        # of 32 bytes, or the 2048-byte cells of the first and last pass).
+        pass_bytes = nr * 2 * 8192 * 32
+        gbps = {k: pass_bytes / (v * 1e-3) / 1e9 for k, v in rec_stages.items() if "pass" in k and v > 0}
+        secondary["recovery_pass_hbm"] = {
+            "bytes_per_pass": pass_bytes, "gb_per_s": gbps,
+            "mean_gb_per_s": sum(gbps.values()) / len(gbps) if gbps else None,
+            "frac_of_8_tb_per_s": (sum(gbps.values()) / len(gbps)) / 8000 if gbps else None}
         # uneven recovery: call sizes U[1,128], per-blob present count U[64,127]
         calls = [rng.randint(1, 128) for _ in range(12)]
         maxn = max(calls)
