@@ -27,17 +27,18 @@ namespace {
 // On the coset every y satisfies y^128 = 7^8192, so the product over all 128
 // cells is y^128 - 1 = 7^8192 - 1, a constant K, and
 //   1 / z(y) = K^-1 * prod over present cells c of (y - omega_128^brev7(c)),
-// which needs no inversion.  Each value is split over two threads (cells
-// 0-63 and 64-127) whose partial products meet in LDS.
-__global__ void __launch_bounds__(512)
+// which needs no inversion.  Block (blob, side) computes one side's 128
+// values; each value is split over 8 threads (16 cells each, so a chain is
+// at most 16 products) whose partial products meet in an LDS tree.  A wave
+// holds one side and one cell range, so its skips are uniform.
+__global__ void __launch_bounds__(1024)
     k_rec_vanishing(Fr *zd, Fr *izc, const uint8_t *present, const Fr *roots) {
     __builtin_amdgcn_s_setprio(3);  // short dependent chains: issue first
-    __shared__ Fr part[256];
+    __shared__ Fr part[8][128];
     const uint32_t blob = blockIdx.x;
-    const uint32_t t = threadIdx.x & 255u;
-    const uint32_t half = threadIdx.x >> 8;
-    const uint32_t k = t & 127u;
-    const bool coset = t >= 128;
+    const bool coset = blockIdx.y != 0;
+    const uint32_t k = threadIdx.x & 127u;
+    const uint32_t q = threadIdx.x >> 7;  // cells [16 q, 16 q + 16)
     Fr x = roots[(64u * k) & 8191u];
     if (coset) x = fr_mul(x, fr_const(cfr::SEVEN_POW64_M));
     const uint8_t *pm = present + 128ull * blob;
@@ -45,15 +46,20 @@ __global__ void __launch_bounds__(512)
     const uint8_t want = coset ? 1 : 0;
     Fr prod = fr_one();
 #pragma unroll 1
-    for (uint32_t c = 64u * half; c < 64u * half + 64u; c++) {
+    for (uint32_t c = 16u * q; c < 16u * q + 16u; c++) {
         if ((pm[c] != 0) != (want != 0)) continue;
         const Fr root = roots[(64u * brev_bits(c, 7)) & 8191u];
         prod = fr_mul(prod, fr_sub(x, root));
     }
-    if (half) part[t] = prod;
+    part[q][k] = prod;
     __syncthreads();
-    if (half) return;
-    prod = fr_mul(prod, part[t]);
+#pragma unroll 1
+    for (uint32_t w = 4; w > 0; w >>= 1) {
+        if (q < w) part[q][k] = fr_mul(part[q][k], part[q + w][k]);
+        __syncthreads();
+    }
+    if (q != 0) return;
+    prod = part[0][k];
     if (coset) izc[blob * 128 + k] = fr_mul(prod, fr_const(cfr::INV_SEVEN_POW8192_MINUS_1_M));
     else zd[blob * 128 + k] = prod;
 }
@@ -180,7 +186,7 @@ void pass_b_cols(void *out, const Fr *in, int out_mode, const Fr *mul_table, con
 
 hipError_t launch_rec_vanishing(Fr *zd, Fr *izc, const uint8_t *present, const Fr *roots,
                                 uint32_t nblobs, hipStream_t s) {
-    hipLaunchKernelGGL(k_rec_vanishing, dim3(nblobs), dim3(512), 0, s, zd, izc, present, roots);
+    hipLaunchKernelGGL(k_rec_vanishing, dim3(nblobs, 2), dim3(1024), 0, s, zd, izc, present, roots);
     return hipGetLastError();
 }
 
