@@ -27,18 +27,20 @@ namespace {
 // On the coset every y satisfies y^128 = 7^8192, so the product over all 128
 // cells is y^128 - 1 = 7^8192 - 1, a constant K, and
 //   1 / z(y) = K^-1 * prod over present cells c of (y - omega_128^brev7(c)),
-// which needs no inversion.  Block (blob, side) computes one side's 128
-// values; each value is split over 8 threads (16 cells each, so a chain is
-// at most 16 products) whose partial products meet in an LDS tree.  A wave
-// holds one side and one cell range, so its skips are uniform.
-__global__ void __launch_bounds__(1024)
+// which needs no inversion.  Block (blob, side, half) computes 64 of one
+// side's 128 values; each value is split over 8 threads (16 cells each, so a
+// chain is at most 16 products) whose partial products meet in an LDS tree.
+// A wave holds one side and one cell range, so its skips are uniform, and the
+// 2 x 2 blocks per blob spread the chains over the CUs (the work is issue-bound).
+__global__ void __launch_bounds__(512)
     k_rec_vanishing(Fr *zd, Fr *izc, const uint8_t *present, const Fr *roots) {
     __builtin_amdgcn_s_setprio(3);  // short dependent chains: issue first
-    __shared__ Fr part[8][128];
+    __shared__ Fr part[8][64];
     const uint32_t blob = blockIdx.x;
     const bool coset = blockIdx.y != 0;
-    const uint32_t k = threadIdx.x & 127u;
-    const uint32_t q = threadIdx.x >> 7;  // cells [16 q, 16 q + 16)
+    const uint32_t l = threadIdx.x & 63u;
+    const uint32_t k = blockIdx.z * 64u + l;
+    const uint32_t q = threadIdx.x >> 6;  // cells [16 q, 16 q + 16)
     Fr x = roots[(64u * k) & 8191u];
     if (coset) x = fr_mul(x, fr_const(cfr::SEVEN_POW64_M));
     const uint8_t *pm = present + 128ull * blob;
@@ -51,15 +53,15 @@ __global__ void __launch_bounds__(1024)
         const Fr root = roots[(64u * brev_bits(c, 7)) & 8191u];
         prod = fr_mul(prod, fr_sub(x, root));
     }
-    part[q][k] = prod;
+    part[q][l] = prod;
     __syncthreads();
 #pragma unroll 1
     for (uint32_t w = 4; w > 0; w >>= 1) {
-        if (q < w) part[q][k] = fr_mul(part[q][k], part[q + w][k]);
+        if (q < w) part[q][l] = fr_mul(part[q][l], part[q + w][l]);
         __syncthreads();
     }
     if (q != 0) return;
-    prod = part[0][k];
+    prod = part[0][l];
     if (coset) izc[blob * 128 + k] = fr_mul(prod, fr_const(cfr::INV_SEVEN_POW8192_MINUS_1_M));
     else zd[blob * 128 + k] = prod;
 }
@@ -186,7 +188,7 @@ void pass_b_cols(void *out, const Fr *in, int out_mode, const Fr *mul_table, con
 
 hipError_t launch_rec_vanishing(Fr *zd, Fr *izc, const uint8_t *present, const Fr *roots,
                                 uint32_t nblobs, hipStream_t s) {
-    hipLaunchKernelGGL(k_rec_vanishing, dim3(nblobs, 2), dim3(1024), 0, s, zd, izc, present, roots);
+    hipLaunchKernelGGL(k_rec_vanishing, dim3(nblobs, 2, 2), dim3(512), 0, s, zd, izc, present, roots);
     return hipGetLastError();
 }
 
