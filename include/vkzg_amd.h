@@ -219,6 +219,12 @@ vkzg_result vkzg_amd_debug_fr_mul(vkzg_prover *p, uint32_t *out, const uint32_t 
  * cycles per product of a single wave alone (its dependency latency). */
 vkzg_result vkzg_amd_debug_fp_mul_rate(vkzg_prover *p, uint32_t waves_per_simd, uint32_t iters,
                                        double *muls_per_s, double *one_wave_cycles);
+/* Test switch: with enable != 0, every later proof pipeline on p sends all of
+ * phase A, phase B and the bucket reduction through their checked recompute
+ * (the path the fast point additions take when they meet a doubling or a
+ * cancellation, which random inputs never reach).  Proofs are unchanged; only
+ * slower. */
+vkzg_result vkzg_amd_debug_force_checked(vkzg_prover *p, int enable);
 /* n device pairings e(P_i, Q_i)^3 (P_i 48-byte compressed G1, Q_i 96-byte
  * compressed G2), each as 12 Fp values of 48 bytes, big-endian, in the order
  * (w^0 real, w^0 imaginary, w^1 real, ...) of Fp12 = Fp2[w]/(w^6 - (1 + u)). */

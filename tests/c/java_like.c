@@ -2,9 +2,14 @@
  * Calls libvkzg.so the way the reference's Java FFM binding does
  * (bindings/java/src/main/java/vkzg/Vkzg.java:53-184): dlopen, dlsym of the
  * eight C symbols, a 4-byte options struct, VKZG_TABLES_PATH set before
- * vkzg_prover_new.  Usage:
- *   java_like LIB --symbols-only          symbol lookup + NULL-argument contract
- *   java_like LIB BLOB PROOFS [N]         prover_new, one N-blob call, memcmp
+ * vkzg_prover_new, then computeCellKzgProofsBatch and recoverCellsBatch.
+ * Usage:
+ *   java_like LIB --symbols-only
+ *       symbol lookup + NULL-argument contract (no GPU needed)
+ *   java_like LIB TABLES BLOBS PROOFS N [CELLS PRESENT WANT M]
+ *       VKZG_TABLES_PATH=TABLES, prover_new, one N-blob proof call memcmp'd
+ *       against PROOFS, and with CELLS given one M-blob recover call
+ *       memcmp'd against WANT; prints "proofs ok" / "recover ok".
  */
 #include <dlfcn.h>
 #include <stdint.h>
@@ -36,7 +41,7 @@ static uint8_t *slurp(const char *path, size_t *len) {
     fseek(f, 0, SEEK_END);
     *len = (size_t)ftell(f);
     fseek(f, 0, SEEK_SET);
-    uint8_t *b = malloc(*len);
+    uint8_t *b = malloc(*len ? *len : 1);
     if (fread(b, 1, *len, f) != *len) {
         fclose(f);
         free(b);
@@ -70,16 +75,18 @@ int main(int argc, char **argv) {
         return 1;
     }
     if (strcmp(argv[2], "--symbols-only") == 0) return 0;
+    if (argc < 6) return 2;
 
     size_t blob_len = 0, proofs_len = 0;
-    uint8_t *blob = slurp(argv[2], &blob_len);
-    uint8_t *want = slurp(argv[3], &proofs_len);
-    const size_t n = argc > 4 ? (size_t)atoi(argv[4]) : 1;
+    uint8_t *blob = slurp(argv[3], &blob_len);
+    uint8_t *want = slurp(argv[4], &proofs_len);
+    const size_t n = (size_t)atoi(argv[5]);
     if (!blob || !want || blob_len != n * 131072 || proofs_len != n * 6144) {
         printf("FAIL: bad input files\n");
         return 1;
     }
-    setenv("VKZG_TABLES_PATH", "/nonexistent/precomputed_tables.bin", 1);
+    /* Vkzg.extractPrecomputedTables: the path is set before the prover exists. */
+    setenv("VKZG_TABLES_PATH", argv[2], 1);
     uint8_t *opts = calloc(1, 4); /* Java allocates exactly 4 bytes */
     options_default(opts);
     void *prover = NULL;
@@ -97,6 +104,31 @@ int main(int argc, char **argv) {
         return 1;
     }
     printf("proofs ok\n");
+
+    if (argc >= 10) {
+        size_t cells_len = 0, present_len = 0, want_len = 0;
+        uint8_t *cells = slurp(argv[6], &cells_len);
+        uint8_t *present = slurp(argv[7], &present_len);
+        uint8_t *want_cells = slurp(argv[8], &want_len);
+        const size_t m = (size_t)atoi(argv[9]);
+        if (!cells || !present || !want_cells || cells_len != m * 262144 || present_len != m * 128 ||
+            want_len != m * 262144) {
+            printf("FAIL: bad recovery input files\n");
+            return 1;
+        }
+        uint8_t *out = malloc(m * 262144);
+        rc = recover(prover, out, cells, present, m);
+        if (rc != 0 || memcmp(out, want_cells, m * 262144) != 0) {
+            printf("FAIL: recover rc=%d (%s) match=%d\n", rc, error_string(rc),
+                   rc == 0 && memcmp(out, want_cells, m * 262144) == 0);
+            return 1;
+        }
+        printf("recover ok\n");
+        free(out);
+        free(cells);
+        free(present);
+        free(want_cells);
+    }
     prover_free(prover);
     free(opts);
     free(proofs);

@@ -29,6 +29,17 @@ def test_library_exports_every_declared_symbol():
         assert hasattr(L, name), name
 
 
+def test_library_exports_only_the_declared_c_abi():
+    """libvkzg.so exports the functions the two headers declare and nothing
+    else: no C++ runtime symbols, no embedded data (a version script keeps
+    them local)."""
+    r = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    exported = {line.split()[-1] for line in r.stdout.splitlines() if line.strip()}
+    declared = declared_functions("vkzg.h") | declared_functions("vkzg_amd.h")
+    assert exported == declared, (sorted(exported - declared), sorted(declared - exported))
+
+
 def test_reference_abi_is_exactly_eight_functions():
     assert declared_functions("vkzg.h") == set(_lib.ABI_SYMBOLS)
 

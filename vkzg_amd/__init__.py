@@ -221,6 +221,11 @@ class Prover:
     def set_graphs(self, enable=True):
         _check(load().vkzg_amd_set_graphs(self._h, 1 if enable else 0))
 
+    def debug_force_checked(self, enable=True):
+        """Send phase A, phase B and the bucket reduction of later proof calls
+        through their checked recompute (a test switch; proofs are unchanged)."""
+        _check(load().vkzg_amd_debug_force_checked(self._h, 1 if enable else 0))
+
     def autotune(self):
         """Run the autotuner; returns its wall time in ms."""
         ms = ctypes.c_double()

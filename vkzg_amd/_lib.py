@@ -52,6 +52,7 @@ EXT_SYMBOLS = [
     "vkzg_amd_debug_g1_msm",
     "vkzg_amd_debug_verify_terms",
     "vkzg_amd_debug_fp_mul_rate",
+    "vkzg_amd_debug_force_checked",
     "vkzg_amd_write_v1_tables",
     "vkzg_amd_verify_challenge",
     "vkzg_amd_sha256",
@@ -180,6 +181,8 @@ def load():
     L.vkzg_amd_set_tuning.argtypes = [vp, sz, ctypes.POINTER(Tuning)]
     L.vkzg_amd_set_graphs.restype = ctypes.c_int
     L.vkzg_amd_set_graphs.argtypes = [vp, ctypes.c_int]
+    L.vkzg_amd_debug_force_checked.restype = ctypes.c_int
+    L.vkzg_amd_debug_force_checked.argtypes = [vp, ctypes.c_int]
     L.vkzg_amd_autotune.restype = ctypes.c_int
     L.vkzg_amd_autotune.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
     L.vkzg_amd_debug_fp_inv.restype = ctypes.c_int

@@ -66,7 +66,10 @@ def build(jobs=8, verbose=True):
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
         objs = list(ex.map(compile_one, SOURCES))
     tmp = LIB + ".tmp"
-    run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + [embed_obj])
+    # Export only the C ABI (vkzg_* and vkzg_amd_*): a version script, so the
+    # extern "C" functions keep default visibility in the objects.
+    run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + [embed_obj] +
+        ["-Wl,--version-script=" + os.path.join(CSRC, "exports.map")])
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)

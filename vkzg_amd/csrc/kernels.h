@@ -67,9 +67,12 @@ hipError_t launch_fk20_scalars(Fr *scal, const uint8_t *blobs, uint64_t blob_str
                                const Fr *roots, const Fr *roots_inv, uint32_t *queue, Fr *coeffs,
                                uint32_t splits, uint32_t nblobs, hipStream_t s);
 // Persistent: `grid` blocks drain (msm, window group) tiles from queue[0].
+// force_checked != 0 (a test switch) sends every tile down the checked
+// recompute, as if the fast additions had met a doubling or cancellation; the
+// same holds for launch_phase_b and launch_bucket_reduce.
 hipError_t launch_phase_a(G1X *part, const Fr *scal, const G1AT *table, uint32_t c, uint32_t W,
                           uint32_t nmsm, uint32_t groups, uint32_t grid, uint32_t *queue,
-                          hipStream_t s);
+                          uint32_t force_checked, hipStream_t s);
 hipError_t launch_ladder(G1J *lad, const G1X *part, uint32_t groups, uint32_t splits, uint32_t nblobs,
                          hipStream_t s);
 hipError_t launch_fold(G1J *fold, const G1J *lad, uint32_t nblobs, hipStream_t s);
@@ -81,11 +84,12 @@ hipError_t launch_glv_phi(G1A *phi, const G1A *pts, uint64_t count, uint32_t *qu
 // buckets; flags[B][128][128] is scratch.
 hipError_t launch_phase_b(G1X *buckets, uint8_t *flags, const G1A *lad_aff, const G1A *lad_phi,
                           const uint32_t *items, const uint32_t *offsets, uint32_t nblobs, uint32_t grid,
-                          uint32_t *queue, hipStream_t s);
+                          uint32_t *queue, uint32_t force_checked, hipStream_t s);
 // `lanes` (8, 16 or 32) lanes per output; persistent (`grid` blocks drain
 // queue[0]), then the recompute of flagged outputs; flags[B][128] is scratch.
 hipError_t launch_bucket_reduce(G1X *cvals, uint8_t *flags, const G1X *buckets, uint32_t nblobs,
-                                uint32_t lanes, uint32_t grid, uint32_t *queue, hipStream_t s);
+                                uint32_t lanes, uint32_t grid, uint32_t *queue, uint32_t force_checked,
+                                hipStream_t s);
 hipError_t launch_finalize(uint8_t *proofs, const G1X *cvals, uint32_t nblobs, hipStream_t s);
 
 // --------------------------------------------------------------- recovery

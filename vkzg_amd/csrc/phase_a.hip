@@ -52,7 +52,7 @@ __device__ __attribute__((noinline)) void phase_a_slow(G1X *out, const Fr *scal,
 // kernel that runs before this one.
 __global__ void __launch_bounds__(256, 2)
     k_phase_a(G1X *part, const Fr *scal, const G1AT *table, uint32_t c, uint32_t W, uint32_t nmsm,
-              uint32_t groups, uint32_t *q) {
+              uint32_t groups, uint32_t *q, uint32_t force_checked) {
     __shared__ uint32_t item_sh;
     const uint32_t t = threadIdx.x % kMsmLanes;
     const uint32_t H = 1u << (c - 1);
@@ -72,7 +72,7 @@ __global__ void __launch_bounds__(256, 2)
         const uint32_t j = msm % 128;
         const uint32_t w0 = g * W / groups, w1 = (g + 1) * W / groups;
         G1X acc = g1x_identity();
-        bool exc = false, started = false;
+        bool exc = force_checked != 0, started = false;
         if (live) {
 #pragma unroll 1
             for (uint32_t qq = 0; qq < 64 / kMsmLanes; qq++) {
@@ -117,12 +117,12 @@ __global__ void __launch_bounds__(256, 2)
 
 hipError_t launch_phase_a(G1X *part, const Fr *scal, const G1AT *table, uint32_t c, uint32_t W,
                           uint32_t nmsm, uint32_t groups, uint32_t grid, uint32_t *queue,
-                          hipStream_t s) {
+                          uint32_t force_checked, hipStream_t s) {
     const uint32_t nitems = (nmsm * groups + kTilesPerItem - 1) / kTilesPerItem;
     if (grid > nitems) grid = nitems;
     if (grid == 0) grid = 1;
     hipLaunchKernelGGL(k_phase_a, dim3(grid), dim3(256), 0, s, part, scal, table, c, W, nmsm, groups,
-                       queue);
+                       queue, force_checked);
     return hipGetLastError();
 }
 

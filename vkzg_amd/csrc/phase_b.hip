@@ -36,7 +36,7 @@ __device__ __attribute__((noinline)) void phase_b_slow(G1X *out, const G1A *L, c
 // calls.
 __global__ void __launch_bounds__(256, 2)
     k_phase_b(G1X *buckets, uint8_t *flags, const G1A *lad_aff, const G1A *lad_phi, const uint32_t *items,
-              const uint32_t *offsets, uint32_t ngroups, uint32_t *q) {
+              const uint32_t *offsets, uint32_t ngroups, uint32_t *q, uint32_t force_checked) {
     const uint32_t lane = threadIdx.x % 64;
     const uint32_t nitems = 128u * ngroups;
 #pragma unroll 1
@@ -53,7 +53,7 @@ __global__ void __launch_bounds__(256, 2)
         const G1A *L = lad_aff + (size_t)(blob * 2 + half) * 64 * kLadPos;
         const G1A *Phi = lad_phi + (size_t)(blob * 2 + half) * 64 * kLadPos;
         G1X acc = g1x_identity();
-        bool exc = false, started = false;
+        bool exc = force_checked != 0, started = false;
 #pragma unroll 1
         for (uint32_t it = lo; it < hi; it++) {
             const uint32_t packed = items[it];
@@ -94,12 +94,12 @@ __global__ void __launch_bounds__(256) k_phase_b_fix(G1X *buckets, const uint8_t
 
 hipError_t launch_phase_b(G1X *buckets, uint8_t *flags, const G1A *lad_aff, const G1A *lad_phi,
                           const uint32_t *items, const uint32_t *offsets, uint32_t nblobs, uint32_t grid,
-                          uint32_t *queue, hipStream_t s) {
+                          uint32_t *queue, uint32_t force_checked, hipStream_t s) {
     const uint32_t ngroups = nblobs * 2;  // 64-lane groups of (blob, a)
     const uint32_t nitems = 128u * ngroups;
     if (grid > (nitems + 3) / 4) grid = (nitems + 3) / 4;
     hipLaunchKernelGGL(k_phase_b, dim3(grid), dim3(256), 0, s, buckets, flags, lad_aff, lad_phi, items,
-                       offsets, ngroups, queue);
+                       offsets, ngroups, queue, force_checked);
     const uint32_t nbuckets = nblobs * 128 * 128;
     hipLaunchKernelGGL(k_phase_b_fix, dim3(blocks_for(nbuckets, 256)), dim3(256), 0, s, buckets, flags,
                        lad_aff, lad_phi, items, offsets, nbuckets);

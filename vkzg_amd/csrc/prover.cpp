@@ -239,7 +239,12 @@ void WorkSet::allocate(uint32_t B) {
 // one stream; the autotuner may pick two, and $VKZG_AMD_SPLITS (1-4) forces a
 // count.
 Tuning tuning_for(const vkzg_prover *p, uint32_t nb) {
-    Tuning t = p->tune[tune_class(nb)];
+    Tuning t;
+    {
+        std::lock_guard<std::mutex> lock(p->tune_mu);
+        t = p->tune[tune_class(nb)];
+    }
+    t.force_checked = p->force_checked.load();
     if (p->forced_splits > 0) t.splits = (uint32_t)p->forced_splits;
     if (t.splits > (uint32_t)kMaxSplits) t.splits = kMaxSplits;
     if (t.splits > nb) t.splits = nb;
@@ -297,7 +302,7 @@ void proofs_range(vkzg_prover *p, WorkSet &ws, uint32_t off, uint32_t nb, uint8_
                            tn.ladder_splits, nb, s));
     clk.mark("fk20_scalars");
     ck(launch_phase_a(u, scal, p->table, p->c, p->W, nb * 128 * tn.ladder_splits, tn.groups,
-                      p->cus * tn.blocks_per_cu, queue, s));
+                      p->cus * tn.blocks_per_cu, queue, tn.force_checked, s));
     clk.mark("phase_a");
     ck(launch_ladder(lad, u, tn.groups, tn.ladder_splits, nb, s));
     clk.mark("ladder");
@@ -308,10 +313,10 @@ void proofs_range(vkzg_prover *p, WorkSet &ws, uint32_t off, uint32_t nb, uint8_
     ck(launch_glv_phi(lad_phi, lad_aff, (uint64_t)nb * npos, ws.queue_b + off, ws.queue_r + off, s));
     clk.mark("normalize");
     ck(launch_phase_b(buckets, ws.bucket_flags + off * 128 * 128, lad_aff, lad_phi, p->kb_items,
-                      p->kb_offsets, nb, p->cus * 2, ws.queue_b + off, s));
+                      p->kb_offsets, nb, p->cus * 2, ws.queue_b + off, tn.force_checked, s));
     clk.mark("phase_b");
     ck(launch_bucket_reduce(cvals, ws.cval_flags + off * 128, buckets, nb, tn.reduce_lanes, p->cus * 2,
-                            ws.queue_r + off, s));
+                            ws.queue_r + off, tn.force_checked, s));
     clk.mark("bucket_reduce");
     ck(launch_finalize(d_proofs, cvals, nb, s));
     clk.mark("finalize");
@@ -379,7 +384,8 @@ namespace {
 uint64_t tuning_word(const Tuning &tn) {
     return (uint64_t)tn.splits | ((uint64_t)tn.groups << 8) | ((uint64_t)tn.blocks_per_cu << 16) |
            ((uint64_t)tn.norm_chunk << 24) | ((uint64_t)tn.rec_cols << 40) |
-           ((uint64_t)tn.ladder_splits << 48) | ((uint64_t)tn.reduce_lanes << 52);
+           ((uint64_t)tn.ladder_splits << 48) | ((uint64_t)tn.reduce_lanes << 52) |
+           ((uint64_t)(tn.force_checked != 0) << 62);
 }
 }  // namespace
 
@@ -492,6 +498,7 @@ void autotune(vkzg_prover *p) {
     for (int c = 0; c < kTuneClasses; c++) {
         const uint32_t nb = std::min<uint32_t>(kTuneClassTop[c], nmax);
         if (c > 0 && kTuneClassTop[c - 1] >= nmax) {
+            std::lock_guard<std::mutex> lock(p->tune_mu);
             p->tune[c] = p->tune[c - 1];  // class beyond the batch size: never used alone
             continue;
         }
@@ -512,6 +519,7 @@ void autotune(vkzg_prover *p) {
         for (uint32_t b : {1u, 2u}) { Tuning t = best; t.blocks_per_cu = b; consider(t); }
         for (uint32_t ch : {0u, 8u, 16u, 32u, 64u}) { Tuning t = best; t.norm_chunk = ch; consider(t); }
         if (nb >= 8) { Tuning t = best; t.splits = 2; consider(t); }
+        std::lock_guard<std::mutex> lock(p->tune_mu);
         p->tune[c] = best;
     }
     // Recovery: NTT sub-transforms per pass block, on random cells with the
@@ -544,6 +552,7 @@ void autotune(vkzg_prover *p) {
     for (int c = 0; c < kTuneClasses; c++) {
         const uint32_t nb = std::min<uint32_t>(kTuneClassTop[c], nmax);
         if (c > 0 && kTuneClassTop[c - 1] >= nmax) {
+            std::lock_guard<std::mutex> lock(p->tune_mu);
             p->tune[c].rec_cols = p->tune[c - 1].rec_cols;
             continue;
         }
@@ -558,6 +567,7 @@ void autotune(vkzg_prover *p) {
                 best_cols = cols;
             }
         }
+        std::lock_guard<std::mutex> lock(p->tune_mu);
         p->tune[c].rec_cols = best_cols;
     }
     (void)hipEventDestroy(e0);
@@ -887,7 +897,11 @@ vkzg_result vkzg_amd_recover_cells_device(vkzg_prover *p, void *d_cells_out, con
 
 vkzg_result vkzg_amd_get_tuning(const vkzg_prover *p, size_t num_blobs, vkzg_amd_tuning *out) {
     if (!p || !out || num_blobs == 0) return VKZG_ERR_BADARGS;
-    const Tuning t = p->tune[tune_class((uint32_t)std::min<size_t>(num_blobs, 0xffffffffu))];
+    Tuning t;
+    {
+        std::lock_guard<std::mutex> lock(p->tune_mu);
+        t = p->tune[tune_class((uint32_t)std::min<size_t>(num_blobs, 0xffffffffu))];
+    }
     out->phase_a_groups = t.groups;
     out->phase_a_blocks_per_cu = t.blocks_per_cu;
     out->normalize_chunk = t.norm_chunk;
@@ -910,6 +924,7 @@ vkzg_result vkzg_amd_set_tuning(vkzg_prover *p, size_t num_blobs, const vkzg_amd
     t.reduce_lanes = in->reduce_lanes;
     if (!tuning_valid(t)) return VKZG_ERR_BADARGS;
     std::lock_guard<std::mutex> lock(p->mu);
+    std::lock_guard<std::mutex> tlock(p->tune_mu);
     p->tune[tune_class((uint32_t)std::min<size_t>(num_blobs, 0xffffffffu))] = t;
     return VKZG_OK;
 }
@@ -918,6 +933,12 @@ vkzg_result vkzg_amd_set_graphs(vkzg_prover *p, int enable) {
     if (!p) return VKZG_ERR_BADARGS;
     std::lock_guard<std::mutex> lock(p->mu);
     p->use_graphs = enable != 0;
+    return VKZG_OK;
+}
+
+vkzg_result vkzg_amd_debug_force_checked(vkzg_prover *p, int enable) {
+    if (!p) return VKZG_ERR_BADARGS;
+    p->force_checked.store(enable ? 1u : 0u);
     return VKZG_OK;
 }
 

@@ -27,7 +27,8 @@ __device__ __attribute__((noinline)) void reduce_slow(G1X *out, const G1X *B) {
 // met by the fast additions) are recomputed by k_reduce_fix.
 template <int LANES>
 __global__ void __launch_bounds__(256, 2)
-    k_bucket_reduce(G1X *cvals, uint8_t *flags, const G1X *buckets, uint32_t nout, uint32_t *q) {
+    k_bucket_reduce(G1X *cvals, uint8_t *flags, const G1X *buckets, uint32_t nout, uint32_t *q,
+                    uint32_t force_checked) {
     constexpr uint32_t per = 128 / LANES;
     constexpr uint32_t teams = 64 / LANES;
     const uint32_t lane = threadIdx.x % 64;
@@ -41,7 +42,7 @@ __global__ void __launch_bounds__(256, 2)
         if (item >= nitems) break;
         const uint32_t o = item * teams + lane / LANES;
         G1X s = g1x_identity(), rs = g1x_identity();
-        bool exc = false;
+        bool exc = force_checked != 0;
         if (o < nout) {
             const G1X *B = buckets + (size_t)o * 128 + l * per;
 #pragma unroll 1
@@ -127,7 +128,8 @@ __global__ void __launch_bounds__(64) k_finalize(uint8_t *proofs, const G1X *cva
 }  // namespace
 
 hipError_t launch_bucket_reduce(G1X *cvals, uint8_t *flags, const G1X *buckets, uint32_t nblobs,
-                                uint32_t lanes, uint32_t grid, uint32_t *queue, hipStream_t s) {
+                                uint32_t lanes, uint32_t grid, uint32_t *queue, uint32_t force_checked,
+                                hipStream_t s) {
     const uint32_t nout = nblobs * 128;
     const uint32_t teams = 64 / (lanes == 16 || lanes == 32 ? lanes : 8);
     const uint32_t waves = (nout + teams - 1) / teams;
@@ -135,15 +137,15 @@ hipError_t launch_bucket_reduce(G1X *cvals, uint8_t *flags, const G1X *buckets, 
     switch (lanes) {
         case 16:
             hipLaunchKernelGGL(k_bucket_reduce<16>, dim3(grid), dim3(256), 0, s, cvals, flags, buckets, nout,
-                               queue);
+                               queue, force_checked);
             break;
         case 32:
             hipLaunchKernelGGL(k_bucket_reduce<32>, dim3(grid), dim3(256), 0, s, cvals, flags, buckets, nout,
-                               queue);
+                               queue, force_checked);
             break;
         default:
             hipLaunchKernelGGL(k_bucket_reduce<8>, dim3(grid), dim3(256), 0, s, cvals, flags, buckets, nout,
-                               queue);
+                               queue, force_checked);
             break;
     }
     hipLaunchKernelGGL(k_reduce_fix, dim3(blocks_for(nout, 256)), dim3(256), 0, s, cvals, flags, buckets,

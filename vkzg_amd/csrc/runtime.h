@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <mutex>
 #include <string>
@@ -31,6 +32,10 @@ struct Tuning {
     uint32_t rec_cols = 8;       // NTT_8192 sub-transforms per pass block: 2, 4, 8 or 16
     uint32_t ladder_splits = 1;  // ladder segments (phase A also computes their starts): 1, 2 or 4
     uint32_t reduce_lanes = 8;   // bucket-reduction lanes per output: 8, 16 or 32
+    // Not a tuning: the test switch that sends phase A, phase B and the bucket
+    // reduction down their checked recompute (vkzg_amd_debug_force_checked).
+    // Set by tuning_for from the prover; kept here so graph keys include it.
+    uint32_t force_checked = 0;
 };
 
 // Batch-size classes: <= 1, <= 4, <= 16, <= 64 and more blobs.
@@ -223,8 +228,13 @@ struct vkzg_prover {
     double setup_ms = 0;
     uint32_t last_splits = 1;
     int forced_splits = 0;  // $VKZG_AMD_SPLITS, 0 = automatic
-    bool use_graphs = true;  // $VKZG_AMD_GRAPHS=0 launches kernel by kernel
+    std::atomic<bool> use_graphs{true};  // $VKZG_AMD_GRAPHS=0 launches kernel by kernel
+    // Tunings per batch-size class.  Written under tune_mu (and mu); read under
+    // tune_mu by tuning_for, which the server's dispatcher thread calls
+    // without holding mu.
+    mutable std::mutex tune_mu;
     vkzg_amd::Tuning tune[vkzg_amd::kTuneClasses];
+    std::atomic<uint32_t> force_checked{0};  // vkzg_amd_debug_force_checked
     bool autotuned = false;
     double autotune_ms = 0;
     vkzg_amd::GraphCache graphs;  // graphs of the synchronous entry points
