@@ -36,6 +36,21 @@ vkzg_result vkzg_amd_recover_cells_device(vkzg_prover *p, void *d_cells_out, con
                                           const uint8_t *h_cell_present, size_t num_blobs,
                                           void *hip_stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory during a call (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc): vkzg_amd_recover_cells_device with the
+ * same arguments and results, whose scratch (2 x 128 + 2 x 8192 Fr per blob)
+ * is a fixed 8.125 MiB file-scope device array instead of the prover's
+ * working set.  The call allocates nothing (no hipMalloc, hipFree or graph).
+ * At most 16 blobs per call: more returns VKZG_ERR_BADARGS.  Every GPU holds
+ * one copy of the array until the process ends, shared by all calls on that
+ * GPU: calls that overlap on one GPU race and must be serialised by the
+ * caller. */
+vkzg_result vkzg_amd_recover_cells_device_noalloc(vkzg_prover *p, void *d_cells_out, const void *d_cells,
+                                                  const void *d_cell_present,
+                                                  const uint8_t *h_cell_present, size_t num_blobs,
+                                                  void *hip_stream);
+
 /* ---------------------------------------------------- cells and proofs
  * The consensus-spec pair of functions around the reference's two entry
  * points (the reference leaves both to the caller, include/vkzg.h:14-18):

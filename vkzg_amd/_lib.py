@@ -25,6 +25,7 @@ EXT_SYMBOLS = [
     "vkzg_amd_prover_new_with_setup",
     "vkzg_amd_compute_proofs_device",
     "vkzg_amd_recover_cells_device",
+    "vkzg_amd_recover_cells_device_noalloc",
     "vkzg_amd_prover_info",
     "vkzg_amd_set_stage_timing",
     "vkzg_amd_last_stage_times",
@@ -147,6 +148,8 @@ def load():
     L.vkzg_amd_compute_proofs_device.argtypes = [vp, vp, vp, sz, vp]
     L.vkzg_amd_recover_cells_device.restype = ctypes.c_int
     L.vkzg_amd_recover_cells_device.argtypes = [vp, vp, vp, vp, vp, sz, vp]
+    L.vkzg_amd_recover_cells_device_noalloc.restype = ctypes.c_int
+    L.vkzg_amd_recover_cells_device_noalloc.argtypes = [vp, vp, vp, vp, vp, sz, vp]
     L.vkzg_amd_prover_info.restype = ctypes.c_int
     L.vkzg_amd_prover_info.argtypes = [vp, ctypes.POINTER(VkzgAmdInfo)]
     L.vkzg_amd_set_stage_timing.restype = ctypes.c_int

@@ -20,7 +20,7 @@ ARCH = os.environ.get("VKZG_AMD_ARCH", "gfx950")
 
 SOURCES = ["setup.hip", "setup_fft.hip", "setup_tables.hip", "fk20_scalars.hip", "phase_a.hip",
            "ladder.hip", "phase_b.hip", "reduce.hip", "recover.hip", "msm.hip", "pairing.hip",
-           "verify.hip", "prover.cpp", "server.cpp", "verify.cpp", "tables_v1.cpp", "sha256.cc"]
+           "verify.hip", "prover.cpp", "noalloc.cpp", "server.cpp", "verify.cpp", "tables_v1.cpp", "sha256.cc"]
 HOST_FLAGS = ["-O3", "-fPIC", "-std=c++17", "-Wall"]  # plain host C++ (.cc): g++
 HIP_FLAGS = [
     "--offload-arch=" + ARCH,

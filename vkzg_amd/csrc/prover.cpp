@@ -283,6 +283,34 @@ bool tuning_valid(const Tuning &t) {
            (t.reduce_lanes == 8 || t.reduce_lanes == 16 || t.reduce_lanes == 32);
 }
 
+void enqueue_recovery_into(vkzg_prover *p, Fr *zd, Fr *izc, Fr *A, Fr *B, uint8_t *d_out,
+                           const uint8_t *d_cells, const uint8_t *d_present, uint32_t nb, hipStream_t s,
+                           StageClock &clk, uint32_t cols) {
+    const Fr *r = p->roots, *ri = p->roots_inv;
+    ck(launch_rec_vanishing(zd, izc, d_present, r, nb, s));
+    clk.mark("vanishing");
+    // (E * Z) -> coefficients, times 7^k / 8192
+    ck(launch_rec_pass_a(A, d_cells, 0, zd, true, r, ri, nb, cols, s));
+    clk.mark("inv_pass_a");
+    ck(launch_rec_pass_b(B, A, 0, p->shift_fwd, nullptr, true, r, ri, nb, cols, s));
+    clk.mark("inv_pass_b");
+    // coset evaluations, divided by Z on the coset
+    ck(launch_rec_pass_a(A, B, 1, nullptr, false, r, ri, nb, cols, s));
+    clk.mark("coset_pass_a");
+    ck(launch_rec_pass_b(B, A, 1, nullptr, izc, false, r, ri, nb, cols, s));
+    clk.mark("coset_pass_b");
+    // coset interpolation: coefficients of P, times 7^-k / 8192
+    ck(launch_rec_pass_a(A, B, 1, nullptr, true, r, ri, nb, cols, s));
+    clk.mark("coset_inv_pass_a");
+    ck(launch_rec_pass_b(B, A, 0, p->shift_inv, nullptr, true, r, ri, nb, cols, s));
+    clk.mark("coset_inv_pass_b");
+    // evaluate P on the whole domain, write cells
+    ck(launch_rec_pass_a(A, B, 1, nullptr, false, r, ri, nb, cols, s));
+    clk.mark("eval_pass_a");
+    ck(launch_rec_pass_b(d_out, A, 2, nullptr, nullptr, false, r, ri, nb, cols, s));
+    clk.mark("eval_pass_b");
+}
+
 namespace {
 
 // One sub-batch: blobs [off, off + nb) of working set ws.
@@ -325,31 +353,9 @@ void proofs_range(vkzg_prover *p, WorkSet &ws, uint32_t off, uint32_t nb, uint8_
 void recovery_range(vkzg_prover *p, WorkSet &ws, uint32_t off, uint32_t nb, uint8_t *d_out,
                     const uint8_t *d_cells, const uint8_t *d_present, hipStream_t s,
                     StageClock &clk, uint32_t cols) {
-    Fr *zd = ws.zd + (size_t)off * 128, *izc = ws.izc + (size_t)off * 128;
-    Fr *A = ws.bufA + (size_t)off * 8192, *B = ws.bufB + (size_t)off * 8192;
-    const Fr *r = p->roots, *ri = p->roots_inv;
-    ck(launch_rec_vanishing(zd, izc, d_present, r, nb, s));
-    clk.mark("vanishing");
-    // (E * Z) -> coefficients, times 7^k / 8192
-    ck(launch_rec_pass_a(A, d_cells, 0, zd, true, r, ri, nb, cols, s));
-    clk.mark("inv_pass_a");
-    ck(launch_rec_pass_b(B, A, 0, p->shift_fwd, nullptr, true, r, ri, nb, cols, s));
-    clk.mark("inv_pass_b");
-    // coset evaluations, divided by Z on the coset
-    ck(launch_rec_pass_a(A, B, 1, nullptr, false, r, ri, nb, cols, s));
-    clk.mark("coset_pass_a");
-    ck(launch_rec_pass_b(B, A, 1, nullptr, izc, false, r, ri, nb, cols, s));
-    clk.mark("coset_pass_b");
-    // coset interpolation: coefficients of P, times 7^-k / 8192
-    ck(launch_rec_pass_a(A, B, 1, nullptr, true, r, ri, nb, cols, s));
-    clk.mark("coset_inv_pass_a");
-    ck(launch_rec_pass_b(B, A, 0, p->shift_inv, nullptr, true, r, ri, nb, cols, s));
-    clk.mark("coset_inv_pass_b");
-    // evaluate P on the whole domain, write cells
-    ck(launch_rec_pass_a(A, B, 1, nullptr, false, r, ri, nb, cols, s));
-    clk.mark("eval_pass_a");
-    ck(launch_rec_pass_b(d_out, A, 2, nullptr, nullptr, false, r, ri, nb, cols, s));
-    clk.mark("eval_pass_b");
+    enqueue_recovery_into(p, ws.zd + (size_t)off * 128, ws.izc + (size_t)off * 128,
+                          ws.bufA + (size_t)off * 8192, ws.bufB + (size_t)off * 8192, d_out, d_cells,
+                          d_present, nb, s, clk, cols);
 }
 
 // Runs body(k, off, n, stream) for each of `splits` sub-batches, forked from
@@ -585,6 +591,7 @@ vkzg_result create(vkzg_prover **out, const vkzg_options *opts, const uint8_t *s
     if (!p) return VKZG_ERR_MALLOC;
     try {
         ck(hipGetDevice(&p->device));
+        noalloc_scratch_setup(p->device);
         hipDeviceProp_t prop;
         ck(hipGetDeviceProperties(&prop, p->device));
         p->name = prop.name;

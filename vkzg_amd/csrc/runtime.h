@@ -325,6 +325,21 @@ void enqueue_recovery(vkzg_prover *p, WorkSet &ws, StreamSet &ss, uint8_t *d_out
                       const uint8_t *d_cells, const uint8_t *d_present, uint32_t nb, hipStream_t s,
                       StageClock &clk, const Tuning &tn, GraphCache *graphs);
 
+// The recovery pipeline of nb blobs on stream s with caller-provided scratch:
+// zd and izc hold nb x 128 Fr, A and B nb x 8192 Fr each.  Every kernel writes
+// each scratch element before it reads it, so the scratch needs no clearing.
+void enqueue_recovery_into(vkzg_prover *p, Fr *zd, Fr *izc, Fr *A, Fr *B, uint8_t *d_out,
+                           const uint8_t *d_cells, const uint8_t *d_present, uint32_t nb, hipStream_t s,
+                           StageClock &clk, uint32_t cols);
+
 bool chunk_missing_ok(const uint8_t *present, uint32_t nb);
+
+// Clears the per-stage times before a call (stage timing accumulates over chunks).
+void reset_stage_times(vkzg_prover *p);
+
+// Resolves, once per device and with that device current, the address of the
+// file-scope scratch array of vkzg_amd_recover_cells_device_noalloc
+// (noalloc.cpp).  Called at prover creation.
+void noalloc_scratch_setup(int device);
 
 }  // namespace vkzg_amd
