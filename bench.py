@@ -27,7 +27,6 @@ R = 0x73EDA753299D7D483339D80809A1D80553BDA402FFFE5BFEFFFFFFFF00000001
 PEAK_INT = 256 * 2.4e9 * 64  # CU x clock x lanes: lane-ops/s (v_mad_u64_u32 issues at full rate)
 FP_M = 300                    # limb-MACs per Fp multiplication (12-limb CIOS)
 FR_M = 136                    # limb-MACs per Fr multiplication (8-limb CIOS)
-W_PROOF = 1.734e9             # limb-MACs per blob of the reference's FK20 as written
 LAD_POS = 17                  # ladder positions 2^(8d) u, d < 17 (GLV halves of the phase-B scalars)
 METRIC = "proof blobs/s @256-blob batch; recovery blobs/s; p50/p99 serve latency"
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -49,6 +48,45 @@ def stage_work(info, nb, items_per_output):
         "fold": nb * 64 * LAD_POS * 2 * add_j * FP_M,
         "phase_b": nb * 128 * items_per_output * madd * FP_M,
         "bucket_reduce": nb * 128 * (2 * 128 + 8) * add * FP_M,
+    }
+
+
+def host_abi_sweep(prover, blobs, nmax, seed, reps=10):
+    """The reference's own convention (bench/bench.cpp:38-60 and its recovery
+    sweep): the plain C ABI on host buffers, host<->device copies included,
+    1, 2, 4, ... nmax blobs per call, one warm-up call then `reps` timed calls,
+    best and mean.  Recovery keeps the first 64 of 128 cells of each blob."""
+    import ctypes
+    import vkzg_amd
+    L = vkzg_amd._lib.load()
+    h = prover.handle
+    src = ctypes.create_string_buffer(blobs, len(blobs))
+    out = ctypes.create_string_buffer(nmax * 6144)
+    rng = random.Random(seed)
+    cells = ctypes.create_string_buffer(rng.randbytes(nmax * 262144), nmax * 262144)
+    present = ctypes.create_string_buffer((bytes([1]) * 64 + bytes(64)) * nmax, nmax * 128)
+    rec_out = ctypes.create_string_buffer(nmax * 262144)
+
+    def sweep(call):
+        res = {}
+        n = 1
+        while n <= nmax:
+            assert call(n) == 0
+            times = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                rc = call(n)
+                times.append(time.perf_counter() - t0)
+                assert rc == 0
+            res[str(n)] = {"best_ms": min(times) * 1e3, "mean_ms": sum(times) / reps * 1e3,
+                           "blobs_per_s_best": n / min(times)}
+            n *= 2
+        return res
+
+    return {
+        "convention": "plain C ABI on host buffers (copies included), 1 warm-up + %d reps, best and mean" % reps,
+        "proofs": sweep(lambda n: L.vkzg_compute_proofs_batch(h, out, src, n)),
+        "recovery_first_64_present": sweep(lambda n: L.vkzg_recover_cells_batch(h, rec_out, cells, present, n)),
     }
 
 
@@ -359,9 +397,18 @@ def main():
     fp_ceiling = {"fp_mul_per_s_by_waves_per_simd": fp_rates, "one_wave_cycles_per_fp_mul": one_wave_cycles,
                   "phase_a_fp_mul_per_s": phase_a_mults / (stage_ms["phase_a"] * 1e-3),
                   "phase_a_over_2_wave_ceiling": phase_a_mults / (stage_ms["phase_a"] * 1e-3) / fp_rates["2"]}
+    # the measured issue rate of v_mad_u64_u32 (8 independent chains per lane)
+    # beside the nominal peak the roofline divides by
+    mad_rates = {str(w): prover.debug_mad_rate(w)[0] for w in (1, 2, 4, 8)}
+    clock_mhz = prover.debug_mad_rate(1, 10)[1]
+    measured_peak = max(mad_rates.values())
+    roofline["measured_peak"] = measured_peak / 1e12
+    roofline["frac_of_measured_peak"] = (dom_work / (stage_ms[dom] * 1e-3)) / measured_peak if dom_work else None
+    mad_issue = {"mad_u64_u32_per_s_by_waves_per_simd": mad_rates, "nominal_clock_mhz": clock_mhz,
+                 "measured_over_nominal_peak": measured_peak / PEAK_INT}
     secondary = {"stage_timed_blobs_per_s": stage_timed_value, "tuning": tuning, "fp_mul_ceiling": fp_ceiling,
+                 "mad_issue_rate": mad_issue,
                  "stage_ms": stage_ms, "stage_frac_of_int_peak": frac,
-                 "pipeline_frac_of_int_peak_vs_reference_work": value * W_PROOF / PEAK_INT,
                  "phase_a_window_bits": info["window_bits"],
                  "table_gib": info["table_bytes"] / 2**30, "setup_ms": info["setup_ms"]}
     if not a.no_secondary:
@@ -460,6 +507,9 @@ def main():
         for _ in range(3):
             prover.recover_cells_and_kzg_proofs_batch(cells_out, bytes(keep), nc)
         secondary["recover_cells_and_proofs_host_%d_blobs_per_s" % nc] = 3 * nc / (time.perf_counter() - t0)
+
+    if not a.no_secondary:
+        secondary["host_abi_sweep"] = host_abi_sweep(prover, blobs, nb, a.seed + 5)
 
     if not a.no_secondary and not a.no_verify:
         secondary["verify"] = verify_bench(a.seed + 4)

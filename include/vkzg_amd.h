@@ -234,6 +234,12 @@ vkzg_result vkzg_amd_debug_fr_mul(vkzg_prover *p, uint32_t *out, const uint32_t 
  * cycles per product of a single wave alone (its dependency latency). */
 vkzg_result vkzg_amd_debug_fp_mul_rate(vkzg_prover *p, uint32_t waves_per_simd, uint32_t iters,
                                        double *muls_per_s, double *one_wave_cycles);
+/* Issue-rate measurement of v_mad_u64_u32 (the 32x32->64 multiply-add every
+ * limb product is): every lane of CUs x 4 x waves_per_simd one-wave blocks
+ * (1-8 waves per SIMD) runs iters x 128 of them in 8 independent chains;
+ * *mads_per_s is the chip's lane rate, *clock_mhz the device's nominal clock. */
+vkzg_result vkzg_amd_debug_mad_rate(vkzg_prover *p, uint32_t waves_per_simd, uint32_t iters,
+                                    double *mads_per_s, double *clock_mhz);
 /* Test switch: with enable != 0, every later proof pipeline on p sends all of
  * phase A, phase B and the bucket reduction through their checked recompute
  * (the path the fast point additions take when they meet a doubling or a

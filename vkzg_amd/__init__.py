@@ -221,6 +221,13 @@ class Prover:
     def set_graphs(self, enable=True):
         _check(load().vkzg_amd_set_graphs(self._h, 1 if enable else 0))
 
+    def debug_mad_rate(self, waves_per_simd, iters=2000):
+        """(v_mad_u64_u32 lane-ops per second over the chip, nominal clock in MHz)."""
+        rate, mhz = ctypes.c_double(), ctypes.c_double()
+        _check(load().vkzg_amd_debug_mad_rate(self._h, waves_per_simd, iters, ctypes.byref(rate),
+                                              ctypes.byref(mhz)))
+        return rate.value, mhz.value
+
     def debug_force_checked(self, enable=True):
         """Send phase A, phase B and the bucket reduction of later proof calls
         through their checked recompute (a test switch; proofs are unchanged)."""

@@ -54,6 +54,7 @@ EXT_SYMBOLS = [
     "vkzg_amd_debug_verify_terms",
     "vkzg_amd_debug_fp_mul_rate",
     "vkzg_amd_debug_force_checked",
+    "vkzg_amd_debug_mad_rate",
     "vkzg_amd_write_v1_tables",
     "vkzg_amd_verify_challenge",
     "vkzg_amd_sha256",
@@ -184,6 +185,9 @@ def load():
     L.vkzg_amd_set_tuning.argtypes = [vp, sz, ctypes.POINTER(Tuning)]
     L.vkzg_amd_set_graphs.restype = ctypes.c_int
     L.vkzg_amd_set_graphs.argtypes = [vp, ctypes.c_int]
+    L.vkzg_amd_debug_mad_rate.restype = ctypes.c_int
+    L.vkzg_amd_debug_mad_rate.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32,
+                                          ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
     L.vkzg_amd_debug_force_checked.restype = ctypes.c_int
     L.vkzg_amd_debug_force_checked.argtypes = [vp, ctypes.c_int]
     L.vkzg_amd_autotune.restype = ctypes.c_int

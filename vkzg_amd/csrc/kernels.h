@@ -113,6 +113,8 @@ hipError_t launch_debug_fp_mul(uint32_t *out, const uint32_t *a, const uint32_t 
 hipError_t launch_debug_fp_inv(uint32_t *out, const uint32_t *a, uint32_t n, hipStream_t s);
 // `blocks` one-wave blocks, each lane n dependent Fp products (rate measurement).
 hipError_t launch_debug_fp_mul_chain(Fp *out, uint32_t blocks, uint32_t n, hipStream_t s);
+// `blocks` one-wave blocks, each lane n x 128 v_mad_u64_u32 in 8 independent chains.
+hipError_t launch_debug_mad_chain(uint64_t *out, uint32_t blocks, uint32_t n, hipStream_t s);
 hipError_t launch_debug_fr_mul(uint32_t *out, const uint32_t *a, const uint32_t *b, uint32_t n,
                                hipStream_t s);
 

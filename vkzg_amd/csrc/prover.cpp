@@ -1084,6 +1084,41 @@ vkzg_result vkzg_amd_debug_fp_mul_rate(vkzg_prover *p, uint32_t waves_per_simd, 
     return rc;
 }
 
+vkzg_result vkzg_amd_debug_mad_rate(vkzg_prover *p, uint32_t waves_per_simd, uint32_t iters,
+                                    double *mads_per_s, double *clock_mhz) {
+    if (!p || !mads_per_s || !clock_mhz || waves_per_simd == 0 || waves_per_simd > 8 || iters == 0)
+        return VKZG_ERR_BADARGS;
+    std::lock_guard<std::mutex> lock(p->mu);
+    DeviceGuard guard(p->device);
+    void *buf = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    vkzg_result rc = VKZG_OK;
+    try {
+        ck(hipMalloc(&buf, 1024 * sizeof(uint64_t)));
+        ck(hipEventCreate(&e0));
+        ck(hipEventCreate(&e1));
+        hipStream_t s = p->stream;
+        int clock_khz = 0;
+        ck(hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeClockRate, p->device));
+        const uint32_t blocks = p->cus * 4 * waves_per_simd;  // one wave per block, 4 SIMDs per CU
+        ck(launch_debug_mad_chain(static_cast<uint64_t *>(buf), blocks, iters, s));  // warm-up
+        ck(hipEventRecord(e0, s));
+        ck(launch_debug_mad_chain(static_cast<uint64_t *>(buf), blocks, iters, s));
+        ck(hipEventRecord(e1, s));
+        ck(hipEventSynchronize(e1));
+        float ms = 0;
+        ck(hipEventElapsedTime(&ms, e0, e1));
+        *mads_per_s = (double)blocks * 64.0 * iters * 128.0 / (ms * 1e-3);
+        *clock_mhz = clock_khz / 1e3;
+    } catch (HipFail &) {
+        rc = VKZG_ERR_GPU;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(buf);
+    return rc;
+}
+
 vkzg_result vkzg_amd_debug_fp_inv(vkzg_prover *p, uint32_t *out, const uint32_t *a, size_t n) {
     return debug_op(p, out, a, a, n, 2);
 }

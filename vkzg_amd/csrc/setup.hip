@@ -134,10 +134,43 @@ __global__ void k_debug_fp_mul_chain(Fp *out, uint32_t n) {
     out[gid & 1023u] = a;
 }
 
+// Issue-rate measurement of v_mad_u64_u32 itself (vkzg_amd_debug_mad_rate):
+// each lane runs 8 independent accumulator chains, 16 products per chain per
+// iteration, against a wave-uniform multiplier.
+__global__ void __launch_bounds__(64) k_debug_mad_chain(uint64_t *out, uint32_t n) {
+    const uint32_t gid = blockIdx.x * blockDim.x + threadIdx.x;
+    uint64_t acc[8];
+    uint32_t x[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        acc[k] = gid + k;
+        x[k] = 0x9E3779B9u * (gid + 17u * k) | 1u;
+    }
+    const uint32_t y = 0x85EBCA6Bu + n;
+#pragma unroll 1
+    for (uint32_t i = 0; i < n; i++) {
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+#pragma unroll
+            for (int k = 0; k < 8; k++)
+                asm volatile("v_mad_u64_u32 %0, vcc, %1, %2, %0" : "+v"(acc[k]) : "v"(x[k]), "s"(y) : "vcc");
+        }
+    }
+    uint64_t sum = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) sum ^= acc[k];
+    out[gid & 1023u] = sum;
+}
+
 }  // namespace
 
 hipError_t launch_debug_fp_mul_chain(Fp *out, uint32_t blocks, uint32_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_debug_fp_mul_chain, dim3(blocks), dim3(64), 0, s, out, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_debug_mad_chain(uint64_t *out, uint32_t blocks, uint32_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_debug_mad_chain, dim3(blocks), dim3(64), 0, s, out, n);
     return hipGetLastError();
 }
 
