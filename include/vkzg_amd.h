@@ -146,7 +146,7 @@ typedef struct {
     uint32_t recovery_cols;         /* recovery NTT_8192 sub-transforms per pass block: 2, 4, 8 or 16 */
     uint32_t ladder_splits;         /* doubling-ladder segments, 1, 2 or 4; phase A computes the start
                                        of each (more phase-A work, shorter dependent chains) */
-    uint32_t reduce_lanes;          /* bucket-reduction lanes per output: 8, 16 or 32 */
+    uint32_t reduce_lanes;          /* bucket-reduction lanes per output: 4, 8, 16 or 32 */
 } vkzg_amd_tuning;
 
 /* The tuning used for a batch of `num_blobs` blobs. */

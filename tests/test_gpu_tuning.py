@@ -26,7 +26,7 @@ def restore_tuning(gpu_prover):
 def test_phase_a_groups_bit_exact(gpu_prover, vectors, restore_tuning, groups):
     blobs = b"".join(vectors.blob(n) for n in NAMES)
     want = b"".join(vectors.proofs(n) for n in NAMES)
-    for bpc, chunk, splits, lanes in ((1, 1, 1, 8), (2, 64, 2, 16), (2, 8, 4, 32), (2, 0, 1, 8)):
+    for bpc, chunk, splits, lanes in ((1, 1, 1, 8), (2, 64, 2, 16), (2, 8, 4, 32), (2, 0, 1, 4)):
         gpu_prover.set_tuning(3, phase_a_groups=groups, phase_a_blocks_per_cu=bpc, normalize_chunk=chunk,
                               ladder_splits=splits, reduce_lanes=lanes)
         assert gpu_prover.get_tuning(3)["phase_a_groups"] == groups
@@ -60,7 +60,7 @@ def test_tuning_rejects_out_of_range(gpu_prover, restore_tuning):
     for bad in (dict(phase_a_groups=0), dict(phase_a_blocks_per_cu=3),
                 dict(normalize_chunk=257), dict(splits=5),
                 dict(recovery_cols=3), dict(recovery_cols=32), dict(ladder_splits=3),
-                dict(phase_a_groups=11), dict(reduce_lanes=4), dict(reduce_lanes=64)):
+                dict(phase_a_groups=11), dict(reduce_lanes=2), dict(reduce_lanes=64)):
         with pytest.raises(vkzg_amd.VkzgError) as e:
             gpu_prover.set_tuning(3, **bad)
         assert e.value.code == vkzg_amd.ERR_BADARGS
@@ -74,7 +74,7 @@ def test_autotune_keeps_results(gpu_prover_small_batch, vectors):
     for n in (1, 4):
         t = p.get_tuning(n)
         assert 1 <= t["phase_a_groups"] <= 10 and 1 <= t["splits"] <= 4 and t["ladder_splits"] in (1, 2, 4)
-        assert t["reduce_lanes"] in (8, 16, 32)
+        assert t["reduce_lanes"] in (4, 8, 16, 32)
     names = [NAMES[i % 3] for i in range(6)]  # chunks of 4 + 2
     blobs = b"".join(vectors.blob(n) for n in names)
     assert p.compute_cell_kzg_proofs_batch(blobs, 6) == b"".join(vectors.proofs(n) for n in names)

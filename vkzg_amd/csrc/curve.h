@@ -19,7 +19,7 @@
 // memory, which measured at several times the kernels' useful HBM traffic).
 // The P == Q fallbacks inside additions are rare and stay out of line.
 #define VKZG_DEV_CALL __device__ __forceinline__
-#define VKZG_DEV_COLD __device__ __attribute__((noinline))
+#define VKZG_DEV_COLD __device__ __forceinline__
 
 namespace vkzg_amd {
 
@@ -237,6 +237,25 @@ VKZG_DEV_CALL G1X g1x_dbl(const G1X &p) {
 
 VKZG_DEV_COLD G1X g1x_dbl_cold(const G1X &p) { return g1x_dbl(p); }
 
+// g1x_dbl without the identity test: the identity (ZZ = ZZZ = 0) doubles to
+// ZZ3 = ZZZ3 = 0, the identity again, so no input has to stay live for a
+// select.  Never flags: doubling has no exceptional case.
+VKZG_DEV G1X g1x_dbl_core(const G1X &p) {
+    const Fp U = fp_dbl(p.y);
+    const Fp V = fp_sqr(U);
+    const Fp W = fp_mul(U, V);
+    G1X r;
+    r.zz = fp_mul(V, p.zz);
+    r.zzz = fp_mul(W, p.zzz);
+    const Fp S = fp_mul(p.x, V);
+    const Fp WY = fp_mul(W, p.y);
+    const Fp X2 = fp_sqr(p.x);
+    const Fp M = fp_add(fp_dbl(X2), X2);
+    r.x = fp_sub(fp_sqr(M), fp_dbl(S));
+    r.y = fp_sub(fp_mul(M, fp_sub(S, r.x)), WY);
+    return r;
+}
+
 // madd-2008-s: XYZZ + affine (b may be the identity)
 VKZG_DEV_CALL G1X g1x_madd(const G1X &a, const G1A &b) {
     if (g1a_is_identity(b)) return a;
@@ -311,13 +330,17 @@ VKZG_DEV G1X g1x_madd_fast(const G1X &a, const G1A &b, bool &exc) {
     return g1x_madd_acc(a, b, false, exc);
 }
 
-VKZG_DEV G1X g1x_add_fast(const G1X &a, const G1X &b, bool &exc) {
-    if (fp_is_zero(a.zz)) return b;
-    if (fp_is_zero(b.zz)) return a;
+// add-2008-s without the identity checks: a and b are not the identity.
+// The products are ordered so that every input coordinate dies as early as
+// possible (ZZ1 ZZ2 and ZZZ1 ZZZ2 first), which keeps the sum within a
+// register budget where two other points are live.
+VKZG_DEV G1X g1x_add_core(const G1X &a, const G1X &b, bool &exc) {
     const Fp U1 = fp_mul(a.x, b.zz);
     const Fp U2 = fp_mul(b.x, a.zz);
+    const Fp ZZ12 = fp_mul(a.zz, b.zz);
     const Fp S1 = fp_mul(a.y, b.zzz);
     const Fp S2 = fp_mul(b.y, a.zzz);
+    const Fp ZZZ12 = fp_mul(a.zzz, b.zzz);
     const Fp P = fp_sub_lazy(U2, U1);
     const Fp R = fp_sub_lazy(S2, S1);
     const Fp PP = fp_sqr(P);
@@ -325,10 +348,22 @@ VKZG_DEV G1X g1x_add_fast(const G1X &a, const G1X &b, bool &exc) {
     const Fp PPP = fp_mul(P, PP);
     const Fp Q = fp_mul(U1, PP);
     G1X r;
+    r.zz = fp_mul(ZZ12, PP);
+    r.zzz = fp_mul(ZZZ12, PPP);
     r.x = fp_sub(fp_sub(fp_sqr(R), PPP), fp_dbl(Q));
     r.y = fp_sub(fp_mul(R, fp_sub_lazy(Q, r.x)), fp_mul(S1, PPP));
-    r.zz = fp_mul(fp_mul(a.zz, b.zz), PP);
-    r.zzz = fp_mul(fp_mul(a.zzz, b.zzz), PPP);
+    return r;
+}
+
+// a + b for hot loops (see g1x_madd_acc for the `exc` contract); either may
+// be the identity.  Branches rather than selects around the sum, so neither
+// input has to stay live to the end.
+VKZG_DEV G1X g1x_add_fast(const G1X &a, const G1X &b, bool &exc) {
+    G1X r = a;
+    if (!fp_is_zero(b.zz)) {
+        if (fp_is_zero(a.zz)) r = b;
+        else r = g1x_add_core(a, b, exc);
+    }
     return r;
 }
 

@@ -486,6 +486,14 @@ VKZG_DEV bool fp_is_zero(const Fp &a) {
 VKZG_DEV bool fp_eq(const Fp &a, const Fp &b) {
     return limbs_eq<kFpLimbs>(fp_canonical(a).v, fp_canonical(b).v);
 }
+// c ? a : b limb by limb (v_cndmask), so a lane-varying choice between two
+// register values never goes through an address.
+VKZG_DEV Fp fp_select(bool c, const Fp &a, const Fp &b) {
+    Fp r;
+#pragma unroll
+    for (int i = 0; i < kFpLimbs; i++) r.v[i] = c ? a.v[i] : b.v[i];
+    return r;
+}
 VKZG_DEV Fp fp_cond_neg(const Fp &a, bool neg) {
     const Fp n = fp_neg(a);
     Fp r;

@@ -85,7 +85,7 @@ hipError_t launch_glv_phi(G1A *phi, const G1A *pts, uint64_t count, uint32_t *qu
 hipError_t launch_phase_b(G1X *buckets, uint8_t *flags, const G1A *lad_aff, const G1A *lad_phi,
                           const uint32_t *items, const uint32_t *offsets, uint32_t nblobs, uint32_t grid,
                           uint32_t *queue, uint32_t force_checked, hipStream_t s);
-// `lanes` (8, 16 or 32) lanes per output; persistent (`grid` blocks drain
+// `lanes` (4, 8, 16 or 32) lanes per output; persistent (`grid` blocks drain
 // queue[0]), then the recompute of flagged outputs; flags[B][128] is scratch.
 hipError_t launch_bucket_reduce(G1X *cvals, uint8_t *flags, const G1X *buckets, uint32_t nblobs,
                                 uint32_t lanes, uint32_t grid, uint32_t *queue, uint32_t force_checked,

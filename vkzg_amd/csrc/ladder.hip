@@ -14,9 +14,9 @@ namespace {
 // proofs, canonical, are unchanged).
 VKZG_DEV G1J g1j_dbl_pair(const G1J &p, uint32_t q) {
     if (fp_is_zero(p.z)) return p;  // uniform in the pair
-    const Fp r1 = fp_sqr(q ? p.y : p.x);
+    const Fp r1 = fp_sqr(fp_select(q != 0, p.y, p.x));
     const Fp A = fp_pair_bcast<0>(r1), B = fp_pair_bcast<1>(r1);
-    const Fp r2 = fp_sqr(q ? fp_add_lazy(p.x, B) : B);
+    const Fp r2 = fp_sqr(fp_select(q != 0, fp_add_lazy(p.x, B), B));
     const Fp C = fp_pair_bcast<0>(r2), T = fp_pair_bcast<1>(r2);
     const Fp E = fp_triple_lazy(A);  // feeds products only
     const Fp F = fp_sqr(E);
@@ -24,7 +24,7 @@ VKZG_DEV G1J g1j_dbl_pair(const G1J &p, uint32_t q) {
     G1J r;
     r.x = fp_sub(F, fp_dbl(D));
     const Fp C8 = fp_dbl(fp_dbl(fp_dbl(C)));
-    const Fp r4 = fp_mul(q ? p.y : E, q ? p.z : fp_sub_lazy(D, r.x));
+    const Fp r4 = fp_mul(fp_select(q != 0, p.y, E), fp_select(q != 0, p.z, fp_sub_lazy(D, r.x)));
     r.y = fp_sub(fp_pair_bcast<0>(r4), C8);
     r.z = fp_dbl(fp_pair_bcast<1>(r4));
     return r;

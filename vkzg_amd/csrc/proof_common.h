@@ -21,4 +21,17 @@ VKZG_DEV G1X g1x_shfl_xor(const G1X &p, int m) {
     return o;
 }
 
+// The point lane (this lane + d) holds (lanes past the wave's end read their own).
+VKZG_DEV G1X g1x_shfl_down(const G1X &p, uint32_t d) {
+    G1X o;
+#pragma unroll
+    for (int k = 0; k < kFpLimbs; k++) {
+        o.x.v[k] = __shfl_down(p.x.v[k], d);
+        o.y.v[k] = __shfl_down(p.y.v[k], d);
+        o.zz.v[k] = __shfl_down(p.zz.v[k], d);
+        o.zzz.v[k] = __shfl_down(p.zzz.v[k], d);
+    }
+    return o;
+}
+
 }  // namespace vkzg_amd

@@ -280,7 +280,7 @@ bool tuning_valid(const Tuning &t) {
            t.norm_chunk <= 256 && t.splits >= 1 && t.splits <= (uint32_t)kMaxSplits &&
            (t.rec_cols == 2 || t.rec_cols == 4 || t.rec_cols == 8 || t.rec_cols == 16) &&
            (t.ladder_splits == 1 || t.ladder_splits == 2 || t.ladder_splits == 4) &&
-           (t.reduce_lanes == 8 || t.reduce_lanes == 16 || t.reduce_lanes == 32);
+           (t.reduce_lanes == 4 || t.reduce_lanes == 8 || t.reduce_lanes == 16 || t.reduce_lanes == 32);
 }
 
 void enqueue_recovery_into(vkzg_prover *p, Fr *zd, Fr *izc, Fr *A, Fr *B, uint8_t *d_out,
@@ -521,7 +521,7 @@ void autotune(vkzg_prover *p) {
         for (uint32_t g : {1u, 2u, 4u, 5u, 10u}) { Tuning t = best; t.groups = g; consider(t); }
         if (nb <= 64)
             for (uint32_t m : {2u, 4u}) { Tuning t = best; t.ladder_splits = m; consider(t); }
-        for (uint32_t r : {16u, 32u}) { Tuning t = best; t.reduce_lanes = r; consider(t); }
+        for (uint32_t r : {4u, 16u, 32u}) { Tuning t = best; t.reduce_lanes = r; consider(t); }
         for (uint32_t b : {1u, 2u}) { Tuning t = best; t.blocks_per_cu = b; consider(t); }
         for (uint32_t ch : {0u, 8u, 16u, 32u, 64u}) { Tuning t = best; t.norm_chunk = ch; consider(t); }
         if (nb >= 8) { Tuning t = best; t.splits = 2; consider(t); }

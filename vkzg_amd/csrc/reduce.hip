@@ -14,12 +14,40 @@ __device__ __attribute__((noinline)) void reduce_slow(G1X *out, const G1X *B) {
     *out = sum;
 }
 
+// One XYZZ point per thread parked in LDS, limb-major (word k of thread t at
+// k * 256 + t), so the running sum waits there while the bucket sum is
+// updated: three live points would not fit 256 VGPRs without spilling.
+struct G1XStash {
+    uint32_t *w;
+    VKZG_DEV void put(const G1X &p) const {
+#pragma unroll
+        for (int i = 0; i < kFpLimbs; i++) {
+            w[(0 * kFpLimbs + i) * 256 + threadIdx.x] = p.x.v[i];
+            w[(1 * kFpLimbs + i) * 256 + threadIdx.x] = p.y.v[i];
+            w[(2 * kFpLimbs + i) * 256 + threadIdx.x] = p.zz.v[i];
+            w[(3 * kFpLimbs + i) * 256 + threadIdx.x] = p.zzz.v[i];
+        }
+    }
+    VKZG_DEV G1X get() const {
+        G1X p;
+#pragma unroll
+        for (int i = 0; i < kFpLimbs; i++) {
+            p.x.v[i] = w[(0 * kFpLimbs + i) * 256 + threadIdx.x];
+            p.y.v[i] = w[(1 * kFpLimbs + i) * 256 + threadIdx.x];
+            p.zz.v[i] = w[(2 * kFpLimbs + i) * 256 + threadIdx.x];
+            p.zzz.v[i] = w[(3 * kFpLimbs + i) * 256 + threadIdx.x];
+        }
+        return p;
+    }
+};
+
 // cvals[blob][a] = sum_{k=1..128} k * buckets[blob][a][k-1]
 // (field.glsl.inc:478-517).  A team of LANES lanes shares an output; lane l
 // owns buckets [per l, per l + per) with per = 128 / LANES:
 // rs_l = sum_i (i+1) B[per l + i] by running sums, s_l = sum_i B[per l + i];
 // the output is sum_l (rs_l + per l * s_l).  More lanes shorten each lane's
-// dependent chain and add total work (the per l * s_l ladders).
+// dependent chain and add total work (the scan over the team's lanes).  The
+// running sum rs_l stays in LDS between its updates (G1XStash).
 //
 // Persistent: every wave takes items of 64 / LANES consecutive outputs from
 // the counter q[0] (cleared by k_glv_phi) until the queue is drained and
@@ -31,6 +59,8 @@ __global__ void __launch_bounds__(256, 2)
                     uint32_t force_checked) {
     constexpr uint32_t per = 128 / LANES;
     constexpr uint32_t teams = 64 / LANES;
+    __shared__ uint32_t stash_words[4 * kFpLimbs * 256];
+    const G1XStash stash{stash_words};
     const uint32_t lane = threadIdx.x % 64;
     const uint32_t l = lane % LANES;
     const uint32_t nitems = (nout + teams - 1) / teams;
@@ -41,28 +71,30 @@ __global__ void __launch_bounds__(256, 2)
         item = __shfl(item, 0);
         if (item >= nitems) break;
         const uint32_t o = item * teams + lane / LANES;
-        G1X s = g1x_identity(), rs = g1x_identity();
+        G1X s = g1x_identity();
+        stash.put(g1x_identity());
         bool exc = force_checked != 0;
         if (o < nout) {
             const G1X *B = buckets + (size_t)o * 128 + l * per;
 #pragma unroll 1
             for (int i = per - 1; i >= 0; i--) {
                 s = g1x_add_fast(s, B[i], exc);
-                rs = g1x_add_fast(rs, s, exc);
+                stash.put(g1x_add_fast(stash.get(), s, exc));
             }
         }
-        // t = (per l) * s by double-and-add over the bits of per l (t never
-        // equals +-s when it is added: it is an even multiple of s there).
-        const uint32_t mult = per * l;
-        G1X t = g1x_identity();
-        if (mult) {
+        // sum_l per l s_l = per * sum_{m >= 1} S_m, with the suffix sums
+        // S_m = sum_{l >= m} s_l of the team (a log-step scan over shuffles),
+        // then log2(per) doublings: no lane-dependent double-and-add chain.
+        G1X x = s;
 #pragma unroll 1
-            for (int bit = 31 - __builtin_clz(mult); bit >= 0; bit--) {
-                t = g1x_dbl(t);
-                if ((mult >> bit) & 1u) t = g1x_add_fast(t, s, exc);
-            }
+        for (uint32_t d = 1; d < LANES; d <<= 1) {
+            const G1X y = g1x_shfl_down(x, d);
+            if (l + d < LANES) x = g1x_add_fast(x, y, exc);
         }
-        G1X v = g1x_add_fast(rs, t, exc);
+        if (l == 0) x = g1x_identity();
+#pragma unroll 1
+        for (uint32_t k = 1; k < per; k <<= 1) x = g1x_dbl_core(x);
+        G1X v = g1x_add_fast(stash.get(), x, exc);
 #pragma unroll 1
         for (int m = 1; m < LANES; m <<= 1) v = g1x_add_fast(v, g1x_shfl_xor(v, m), exc);
         const uint64_t flagged = __ballot(exc);
@@ -131,10 +163,14 @@ hipError_t launch_bucket_reduce(G1X *cvals, uint8_t *flags, const G1X *buckets, 
                                 uint32_t lanes, uint32_t grid, uint32_t *queue, uint32_t force_checked,
                                 hipStream_t s) {
     const uint32_t nout = nblobs * 128;
-    const uint32_t teams = 64 / (lanes == 16 || lanes == 32 ? lanes : 8);
+    const uint32_t teams = 64 / (lanes == 4 || lanes == 16 || lanes == 32 ? lanes : 8);
     const uint32_t waves = (nout + teams - 1) / teams;
     if (grid > (waves + 3) / 4) grid = (waves + 3) / 4;
     switch (lanes) {
+        case 4:
+            hipLaunchKernelGGL(k_bucket_reduce<4>, dim3(grid), dim3(256), 0, s, cvals, flags, buckets, nout,
+                               queue, force_checked);
+            break;
         case 16:
             hipLaunchKernelGGL(k_bucket_reduce<16>, dim3(grid), dim3(256), 0, s, cvals, flags, buckets, nout,
                                queue, force_checked);

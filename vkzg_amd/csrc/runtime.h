@@ -31,7 +31,7 @@ struct Tuning {
     uint32_t splits = 1;         // sub-streams, 1..kMaxSplits
     uint32_t rec_cols = 8;       // NTT_8192 sub-transforms per pass block: 2, 4, 8 or 16
     uint32_t ladder_splits = 1;  // ladder segments (phase A also computes their starts): 1, 2 or 4
-    uint32_t reduce_lanes = 8;   // bucket-reduction lanes per output: 8, 16 or 32
+    uint32_t reduce_lanes = 8;   // bucket-reduction lanes per output: 4, 8, 16 or 32
     // Not a tuning: the test switch that sends phase A, phase B and the bucket
     // reduction down their checked recompute (vkzg_amd_debug_force_checked).
     // Set by tuning_for from the prover; kept here so graph keys include it.

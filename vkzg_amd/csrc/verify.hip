@@ -28,19 +28,24 @@ VKZG_DEV bool g1x_eq(const G1X &a, const G1X &b) {
 // [z^2]P == phi(P) + P.  Sound for every point of E(Fp): phi^2 + phi + 1 = 0,
 // so (phi + LAMBDA + 1)(phi - LAMBDA) = -(LAMBDA^2 + LAMBDA + 1) = -r, and
 // (phi - LAMBDA)Q = 0 forces [r]Q = 0.  [z^2]P costs two 64-bit chains.
+// The checked additions as calls: inlined into the two chains below they
+// need more registers than a 64-thread kernel gets without spilling.
+__device__ __attribute__((noinline)) G1X g1x_madd_call(const G1X &a, const G1A &b) { return g1x_madd(a, b); }
+__device__ __attribute__((noinline)) G1X g1x_add_call(const G1X &a, const G1X &b) { return g1x_add(a, b); }
+
 VKZG_DEV bool g1_in_subgroup(const G1A &p) {
     if (g1a_is_identity(p)) return true;
     G1X a = g1x_from_affine(p);
 #pragma unroll 1
     for (int i = 62; i >= 0; i--) {
         a = g1x_dbl(a);
-        if ((kZAbs >> i) & 1u) a = g1x_madd(a, p);
+        if ((kZAbs >> i) & 1u) a = g1x_madd_call(a, p);
     }
     G1X b = a;
 #pragma unroll 1
     for (int i = 62; i >= 0; i--) {
         b = g1x_dbl(b);
-        if ((kZAbs >> i) & 1u) b = g1x_add(b, a);
+        if ((kZAbs >> i) & 1u) b = g1x_add_call(b, a);
     }
     G1A phi = p;
     phi.x = fp_mul(p.x, fp_const(cfp::BETA_M));
