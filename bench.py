@@ -261,7 +261,7 @@ def serve_bench(prover, rng, seconds, rate, probe_rate, slots):
     # size and kernels launched one by one (no per-size tuning, no graphs)
     saved = {n: prover.get_tuning(n) for n in (1, 4, 16, 64, 256)}
     fixed = dict(phase_a_groups=1, phase_a_blocks_per_cu=2, normalize_chunk=32, splits=1,
-                 recovery_cols=8, ladder_splits=1, reduce_lanes=8)
+                 recovery_waves=1, ladder_splits=1, reduce_lanes=8)
     for n in saved:
         prover.set_tuning(n, **fixed)
     prover.set_graphs(False)
@@ -442,10 +442,11 @@ def main():
         rec_stages = prover.last_stage_times()
         secondary["recovery_stage_ms"] = rec_stages
         prover.set_stage_timing(False)
-        # Each NTT pass reads 256 KiB and writes 256 KiB per blob (8192 values
-        # of 32 bytes, or the 2048-byte cells of the first and last pass).
+        # Each of the five recovery kernels reads 256 KiB and writes 256 KiB
+        # per blob (8192 values of 32 bytes, or the 2048-byte cells of the
+        # first and last), plus the 128-byte mask of the first and third.
         pass_bytes = nr * 2 * 8192 * 32
-        gbps = {k: pass_bytes / (v * 1e-3) / 1e9 for k, v in rec_stages.items() if "pass" in k and v > 0}
+        gbps = {k: pass_bytes / (v * 1e-3) / 1e9 for k, v in rec_stages.items() if v > 0}
         secondary["recovery_pass_hbm"] = {
             "bytes_per_pass": pass_bytes, "gb_per_s": gbps,
             "mean_gb_per_s": sum(gbps.values()) / len(gbps) if gbps else None,

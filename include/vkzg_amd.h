@@ -30,7 +30,9 @@ vkzg_result vkzg_amd_compute_proofs_device(vkzg_prover *p, void *d_proofs, const
 
 /* vkzg_recover_cells_batch on device pointers.  `h_cell_present` is the host
  * copy of the presence mask (num_blobs x 128 bytes) used for argument
- * validation; `d_cell_present` is the same mask in device memory. */
+ * validation; `d_cell_present` is the same mask in device memory.  The three
+ * device pointers must be 16-byte aligned (hipMalloc's are); others return
+ * VKZG_ERR_BADARGS. */
 vkzg_result vkzg_amd_recover_cells_device(vkzg_prover *p, void *d_cells_out, const void *d_cells,
                                           const void *d_cell_present,
                                           const uint8_t *h_cell_present, size_t num_blobs,
@@ -39,8 +41,8 @@ vkzg_result vkzg_amd_recover_cells_device(vkzg_prover *p, void *d_cells_out, con
 /* No counterpart in the reference.  For callers that cannot allocate device
  * memory during a call (code under HIP graph capture, tight serving loops,
  * harnesses that forbid hipMalloc): vkzg_amd_recover_cells_device with the
- * same arguments and results, whose scratch (2 x 128 + 2 x 8192 Fr per blob)
- * is a fixed 8.125 MiB file-scope device array instead of the prover's
+ * same arguments and results, whose scratch (2 x 8192 Fr per blob) is a
+ * fixed 8 MiB file-scope device array instead of the prover's
  * working set.  The call allocates nothing (no hipMalloc, hipFree or graph).
  * At most 16 blobs per call: more returns VKZG_ERR_BADARGS.  Every GPU holds
  * one copy of the array until the process ends, shared by all calls on that
@@ -143,10 +145,11 @@ typedef struct {
     uint32_t normalize_chunk;       /* points per batched inversion, 1-256; 0 = the smallest chunk
                                        whose lanes fit in one wave per SIMD (the default) */
     uint32_t splits;                /* sub-streams a batch is spread over, 1-4 */
-    uint32_t recovery_cols;         /* recovery NTT_8192 sub-transforms per pass block: 2, 4, 8 or 16 */
+    uint32_t recovery_waves;        /* recovery work items (one wave each) per block: 1, 2 or 4 */
     uint32_t ladder_splits;         /* doubling-ladder segments, 1, 2 or 4; phase A computes the start
                                        of each (more phase-A work, shorter dependent chains) */
     uint32_t reduce_lanes;          /* bucket-reduction lanes per output: 4, 8, 16 or 32 */
+    uint32_t recovery_radix;        /* recovery register radix: 4 (4 values per lane) or 8 (8) */
 } vkzg_amd_tuning;
 
 /* The tuning used for a batch of `num_blobs` blobs. */

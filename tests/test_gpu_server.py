@@ -138,7 +138,8 @@ def test_serving_mix_at_full_size(gpu_prover, vectors, cells_of):
             nb = local.choice([1, 4, 16, 64, 256])
             gpu_prover.set_tuning(nb, phase_a_groups=local.choice([1, 2, 4]),
                                   reduce_lanes=local.choice([8, 16]),
-                                  recovery_cols=local.choice([2, 4]))
+                                  recovery_waves=local.choice([1, 2, 4]),
+                                  recovery_radix=local.choice([4, 8]))
             stop.wait(0.005)
 
     side = [threading.Thread(target=sync_caller), threading.Thread(target=tuner)]

@@ -41,16 +41,16 @@ def test_splits_and_odd_sizes_bit_exact(gpu_prover, vectors, restore_tuning):
     assert gpu_prover.compute_cell_kzg_proofs_batch(blobs, 37) == want
 
 
-@pytest.mark.parametrize("cols", [2, 4, 8, 16])
-def test_recovery_cols_bit_exact(gpu_prover, vectors, cells_of, restore_tuning, cols):
-    rng = random.Random(cols)
+@pytest.mark.parametrize("radix,waves", [(4, 1), (4, 2), (4, 4), (8, 1), (8, 2), (8, 4)])
+def test_recovery_radix_and_waves_bit_exact(gpu_prover, vectors, cells_of, restore_tuning, radix, waves):
+    rng = random.Random(radix * waves)
     names = [NAMES[i % 3] for i in range(3)]
     cells = b"".join(cells_of(n) for n in names)
     present = bytearray(128 * 3)
     for b in range(3):
         for c in rng.sample(range(128), 64 + 20 * b):
             present[128 * b + c] = 1
-    gpu_prover.set_tuning(3, recovery_cols=cols)
+    gpu_prover.set_tuning(3, recovery_waves=waves, recovery_radix=radix)
     assert gpu_prover.recover_cells_batch(cells, bytes(present), 3) == cells
     out, proofs = gpu_prover.compute_cells_and_kzg_proofs_batch(b"".join(vectors.blob(n) for n in names), 3)
     assert out == cells and proofs == b"".join(vectors.proofs(n) for n in names)
@@ -59,7 +59,7 @@ def test_recovery_cols_bit_exact(gpu_prover, vectors, cells_of, restore_tuning, 
 def test_tuning_rejects_out_of_range(gpu_prover, restore_tuning):
     for bad in (dict(phase_a_groups=0), dict(phase_a_blocks_per_cu=3),
                 dict(normalize_chunk=257), dict(splits=5),
-                dict(recovery_cols=3), dict(recovery_cols=32), dict(ladder_splits=3),
+                dict(recovery_waves=3), dict(recovery_waves=8), dict(recovery_radix=2), dict(ladder_splits=3),
                 dict(phase_a_groups=11), dict(reduce_lanes=2), dict(reduce_lanes=64)):
         with pytest.raises(vkzg_amd.VkzgError) as e:
             gpu_prover.set_tuning(3, **bad)

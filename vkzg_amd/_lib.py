@@ -90,9 +90,10 @@ class Tuning(ctypes.Structure):
         ("phase_a_blocks_per_cu", ctypes.c_uint32),
         ("normalize_chunk", ctypes.c_uint32),
         ("splits", ctypes.c_uint32),
-        ("recovery_cols", ctypes.c_uint32),
+        ("recovery_waves", ctypes.c_uint32),
         ("ladder_splits", ctypes.c_uint32),
         ("reduce_lanes", ctypes.c_uint32),
+        ("recovery_radix", ctypes.c_uint32),
     ]
 
 

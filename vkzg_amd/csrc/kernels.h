@@ -93,19 +93,23 @@ hipError_t launch_bucket_reduce(G1X *cvals, uint8_t *flags, const G1X *buckets, 
 hipError_t launch_finalize(uint8_t *proofs, const G1X *cvals, uint32_t nblobs, hipStream_t s);
 
 // --------------------------------------------------------------- recovery
-hipError_t launch_rec_vanishing(Fr *zd, Fr *izc, const uint8_t *present, const Fr *roots,
-                                uint32_t nblobs, hipStream_t s);
-// in_mode 0: `in` is cell bytes, multiplied by zd on load; 1: `in` is Fr.
-// Four-step NTT_8192 passes; `cols` (2, 4, 8 or 16) sub-transforms per block.
-// Pass B out_mode 0: multiply by mul_table[k], store Fr; 1: multiply by
-// izc[k % 128], store Fr; 2: store canonical big-endian bytes at the cell wire
-// position.
-hipError_t launch_rec_pass_a(Fr *out, const void *in, int in_mode, const Fr *zd, bool inverse,
-                             const Fr *roots, const Fr *roots_inv, uint32_t nblobs, uint32_t cols,
-                             hipStream_t s);
-hipError_t launch_rec_pass_b(void *out, const Fr *in, int out_mode, const Fr *mul_table,
-                             const Fr *izc, bool inverse, const Fr *roots, const Fr *roots_inv,
-                             uint32_t nblobs, uint32_t cols, hipStream_t s);
+// The four transforms of a recovery as five kernels (recover.hip), each
+// over one-wave work items: 16 per blob with 8 values per lane (radix 8) or
+// 32 per blob with 4 (radix 4); `waves` (1, 2 or 4) items per block.  A and
+// B are per-blob Fr[8192] intermediates in the layout of the chosen radix.
+// in_mode 0: `in` is the cells, the first inverse pass with the vanishing
+// factor; in_mode 1: `in` is Fr coefficients in natural order (Montgomery),
+// the first forward pass of their evaluation (followed by launch_rec_last).
+hipError_t launch_rec_first(Fr *out, const void *in, int in_mode, const uint8_t *present, const Fr *roots,
+                            const Fr *roots_inv, uint32_t nblobs, uint32_t radix, uint32_t waves, hipStream_t s);
+// stage 0: * shift (7^k / 8192); stage 1: * 1/Z on the coset (from present);
+// stage 2: * shift (7^-k / 8192).
+hipError_t launch_rec_mid(Fr *out, const Fr *in, int stage, const uint8_t *present, const Fr *roots,
+                          const Fr *roots_inv, const Fr *shift, uint32_t nblobs, uint32_t radix,
+                          uint32_t waves, hipStream_t s);
+// Canonical big-endian cell bytes.
+hipError_t launch_rec_last(uint8_t *cells, const Fr *in, const Fr *roots, uint32_t nblobs, uint32_t radix,
+                           uint32_t waves, hipStream_t s);
 
 // ------------------------------------------------------------ test hooks
 hipError_t launch_debug_fp_mul(uint32_t *out, const uint32_t *a, const uint32_t *b, uint32_t n,

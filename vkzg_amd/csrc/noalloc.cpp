@@ -2,9 +2,9 @@
 // (vkzg_amd_recover_cells_device_noalloc, include/vkzg_amd.h).
 //
 // The recovery kernels need scratch beyond the caller's inputs and outputs:
-// the vanishing values zd and 1/z on the coset (128 Fr each per blob) and the
-// two ping-pong transform buffers (8192 Fr each per blob).  Here that scratch
-// is one file-scope device array, `scratch`, sized for kNoAllocMaxBlobs blobs;
+// the two ping-pong transform buffers (8192 Fr each per blob).  Here that
+// scratch is one file-scope device array, `scratch`, sized for
+// kNoAllocMaxBlobs blobs;
 // every GPU the library runs on holds its own copy until the process ends.
 // Host code never uses the array's name as an address: the device address of
 // each GPU's copy is resolved once per device by hipGetSymbolAddress (at
@@ -24,9 +24,8 @@ using namespace vkzg_amd;
 namespace {
 
 constexpr uint32_t kNoAllocMaxBlobs = 16;
-constexpr size_t kZBytes = (size_t)kNoAllocMaxBlobs * 128 * sizeof(Fr);     // zd, izc: 64 KiB each
 constexpr size_t kBufBytes = (size_t)kNoAllocMaxBlobs * 8192 * sizeof(Fr);  // A, B: 4 MiB each
-constexpr size_t kScratchBytes = 2 * kZBytes + 2 * kBufBytes;               // 8.125 MiB
+constexpr size_t kScratchBytes = 2 * kBufBytes;                             // 8 MiB
 constexpr int kMaxDevices = 64;
 
 }  // namespace
@@ -67,22 +66,21 @@ extern "C" vkzg_result vkzg_amd_recover_cells_device_noalloc(vkzg_prover *p, voi
     if (num_blobs == 0) return VKZG_OK;
     if (num_blobs > kNoAllocMaxBlobs) return VKZG_ERR_BADARGS;
     if (!chunk_missing_ok(h_present, (uint32_t)num_blobs)) return VKZG_ERR_BADARGS;
+    if (!aligned16(d_cells_out) || !aligned16(d_cells) || !aligned16(d_present)) return VKZG_ERR_BADARGS;
     std::lock_guard<std::mutex> lock(p->mu);
     DeviceGuard guard(p->device);
     reset_stage_times(p);
     hipStream_t s = hip_stream ? (hipStream_t)hip_stream : p->stream;
     try {
         uint8_t *base = static_cast<uint8_t *>(scratch_ptr(p->device));
-        Fr *zd = reinterpret_cast<Fr *>(base);
-        Fr *izc = reinterpret_cast<Fr *>(base + kZBytes);
-        Fr *A = reinterpret_cast<Fr *>(base + 2 * kZBytes);
-        Fr *B = reinterpret_cast<Fr *>(base + 2 * kZBytes + kBufBytes);
+        Fr *A = reinterpret_cast<Fr *>(base);
+        Fr *B = reinterpret_cast<Fr *>(base + kBufBytes);
         const uint32_t nb = (uint32_t)num_blobs;
         StageClock clk(p, s, true);
         const Tuning tn = tuning_for(p, nb);
-        enqueue_recovery_into(p, zd, izc, A, B, static_cast<uint8_t *>(d_cells_out),
+        enqueue_recovery_into(p, A, B, static_cast<uint8_t *>(d_cells_out),
                               static_cast<const uint8_t *>(d_cells), static_cast<const uint8_t *>(d_present),
-                              nb, s, clk, tn.rec_cols);
+                              nb, s, clk, tn.rec_radix, tn.rec_waves);
         ck(hipStreamSynchronize(s));
         clk.finish();
     } catch (HipFail &) {

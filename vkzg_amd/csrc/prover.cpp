@@ -197,7 +197,7 @@ uint64_t WorkSet::bytes_per_blob() {
            kMaxLadSplits * 128 * kMaxGroups * sizeof(G1X) +
            2 * npos * sizeof(G1J) + 2 * npos * sizeof(G1A) + npos * sizeof(Fp) +
            128 * 128 * (sizeof(G1X) + 1) + 128 * sizeof(G1X) + 2 * kCellsBytes + 128 +
-           2 * 128 * sizeof(Fr) + 2 * 8192 * sizeof(Fr);
+           2 * 8192 * sizeof(Fr);
 }
 
 void WorkSet::allocate(uint32_t B) {
@@ -225,8 +225,6 @@ void WorkSet::allocate(uint32_t B) {
     cells = arena.dalloc<uint8_t>((size_t)B * kCellsBytes);
     cells_out = arena.dalloc<uint8_t>((size_t)B * kCellsBytes);
     present = arena.dalloc<uint8_t>((size_t)B * 128);
-    zd = arena.dalloc<Fr>((size_t)B * 128);
-    izc = arena.dalloc<Fr>((size_t)B * 128);
     bufA = arena.dalloc<Fr>((size_t)B * 8192);
     bufB = arena.dalloc<Fr>((size_t)B * 8192);
 }
@@ -278,37 +276,30 @@ uint32_t fit_norm_chunk(uint64_t count, uint32_t cus) {
 bool tuning_valid(const Tuning &t) {
     return t.groups >= 1 && t.groups <= kMaxGroups && t.blocks_per_cu >= 1 && t.blocks_per_cu <= 2 &&
            t.norm_chunk <= 256 && t.splits >= 1 && t.splits <= (uint32_t)kMaxSplits &&
-           (t.rec_cols == 2 || t.rec_cols == 4 || t.rec_cols == 8 || t.rec_cols == 16) &&
+           (t.rec_waves == 1 || t.rec_waves == 2 || t.rec_waves == 4) && (t.rec_radix == 4 || t.rec_radix == 8) &&
            (t.ladder_splits == 1 || t.ladder_splits == 2 || t.ladder_splits == 4) &&
            (t.reduce_lanes == 4 || t.reduce_lanes == 8 || t.reduce_lanes == 16 || t.reduce_lanes == 32);
 }
 
-void enqueue_recovery_into(vkzg_prover *p, Fr *zd, Fr *izc, Fr *A, Fr *B, uint8_t *d_out,
-                           const uint8_t *d_cells, const uint8_t *d_present, uint32_t nb, hipStream_t s,
-                           StageClock &clk, uint32_t cols) {
+void enqueue_recovery_into(vkzg_prover *p, Fr *A, Fr *B, uint8_t *d_out, const uint8_t *d_cells,
+                           const uint8_t *d_present, uint32_t nb, hipStream_t s, StageClock &clk,
+                           uint32_t radix, uint32_t waves) {
     const Fr *r = p->roots, *ri = p->roots_inv;
-    ck(launch_rec_vanishing(zd, izc, d_present, r, nb, s));
-    clk.mark("vanishing");
-    // (E * Z) -> coefficients, times 7^k / 8192
-    ck(launch_rec_pass_a(A, d_cells, 0, zd, true, r, ri, nb, cols, s));
-    clk.mark("inv_pass_a");
-    ck(launch_rec_pass_b(B, A, 0, p->shift_fwd, nullptr, true, r, ri, nb, cols, s));
-    clk.mark("inv_pass_b");
-    // coset evaluations, divided by Z on the coset
-    ck(launch_rec_pass_a(A, B, 1, nullptr, false, r, ri, nb, cols, s));
-    clk.mark("coset_pass_a");
-    ck(launch_rec_pass_b(B, A, 1, nullptr, izc, false, r, ri, nb, cols, s));
-    clk.mark("coset_pass_b");
-    // coset interpolation: coefficients of P, times 7^-k / 8192
-    ck(launch_rec_pass_a(A, B, 1, nullptr, true, r, ri, nb, cols, s));
-    clk.mark("coset_inv_pass_a");
-    ck(launch_rec_pass_b(B, A, 0, p->shift_inv, nullptr, true, r, ri, nb, cols, s));
-    clk.mark("coset_inv_pass_b");
-    // evaluate P on the whole domain, write cells
-    ck(launch_rec_pass_a(A, B, 1, nullptr, false, r, ri, nb, cols, s));
-    clk.mark("eval_pass_a");
-    ck(launch_rec_pass_b(d_out, A, 2, nullptr, nullptr, false, r, ri, nb, cols, s));
-    clk.mark("eval_pass_b");
+    // T1 pass A: (E * Z) -> inverse
+    ck(launch_rec_first(A, d_cells, 0, d_present, r, ri, nb, radix, waves, s));
+    clk.mark("t1_a");
+    // T1 pass B, * 7^k / 8192 (coefficients on the coset), T2 pass A
+    ck(launch_rec_mid(B, A, 0, d_present, r, ri, p->shift_fwd, nb, radix, waves, s));
+    clk.mark("t1_b_t2_a");
+    // T2 pass B (coset evaluations), / Z on the coset, T3 pass A
+    ck(launch_rec_mid(A, B, 1, d_present, r, ri, nullptr, nb, radix, waves, s));
+    clk.mark("t2_b_t3_a");
+    // T3 pass B, * 7^-k / 8192 (coefficients of P), T4 pass A
+    ck(launch_rec_mid(B, A, 2, d_present, r, ri, p->shift_inv, nb, radix, waves, s));
+    clk.mark("t3_b_t4_a");
+    // T4 pass B: P on the whole domain, as cells
+    ck(launch_rec_last(d_out, B, r, nb, radix, waves, s));
+    clk.mark("t4_b");
 }
 
 namespace {
@@ -352,10 +343,9 @@ void proofs_range(vkzg_prover *p, WorkSet &ws, uint32_t off, uint32_t nb, uint8_
 
 void recovery_range(vkzg_prover *p, WorkSet &ws, uint32_t off, uint32_t nb, uint8_t *d_out,
                     const uint8_t *d_cells, const uint8_t *d_present, hipStream_t s,
-                    StageClock &clk, uint32_t cols) {
-    enqueue_recovery_into(p, ws.zd + (size_t)off * 128, ws.izc + (size_t)off * 128,
-                          ws.bufA + (size_t)off * 8192, ws.bufB + (size_t)off * 8192, d_out, d_cells,
-                          d_present, nb, s, clk, cols);
+                    StageClock &clk, const Tuning &tn) {
+    enqueue_recovery_into(p, ws.bufA + (size_t)off * 8192, ws.bufB + (size_t)off * 8192, d_out, d_cells,
+                          d_present, nb, s, clk, tn.rec_radix, tn.rec_waves);
 }
 
 // Runs body(k, off, n, stream) for each of `splits` sub-batches, forked from
@@ -389,9 +379,9 @@ void split_over_streams(StreamSet &ss, uint32_t nb, uint32_t splits, hipStream_t
 namespace {
 uint64_t tuning_word(const Tuning &tn) {
     return (uint64_t)tn.splits | ((uint64_t)tn.groups << 8) | ((uint64_t)tn.blocks_per_cu << 16) |
-           ((uint64_t)tn.norm_chunk << 24) | ((uint64_t)tn.rec_cols << 40) |
+           ((uint64_t)tn.norm_chunk << 24) | ((uint64_t)tn.rec_waves << 40) |
            ((uint64_t)tn.ladder_splits << 48) | ((uint64_t)tn.reduce_lanes << 52) |
-           ((uint64_t)(tn.force_checked != 0) << 62);
+           ((uint64_t)(tn.rec_radix == 8) << 61) | ((uint64_t)(tn.force_checked != 0) << 62);
 }
 }  // namespace
 
@@ -419,7 +409,7 @@ void enqueue_recovery(vkzg_prover *p, WorkSet &ws, StreamSet &ss, uint8_t *d_out
         split_over_streams(ss, nb, tn.splits, s0, clk, [&](uint32_t k, uint32_t off, uint32_t n, hipStream_t q) {
             recovery_range(p, ws, off, n, d_out + (size_t)off * kCellsBytes,
                            d_cells + (size_t)off * kCellsBytes, d_present + (size_t)off * 128, q,
-                           k == 0 ? clk : off_clock, tn.rec_cols);
+                           k == 0 ? clk : off_clock, tn);
         });
     };
     if (!graphs) return body(s);
@@ -441,10 +431,8 @@ void enqueue_cells_and_proofs(vkzg_prover *p, WorkSet &ws, uint8_t *d_cells, uin
         proofs_range(p, ws, 0, nb, d_proofs, d_blobs, q, off_clock, tn, blob_stride,
                      d_cells ? ws.bufA : nullptr);
         if (d_cells) {
-            ck(launch_rec_pass_a(ws.bufB, ws.bufA, 1, nullptr, false, p->roots, p->roots_inv, nb,
-                                 tn.rec_cols, q));
-            ck(launch_rec_pass_b(d_cells, ws.bufB, 2, nullptr, nullptr, false, p->roots, p->roots_inv,
-                                 nb, tn.rec_cols, q));
+            ck(launch_rec_first(ws.bufB, ws.bufA, 1, nullptr, p->roots, p->roots_inv, nb, tn.rec_radix, tn.rec_waves, q));
+            ck(launch_rec_last(d_cells, ws.bufB, p->roots, nb, tn.rec_radix, tn.rec_waves, q));
         }
     };
     if (!graphs) return body(s);
@@ -559,22 +547,28 @@ void autotune(vkzg_prover *p) {
         const uint32_t nb = std::min<uint32_t>(kTuneClassTop[c], nmax);
         if (c > 0 && kTuneClassTop[c - 1] >= nmax) {
             std::lock_guard<std::mutex> lock(p->tune_mu);
-            p->tune[c].rec_cols = p->tune[c - 1].rec_cols;
+            p->tune[c].rec_waves = p->tune[c - 1].rec_waves;
+            p->tune[c].rec_radix = p->tune[c - 1].rec_radix;
             continue;
         }
         Tuning t = p->tune[c];
         float best_ms = 1e30f;
-        uint32_t best_cols = t.rec_cols;
-        for (uint32_t cols : {2u, 4u, 8u, 16u}) {
-            t.rec_cols = cols;
-            const float ms = time_rec(nb, t);
-            if (ms < best_ms * 0.98f) {
-                best_ms = ms;
-                best_cols = cols;
+        uint32_t best_waves = t.rec_waves, best_radix = t.rec_radix;
+        for (uint32_t radix : {4u, 8u}) {
+            for (uint32_t waves : {1u, 2u, 4u}) {
+                t.rec_radix = radix;
+                t.rec_waves = waves;
+                const float ms = time_rec(nb, t);
+                if (ms < best_ms * 0.98f) {
+                    best_ms = ms;
+                    best_waves = waves;
+                    best_radix = radix;
+                }
             }
         }
         std::lock_guard<std::mutex> lock(p->tune_mu);
-        p->tune[c].rec_cols = best_cols;
+        p->tune[c].rec_waves = best_waves;
+        p->tune[c].rec_radix = best_radix;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -878,6 +872,7 @@ vkzg_result vkzg_amd_recover_cells_device(vkzg_prover *p, void *d_cells_out, con
                                           size_t num_blobs, void *hip_stream) {
     if (!p || !d_cells_out || !d_cells || !d_present || !h_present) return VKZG_ERR_BADARGS;
     if (num_blobs == 0) return VKZG_OK;
+    if (!aligned16(d_cells_out) || !aligned16(d_cells) || !aligned16(d_present)) return VKZG_ERR_BADARGS;
     std::lock_guard<std::mutex> lock(p->mu);
     DeviceGuard guard(p->device);
     reset_stage_times(p);
@@ -913,7 +908,8 @@ vkzg_result vkzg_amd_get_tuning(const vkzg_prover *p, size_t num_blobs, vkzg_amd
     out->phase_a_blocks_per_cu = t.blocks_per_cu;
     out->normalize_chunk = t.norm_chunk;
     out->splits = t.splits;
-    out->recovery_cols = t.rec_cols;
+    out->recovery_waves = t.rec_waves;
+    out->recovery_radix = t.rec_radix;
     out->ladder_splits = t.ladder_splits;
     out->reduce_lanes = t.reduce_lanes;
     return VKZG_OK;
@@ -926,7 +922,8 @@ vkzg_result vkzg_amd_set_tuning(vkzg_prover *p, size_t num_blobs, const vkzg_amd
     t.blocks_per_cu = in->phase_a_blocks_per_cu;
     t.norm_chunk = in->normalize_chunk;
     t.splits = in->splits;
-    t.rec_cols = in->recovery_cols;
+    t.rec_waves = in->recovery_waves;
+    t.rec_radix = in->recovery_radix;
     t.ladder_splits = in->ladder_splits;
     t.reduce_lanes = in->reduce_lanes;
     if (!tuning_valid(t)) return VKZG_ERR_BADARGS;

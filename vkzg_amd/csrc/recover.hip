@@ -5,15 +5,38 @@
 // of Z's 8192-point evaluations take only 128 distinct values:
 //   Z(omega^k)     = z(omega_128^(k mod 128))            (zero on missing cells)
 //   Z(7 omega^k)   = z(7^64 omega_128^(k mod 128))
-// k_rec_vanishing computes those 2 x 128 values (and the 128 inverses) from
-// the product form of z, which removes two of the six transforms and the
-// per-element Fermat inversion of k_ew_div.comp.  The remaining four
-// transforms run as two four-step passes each (8192 = 64 x 128) with every
-// pointwise product fused into a pass:
-//   pass A/B inverse   : load cells * Z(omega^k)          -> * 7^k / 8192
-//   pass A/B forward   :                                  -> * 1 / Z(7 omega^k)
-//   pass A/B inverse   :                                  -> * 7^-k / 8192
-//   pass A/B forward   :                                  -> canonical cell bytes
+// which removes two of the six transforms and the per-element Fermat
+// inversion of k_ew_div.comp.  The remaining four transforms T1 (inverse),
+// T2 (forward), T3 (inverse) and T4 (forward) are four-step NTT_8192s,
+// 8192 = 64 x 128, over x[n1 + 64 n2]:
+//   pass A  y[n1][k2] = omega^(s n1 k2) sum_n2 x[n1 + 64 n2] omega_128^(s n2 k2)
+//   pass B  X[k2 + 128 k1] = sum_n1 y[n1][k2] omega_64^(s n1 k1)
+// Pass B of one transform and pass A of the next meet in registers: the
+// pass-A column n1 = c of the next transform holds X[c + 64 n2], i.e. the
+// pass-B columns k2 = c (n2 even) and k2 = c + 64 (n2 odd).  So one wave that
+// owns the column pair (c, c + 64) runs both, and the four transforms take
+// five kernels instead of eight passes:
+//   k_rec_first          T1 pass A: cells * Z(omega^k), reduced mod r
+//   k_rec_mid<0>         T1 pass B, * 7^k / 8192, T2 pass A
+//   k_rec_mid<1>         T2 pass B, * 1 / Z(7 omega^k), T3 pass A
+//   k_rec_mid<2>         T3 pass B, * 7^-k / 8192, T4 pass A
+//   k_rec_last           T4 pass B -> canonical cell bytes
+// The vanishing values each wave needs (8 of Z on the domain in the first
+// kernel, 8 of 1/Z on the coset in k_rec_mid<1>) are computed by that wave.
+//
+// Work items are waves: item (blob, q) owns the pass-A columns n1 in
+// [4q, 4q + 4) and the pass-B columns k2 in [4q, 4q + 4) and
+// [64 + 4q, 64 + 4q + 4); 16 items per blob.  Every lane holds 8 values and
+// runs radix-8 (and radix-4) butterflies in registers; values change lanes
+// through a wave-private 16 KiB LDS region, with no barrier beyond the block
+// (of 1, 2 or 4 such waves).
+//
+// Between kernels y[n1][k2] is stored per blob as S[q][n1][h][j] with
+// k2 = 4q + j + 64h (q < 16, n1 < 64, h < 2, j < 4), so the 512 values a wave
+// reads next are one contiguous 16 KiB block.  Cell wire position w holds the
+// evaluation at natural index brev13(w); with i = n1 + 64 n2 that is
+// w = 128 brev6(n1) + brev7(n2), so a pass-A column is two whole cells, and
+// a pass-B column k2 is the one cell brev7(k2).
 #include "kernels.h"
 #include "ntt.h"
 
@@ -21,200 +44,646 @@ namespace vkzg_amd {
 
 namespace {
 
-// Per blob: zd[k] = z(omega_128^k), izc[k] = 1 / z(7^64 omega_128^k),
-// z(y) = prod over missing wire cells c of (y - omega_128^brev7(c))
-// (k_vanishing_poly.comp, src/vkzg.cpp:1043-1050).
-// On the coset every y satisfies y^128 = 7^8192, so the product over all 128
-// cells is y^128 - 1 = 7^8192 - 1, a constant K, and
-//   1 / z(y) = K^-1 * prod over present cells c of (y - omega_128^brev7(c)),
-// which needs no inversion.  Block (blob, side, half) computes 64 of one
-// side's 128 values; each value is split over 8 threads (16 cells each, so a
-// chain is at most 16 products) whose partial products meet in an LDS tree.
-// A wave holds one side and one cell range, so its skips are uniform, and the
-// 2 x 2 blocks per blob spread the chains over the CUs (the work is issue-bound).
-__global__ void __launch_bounds__(512)
-    k_rec_vanishing(Fr *zd, Fr *izc, const uint8_t *present, const Fr *roots) {
-    __builtin_amdgcn_s_setprio(3);  // short dependent chains: issue first
-    __shared__ Fr part[8][64];
-    const uint32_t blob = blockIdx.x;
-    const bool coset = blockIdx.y != 0;
-    const uint32_t l = threadIdx.x & 63u;
-    const uint32_t k = blockIdx.z * 64u + l;
-    const uint32_t q = threadIdx.x >> 6;  // cells [16 q, 16 q + 16)
-    Fr x = roots[(64u * k) & 8191u];
-    if (coset) x = fr_mul(x, fr_const(cfr::SEVEN_POW64_M));
-    const uint8_t *pm = present + 128ull * blob;
-    // domain side: factors of missing cells; coset side: factors of present cells
-    const uint8_t want = coset ? 1 : 0;
+constexpr int kItemsPerBlob = 16;
+
+// (u, t) <- (u + t w, u - t w), and the same with w = 1.
+VKZG_DEV void bfly(Fr &u, Fr &t, const Fr &w) {
+    const Fr tw = fr_mul(t, w);
+    t = fr_sub(u, tw);
+    u = fr_add(u, tw);
+}
+VKZG_DEV void bfly1(Fr &u, Fr &t) {
+    const Fr s = fr_add(u, t);
+    t = fr_sub(u, t);
+    u = s;
+}
+
+// out[e] = sum_b v[b] w8^(b e) in place (natural order in and out): radix-2
+// decimation in time over the bit-reversed inputs; w1, w2, w3 are w8^1..3 of
+// the transform's direction.  5 products.
+VKZG_DEV void dft8(Fr (&v)[8], const Fr &w1, const Fr &w2, const Fr &w3) {
+    Fr x0 = v[0], x1 = v[4], x2 = v[2], x3 = v[6], x4 = v[1], x5 = v[5], x6 = v[3], x7 = v[7];
+    bfly1(x0, x1);
+    bfly1(x2, x3);
+    bfly1(x4, x5);
+    bfly1(x6, x7);
+    bfly1(x0, x2);
+    bfly(x1, x3, w2);
+    bfly1(x4, x6);
+    bfly(x5, x7, w2);
+    bfly1(x0, x4);
+    bfly(x1, x5, w1);
+    bfly(x2, x6, w2);
+    bfly(x3, x7, w3);
+    v[0] = x0, v[1] = x1, v[2] = x2, v[3] = x3, v[4] = x4, v[5] = x5, v[6] = x6, v[7] = x7;
+}
+
+// out[g] = sum_a v[a] w4^(a g), w4 of the transform's direction.  1 product.
+VKZG_DEV void dft4(Fr &v0, Fr &v1, Fr &v2, Fr &v3, const Fr &w4) {
+    Fr x0 = v0, x1 = v2, x2 = v1, x3 = v3;
+    bfly1(x0, x1);
+    bfly1(x2, x3);
+    bfly1(x0, x2);
+    bfly(x1, x3, w4);
+    v0 = x0, v1 = x1, v2 = x2, v3 = x3;
+}
+
+// A wave's 512-value LDS exchange region, as pairs of 16-byte halves; the
+// position of logical index idx swaps bits 3-5 with bits 6-8 xor'ed in, which
+// spreads the strided patterns below over the banks.
+struct Exch {
+    uint4 *base;
+    VKZG_DEV static uint32_t pos(uint32_t idx) { return idx ^ (((idx >> 6) & 7u) << 3); }
+    VKZG_DEV void put(uint32_t idx, const Fr &v) const {
+        uint4 *p = base + 2 * pos(idx);
+        p[0] = make_uint4(v.v[0], v.v[1], v.v[2], v.v[3]);
+        p[1] = make_uint4(v.v[4], v.v[5], v.v[6], v.v[7]);
+    }
+    VKZG_DEV Fr get(uint32_t idx) const {
+        const uint4 *p = base + 2 * pos(idx);
+        const uint4 a = p[0], b = p[1];
+        Fr r;
+        r.v[0] = a.x, r.v[1] = a.y, r.v[2] = a.z, r.v[3] = a.w;
+        r.v[4] = b.x, r.v[5] = b.y, r.v[6] = b.z, r.v[7] = b.w;
+        return r;
+    }
+};
+
+// Writes and reads of an exchange region by the lanes of one wave: the
+// region is the wave's own, so a block-wide barrier (blocks of 1-4 waves)
+// only has to order each wave's LDS accesses.
+VKZG_DEV void exch_sync() { __syncthreads(); }
+
+VKZG_DEV Fr ld_fr(const Fr *p) {
+    const uint4 *q = reinterpret_cast<const uint4 *>(p);
+    const uint4 a = q[0], b = q[1];
+    Fr r;
+    r.v[0] = a.x, r.v[1] = a.y, r.v[2] = a.z, r.v[3] = a.w;
+    r.v[4] = b.x, r.v[5] = b.y, r.v[6] = b.z, r.v[7] = b.w;
+    return r;
+}
+VKZG_DEV void st_fr(Fr *p, const Fr &v) {
+    uint4 *q = reinterpret_cast<uint4 *>(p);
+    q[0] = make_uint4(v.v[0], v.v[1], v.v[2], v.v[3]);
+    q[1] = make_uint4(v.v[4], v.v[5], v.v[6], v.v[7]);
+}
+
+VKZG_DEV Fr fr_shfl(const Fr &v, int src) {
+    Fr r;
+#pragma unroll
+    for (int i = 0; i < 8; i++) r.v[i] = (uint32_t)__shfl((int)v.v[i], src);
+    return r;
+}
+VKZG_DEV Fr fr_shfl_xor(const Fr &v, int m) {
+    Fr r;
+#pragma unroll
+    for (int i = 0; i < 8; i++) r.v[i] = (uint32_t)__shfl_xor((int)v.v[i], m);
+    return r;
+}
+// The transform's direction: w = omega_8192^(+-1) and its table of powers.
+struct Dir {
+    const Fr *tab;  // tab[k] = w^k
+    Fr w8_1, w8_2, w8_3;
+    VKZG_DEV explicit Dir(const Fr *t) : tab(t) {
+        w8_1 = tab[1024];
+        w8_2 = tab[2048];
+        w8_3 = tab[3072];
+    }
+    VKZG_DEV Fr pow(uint32_t k) const { return tab[k & 8191u]; }
+};
+
+// Pass B on the wave's 8 columns cc (k2 = 4q + (cc & 3) + 64 (cc >> 2)).
+// In: lane L = 8a + cc holds v[b] = y[a + 8b][cc].  Out: lane L = 8e + cc
+// holds v[f] = X[e + 8f] of its column (the pass-B output k1 = e + 8f).
+VKZG_DEV void pass_b(Fr (&v)[8], const Dir &d, const Exch &x, uint32_t lane) {
+    const uint32_t cc = lane & 7u, a = lane >> 3;
+    dft8(v, d.w8_1, d.w8_2, d.w8_3);
+#pragma unroll
+    for (int e = 1; e < 8; e++) v[e] = fr_mul(v[e], d.pow(128u * a * (uint32_t)e));
+    exch_sync();
+#pragma unroll
+    for (int e = 0; e < 8; e++) x.put(cc * 64u + (uint32_t)e * 8u + a, v[e]);
+    exch_sync();
+    const uint32_t e = lane >> 3;
+#pragma unroll
+    for (int aa = 0; aa < 8; aa++) v[aa] = x.get(cc * 64u + e * 8u + (uint32_t)aa);
+    dft8(v, d.w8_1, d.w8_2, d.w8_3);
+}
+
+// Pass A on the wave's 4 columns c (n1 = 4q + c), with its four-step
+// twiddle.  In: lane L = 16c + a holds v[b] = x[a + 16b] of column c.  Out:
+// v[r] = y[n1][k2] with k2 = pass_a_k2(lane, r).
+VKZG_DEV uint32_t pass_a_k2(uint32_t lane, int r) {
+    const uint32_t m = lane & 15u;
+    return (m >> 1) + 8u * (2u * (m & 1u) + (uint32_t)(r >> 2)) + 32u * (uint32_t)(r & 3);
+}
+VKZG_DEV void pass_a(Fr (&v)[8], const Dir &d, const Exch &x, uint32_t lane, uint32_t n1) {
+    const uint32_t c = lane >> 4, a = lane & 15u;
+    // DFT_8 over b, twiddle w128^(a e)
+    dft8(v, d.w8_1, d.w8_2, d.w8_3);
+#pragma unroll
+    for (int e = 1; e < 8; e++) v[e] = fr_mul(v[e], d.pow(64u * a * (uint32_t)e));
+    exch_sync();
+#pragma unroll
+    for (int e = 0; e < 8; e++) x.put(c * 128u + a * 8u + (uint32_t)e, v[e]);
+    exch_sync();
+    // DFT_16 over a = a0 + 4 a1 for e = m >> 1: first DFT_4 over a1 for this
+    // lane's two a0 = 2 (m & 1) + j, twiddle w16^(a0 g)
+    const uint32_t m = lane & 15u, e = m >> 1, odd = m & 1u;
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        const uint32_t a0 = 2u * odd + (uint32_t)j;
+        Fr g0 = x.get(c * 128u + (a0 + 0u) * 8u + e);
+        Fr g1 = x.get(c * 128u + (a0 + 4u) * 8u + e);
+        Fr g2 = x.get(c * 128u + (a0 + 8u) * 8u + e);
+        Fr g3 = x.get(c * 128u + (a0 + 12u) * 8u + e);
+        dft4(g0, g1, g2, g3, d.w8_2);
+        v[j * 4 + 0] = g0;
+        v[j * 4 + 1] = fr_mul(g1, d.pow(512u * a0));
+        v[j * 4 + 2] = fr_mul(g2, d.pow(1024u * a0));
+        v[j * 4 + 3] = fr_mul(g3, d.pow(1536u * a0));
+    }
+    // v[4 j + g] = W[e][a0 = 2 odd + j][g]; the DFT_4 over a0 needs, for this
+    // lane's g = 2 odd + t, the W of all four a0: a second exchange (half of
+    // it with the pair lane m ^ 1).
+    exch_sync();
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+        for (int g = 0; g < 4; g++) x.put(c * 128u + e * 16u + (2u * odd + (uint32_t)j) * 4u + (uint32_t)g, v[j * 4 + g]);
+    exch_sync();
+    Fr out[8];
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+        const uint32_t g = 2u * odd + (uint32_t)t;
+        Fr c0 = x.get(c * 128u + e * 16u + 0u * 4u + g);
+        Fr c1 = x.get(c * 128u + e * 16u + 1u * 4u + g);
+        Fr c2 = x.get(c * 128u + e * 16u + 2u * 4u + g);
+        Fr c3 = x.get(c * 128u + e * 16u + 3u * 4u + g);
+        dft4(c0, c1, c2, c3, d.w8_2);
+        out[t * 4 + 0] = c0;
+        out[t * 4 + 1] = c1;
+        out[t * 4 + 2] = c2;
+        out[t * 4 + 3] = c3;
+    }
+#pragma unroll
+    for (int r = 0; r < 8; r++) v[r] = fr_mul(out[r], d.pow(n1 * pass_a_k2(lane, r)));
+}
+
+// Index of y[n1][k2] in a blob's stored intermediate S[q][n1][h][j].
+VKZG_DEV uint32_t s_index(uint32_t n1, uint32_t k2) {
+    return ((k2 & 63u) >> 2) * 512u + n1 * 8u + (k2 & 3u) + 4u * (k2 >> 6);
+}
+
+// Product over the cells c with (present[c] != 0) == want of (x - w128^brev7(c)),
+// for 64 / PARTS values per wave: lane L computes part L % PARTS (cells
+// [128 / PARTS * part, +128 / PARTS)) of value L / PARTS; the PARTS lanes of
+// a value combine by xor shuffles, so each of them ends with the whole
+// product.
+template <int PARTS>
+VKZG_DEV Fr vanishing_product(Fr x, const uint8_t *present, bool want, const Fr *roots, uint32_t lane) {
+    constexpr int kCells = 128 / PARTS;  // 16 or 8
+    const uint32_t part = lane & (PARTS - 1u);
+    uint32_t w[kCells / 4];
+    if constexpr (kCells == 16) {
+        const uint4 bits = *reinterpret_cast<const uint4 *>(present + kCells * part);
+        w[0] = bits.x, w[1] = bits.y, w[2] = bits.z, w[3] = bits.w;
+    } else {
+        const uint2 bits = *reinterpret_cast<const uint2 *>(present + kCells * part);
+        w[0] = bits.x, w[1] = bits.y;
+    }
     Fr prod = fr_one();
-#pragma unroll 1
-    for (uint32_t c = 16u * q; c < 16u * q + 16u; c++) {
-        if ((pm[c] != 0) != (want != 0)) continue;
-        const Fr root = roots[(64u * brev_bits(c, 7)) & 8191u];
-        prod = fr_mul(prod, fr_sub(x, root));
-    }
-    part[q][l] = prod;
-    __syncthreads();
-#pragma unroll 1
-    for (uint32_t w = 4; w > 0; w >>= 1) {
-        if (q < w) part[q][l] = fr_mul(part[q][l], part[q + w][l]);
-        __syncthreads();
-    }
-    if (q != 0) return;
-    prod = part[0][l];
-    if (coset) izc[blob * 128 + k] = fr_mul(prod, fr_const(cfr::INV_SEVEN_POW8192_MINUS_1_M));
-    else zd[blob * 128 + k] = prod;
-}
-
-// Threads of a pass block: one per butterfly of a stage, at most 256.
-template <int COLS, int N>
-constexpr int pass_threads() {
-    return COLS * (N / 2) < 256 ? COLS * (N / 2) : 256;
-}
-
-// Pass A: for each n1 < 64, DFT_128 over n2 of x[n1 + 64 n2], times
-// omega^(+-n1 k2), stored at y[n1 * 128 + k2].  Block = COLS consecutive n1.
-template <int IN_MODE, bool INVERSE, int COLS>
-__global__ void __launch_bounds__((pass_threads<COLS, 128>()))
-    k_rec_pass_a(Fr *out, const void *in, const Fr *zd, const Fr *roots_dir) {
-    constexpr int n = 128;
-    constexpr int nthreads = pass_threads<COLS, n>();
-    __shared__ uint32_t lds[8 * COLS * n];
-    constexpr int pitch = COLS * n;
-    const uint32_t blob = blockIdx.y;
-    const uint32_t n1_0 = blockIdx.x * COLS;
-    const int tid = threadIdx.x;
-#pragma unroll 1
-    for (int e = tid; e < COLS * n; e += nthreads) {
-        const int c = e % COLS, n2 = e / COLS;
-        const uint32_t idx = n1_0 + c + 64u * n2;  // natural evaluation index
-        Fr v;
-        if (IN_MODE == 0) {
-            // wire element brev13(idx) of the blob's cells, reduced mod r, times z
-            const uint32_t *cw = reinterpret_cast<const uint32_t *>(in) +
-                                 ((size_t)blob * 8192 + brev_bits(idx, 13)) * 8;
-            uint32_t w[8];
 #pragma unroll
-            for (int k = 0; k < 8; k++) w[k] = cw[k];
-            v = fr_mul(fr_from_any(fr_from_be_words(w)), zd[blob * 128 + (idx & 127u)]);
-        } else {
-            v = reinterpret_cast<const Fr *>(in)[(size_t)blob * 8192 + idx];
-        }
-        lds_store_fr(lds, pitch, c * n + (int)brev_bits(n2, 7), v);
-    }
-    __syncthreads();
-    lds_ntt_dit<7>(lds, pitch, COLS, roots_dir, tid, nthreads);
-#pragma unroll 1
-    for (int e = tid; e < COLS * n; e += nthreads) {
-        const int k2 = e % n, c = e / n;
-        const uint32_t n1 = n1_0 + c;
-        const Fr v = fr_mul(lds_load_fr(lds, pitch, c * n + k2), roots_dir[(n1 * k2) & 8191u]);
-        out[(size_t)blob * 8192 + n1 * 128 + k2] = v;
-    }
-}
-
-// Pass B: for each k2 < 128, DFT_64 over n1 of y[n1 * 128 + k2], giving
-// X[k2 + 128 k1].  Block = COLS consecutive k2.
-template <int OUT_MODE, bool INVERSE, int COLS>
-__global__ void __launch_bounds__((pass_threads<COLS, 64>()))
-    k_rec_pass_b(void *out, const Fr *in, const Fr *mul_table, const Fr *izc, const Fr *roots_dir) {
-    constexpr int n = 64;
-    constexpr int nthreads = pass_threads<COLS, n>();
-    __shared__ uint32_t lds[8 * COLS * n];
-    constexpr int pitch = COLS * n;
-    const uint32_t blob = blockIdx.y;
-    const uint32_t k2_0 = blockIdx.x * COLS;
-    const int tid = threadIdx.x;
-#pragma unroll 1
-    for (int e = tid; e < COLS * n; e += nthreads) {
-        const int c = e % COLS, n1 = e / COLS;
-        const Fr v = in[(size_t)blob * 8192 + n1 * 128 + k2_0 + c];
-        lds_store_fr(lds, pitch, c * n + (int)brev_bits(n1, 6), v);
-    }
-    __syncthreads();
-    lds_ntt_dit<6>(lds, pitch, COLS, roots_dir, tid, nthreads);
-#pragma unroll 1
-    for (int e = tid; e < COLS * n; e += nthreads) {
-        const int c = e % COLS, k1 = e / COLS;
-        const uint32_t k = k2_0 + c + 128u * k1;  // natural output index
-        Fr v = lds_load_fr(lds, pitch, c * n + k1);
-        if (OUT_MODE == 0) {
-            reinterpret_cast<Fr *>(out)[(size_t)blob * 8192 + k] = fr_mul(v, mul_table[k]);
-        } else if (OUT_MODE == 1) {
-            reinterpret_cast<Fr *>(out)[(size_t)blob * 8192 + k] =
-                fr_mul(v, izc[blob * 128 + (k & 127u)]);
-        } else {
-            uint32_t w[8];
-            fr_to_be_words(w, fr_to_canonical(v));
-            uint32_t *dst = reinterpret_cast<uint32_t *>(out) +
-                            ((size_t)blob * 8192 + brev_bits(k, 13)) * 8;
-#pragma unroll
-            for (int q = 0; q < 8; q++) dst[q] = w[q];
+    for (int k = 0; k < kCells; k++) {
+        const bool pres = ((w[k >> 2] >> (8 * (k & 3))) & 0xffu) != 0;
+        if (pres == want) {
+            const uint32_t cell = kCells * part + (uint32_t)k;
+            prod = fr_mul(prod, fr_sub(x, roots[(64u * brev_bits(cell, 7)) & 8191u]));
         }
     }
+#pragma unroll
+    for (int m = 1; m < PARTS; m <<= 1) prod = fr_mul(prod, fr_shfl_xor(prod, m));
+    return prod;
 }
 
-template <int IN_MODE, bool INV, int COLS>
-void launch_a(Fr *out, const void *in, const Fr *zd, const Fr *r, uint32_t nblobs, hipStream_t s) {
-    hipLaunchKernelGGL((k_rec_pass_a<IN_MODE, INV, COLS>), dim3(64 / COLS, nblobs),
-                       dim3(pass_threads<COLS, 128>()), 0, s, out, in, zd, r);
+// T1 pass A.  IN_MODE 0: `in` is the blobs' cell bytes (reduced mod r) and
+// the transform is inverse, on E(omega^i) Z(omega^i) with Z from the missing
+// cells.  IN_MODE 1: `in` is Fr coefficients in natural order (the blob
+// polynomial) and the transform is forward, with no factor: the first half
+// of the evaluation of a polynomial.  Writes S.
+template <int IN_MODE>
+__global__ void __launch_bounds__(256, 2)
+    k_rec_first(Fr *out, const void *in, const uint8_t *present, const Fr *roots, const Fr *roots_inv,
+                uint32_t nitems) {
+    extern __shared__ uint4 lds[];  // 16 KiB per wave
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint32_t item = blockIdx.x * (blockDim.x >> 6) + wave;
+    if (item >= nitems) return;  // whole waves: the block barriers below see only live waves' peers exit
+    const uint32_t blob = item / kItemsPerBlob, q = item % kItemsPerBlob;
+    const Exch x{lds + 1024 * wave};
+    const uint32_t c = lane >> 4, a = lane & 15u, n1 = 4u * q + c;
+    Fr v[8];
+    if (IN_MODE == 0) {
+        // Z(omega^i) for i & 127 = n1 + 64 (a & 1): value v = c + 4 (a & 1)
+        // of the 8 this wave computes (k = 4q + (v & 3) + 64 (v >> 2)).
+        const uint32_t vv = lane >> 3;
+        const uint32_t k = 4u * q + (vv & 3u) + 64u * (vv >> 2);
+        const Fr zd = vanishing_product<8>(roots[(64u * k) & 8191u], present + 128ull * blob, false, roots, lane);
+        // times R^2: one product takes a canonical element to E Z in Montgomery form
+        const Fr zr = fr_mul(fr_shfl(zd, (int)((c + 4u * (a & 1u)) * 8u)), fr_const(cfr::R2));
+        // x[a + 16b] sits at wire 128 brev6(n1) + 8 brev4(a) + brev3(b):
+        // 256 contiguous bytes per lane
+        const uint32_t *cw = reinterpret_cast<const uint32_t *>(in) +
+                             ((size_t)blob * 8192 + 128u * brev_bits(n1, 6) + 8u * brev_bits(a, 4)) * 8;
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const uint32_t w = brev_bits((uint32_t)b, 3);
+            const uint4 lo = *reinterpret_cast<const uint4 *>(cw + 8 * w);
+            const uint4 hi = *reinterpret_cast<const uint4 *>(cw + 8 * w + 4);
+            const uint32_t words[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+            Fr e = fr_from_be_words(words);
+            reduce_once<8>(e.v, e.v, cfr::P);
+            reduce_once<8>(e.v, e.v, cfr::P);
+            v[b] = fr_mul(e, zr);
+        }
+        pass_a(v, Dir(roots_inv), x, lane, n1);
+    } else {
+        const Fr *src = reinterpret_cast<const Fr *>(in) + (size_t)blob * 8192;
+#pragma unroll
+        for (int b = 0; b < 8; b++) v[b] = ld_fr(src + n1 + 64u * (a + 16u * (uint32_t)b));
+        pass_a(v, Dir(roots), x, lane, n1);
+    }
+    Fr *dst = out + (size_t)blob * 8192;
+#pragma unroll
+    for (int r = 0; r < 8; r++) st_fr(dst + s_index(n1, pass_a_k2(lane, r)), v[r]);
 }
 
-template <int OUT_MODE, bool INV, int COLS>
-void launch_b(void *out, const Fr *in, const Fr *mul_table, const Fr *izc, const Fr *r, uint32_t nblobs,
-              hipStream_t s) {
-    hipLaunchKernelGGL((k_rec_pass_b<OUT_MODE, INV, COLS>), dim3(128 / COLS, nblobs),
-                       dim3(pass_threads<COLS, 64>()), 0, s, out, in, mul_table, izc, r);
+// Pass B of T1 / T2 / T3, its pointwise factor, and pass A of the next
+// transform.  STAGE 0: inverse, * 7^k / 8192, forward.  STAGE 1: forward,
+// * 1 / Z(7 omega^k), inverse.  STAGE 2: inverse, * 7^-k / 8192, forward.
+template <int STAGE>
+__global__ void __launch_bounds__(256, 2)
+    k_rec_mid(Fr *out, const Fr *in, const uint8_t *present, const Fr *roots, const Fr *roots_inv,
+              const Fr *shift, uint32_t nitems) {
+    extern __shared__ uint4 lds[];  // 16 KiB per wave
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint32_t item = blockIdx.x * (blockDim.x >> 6) + wave;
+    if (item >= nitems) return;
+    const uint32_t blob = item / kItemsPerBlob, q = item % kItemsPerBlob;
+    const Exch x{lds + 1024 * wave};
+    const Fr *src = in + (size_t)blob * 8192 + q * 512u;
+    Fr v[8];
+#pragma unroll
+    for (int b = 0; b < 8; b++) v[b] = ld_fr(src + 64u * (uint32_t)b + lane);
+    const bool b_inverse = STAGE != 1;
+    pass_b(v, Dir(b_inverse ? roots_inv : roots), x, lane);
+    // lane L = 8e + cc holds X[k2 + 128 (e + 8f)], k2 = 4q + (cc & 3) + 64 (cc >> 2)
+    const uint32_t cc = lane & 7u, e = lane >> 3;
+    const uint32_t k2 = 4u * q + (cc & 3u) + 64u * (cc >> 2);
+    if (STAGE == 1) {
+        // 1 / Z(7 omega^k) = K^-1 prod over present cells (y - w128^brev7(c)),
+        // y = 7^64 w128^k2, K = 7^8192 - 1 (the product over all 128 cells)
+        const uint32_t vv = lane >> 3;
+        const uint32_t kv = 4u * q + (vv & 3u) + 64u * (vv >> 2);
+        const Fr y = fr_mul(roots[(64u * kv) & 8191u], fr_const(cfr::SEVEN_POW64_M));
+        const Fr p = vanishing_product<8>(y, present + 128ull * blob, true, roots, lane);
+        const Fr izc = fr_mul(fr_shfl(p, (int)(cc * 8u)), fr_const(cfr::INV_SEVEN_POW8192_MINUS_1_M));
+#pragma unroll
+        for (int f = 0; f < 8; f++) v[f] = fr_mul(v[f], izc);
+    } else {
+#pragma unroll
+        for (int f = 0; f < 8; f++) v[f] = fr_mul(v[f], shift[k2 + 128u * (e + 8u * (uint32_t)f)]);
+    }
+    // next pass A column c = cc & 3 takes x[n2] = X'[k1 = n2 >> 1] of the
+    // pass-B column with h = n2 & 1 = cc >> 2
+    exch_sync();
+#pragma unroll
+    for (int f = 0; f < 8; f++) x.put((cc & 3u) * 128u + 2u * (e + 8u * (uint32_t)f) + (cc >> 2), v[f]);
+    exch_sync();
+    const uint32_t c = lane >> 4, a = lane & 15u, n1 = 4u * q + c;
+#pragma unroll
+    for (int b = 0; b < 8; b++) v[b] = x.get(c * 128u + a + 16u * (uint32_t)b);
+    pass_a(v, Dir(b_inverse ? roots : roots_inv), x, lane, n1);
+    Fr *dst = out + (size_t)blob * 8192;
+#pragma unroll
+    for (int r = 0; r < 8; r++) st_fr(dst + s_index(n1, pass_a_k2(lane, r)), v[r]);
 }
 
-template <int COLS>
-void pass_a_cols(Fr *out, const void *in, int in_mode, const Fr *zd, bool inverse, const Fr *r,
-                 uint32_t nblobs, hipStream_t s) {
-    if (in_mode == 0 && inverse) launch_a<0, true, COLS>(out, in, zd, r, nblobs, s);
-    else if (in_mode == 0) launch_a<0, false, COLS>(out, in, zd, r, nblobs, s);
-    else if (inverse) launch_a<1, true, COLS>(out, in, zd, r, nblobs, s);
-    else launch_a<1, false, COLS>(out, in, zd, r, nblobs, s);
+// T4 pass B: canonical big-endian bytes.  Lane L = 8e + cc writes the
+// elements k1 = e + 8f of cell brev7(k2) at positions 8 brev3(e) + brev3(f):
+// 256 contiguous bytes.
+__global__ void __launch_bounds__(256, 2)
+    k_rec_last(uint8_t *cells, const Fr *in, const Fr *roots, uint32_t nitems) {
+    extern __shared__ uint4 lds[];  // 16 KiB per wave
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint32_t item = blockIdx.x * (blockDim.x >> 6) + wave;
+    if (item >= nitems) return;
+    const uint32_t blob = item / kItemsPerBlob, q = item % kItemsPerBlob;
+    const Exch x{lds + 1024 * wave};
+    const Fr *src = in + (size_t)blob * 8192 + q * 512u;
+    Fr v[8];
+#pragma unroll
+    for (int b = 0; b < 8; b++) v[b] = ld_fr(src + 64u * (uint32_t)b + lane);
+    pass_b(v, Dir(roots), x, lane);
+    const uint32_t cc = lane & 7u, e = lane >> 3;
+    const uint32_t k2 = 4u * q + (cc & 3u) + 64u * (cc >> 2);
+    uint32_t *dst = reinterpret_cast<uint32_t *>(cells) +
+                    ((size_t)blob * 8192 + 64u * brev_bits(k2, 7) + 8u * brev_bits(e, 3)) * 8;
+    Fr one = fr_zero();
+    one.v[0] = 1;
+#pragma unroll
+    for (int f = 0; f < 8; f++) {
+        uint32_t w[8];
+        fr_to_be_words(w, fr_mul(v[f], one));  // Montgomery -> canonical
+        uint4 *p = reinterpret_cast<uint4 *>(dst + 8u * brev_bits((uint32_t)f, 3));
+        p[0] = make_uint4(w[0], w[1], w[2], w[3]);
+        p[1] = make_uint4(w[4], w[5], w[6], w[7]);
+    }
 }
 
-template <int COLS>
-void pass_b_cols(void *out, const Fr *in, int out_mode, const Fr *mul_table, const Fr *izc, const Fr *r,
-                 uint32_t nblobs, hipStream_t s) {
-    if (out_mode == 0) launch_b<0, true, COLS>(out, in, mul_table, izc, r, nblobs, s);
-    else if (out_mode == 1) launch_b<1, false, COLS>(out, in, mul_table, izc, r, nblobs, s);
-    else launch_b<2, false, COLS>(out, in, mul_table, izc, r, nblobs, s);
+// ------------------------------------------------------------------ radix 4
+//
+// The same five kernels with 4 values per lane: item (blob, q) for q < 32
+// owns the pass-A columns n1 in {2q, 2q + 1} and the pass-B columns k2 in
+// {2q, 2q + 1, 64 + 2q, 65 + 2q}, so a batch has twice as many (shorter)
+// waves: more of them share each SIMD, which is what a v_mad_u64_u32 stream
+// needs to issue at full rate.  The transforms run as Stockham stages
+// (natural order in and out): thread j of a column of length N reads
+// d[j + r N / R] (r < R), multiplies by w_(Ns R)^((j mod Ns) r), takes the
+// DFT_R and writes d'[(j / Ns) Ns R + j mod Ns + r Ns], for radices
+// 4, 4, 4 (N = 64) and 4, 4, 4, 2 (N = 128), through a 256-value LDS region
+// per wave.  Between kernels y[n1][k2] is stored per blob as
+// S4[q][n1][h][j] with k2 = 2q + j + 64h (q < 32, h, j < 2).
+constexpr int kItemsPerBlob4 = 32;
+
+struct Exch4 {
+    uint4 *base;
+    VKZG_DEV static uint32_t pos(uint32_t idx) { return idx ^ ((idx >> 5) & 7u); }
+    VKZG_DEV void put(uint32_t idx, const Fr &v) const {
+        uint4 *p = base + 2 * pos(idx);
+        p[0] = make_uint4(v.v[0], v.v[1], v.v[2], v.v[3]);
+        p[1] = make_uint4(v.v[4], v.v[5], v.v[6], v.v[7]);
+    }
+    VKZG_DEV Fr get(uint32_t idx) const {
+        const uint4 *p = base + 2 * pos(idx);
+        const uint4 a = p[0], b = p[1];
+        Fr r;
+        r.v[0] = a.x, r.v[1] = a.y, r.v[2] = a.z, r.v[3] = a.w;
+        r.v[4] = b.x, r.v[5] = b.y, r.v[6] = b.z, r.v[7] = b.w;
+        return r;
+    }
+};
+
+VKZG_DEV uint32_t s4_index(uint32_t n1, uint32_t k2) {
+    return ((k2 & 63u) >> 1) * 256u + n1 * 4u + (k2 & 1u) + 2u * (k2 >> 6);
 }
+
+// One radix-4 Stockham stage of a column at LDS offset `col` (length N,
+// N / 4 threads, thread j): v <- d[j + r N/4] twiddled and transformed; the
+// caller stores v at (j / Ns) 4 Ns + j mod Ns + r Ns.
+template <int N, int NS>
+VKZG_DEV void stage4_load(Fr (&v)[4], const Dir &d, const Exch4 &x, uint32_t col, uint32_t j) {
+#pragma unroll
+    for (int r = 0; r < 4; r++) v[r] = x.get(col + j + (uint32_t)r * (N / 4));
+    if (NS > 1) {
+        const uint32_t k = j % NS;
+#pragma unroll
+        for (int r = 1; r < 4; r++) v[r] = fr_mul(v[r], d.pow((8192u / (NS * 4)) * k * (uint32_t)r));
+    }
+    dft4(v[0], v[1], v[2], v[3], d.w8_2);
+}
+template <int NS>
+VKZG_DEV void stage4_store(const Fr (&v)[4], const Exch4 &x, uint32_t col, uint32_t j) {
+    const uint32_t base = (j / NS) * NS * 4u + j % NS;
+#pragma unroll
+    for (int r = 0; r < 4; r++) x.put(col + base + (uint32_t)r * NS, v[r]);
+}
+
+// Pass B (DFT_64) of the wave's 4 columns cc (k2 = 2q + (cc & 1) + 64 (cc >> 1)):
+// lane L = 4j + cc enters with v[r] = y[j + 16r][cc] and leaves with
+// v[r] = X[j + 16r] of its column.
+VKZG_DEV void pass_b4(Fr (&v)[4], const Dir &d, const Exch4 &x, uint32_t lane) {
+    const uint32_t cc = lane & 3u, j = lane >> 2, col = cc * 64u;
+    dft4(v[0], v[1], v[2], v[3], d.w8_2);
+    exch_sync();
+    stage4_store<1>(v, x, col, j);
+    exch_sync();
+    stage4_load<64, 4>(v, d, x, col, j);
+    exch_sync();
+    stage4_store<4>(v, x, col, j);
+    exch_sync();
+    stage4_load<64, 16>(v, d, x, col, j);  // NS = 16: output index j + 16 r
+}
+
+// Pass A (DFT_128) of the wave's 2 columns c (n1 = 2q + c), with its
+// four-step twiddle: lane L = 32c + j enters with v[r] = x[j + 32r] and
+// leaves with v[t] = y[n1][k2] for k2 = pass_a4_k2(j, t).
+VKZG_DEV uint32_t pass_a4_k2(uint32_t j, int t) {
+    // t = 0, 1: j' = j, outputs j', j' + 64; t = 2, 3: j' = j + 32
+    return j + 32u * (uint32_t)(t >> 1) + 64u * (uint32_t)(t & 1);
+}
+VKZG_DEV void pass_a4(Fr (&v)[4], const Dir &d, const Exch4 &x, uint32_t lane, uint32_t n1) {
+    const uint32_t c = lane >> 5, j = lane & 31u, col = c * 128u;
+    dft4(v[0], v[1], v[2], v[3], d.w8_2);
+    exch_sync();
+    stage4_store<1>(v, x, col, j);
+    exch_sync();
+    stage4_load<128, 4>(v, d, x, col, j);
+    exch_sync();
+    stage4_store<4>(v, x, col, j);
+    exch_sync();
+    stage4_load<128, 16>(v, d, x, col, j);
+    exch_sync();
+    stage4_store<16>(v, x, col, j);
+    exch_sync();
+    // radix-2 stage (Ns = 64): j' = j and j + 32, d[j'] and d[j' + 64],
+    // twiddle w128^j' = w^(64 j')
+    Fr u0 = x.get(col + j), u1 = x.get(col + j + 64u);
+    Fr u2 = x.get(col + j + 32u), u3 = x.get(col + j + 96u);
+    bfly(u0, u1, d.pow(64u * j));
+    bfly(u2, u3, d.pow(64u * (j + 32u)));
+    v[0] = fr_mul(u0, d.pow(n1 * pass_a4_k2(j, 0)));
+    v[1] = fr_mul(u1, d.pow(n1 * pass_a4_k2(j, 1)));
+    v[2] = fr_mul(u2, d.pow(n1 * pass_a4_k2(j, 2)));
+    v[3] = fr_mul(u3, d.pow(n1 * pass_a4_k2(j, 3)));
+}
+
+template <int IN_MODE>
+__global__ void __launch_bounds__(256, 2)
+    k_rec4_first(Fr *out, const void *in, const uint8_t *present, const Fr *roots, const Fr *roots_inv,
+                 uint32_t nitems) {
+    extern __shared__ uint4 lds[];  // 8 KiB per wave
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint32_t item = blockIdx.x * (blockDim.x >> 6) + wave;
+    if (item >= nitems) return;
+    const uint32_t blob = item / kItemsPerBlob4, q = item % kItemsPerBlob4;
+    const Exch4 x{lds + 512 * wave};
+    const uint32_t c = lane >> 5, j = lane & 31u, n1 = 2u * q + c;
+    Fr v[4];
+    if (IN_MODE == 0) {
+        // Z(omega^i) for i & 127 = n1 + 64 (j & 1): value vv = c + 2 (j & 1)
+        // of the 4 this wave computes (k = 2q + (vv & 1) + 64 (vv >> 1)).
+        const uint32_t vv = lane >> 4;
+        const uint32_t k = 2u * q + (vv & 1u) + 64u * (vv >> 1);
+        const Fr zd = vanishing_product<16>(roots[(64u * k) & 8191u], present + 128ull * blob, false, roots, lane);
+        const Fr zr = fr_mul(fr_shfl(zd, (int)((c + 2u * (j & 1u)) * 16u)), fr_const(cfr::R2));
+        // x[j + 32r] sits at wire 128 brev6(n1) + 4 brev5(j) + brev2(r)
+        const uint32_t *cw = reinterpret_cast<const uint32_t *>(in) +
+                             ((size_t)blob * 8192 + 128u * brev_bits(n1, 6) + 4u * brev_bits(j, 5)) * 8;
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const uint32_t w = brev_bits((uint32_t)r, 2);
+            const uint4 lo = *reinterpret_cast<const uint4 *>(cw + 8 * w);
+            const uint4 hi = *reinterpret_cast<const uint4 *>(cw + 8 * w + 4);
+            const uint32_t words[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+            Fr e = fr_from_be_words(words);
+            reduce_once<8>(e.v, e.v, cfr::P);
+            reduce_once<8>(e.v, e.v, cfr::P);
+            v[r] = fr_mul(e, zr);
+        }
+        pass_a4(v, Dir(roots_inv), x, lane, n1);
+    } else {
+        const Fr *src = reinterpret_cast<const Fr *>(in) + (size_t)blob * 8192;
+#pragma unroll
+        for (int r = 0; r < 4; r++) v[r] = ld_fr(src + n1 + 64u * (j + 32u * (uint32_t)r));
+        pass_a4(v, Dir(roots), x, lane, n1);
+    }
+    Fr *dst = out + (size_t)blob * 8192;
+#pragma unroll
+    for (int t = 0; t < 4; t++) st_fr(dst + s4_index(n1, pass_a4_k2(j, t)), v[t]);
+}
+
+template <int STAGE>
+__global__ void __launch_bounds__(256, 2)
+    k_rec4_mid(Fr *out, const Fr *in, const uint8_t *present, const Fr *roots, const Fr *roots_inv,
+               const Fr *shift, uint32_t nitems) {
+    extern __shared__ uint4 lds[];
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint32_t item = blockIdx.x * (blockDim.x >> 6) + wave;
+    if (item >= nitems) return;
+    const uint32_t blob = item / kItemsPerBlob4, q = item % kItemsPerBlob4;
+    const Exch4 x{lds + 512 * wave};
+    const Fr *src = in + (size_t)blob * 8192 + q * 256u;
+    Fr v[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) v[r] = ld_fr(src + 64u * (uint32_t)r + lane);
+    const bool b_inverse = STAGE != 1;
+    pass_b4(v, Dir(b_inverse ? roots_inv : roots), x, lane);
+    // lane L = 4j + cc holds X[k2 + 128 (j + 16 r)], k2 = 2q + (cc & 1) + 64 (cc >> 1)
+    const uint32_t cc = lane & 3u, jb = lane >> 2;
+    const uint32_t k2 = 2u * q + (cc & 1u) + 64u * (cc >> 1);
+    if (STAGE == 1) {
+        const uint32_t vv = lane >> 4;
+        const uint32_t kv = 2u * q + (vv & 1u) + 64u * (vv >> 1);
+        const Fr y = fr_mul(roots[(64u * kv) & 8191u], fr_const(cfr::SEVEN_POW64_M));
+        const Fr p = vanishing_product<16>(y, present + 128ull * blob, true, roots, lane);
+        const Fr izc = fr_mul(fr_shfl(p, (int)(cc * 16u)), fr_const(cfr::INV_SEVEN_POW8192_MINUS_1_M));
+#pragma unroll
+        for (int r = 0; r < 4; r++) v[r] = fr_mul(v[r], izc);
+    } else {
+#pragma unroll
+        for (int r = 0; r < 4; r++) v[r] = fr_mul(v[r], shift[k2 + 128u * (jb + 16u * (uint32_t)r)]);
+    }
+    // next pass A column c = cc & 1 takes x[n2] = X'[k1 = n2 >> 1] of the
+    // pass-B column with h = n2 & 1 = cc >> 1
+    exch_sync();
+#pragma unroll
+    for (int r = 0; r < 4; r++) x.put((cc & 1u) * 128u + 2u * (jb + 16u * (uint32_t)r) + (cc >> 1), v[r]);
+    exch_sync();
+    const uint32_t c = lane >> 5, j = lane & 31u, n1 = 2u * q + c;
+#pragma unroll
+    for (int r = 0; r < 4; r++) v[r] = x.get(c * 128u + j + 32u * (uint32_t)r);
+    pass_a4(v, Dir(b_inverse ? roots : roots_inv), x, lane, n1);
+    Fr *dst = out + (size_t)blob * 8192;
+#pragma unroll
+    for (int t = 0; t < 4; t++) st_fr(dst + s4_index(n1, pass_a4_k2(j, t)), v[t]);
+}
+
+__global__ void __launch_bounds__(256, 2)
+    k_rec4_last(uint8_t *cells, const Fr *in, const Fr *roots, uint32_t nitems) {
+    extern __shared__ uint4 lds[];
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint32_t item = blockIdx.x * (blockDim.x >> 6) + wave;
+    if (item >= nitems) return;
+    const uint32_t blob = item / kItemsPerBlob4, q = item % kItemsPerBlob4;
+    const Exch4 x{lds + 512 * wave};
+    const Fr *src = in + (size_t)blob * 8192 + q * 256u;
+    Fr v[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) v[r] = ld_fr(src + 64u * (uint32_t)r + lane);
+    pass_b4(v, Dir(roots), x, lane);
+    // X[k1 = j + 16r] of cell brev7(k2) at element 4 brev4(j) + brev2(r)
+    const uint32_t cc = lane & 3u, j = lane >> 2;
+    const uint32_t k2 = 2u * q + (cc & 1u) + 64u * (cc >> 1);
+    uint32_t *dst = reinterpret_cast<uint32_t *>(cells) +
+                    ((size_t)blob * 8192 + 64u * brev_bits(k2, 7) + 4u * brev_bits(j, 4)) * 8;
+    Fr one = fr_zero();
+    one.v[0] = 1;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        uint32_t w[8];
+        fr_to_be_words(w, fr_mul(v[r], one));  // Montgomery -> canonical
+        uint4 *p = reinterpret_cast<uint4 *>(dst + 8u * brev_bits((uint32_t)r, 2));
+        p[0] = make_uint4(w[0], w[1], w[2], w[3]);
+        p[1] = make_uint4(w[4], w[5], w[6], w[7]);
+    }
+}
+
+uint32_t rec_grid(uint32_t nitems, uint32_t waves) { return (nitems + waves - 1) / waves; }
+constexpr uint32_t kExchBytes = 1024 * sizeof(uint4);  // LDS per wave
 
 }  // namespace
 
-hipError_t launch_rec_vanishing(Fr *zd, Fr *izc, const uint8_t *present, const Fr *roots,
-                                uint32_t nblobs, hipStream_t s) {
-    hipLaunchKernelGGL(k_rec_vanishing, dim3(nblobs, 2, 2), dim3(512), 0, s, zd, izc, present, roots);
+hipError_t launch_rec_first(Fr *out, const void *in, int in_mode, const uint8_t *present, const Fr *roots,
+                            const Fr *roots_inv, uint32_t nblobs, uint32_t radix, uint32_t waves, hipStream_t s) {
+    const bool r4 = radix == 4;
+    const uint32_t nitems = nblobs * (r4 ? kItemsPerBlob4 : kItemsPerBlob);
+    const dim3 grid(rec_grid(nitems, waves)), block(64 * waves);
+    const uint32_t lds = (r4 ? kExchBytes / 2 : kExchBytes) * waves;
+    if (r4 && in_mode == 0)
+        hipLaunchKernelGGL(k_rec4_first<0>, grid, block, lds, s, out, in, present, roots, roots_inv, nitems);
+    else if (r4)
+        hipLaunchKernelGGL(k_rec4_first<1>, grid, block, lds, s, out, in, present, roots, roots_inv, nitems);
+    else if (in_mode == 0)
+        hipLaunchKernelGGL(k_rec_first<0>, grid, block, lds, s, out, in, present, roots, roots_inv, nitems);
+    else
+        hipLaunchKernelGGL(k_rec_first<1>, grid, block, lds, s, out, in, present, roots, roots_inv, nitems);
     return hipGetLastError();
 }
 
-hipError_t launch_rec_pass_a(Fr *out, const void *in, int in_mode, const Fr *zd, bool inverse,
-                             const Fr *roots, const Fr *roots_inv, uint32_t nblobs, uint32_t cols,
-                             hipStream_t s) {
-    const Fr *r = inverse ? roots_inv : roots;
-    switch (cols) {
-        case 4: pass_a_cols<4>(out, in, in_mode, zd, inverse, r, nblobs, s); break;
-        case 16: pass_a_cols<16>(out, in, in_mode, zd, inverse, r, nblobs, s); break;
-        case 2: pass_a_cols<2>(out, in, in_mode, zd, inverse, r, nblobs, s); break;
-        default: pass_a_cols<8>(out, in, in_mode, zd, inverse, r, nblobs, s); break;
+hipError_t launch_rec_mid(Fr *out, const Fr *in, int stage, const uint8_t *present, const Fr *roots,
+                          const Fr *roots_inv, const Fr *shift, uint32_t nblobs, uint32_t radix,
+                          uint32_t waves, hipStream_t s) {
+    const bool r4 = radix == 4;
+    const uint32_t nitems = nblobs * (r4 ? kItemsPerBlob4 : kItemsPerBlob);
+    const dim3 grid(rec_grid(nitems, waves)), block(64 * waves);
+    const uint32_t lds = (r4 ? kExchBytes / 2 : kExchBytes) * waves;
+    if (r4) {
+        if (stage == 0)
+            hipLaunchKernelGGL(k_rec4_mid<0>, grid, block, lds, s, out, in, present, roots, roots_inv, shift, nitems);
+        else if (stage == 1)
+            hipLaunchKernelGGL(k_rec4_mid<1>, grid, block, lds, s, out, in, present, roots, roots_inv, shift, nitems);
+        else
+            hipLaunchKernelGGL(k_rec4_mid<2>, grid, block, lds, s, out, in, present, roots, roots_inv, shift, nitems);
+    } else {
+        if (stage == 0)
+            hipLaunchKernelGGL(k_rec_mid<0>, grid, block, lds, s, out, in, present, roots, roots_inv, shift, nitems);
+        else if (stage == 1)
+            hipLaunchKernelGGL(k_rec_mid<1>, grid, block, lds, s, out, in, present, roots, roots_inv, shift, nitems);
+        else
+            hipLaunchKernelGGL(k_rec_mid<2>, grid, block, lds, s, out, in, present, roots, roots_inv, shift, nitems);
     }
     return hipGetLastError();
 }
 
-hipError_t launch_rec_pass_b(void *out, const Fr *in, int out_mode, const Fr *mul_table,
-                             const Fr *izc, bool inverse, const Fr *roots, const Fr *roots_inv,
-                             uint32_t nblobs, uint32_t cols, hipStream_t s) {
-    const Fr *r = inverse ? roots_inv : roots;
-    switch (cols) {
-        case 4: pass_b_cols<4>(out, in, out_mode, mul_table, izc, r, nblobs, s); break;
-        case 16: pass_b_cols<16>(out, in, out_mode, mul_table, izc, r, nblobs, s); break;
-        case 2: pass_b_cols<2>(out, in, out_mode, mul_table, izc, r, nblobs, s); break;
-        default: pass_b_cols<8>(out, in, out_mode, mul_table, izc, r, nblobs, s); break;
-    }
+hipError_t launch_rec_last(uint8_t *cells, const Fr *in, const Fr *roots, uint32_t nblobs, uint32_t radix,
+                           uint32_t waves, hipStream_t s) {
+    const bool r4 = radix == 4;
+    const uint32_t nitems = nblobs * (r4 ? kItemsPerBlob4 : kItemsPerBlob);
+    const dim3 grid(rec_grid(nitems, waves)), block(64 * waves);
+    const uint32_t lds = (r4 ? kExchBytes / 2 : kExchBytes) * waves;
+    if (r4) hipLaunchKernelGGL(k_rec4_last, grid, block, lds, s, cells, in, roots, nitems);
+    else hipLaunchKernelGGL(k_rec_last, grid, block, lds, s, cells, in, roots, nitems);
     return hipGetLastError();
 }
 

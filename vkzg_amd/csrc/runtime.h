@@ -29,7 +29,8 @@ struct Tuning {
     uint32_t blocks_per_cu = 2;  // persistent phase-A grid = CUs x this, 1..2
     uint32_t norm_chunk = 0;     // points per batched inversion in the normalisation (0: fitted to the chip)
     uint32_t splits = 1;         // sub-streams, 1..kMaxSplits
-    uint32_t rec_cols = 8;       // NTT_8192 sub-transforms per pass block: 2, 4, 8 or 16
+    uint32_t rec_waves = 1;      // recovery work items (waves) per block: 1, 2 or 4
+    uint32_t rec_radix = 4;      // recovery register radix: 4 (4 values per lane) or 8 (8 values)
     uint32_t ladder_splits = 1;  // ladder segments (phase A also computes their starts): 1, 2 or 4
     uint32_t reduce_lanes = 8;   // bucket-reduction lanes per output: 4, 8, 16 or 32
     // Not a tuning: the test switch that sends phase A, phase B and the bucket
@@ -116,7 +117,7 @@ struct WorkSet {
     Fp *prefix = nullptr;
     // recovery
     uint8_t *cells = nullptr, *present = nullptr, *cells_out = nullptr;
-    Fr *zd = nullptr, *izc = nullptr, *bufA = nullptr, *bufB = nullptr;
+    Fr *bufA = nullptr, *bufB = nullptr;
 
     void allocate(uint32_t blobs_cap);
     ~WorkSet() { arena.release(); }
@@ -326,13 +327,16 @@ void enqueue_recovery(vkzg_prover *p, WorkSet &ws, StreamSet &ss, uint8_t *d_out
                       StageClock &clk, const Tuning &tn, GraphCache *graphs);
 
 // The recovery pipeline of nb blobs on stream s with caller-provided scratch:
-// zd and izc hold nb x 128 Fr, A and B nb x 8192 Fr each.  Every kernel writes
-// each scratch element before it reads it, so the scratch needs no clearing.
-void enqueue_recovery_into(vkzg_prover *p, Fr *zd, Fr *izc, Fr *A, Fr *B, uint8_t *d_out,
-                           const uint8_t *d_cells, const uint8_t *d_present, uint32_t nb, hipStream_t s,
-                           StageClock &clk, uint32_t cols);
+// A and B hold nb x 8192 Fr each.  Every kernel writes each scratch element
+// before any kernel reads it, so the scratch needs no clearing.  The cell,
+// mask and output buffers must be 16-byte aligned (vector accesses).
+void enqueue_recovery_into(vkzg_prover *p, Fr *A, Fr *B, uint8_t *d_out, const uint8_t *d_cells,
+                           const uint8_t *d_present, uint32_t nb, hipStream_t s, StageClock &clk,
+                           uint32_t radix, uint32_t waves);
 
 bool chunk_missing_ok(const uint8_t *present, uint32_t nb);
+
+inline bool aligned16(const void *p) { return ((uintptr_t)p & 15u) == 0; }
 
 // Clears the per-stage times before a call (stage timing accumulates over chunks).
 void reset_stage_times(vkzg_prover *p);
