@@ -129,6 +129,8 @@ typedef struct {
     uint32_t autotuned;        /* 1 after vkzg_amd_autotune */
     double autotune_ms;        /* its wall time */
     uint32_t phase_b_items;    /* bucket items of both phase-B halves (each output uses one half) */
+    uint64_t resident_table_bytes; /* the main phase-A table plus any smaller ones the autotuner kept */
+    uint32_t extra_table_windows;  /* bit c set: a smaller phase-A table of window c is resident */
 } vkzg_amd_info;
 
 /* Fills `out` (never fails for a non-NULL prover). */
@@ -150,7 +152,16 @@ typedef struct {
                                        of each (more phase-A work, shorter dependent chains) */
     uint32_t reduce_lanes;          /* bucket-reduction lanes per output: 4, 8, 16 or 32 */
     uint32_t recovery_radix;        /* recovery register radix: 4 (4 values per lane) or 8 (8) */
+    uint32_t phase_a_window_bits;   /* phase-A table window: 0 = the main table, else a smaller
+                                       resident one (vkzg_amd_info.extra_table_windows) */
 } vkzg_amd_tuning;
+
+/* Builds (if not yet resident) a phase-A table of window `window_bits`
+ * (10-13, below the main table's) beside the main one, so that tunings may
+ * name it (phase_a_window_bits); it stays until the prover is freed.  The
+ * autotuner builds such candidates itself and keeps those a size class picks.
+ * BADARGS out of range, MALLOC when it does not fit in free device memory. */
+vkzg_result vkzg_amd_load_window_table(vkzg_prover *p, uint32_t window_bits);
 
 /* The tuning used for a batch of `num_blobs` blobs. */
 vkzg_result vkzg_amd_get_tuning(const vkzg_prover *p, size_t num_blobs, vkzg_amd_tuning *out);
@@ -171,8 +182,10 @@ vkzg_result vkzg_amd_autotune(vkzg_prover *p, double *ms_out);
  * dispatcher packs queued requests (oldest first, one kind per batch) into
  * device batches of up to `batch_blobs` blobs and runs up to `slots` batches
  * at once, each with its own device working set; requests arriving while
- * batches run join the next free slot.  `hbm_budget_bytes` caps the working
- * sets (fewer slots, then smaller batches).  Request buffers belong to the
+ * batches run join the next free slot.  `hbm_budget_bytes` caps the device
+ * memory of the prover's resident phase-A tables plus the slots' working
+ * sets (fewer slots, then smaller batches); a budget the tables alone fill
+ * is VKZG_ERR_MALLOC.  Request buffers belong to the
  * caller and must stay valid until vkzg_amd_server_wait returns.  A request
  * with an invalid blob fails alone (VKZG_ERR_INVALID_BLOB, its outputs
  * unspecified); more than 64 missing cells is rejected at submission.
@@ -182,7 +195,7 @@ typedef struct vkzg_amd_server vkzg_amd_server;
 typedef struct {
     uint32_t batch_blobs;      /* blobs per device batch; 0 = the prover's max_batch_size */
     uint32_t slots;            /* batches in flight; 0 = 2 */
-    uint64_t hbm_budget_bytes; /* device memory for the slots; 0 = no limit */
+    uint64_t hbm_budget_bytes; /* device memory for the resident tables and the slots; 0 = no limit */
 } vkzg_amd_server_options;
 
 typedef struct {

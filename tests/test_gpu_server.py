@@ -202,9 +202,14 @@ def test_large_request_spans_batches_and_budget(gpu_prover, vectors):
     want = vectors.proofs("compute_cells_and_kzg_proofs_case_valid_3")
     with vkzg_amd.Server(gpu_prover, batch_blobs=1, slots=1) as one:
         per_blob = one.stats()["slot_bytes"]
-    # a budget of 10.5 blobs of working set: one slot of 10 blobs
+    # a budget of the resident tables plus 10.5 blobs of working set: one slot
+    # of 10 blobs; the tables alone do not leave room for any slot
+    tables = gpu_prover.info()["resident_table_bytes"]
+    with pytest.raises(vkzg_amd.VkzgError) as e:
+        vkzg_amd.Server(gpu_prover, hbm_budget_bytes=tables)
+    assert e.value.code == vkzg_amd.ERR_MALLOC
     with vkzg_amd.Server(gpu_prover, batch_blobs=64, slots=4,
-                         hbm_budget_bytes=per_blob * 21 // 2) as srv:
+                         hbm_budget_bytes=tables + per_blob * 21 // 2) as srv:
         st = srv.stats()
         assert st["slots"] == 1 and st["batch_blobs"] == 10
         t = srv.submit_proofs(blob * 23, 23)

@@ -56,6 +56,38 @@ def test_recovery_radix_and_waves_bit_exact(gpu_prover, vectors, cells_of, resto
     assert out == cells and proofs == b"".join(vectors.proofs(n) for n in names)
 
 
+def test_smaller_window_table_bit_exact(gpu_prover_small_batch, gpu_seeded_prover, vectors):
+    """A second, smaller phase-A table beside the main one: tunings that name
+    it give the same proofs; windows outside 10-13, at or above the main
+    table's, or not resident are rejected."""
+    p = gpu_seeded_prover
+    main = p.info()["window_bits"]
+    assert main > 10
+    saved = p.get_tuning(3)
+    with pytest.raises(vkzg_amd.VkzgError) as e:
+        p.set_tuning(3, phase_a_window_bits=10)  # not resident yet
+    assert e.value.code == vkzg_amd.ERR_BADARGS
+    for bad in (9, 14, main):
+        with pytest.raises(vkzg_amd.VkzgError):
+            p.load_window_table(bad)
+    p.load_window_table(10)
+    info = p.info()
+    assert info["extra_table_windows"] & (1 << 10)
+    assert info["resident_table_bytes"] > info["table_bytes"]
+    blobs = b"".join(vectors.blob(n) for n in NAMES)
+    want = p.compute_cell_kzg_proofs_batch(blobs, 3)
+    try:
+        p.set_tuning(3, phase_a_window_bits=10)
+        assert p.get_tuning(3)["phase_a_window_bits"] == 10
+        assert p.compute_cell_kzg_proofs_batch(blobs, 3) == want
+    finally:
+        p.set_tuning(3, **saved)
+    # the main 10-bit table of the small-batch prover gives the same proofs
+    # of the mainnet vectors as its default
+    assert gpu_prover_small_batch.compute_cell_kzg_proofs_batch(blobs, 3) == \
+        b"".join(vectors.proofs(n) for n in NAMES)
+
+
 def test_tuning_rejects_out_of_range(gpu_prover, restore_tuning):
     for bad in (dict(phase_a_groups=0), dict(phase_a_blocks_per_cu=3),
                 dict(normalize_chunk=257), dict(splits=5),

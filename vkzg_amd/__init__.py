@@ -221,6 +221,10 @@ class Prover:
     def set_graphs(self, enable=True):
         _check(load().vkzg_amd_set_graphs(self._h, 1 if enable else 0))
 
+    def load_window_table(self, window_bits):
+        """Keep a smaller phase-A table (window 10-13) resident for tunings to name."""
+        _check(load().vkzg_amd_load_window_table(self._h, window_bits))
+
     def debug_mad_rate(self, waves_per_simd, iters=2000):
         """(v_mad_u64_u32 lane-ops per second over the chip, nominal clock in MHz)."""
         rate, mhz = ctypes.c_double(), ctypes.c_double()

@@ -46,6 +46,7 @@ EXT_SYMBOLS = [
     "vkzg_amd_set_tuning",
     "vkzg_amd_autotune",
     "vkzg_amd_set_graphs",
+    "vkzg_amd_load_window_table",
     "vkzg_amd_load_g2_setup",
     "vkzg_amd_verify_cell_kzg_proof_batch",
     "vkzg_amd_blob_to_commitments",
@@ -81,6 +82,8 @@ class VkzgAmdInfo(ctypes.Structure):
         ("autotuned", ctypes.c_uint32),
         ("autotune_ms", ctypes.c_double),
         ("phase_b_items", ctypes.c_uint32),
+        ("resident_table_bytes", ctypes.c_uint64),
+        ("extra_table_windows", ctypes.c_uint32),
     ]
 
 
@@ -94,6 +97,7 @@ class Tuning(ctypes.Structure):
         ("ladder_splits", ctypes.c_uint32),
         ("reduce_lanes", ctypes.c_uint32),
         ("recovery_radix", ctypes.c_uint32),
+        ("phase_a_window_bits", ctypes.c_uint32),
     ]
 
 
@@ -189,6 +193,8 @@ def load():
     L.vkzg_amd_debug_mad_rate.restype = ctypes.c_int
     L.vkzg_amd_debug_mad_rate.argtypes = [vp, ctypes.c_uint32, ctypes.c_uint32,
                                           ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
+    L.vkzg_amd_load_window_table.restype = ctypes.c_int
+    L.vkzg_amd_load_window_table.argtypes = [vp, ctypes.c_uint32]
     L.vkzg_amd_debug_force_checked.restype = ctypes.c_int
     L.vkzg_amd_debug_force_checked.argtypes = [vp, ctypes.c_int]
     L.vkzg_amd_autotune.restype = ctypes.c_int

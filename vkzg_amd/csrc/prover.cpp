@@ -111,6 +111,52 @@ void build_phase_b_items(const Fr *kpm, std::vector<uint32_t> &items, std::vecto
     }
 }
 
+// Fills t with a freshly built phase-A table of window c: the W chains
+// 2^(cw) xext (affine), then every multiple d * 2^(cw) xext, d <= 2^(c-1).
+void build_phase_a_table(vkzg_prover *p, PhaseATable &t, uint32_t c) {
+    hipStream_t s = p->stream;
+    t.c = c;
+    t.W = (256 + c - 1) / c;
+    t.H = 1u << (c - 1);
+    t.bytes = table_bytes_for(c);
+    t.table = p->arena.dalloc<G1AT>(t.bytes / sizeof(G1AT));
+    G1J *pwj = p->arena.dalloc<G1J>((size_t)t.W * 8192);
+    Fp *pw_prefix = p->arena.dalloc<Fp>((size_t)t.W * 8192);
+    G1A *pw = p->arena.dalloc<G1A>((size_t)t.W * 8192);
+    ck(launch_table_chains(pwj, p->xext, c, t.W, s));
+    ck(launch_jac_to_affine(pw, pwj, pw_prefix, (uint64_t)t.W * 8192, 16, s));
+    ck(hipStreamSynchronize(s));
+    p->arena.dfree(pwj);
+    p->arena.dfree(pw_prefix);
+    const uint32_t S = t.H < 64 ? t.H : 64;
+    const uint64_t total_threads = 8192ull * t.W * (t.H / S);
+    const uint32_t slice = 1u << 18;
+    G1J *scratch = p->arena.dalloc<G1J>((size_t)S * slice);
+    Fp *pref = p->arena.dalloc<Fp>((size_t)S * slice);
+    for (uint64_t t0 = 0; t0 < total_threads; t0 += slice) {
+        const uint32_t n = (uint32_t)std::min<uint64_t>(slice, total_threads - t0);
+        ck(launch_table_multiples(t.table, pw, c, t.W, t0, n, scratch, pref, s));
+    }
+    ck(hipStreamSynchronize(s));
+    p->arena.dfree(scratch);
+    p->arena.dfree(pref);
+    p->arena.dfree(pw);
+}
+
+PhaseATable phase_a_table(const vkzg_prover *p, uint32_t window_bits) {
+    if (window_bits >= kExtraWindowMin && window_bits <= kExtraWindowMax && window_bits != p->c) {
+        const PhaseATable &t = p->extra_tables[window_bits - kExtraWindowMin];
+        if (t.table) return t;
+    }
+    PhaseATable m;
+    m.c = p->c;
+    m.W = p->W;
+    m.H = p->H;
+    m.bytes = p->table_bytes;
+    m.table = p->table;
+    return m;
+}
+
 void build_tables(vkzg_prover *p, const uint8_t *setup) {
     hipStream_t s = p->stream;
     p->roots = p->arena.dalloc<Fr>(8192);
@@ -156,34 +202,13 @@ void build_tables(vkzg_prover *p, const uint8_t *setup) {
     // Phase-A multiples table.
     size_t free_b = 0, total_b = 0;
     ck(hipMemGetInfo(&free_b, &total_b));
-    p->c = choose_window(free_b);
-    p->W = (256 + p->c - 1) / p->c;
-    p->H = 1u << (p->c - 1);
-    p->table_bytes = table_bytes_for(p->c);
-    p->table = p->arena.dalloc<G1AT>(p->table_bytes / sizeof(G1AT));
-
-    G1J *pwj = p->arena.dalloc<G1J>((size_t)p->W * 8192);
-    Fp *pw_prefix = p->arena.dalloc<Fp>((size_t)p->W * 8192);
-    G1A *pw = p->arena.dalloc<G1A>((size_t)p->W * 8192);
-    ck(launch_table_chains(pwj, p->xext, p->c, p->W, s));
-    ck(launch_jac_to_affine(pw, pwj, pw_prefix, (uint64_t)p->W * 8192, 16, s));
-    ck(hipStreamSynchronize(s));
-    p->arena.dfree(pwj);
-    p->arena.dfree(pw_prefix);
-
-    const uint32_t S = p->H < 64 ? p->H : 64;
-    const uint64_t total_threads = 8192ull * p->W * (p->H / S);
-    const uint32_t slice = 1u << 18;
-    G1J *scratch = p->arena.dalloc<G1J>((size_t)S * slice);
-    Fp *pref = p->arena.dalloc<Fp>((size_t)S * slice);
-    for (uint64_t t0 = 0; t0 < total_threads; t0 += slice) {
-        const uint32_t n = (uint32_t)std::min<uint64_t>(slice, total_threads - t0);
-        ck(launch_table_multiples(p->table, pw, p->c, p->W, t0, n, scratch, pref, s));
-    }
-    ck(hipStreamSynchronize(s));
-    p->arena.dfree(scratch);
-    p->arena.dfree(pref);
-    p->arena.dfree(pw);
+    PhaseATable main_table;
+    build_phase_a_table(p, main_table, choose_window(free_b));
+    p->c = main_table.c;
+    p->W = main_table.W;
+    p->H = main_table.H;
+    p->table_bytes = main_table.bytes;
+    p->table = main_table.table;
     p->arena.dfree(xj);
     p->arena.dfree(prefix);
     p->arena.dfree(d_kpm);
@@ -278,7 +303,8 @@ bool tuning_valid(const Tuning &t) {
            t.norm_chunk <= 256 && t.splits >= 1 && t.splits <= (uint32_t)kMaxSplits &&
            (t.rec_waves == 1 || t.rec_waves == 2 || t.rec_waves == 4) && (t.rec_radix == 4 || t.rec_radix == 8) &&
            (t.ladder_splits == 1 || t.ladder_splits == 2 || t.ladder_splits == 4) &&
-           (t.reduce_lanes == 4 || t.reduce_lanes == 8 || t.reduce_lanes == 16 || t.reduce_lanes == 32);
+           (t.reduce_lanes == 4 || t.reduce_lanes == 8 || t.reduce_lanes == 16 || t.reduce_lanes == 32) &&
+           (t.window_bits == 0 || (t.window_bits >= kExtraWindowMin && t.window_bits <= kExtraWindowMax));
 }
 
 void enqueue_recovery_into(vkzg_prover *p, Fr *A, Fr *B, uint8_t *d_out, const uint8_t *d_cells,
@@ -320,7 +346,8 @@ void proofs_range(vkzg_prover *p, WorkSet &ws, uint32_t off, uint32_t nb, uint8_
     ck(launch_fk20_scalars(scal, d_blobs, blob_stride, err, p->roots, p->roots_inv, queue, coeffs,
                            tn.ladder_splits, nb, s));
     clk.mark("fk20_scalars");
-    ck(launch_phase_a(u, scal, p->table, p->c, p->W, nb * 128 * tn.ladder_splits, tn.groups,
+    const PhaseATable tab = phase_a_table(p, tn.window_bits);
+    ck(launch_phase_a(u, scal, tab.table, tab.c, tab.W, nb * 128 * tn.ladder_splits, tn.groups,
                       p->cus * tn.blocks_per_cu, queue, tn.force_checked, s));
     clk.mark("phase_a");
     ck(launch_ladder(lad, u, tn.groups, tn.ladder_splits, nb, s));
@@ -377,11 +404,13 @@ void split_over_streams(StreamSet &ss, uint32_t nb, uint32_t splits, hipStream_t
 }  // namespace
 
 namespace {
+// Every launch-shape field in its own bit range (graph cache key).
 uint64_t tuning_word(const Tuning &tn) {
     return (uint64_t)tn.splits | ((uint64_t)tn.groups << 8) | ((uint64_t)tn.blocks_per_cu << 16) |
-           ((uint64_t)tn.norm_chunk << 24) | ((uint64_t)tn.rec_waves << 40) |
-           ((uint64_t)tn.ladder_splits << 48) | ((uint64_t)tn.reduce_lanes << 52) |
-           ((uint64_t)(tn.rec_radix == 8) << 61) | ((uint64_t)(tn.force_checked != 0) << 62);
+           ((uint64_t)tn.rec_waves << 20) | ((uint64_t)tn.norm_chunk << 24) |
+           ((uint64_t)tn.ladder_splits << 36) | ((uint64_t)tn.reduce_lanes << 40) |
+           ((uint64_t)tn.rec_radix << 48) | ((uint64_t)tn.window_bits << 52) |
+           ((uint64_t)(tn.force_checked != 0) << 56);
 }
 }  // namespace
 
@@ -436,7 +465,7 @@ void enqueue_cells_and_proofs(vkzg_prover *p, WorkSet &ws, uint8_t *d_cells, uin
         }
     };
     if (!graphs) return body(s);
-    const GraphKey key{{3u | ((uint64_t)(uintptr_t)d_cells << 2), nb, tuning_word(tn) | ((uint64_t)blob_stride << 32),
+    const GraphKey key{{3u | ((uint64_t)(uintptr_t)d_cells << 2), nb | ((uint64_t)blob_stride << 32), tuning_word(tn),
                         (uint64_t)(uintptr_t)d_proofs, (uint64_t)(uintptr_t)d_blobs,
                         (uint64_t)(uintptr_t)&ws}};
     enqueue_maybe_graph(p, *graphs, key, s, off_clock, body);
@@ -489,6 +518,21 @@ void autotune(vkzg_prover *p) {
         }
         return best;
     };
+    // Candidate smaller phase-A windows (c - 2 and c - 4 of the main table, within
+    // 10..13): built now when they fit in half the free memory, kept only if
+    // some class picks one.
+    std::vector<uint32_t> windows;
+    for (uint32_t w : {p->c - 2, p->c - 4}) {
+        if (w < kExtraWindowMin || w > kExtraWindowMax || w >= p->c) continue;
+        PhaseATable &slot = p->extra_tables[w - kExtraWindowMin];
+        if (!slot.table) {
+            size_t free_b = 0, total_b = 0;
+            ck(hipMemGetInfo(&free_b, &total_b));
+            if (table_bytes_for(w) > free_b / 2) continue;
+            build_phase_a_table(p, slot, w);
+        }
+        windows.push_back(w);
+    }
     for (int c = 0; c < kTuneClasses; c++) {
         const uint32_t nb = std::min<uint32_t>(kTuneClassTop[c], nmax);
         if (c > 0 && kTuneClassTop[c - 1] >= nmax) {
@@ -513,8 +557,20 @@ void autotune(vkzg_prover *p) {
         for (uint32_t b : {1u, 2u}) { Tuning t = best; t.blocks_per_cu = b; consider(t); }
         for (uint32_t ch : {0u, 8u, 16u, 32u, 64u}) { Tuning t = best; t.norm_chunk = ch; consider(t); }
         if (nb >= 8) { Tuning t = best; t.splits = 2; consider(t); }
+        for (uint32_t w : windows) { Tuning t = best; t.window_bits = w; consider(t); }
+        { Tuning t = best; t.window_bits = 0; consider(t); }
+        if (best.window_bits) p->extra_named[best.window_bits - kExtraWindowMin] = true;
         std::lock_guard<std::mutex> lock(p->tune_mu);
         p->tune[c] = best;
+    }
+    // Smaller tables no tuning has ever named go (a named one may be in use by
+    // a tuning another thread has copied, so it stays until the prover goes).
+    for (uint32_t w : windows) {
+        PhaseATable &slot = p->extra_tables[w - kExtraWindowMin];
+        if (!p->extra_named[w - kExtraWindowMin] && slot.table) {
+            p->arena.dfree(slot.table);
+            slot = PhaseATable{};
+        }
     }
     // Recovery: NTT sub-transforms per pass block, on random cells with the
     // first 64 of 128 present.
@@ -912,6 +968,7 @@ vkzg_result vkzg_amd_get_tuning(const vkzg_prover *p, size_t num_blobs, vkzg_amd
     out->recovery_radix = t.rec_radix;
     out->ladder_splits = t.ladder_splits;
     out->reduce_lanes = t.reduce_lanes;
+    out->phase_a_window_bits = t.window_bits;
     return VKZG_OK;
 }
 
@@ -926,10 +983,39 @@ vkzg_result vkzg_amd_set_tuning(vkzg_prover *p, size_t num_blobs, const vkzg_amd
     t.rec_radix = in->recovery_radix;
     t.ladder_splits = in->ladder_splits;
     t.reduce_lanes = in->reduce_lanes;
+    t.window_bits = in->phase_a_window_bits == p->c ? 0u : in->phase_a_window_bits;
     if (!tuning_valid(t)) return VKZG_ERR_BADARGS;
     std::lock_guard<std::mutex> lock(p->mu);
+    if (t.window_bits) {  // only a resident smaller table
+        if (!p->extra_tables[t.window_bits - kExtraWindowMin].table) return VKZG_ERR_BADARGS;
+        p->extra_named[t.window_bits - kExtraWindowMin] = true;
+    }
     std::lock_guard<std::mutex> tlock(p->tune_mu);
     p->tune[tune_class((uint32_t)std::min<size_t>(num_blobs, 0xffffffffu))] = t;
+    return VKZG_OK;
+}
+
+vkzg_result vkzg_amd_load_window_table(vkzg_prover *p, uint32_t window_bits) {
+    if (!p || window_bits < kExtraWindowMin || window_bits > kExtraWindowMax) return VKZG_ERR_BADARGS;
+    std::lock_guard<std::mutex> lock(p->mu);
+    if (window_bits >= p->c) return VKZG_ERR_BADARGS;
+    DeviceGuard guard(p->device);
+    PhaseATable &slot = p->extra_tables[window_bits - kExtraWindowMin];
+    if (slot.table) {
+        p->extra_named[window_bits - kExtraWindowMin] = true;
+        return VKZG_OK;
+    }
+    try {
+        size_t free_b = 0, total_b = 0;
+        ck(hipMemGetInfo(&free_b, &total_b));
+        if (table_bytes_for(window_bits) > free_b) return VKZG_ERR_MALLOC;
+        build_phase_a_table(p, slot, window_bits);
+    } catch (HipFail &) {
+        if (slot.table) p->arena.dfree(slot.table);
+        slot = PhaseATable{};
+        return VKZG_ERR_GPU;
+    }
+    p->extra_named[window_bits - kExtraWindowMin] = true;
     return VKZG_OK;
 }
 
@@ -964,6 +1050,13 @@ vkzg_result vkzg_amd_prover_info(const vkzg_prover *p, vkzg_amd_info *out) {
     out->window_bits = p->c;
     out->windows = p->W;
     out->table_bytes = p->table_bytes;
+    out->resident_table_bytes = p->table_bytes;
+    out->extra_table_windows = 0;
+    for (uint32_t i = 0; i <= kExtraWindowMax - kExtraWindowMin; i++)
+        if (p->extra_tables[i].table) {
+            out->resident_table_bytes += p->extra_tables[i].bytes;
+            out->extra_table_windows |= 1u << (kExtraWindowMin + i);
+        }
     out->working_bytes = WorkSet::bytes_per_blob() * p->max_batch;
     out->last_splits = p->last_splits;
     out->graphs_enabled = p->use_graphs ? 1u : 0u;

@@ -252,15 +252,17 @@ VKZG_DEV Fr vanishing_product(Fr x, const uint8_t *present, bool want, const Fr 
         const uint2 bits = *reinterpret_cast<const uint2 *>(present + kCells * part);
         w[0] = bits.x, w[1] = bits.y;
     }
-    Fr prod = fr_one();
+    // four independent chains (cells k mod 4), so a lane's products overlap
+    Fr acc[4] = {fr_one(), fr_one(), fr_one(), fr_one()};
 #pragma unroll
     for (int k = 0; k < kCells; k++) {
         const bool pres = ((w[k >> 2] >> (8 * (k & 3))) & 0xffu) != 0;
         if (pres == want) {
             const uint32_t cell = kCells * part + (uint32_t)k;
-            prod = fr_mul(prod, fr_sub(x, roots[(64u * brev_bits(cell, 7)) & 8191u]));
+            acc[k & 3] = fr_mul(acc[k & 3], fr_sub(x, roots[(64u * brev_bits(cell, 7)) & 8191u]));
         }
     }
+    Fr prod = fr_mul(fr_mul(acc[0], acc[1]), fr_mul(acc[2], acc[3]));
 #pragma unroll
     for (int m = 1; m < PARTS; m <<= 1) prod = fr_mul(prod, fr_shfl_xor(prod, m));
     return prod;
@@ -523,18 +525,24 @@ __global__ void __launch_bounds__(256, 2)
     if (IN_MODE == 0) {
         // Z(omega^i) for i & 127 = n1 + 64 (j & 1): value vv = c + 2 (j & 1)
         // of the 4 this wave computes (k = 2q + (vv & 1) + 64 (vv >> 1)).
+        // x[j + 32r] sits at wire 128 brev6(n1) + 4 brev5(j) + brev2(r): the
+        // loads go out first, the vanishing value is computed meanwhile
+        const uint32_t *cw = reinterpret_cast<const uint32_t *>(in) +
+                             ((size_t)blob * 8192 + 128u * brev_bits(n1, 6) + 4u * brev_bits(j, 5)) * 8;
+        uint4 raw[4][2];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const uint32_t w = brev_bits((uint32_t)r, 2);
+            raw[r][0] = *reinterpret_cast<const uint4 *>(cw + 8 * w);
+            raw[r][1] = *reinterpret_cast<const uint4 *>(cw + 8 * w + 4);
+        }
         const uint32_t vv = lane >> 4;
         const uint32_t k = 2u * q + (vv & 1u) + 64u * (vv >> 1);
         const Fr zd = vanishing_product<16>(roots[(64u * k) & 8191u], present + 128ull * blob, false, roots, lane);
         const Fr zr = fr_mul(fr_shfl(zd, (int)((c + 2u * (j & 1u)) * 16u)), fr_const(cfr::R2));
-        // x[j + 32r] sits at wire 128 brev6(n1) + 4 brev5(j) + brev2(r)
-        const uint32_t *cw = reinterpret_cast<const uint32_t *>(in) +
-                             ((size_t)blob * 8192 + 128u * brev_bits(n1, 6) + 4u * brev_bits(j, 5)) * 8;
 #pragma unroll
         for (int r = 0; r < 4; r++) {
-            const uint32_t w = brev_bits((uint32_t)r, 2);
-            const uint4 lo = *reinterpret_cast<const uint4 *>(cw + 8 * w);
-            const uint4 hi = *reinterpret_cast<const uint4 *>(cw + 8 * w + 4);
+            const uint4 lo = raw[r][0], hi = raw[r][1];
             const uint32_t words[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
             Fr e = fr_from_be_words(words);
             reduce_once<8>(e.v, e.v, cfr::P);
@@ -568,16 +576,19 @@ __global__ void __launch_bounds__(256, 2)
 #pragma unroll
     for (int r = 0; r < 4; r++) v[r] = ld_fr(src + 64u * (uint32_t)r + lane);
     const bool b_inverse = STAGE != 1;
-    pass_b4(v, Dir(b_inverse ? roots_inv : roots), x, lane);
-    // lane L = 4j + cc holds X[k2 + 128 (j + 16 r)], k2 = 2q + (cc & 1) + 64 (cc >> 1)
+    // lane L = 4j + cc ends pass B with X[k2 + 128 (j + 16 r)], k2 = 2q + (cc & 1) + 64 (cc >> 1)
     const uint32_t cc = lane & 3u, jb = lane >> 2;
     const uint32_t k2 = 2u * q + (cc & 1u) + 64u * (cc >> 1);
-    if (STAGE == 1) {
+    Fr izc;
+    if (STAGE == 1) {  // before pass B: overlaps the loads above
         const uint32_t vv = lane >> 4;
         const uint32_t kv = 2u * q + (vv & 1u) + 64u * (vv >> 1);
         const Fr y = fr_mul(roots[(64u * kv) & 8191u], fr_const(cfr::SEVEN_POW64_M));
         const Fr p = vanishing_product<16>(y, present + 128ull * blob, true, roots, lane);
-        const Fr izc = fr_mul(fr_shfl(p, (int)(cc * 16u)), fr_const(cfr::INV_SEVEN_POW8192_MINUS_1_M));
+        izc = fr_mul(fr_shfl(p, (int)(cc * 16u)), fr_const(cfr::INV_SEVEN_POW8192_MINUS_1_M));
+    }
+    pass_b4(v, Dir(b_inverse ? roots_inv : roots), x, lane);
+    if (STAGE == 1) {
 #pragma unroll
         for (int r = 0; r < 4; r++) v[r] = fr_mul(v[r], izc);
     } else {

@@ -33,11 +33,23 @@ struct Tuning {
     uint32_t rec_radix = 4;      // recovery register radix: 4 (4 values per lane) or 8 (8 values)
     uint32_t ladder_splits = 1;  // ladder segments (phase A also computes their starts): 1, 2 or 4
     uint32_t reduce_lanes = 8;   // bucket-reduction lanes per output: 4, 8, 16 or 32
+    uint32_t window_bits = 0;    // phase-A table: 0 = the prover's main table, else a resident
+                                 // smaller one of this window (10..13, below the main one)
     // Not a tuning: the test switch that sends phase A, phase B and the bucket
     // reduction down their checked recompute (vkzg_amd_debug_force_checked).
     // Set by tuning_for from the prover; kept here so graph keys include it.
     uint32_t force_checked = 0;
 };
+
+// A resident phase-A multiples table of window c: W = ceil(256 / c) windows
+// of H = 2^(c-1) multiples (table[128][64][W][H], one 128-byte line each).
+struct PhaseATable {
+    uint32_t c = 0, W = 0, H = 0;
+    uint64_t bytes = 0;
+    G1AT *table = nullptr;
+};
+// Smaller tables the autotuner may keep beside the main one, by window.
+constexpr uint32_t kExtraWindowMin = 10, kExtraWindowMax = 13;
 
 // Batch-size classes: <= 1, <= 4, <= 16, <= 64 and more blobs.
 constexpr int kTuneClasses = 5;
@@ -244,9 +256,15 @@ struct vkzg_prover {
     vkzg_amd::DeviceArena arena;
     vkzg_amd::Fr *roots = nullptr, *roots_inv = nullptr, *shift_fwd = nullptr, *shift_inv = nullptr;
     vkzg_amd::G1A *srs = nullptr, *xext = nullptr;
-    vkzg_amd::G1AT *table = nullptr;
+    vkzg_amd::G1AT *table = nullptr;  // the main phase-A table (window c)
     uint32_t c = 0, W = 0, H = 0;
     uint64_t table_bytes = 0;
+    // Smaller phase-A tables kept by the autotuner (window kExtraWindowMin + i).
+    // A slot is filled under mu before any tuning names it and is freed only
+    // if no tuning ever named it, so a tuning copied by another thread always
+    // finds its table.
+    vkzg_amd::PhaseATable extra_tables[vkzg_amd::kExtraWindowMax - vkzg_amd::kExtraWindowMin + 1];
+    bool extra_named[vkzg_amd::kExtraWindowMax - vkzg_amd::kExtraWindowMin + 1] = {};
     uint32_t *kb_items = nullptr, *kb_offsets = nullptr;
     uint32_t kb_count = 0;
 
@@ -300,6 +318,9 @@ struct StageClock {
 
 // The tuning for a batch of nb blobs ($VKZG_AMD_SPLITS overrides its splits).
 Tuning tuning_for(const vkzg_prover *p, uint32_t nb);
+
+// The phase-A table a tuning names (window_bits 0: the main one).
+PhaseATable phase_a_table(const vkzg_prover *p, uint32_t window_bits);
 
 // Enqueue a pipeline through `cache` when graphs are on and stage timing is
 // off (timing records events between stages), else launch it directly.

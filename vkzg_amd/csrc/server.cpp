@@ -307,9 +307,17 @@ vkzg_result vkzg_amd_server_new(vkzg_amd_server **out, vkzg_prover *p,
     const uint64_t budget = opts ? opts->hbm_budget_bytes : 0;
     const uint64_t per_blob = WorkSet::bytes_per_blob();
     if (budget) {
-        // Admission by HBM budget: as many slots as fit, and smaller batches
-        // when even one full slot would not.
-        const uint64_t fit_blobs = budget / per_blob;
+        // Admission by HBM budget: the prover's resident phase-A tables count
+        // against it; then as many slots as fit, and smaller batches when
+        // even one full slot would not.
+        uint64_t tables = 0;
+        {
+            std::lock_guard<std::mutex> lock(p->mu);
+            tables = p->table_bytes;
+            for (const auto &t : p->extra_tables) tables += t.bytes;
+        }
+        if (budget <= tables) return VKZG_ERR_MALLOC;
+        const uint64_t fit_blobs = (budget - tables) / per_blob;
         if (fit_blobs == 0) return VKZG_ERR_MALLOC;
         if (fit_blobs < sv->cap) sv->cap = (uint32_t)fit_blobs;
         nslots = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(nslots, fit_blobs / sv->cap));
