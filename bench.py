@@ -27,6 +27,9 @@ R = 0x73EDA753299D7D483339D80809A1D80553BDA402FFFE5BFEFFFFFFFF00000001
 PEAK_INT = 256 * 2.4e9 * 64  # CU x clock x lanes: lane-ops/s (v_mad_u64_u32 issues at full rate)
 FP_M = 300                    # limb-MACs per Fp multiplication (12-limb CIOS)
 FR_M = 136                    # limb-MACs per Fr multiplication (8-limb CIOS)
+# Algorithmic work of one blob's recovery (lean form: 6 NTT_8192, pointwise
+# products, batch-inverted division, vanishing polynomial, I/O), in limb-MACs.
+W_REC = 6.38e7
 LAD_POS = 17                  # ladder positions 2^(8d) u, d < 17 (GLV halves of the phase-B scalars)
 METRIC = "proof blobs/s @256-blob batch; recovery blobs/s; p50/p99 serve latency"
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -412,16 +415,30 @@ def main():
                  "phase_a_window_bits": info["window_bits"],
                  "table_gib": info["table_bytes"] / 2**30, "setup_ms": info["setup_ms"]}
     if not a.no_secondary:
-        # recoverCellsBatch, 64 blobs, exactly 64 of 128 cells present
+        # recoverCellsBatch, 64 blobs, exactly 64 of 128 cells present.  The
+        # cells are the extension of the seeded blobs (this library's cells
+        # and proofs call); the missing cells hold random bytes, and every
+        # recovered blob is compared with its original cells afterwards (the
+        # encode -> erase -> decode round trip).
         rng = random.Random(a.seed + 2)
         nr = 64
         from vkzg_amd import BYTES_PER_CELL
-        cells = rng.randbytes(nr * 128 * BYTES_PER_CELL)
-        masks = []
-        for _ in range(nr):
-            keep = set(rng.sample(range(128), 64))
-            masks.append(bytes(1 if k in keep else 0 for k in range(128)))
-        present = b"".join(masks)
+        cell_bytes = 128 * BYTES_PER_CELL
+        ncells = min(nb, 128)
+        full = b"".join(prover.compute_cells_and_kzg_proofs_batch(blobs[i * 131072:(i + 64) * 131072],
+                                                                  min(64, ncells - i))[0]
+                        for i in range(0, ncells, 64))
+
+        def erase(b, keep):
+            c = bytearray(rng.randbytes(cell_bytes))
+            for k in keep:
+                c[2048 * k:2048 * (k + 1)] = full[b * cell_bytes + 2048 * k:b * cell_bytes + 2048 * (k + 1)]
+            return bytes(c), bytes(1 if k in keep else 0 for k in range(128))
+
+        pairs = [erase(b % ncells, set(rng.sample(range(128), 64))) for b in range(nr)]
+        cells = b"".join(p_[0] for p_ in pairs)
+        present = b"".join(p_[1] for p_ in pairs)
+        want = b"".join(full[(b % ncells) * cell_bytes:((b % ncells) + 1) * cell_bytes] for b in range(nr))
         d_cells = hip.DeviceBuffer.from_bytes(cells)
         d_present = hip.DeviceBuffer.from_bytes(present)
         d_out = hip.DeviceBuffer(len(cells))
@@ -434,8 +451,12 @@ def main():
             prover.recover_cells_device(d_out.ptr, d_cells.ptr, d_present.ptr, present, nr,
                                         stream.handle)
         rec_dt = (time.perf_counter() - t0) / reps
+        if d_out.to_bytes() != want:
+            raise SystemExit("recovery of the 64-blob workload returned wrong cells")
         secondary["recovery_64_blobs_per_s"] = nr / rec_dt
+        secondary["recovery_64_frac_of_int_peak"] = nr / rec_dt * W_REC / PEAK_INT
         secondary["recovery_64_ms"] = rec_dt * 1e3
+        secondary["recovery_checked"] = "every blob of the 64-blob and the last uneven call equals its original cells"
         prover.set_stage_timing(True)
         prover.recover_cells_device(d_out.ptr, d_cells.ptr, d_present.ptr, present, nr,
                                     stream.handle)
@@ -454,14 +475,11 @@ def main():
         # uneven recovery: call sizes U[1,128], per-blob present count U[64,127]
         calls = [rng.randint(1, 128) for _ in range(12)]
         maxn = max(calls)
-        ucells = rng.randbytes(maxn * 128 * BYTES_PER_CELL)
+        pairs = [erase(b % ncells, set(rng.sample(range(128), rng.randint(64, 127)))) for b in range(maxn)]
+        ucells = b"".join(p_[0] for p_ in pairs)
+        umask = b"".join(p_[1] for p_ in pairs)
         d_ucells = hip.DeviceBuffer.from_bytes(ucells)
         d_uout = hip.DeviceBuffer(len(ucells))
-        masks = []
-        for _ in range(maxn):
-            keep = set(rng.sample(range(128), rng.randint(64, 127)))
-            masks.append(bytes(1 if k in keep else 0 for k in range(128)))
-        umask = b"".join(masks)
         d_umask = hip.DeviceBuffer.from_bytes(umask)
         for n in calls:  # first use of each size captures its graph
             prover.recover_cells_device(d_uout.ptr, d_ucells.ptr, d_umask.ptr, umask[:128 * n], n,
@@ -471,6 +489,11 @@ def main():
             prover.recover_cells_device(d_uout.ptr, d_ucells.ptr, d_umask.ptr, umask[:128 * n], n,
                                         stream.handle)
         secondary["uneven_recovery_blobs_per_s"] = sum(calls) / (time.perf_counter() - t0)
+        secondary["uneven_recovery_frac_of_int_peak"] = secondary["uneven_recovery_blobs_per_s"] * W_REC / PEAK_INT
+        last = calls[-1]
+        if d_uout.to_bytes(last * cell_bytes) != b"".join(
+                full[(b % ncells) * cell_bytes:((b % ncells) + 1) * cell_bytes] for b in range(last)):
+            raise SystemExit("uneven recovery returned wrong cells")
         secondary["uneven_recovery_calls"] = calls
         # plain form: one blob per call through the host C ABI
         one = blobs[:131072]
