@@ -99,6 +99,122 @@ VKZG_DEV void mac_vs(uint64_t &acc, uint32_t &hi, uint32_t x, uint32_t y) {
         : "v"(x), "s"(y)
         : "vcc");
 }
+// Both products of one FIPS step, a[j] * b[i - j] and m[j] * p[i - j], in one
+// block: the compiler separates consecutive asm blocks with an s_nop.
+VKZG_DEV void mac_pair(uint64_t &acc, uint32_t &hi, uint32_t x, uint32_t y, uint32_t mx,
+                       uint32_t py) {
+    asm("v_mad_u64_u32 %0, vcc, %2, %3, %0\n\t"
+        "v_addc_co_u32_e32 %1, vcc, 0, %1, vcc\n\t"
+        "v_mad_u64_u32 %0, vcc, %4, %5, %0\n\t"
+        "v_addc_co_u32_e32 %1, vcc, 0, %1, vcc"
+        : "+v"(acc), "+v"(hi)
+        : "v"(x), "v"(y), "v"(mx), "s"(py)
+        : "vcc");
+}
+
+// Fr (mod r) carry chains written out.  The carry of v_add_co/v_addc and the
+// borrow of v_sub_co/v_subb pass through VCC, and a VALU carry read of VCC
+// needs no wait states; VCC read as a v_cndmask select does, so one s_nop
+// (two wait states) sits before the selects.  A VALU instruction reads at
+// most one scalar value (VCC, an SGPR or a literal), so r's limbs 2..7 enter
+// the borrow chain from VGPRs; limbs 0 and 1 are the inline constants 1 and
+// -1.  The compiler's own lowering of the same chains goes through 64-bit
+// adds, about twice the instructions.
+//
+// r = t - r_mod when t >= r_mod, else t (t < 2^256).
+VKZG_DEV void fr_reduce_once(uint32_t *r, const uint32_t *t) {
+    uint32_t u[8];
+    asm("v_subrev_co_u32_e32 %[u0], vcc, 1, %[t0]\n\t"
+        "v_subbrev_co_u32_e32 %[u1], vcc, -1, %[t1], vcc\n\t"
+        "v_subb_co_u32_e32 %[u2], vcc, %[t2], %[p2], vcc\n\t"
+        "v_subb_co_u32_e32 %[u3], vcc, %[t3], %[p3], vcc\n\t"
+        "v_subb_co_u32_e32 %[u4], vcc, %[t4], %[p4], vcc\n\t"
+        "v_subb_co_u32_e32 %[u5], vcc, %[t5], %[p5], vcc\n\t"
+        "v_subb_co_u32_e32 %[u6], vcc, %[t6], %[p6], vcc\n\t"
+        "v_subb_co_u32_e32 %[u7], vcc, %[t7], %[p7], vcc\n\t"
+        "s_nop 1\n\t"
+        "v_cndmask_b32_e32 %[r0], %[u0], %[t0], vcc\n\t"
+        "v_cndmask_b32_e32 %[r1], %[u1], %[t1], vcc\n\t"
+        "v_cndmask_b32_e32 %[r2], %[u2], %[t2], vcc\n\t"
+        "v_cndmask_b32_e32 %[r3], %[u3], %[t3], vcc\n\t"
+        "v_cndmask_b32_e32 %[r4], %[u4], %[t4], vcc\n\t"
+        "v_cndmask_b32_e32 %[r5], %[u5], %[t5], vcc\n\t"
+        "v_cndmask_b32_e32 %[r6], %[u6], %[t6], vcc\n\t"
+        "v_cndmask_b32_e32 %[r7], %[u7], %[t7], vcc"
+        : [r0] "=&v"(r[0]), [r1] "=&v"(r[1]), [r2] "=&v"(r[2]), [r3] "=&v"(r[3]),
+          [r4] "=&v"(r[4]), [r5] "=&v"(r[5]), [r6] "=&v"(r[6]), [r7] "=&v"(r[7]),
+          [u0] "=&v"(u[0]), [u1] "=&v"(u[1]), [u2] "=&v"(u[2]), [u3] "=&v"(u[3]),
+          [u4] "=&v"(u[4]), [u5] "=&v"(u[5]), [u6] "=&v"(u[6]), [u7] "=&v"(u[7])
+        : [t0] "v"(t[0]), [t1] "v"(t[1]), [t2] "v"(t[2]), [t3] "v"(t[3]), [t4] "v"(t[4]),
+          [t5] "v"(t[5]), [t6] "v"(t[6]), [t7] "v"(t[7]), [p2] "v"(cfr::P[2]),
+          [p3] "v"(cfr::P[3]), [p4] "v"(cfr::P[4]), [p5] "v"(cfr::P[5]), [p6] "v"(cfr::P[6]),
+          [p7] "v"(cfr::P[7])
+        : "vcc");
+}
+
+// r = a + b mod r_mod for a, b < r_mod (a + b < 2^256: no carry out).
+VKZG_DEV void fr_add_limbs(uint32_t *r, const uint32_t *a, const uint32_t *b) {
+    uint32_t t[8];
+    asm("v_add_co_u32_e32 %[t0], vcc, %[a0], %[b0]\n\t"
+        "v_addc_co_u32_e32 %[t1], vcc, %[a1], %[b1], vcc\n\t"
+        "v_addc_co_u32_e32 %[t2], vcc, %[a2], %[b2], vcc\n\t"
+        "v_addc_co_u32_e32 %[t3], vcc, %[a3], %[b3], vcc\n\t"
+        "v_addc_co_u32_e32 %[t4], vcc, %[a4], %[b4], vcc\n\t"
+        "v_addc_co_u32_e32 %[t5], vcc, %[a5], %[b5], vcc\n\t"
+        "v_addc_co_u32_e32 %[t6], vcc, %[a6], %[b6], vcc\n\t"
+        "v_addc_co_u32_e32 %[t7], vcc, %[a7], %[b7], vcc"
+        : [t0] "=&v"(t[0]), [t1] "=&v"(t[1]), [t2] "=&v"(t[2]), [t3] "=&v"(t[3]),
+          [t4] "=&v"(t[4]), [t5] "=&v"(t[5]), [t6] "=&v"(t[6]), [t7] "=&v"(t[7])
+        : [a0] "v"(a[0]), [a1] "v"(a[1]), [a2] "v"(a[2]), [a3] "v"(a[3]), [a4] "v"(a[4]),
+          [a5] "v"(a[5]), [a6] "v"(a[6]), [a7] "v"(a[7]), [b0] "v"(b[0]), [b1] "v"(b[1]),
+          [b2] "v"(b[2]), [b3] "v"(b[3]), [b4] "v"(b[4]), [b5] "v"(b[5]), [b6] "v"(b[6]),
+          [b7] "v"(b[7])
+        : "vcc");
+    fr_reduce_once(r, t);
+}
+
+// r = a - b mod r_mod for a, b < r_mod: the difference, plus r_mod masked by
+// the final borrow.
+VKZG_DEV void fr_sub_limbs(uint32_t *r, const uint32_t *a, const uint32_t *b) {
+    uint32_t t[8], mask, c;
+    asm("v_sub_co_u32_e32 %[t0], vcc, %[a0], %[b0]\n\t"
+        "v_subb_co_u32_e32 %[t1], vcc, %[a1], %[b1], vcc\n\t"
+        "v_subb_co_u32_e32 %[t2], vcc, %[a2], %[b2], vcc\n\t"
+        "v_subb_co_u32_e32 %[t3], vcc, %[a3], %[b3], vcc\n\t"
+        "v_subb_co_u32_e32 %[t4], vcc, %[a4], %[b4], vcc\n\t"
+        "v_subb_co_u32_e32 %[t5], vcc, %[a5], %[b5], vcc\n\t"
+        "v_subb_co_u32_e32 %[t6], vcc, %[a6], %[b6], vcc\n\t"
+        "v_subb_co_u32_e32 %[t7], vcc, %[a7], %[b7], vcc\n\t"
+        "s_nop 1\n\t"
+        "v_cndmask_b32_e64 %[m], 0, -1, vcc"
+        : [t0] "=&v"(t[0]), [t1] "=&v"(t[1]), [t2] "=&v"(t[2]), [t3] "=&v"(t[3]),
+          [t4] "=&v"(t[4]), [t5] "=&v"(t[5]), [t6] "=&v"(t[6]), [t7] "=&v"(t[7]), [m] "=&v"(mask)
+        : [a0] "v"(a[0]), [a1] "v"(a[1]), [a2] "v"(a[2]), [a3] "v"(a[3]), [a4] "v"(a[4]),
+          [a5] "v"(a[5]), [a6] "v"(a[6]), [a7] "v"(a[7]), [b0] "v"(b[0]), [b1] "v"(b[1]),
+          [b2] "v"(b[2]), [b3] "v"(b[3]), [b4] "v"(b[4]), [b5] "v"(b[5]), [b6] "v"(b[6]),
+          [b7] "v"(b[7])
+        : "vcc");
+    asm("v_and_b32_e32 %[c], 1, %[m]\n\t"
+        "v_add_co_u32_e32 %[r0], vcc, %[c], %[t0]\n\t"
+        "v_addc_co_u32_e32 %[r1], vcc, %[m], %[t1], vcc\n\t"
+        "v_and_b32_e32 %[c], 0xfffe5bfe, %[m]\n\t"
+        "v_addc_co_u32_e32 %[r2], vcc, %[c], %[t2], vcc\n\t"
+        "v_and_b32_e32 %[c], 0x53bda402, %[m]\n\t"
+        "v_addc_co_u32_e32 %[r3], vcc, %[c], %[t3], vcc\n\t"
+        "v_and_b32_e32 %[c], 0x09a1d805, %[m]\n\t"
+        "v_addc_co_u32_e32 %[r4], vcc, %[c], %[t4], vcc\n\t"
+        "v_and_b32_e32 %[c], 0x3339d808, %[m]\n\t"
+        "v_addc_co_u32_e32 %[r5], vcc, %[c], %[t5], vcc\n\t"
+        "v_and_b32_e32 %[c], 0x299d7d48, %[m]\n\t"
+        "v_addc_co_u32_e32 %[r6], vcc, %[c], %[t6], vcc\n\t"
+        "v_and_b32_e32 %[c], 0x73eda753, %[m]\n\t"
+        "v_addc_co_u32_e32 %[r7], vcc, %[c], %[t7], vcc"
+        : [r0] "=&v"(r[0]), [r1] "=&v"(r[1]), [r2] "=&v"(r[2]), [r3] "=&v"(r[3]),
+          [r4] "=&v"(r[4]), [r5] "=&v"(r[5]), [r6] "=&v"(r[6]), [r7] "=&v"(r[7]), [c] "=&v"(c)
+        : [t0] "v"(t[0]), [t1] "v"(t[1]), [t2] "v"(t[2]), [t3] "v"(t[3]), [t4] "v"(t[4]),
+          [t5] "v"(t[5]), [t6] "v"(t[6]), [t7] "v"(t[7]), [m] "v"(mask)
+        : "vcc");
+}
 
 // Montgomery multiplication by finely integrated product scanning (FIPS):
 // column k of a*b and of m*p are summed together in a 96-bit accumulator,
@@ -114,10 +230,7 @@ VKZG_DEV void mont_mul(uint32_t *r, const uint32_t *a, const uint32_t *b, const 
 #pragma unroll
     for (int i = 0; i < N; i++) {
 #pragma unroll
-        for (int j = 0; j < i; j++) {
-            mac_vv(acc, hi, a[j], b[i - j]);
-            mac_vs(acc, hi, m[j], p[i - j]);
-        }
+        for (int j = 0; j < i; j++) mac_pair(acc, hi, a[j], b[i - j], m[j], p[i - j]);
         mac_vv(acc, hi, a[i], b[0]);
         m[i] = (uint32_t)acc * n0;
         mac_vs(acc, hi, m[i], p[0]);
@@ -128,15 +241,16 @@ VKZG_DEV void mont_mul(uint32_t *r, const uint32_t *a, const uint32_t *b, const 
 #pragma unroll
     for (int i = N; i < 2 * N; i++) {
 #pragma unroll
-        for (int j = i - N + 1; j < N; j++) {
-            mac_vv(acc, hi, a[j], b[i - j]);
-            mac_vs(acc, hi, m[j], p[i - j]);
-        }
+        for (int j = i - N + 1; j < N; j++) mac_pair(acc, hi, a[j], b[i - j], m[j], p[i - j]);
         t[i - N] = (uint32_t)acc;
         acc = (acc >> 32) | ((uint64_t)hi << 32);
         hi = 0;
     }
-    reduce_once<N>(r, t, p);
+    if constexpr (N == 8) {
+        fr_reduce_once(r, t);
+    } else {
+        reduce_once<N>(r, t, p);
+    }
 }
 
 // The same product by operand scanning (CIOS) in plain C++; the test hooks
@@ -226,12 +340,12 @@ VKZG_DEV Fr fr_mul(const Fr &a, const Fr &b) {
 VKZG_DEV Fr fr_sqr(const Fr &a) { return fr_mul(a, a); }
 VKZG_DEV Fr fr_add(const Fr &a, const Fr &b) {
     Fr r;
-    add_mod<8>(r.v, a.v, b.v, cfr::P);
+    fr_add_limbs(r.v, a.v, b.v);
     return r;
 }
 VKZG_DEV Fr fr_sub(const Fr &a, const Fr &b) {
     Fr r;
-    sub_mod<8>(r.v, a.v, b.v, cfr::P);
+    fr_sub_limbs(r.v, a.v, b.v);
     return r;
 }
 VKZG_DEV Fr fr_neg(const Fr &a) { return fr_sub(fr_zero(), a); }
@@ -250,8 +364,8 @@ VKZG_DEV Fr fr_from_canonical(const Fr &a) { return fr_mul(a, fr_const(cfr::R2))
 // non-canonical cell bytes mod r).
 VKZG_DEV Fr fr_from_any(const Fr &a) {
     Fr t;
-    reduce_once<8>(t.v, a.v, cfr::P);
-    reduce_once<8>(t.v, t.v, cfr::P);
+    fr_reduce_once(t.v, a.v);
+    fr_reduce_once(t.v, t.v);
     return fr_from_canonical(t);
 }
 

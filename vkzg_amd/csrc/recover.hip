@@ -304,8 +304,8 @@ __global__ void __launch_bounds__(256, 2)
             const uint4 hi = *reinterpret_cast<const uint4 *>(cw + 8 * w + 4);
             const uint32_t words[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
             Fr e = fr_from_be_words(words);
-            reduce_once<8>(e.v, e.v, cfr::P);
-            reduce_once<8>(e.v, e.v, cfr::P);
+            fr_reduce_once(e.v, e.v);
+            fr_reduce_once(e.v, e.v);
             v[b] = fr_mul(e, zr);
         }
         pass_a(v, Dir(roots_inv), x, lane, n1);
@@ -545,8 +545,8 @@ __global__ void __launch_bounds__(256, 2)
             const uint4 lo = raw[r][0], hi = raw[r][1];
             const uint32_t words[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
             Fr e = fr_from_be_words(words);
-            reduce_once<8>(e.v, e.v, cfr::P);
-            reduce_once<8>(e.v, e.v, cfr::P);
+            fr_reduce_once(e.v, e.v);
+            fr_reduce_once(e.v, e.v);
             v[r] = fr_mul(e, zr);
         }
         pass_a4(v, Dir(roots_inv), x, lane, n1);
