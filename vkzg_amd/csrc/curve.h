@@ -248,11 +248,10 @@ VKZG_DEV G1X g1x_dbl_core(const G1X &p) {
     r.zz = fp_mul(V, p.zz);
     r.zzz = fp_mul(W, p.zzz);
     const Fp S = fp_mul(p.x, V);
-    const Fp WY = fp_mul(W, p.y);
     const Fp X2 = fp_sqr(p.x);
     const Fp M = fp_add(fp_dbl(X2), X2);
     r.x = fp_sub(fp_sqr(M), fp_dbl(S));
-    r.y = fp_sub(fp_mul(M, fp_sub(S, r.x)), WY);
+    r.y = fp_mul_sum(M, fp_sub_lazy(S, r.x), p.y, fp_neg_lazy(W));
     return r;
 }
 
@@ -305,7 +304,7 @@ VKZG_DEV G1X g1x_madd_acc(const G1X &a, const G1A &b, bool neg, bool &exc) {
     const Fp Q = fp_mul(a.x, PP);
     G1X r;
     r.x = fp_sub(fp_sub(fp_sqr(R), PPP), fp_dbl(Q));
-    r.y = fp_sub(fp_mul(R, fp_sub_lazy(Q, r.x)), fp_mul(a.y, PPP));
+    r.y = fp_mul_sum(R, fp_sub_lazy(Q, r.x), a.y, fp_neg_lazy(PPP));
     r.zz = fp_mul(a.zz, PP);
     r.zzz = fp_mul(a.zzz, PPP);
     return r;
@@ -351,7 +350,7 @@ VKZG_DEV G1X g1x_add_core(const G1X &a, const G1X &b, bool &exc) {
     r.zz = fp_mul(ZZ12, PP);
     r.zzz = fp_mul(ZZZ12, PPP);
     r.x = fp_sub(fp_sub(fp_sqr(R), PPP), fp_dbl(Q));
-    r.y = fp_sub(fp_mul(R, fp_sub_lazy(Q, r.x)), fp_mul(S1, PPP));
+    r.y = fp_mul_sum(R, fp_sub_lazy(Q, r.x), S1, fp_neg_lazy(PPP));
     return r;
 }
 

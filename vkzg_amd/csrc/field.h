@@ -529,6 +529,53 @@ VKZG_DEV Fp fp_neg_lazy_if(const Fp &a, bool neg) {
     for (int i = 0; i < kFpLimbs; i++) r.v[i] = neg ? cfp::P4S[i] - a.v[i] : a.v[i];
     return r;
 }
+// 4p - a limb by limb for a normalised a < 2p: limbs below 2^29, value in
+// (2p, 4p], congruent to -a.
+VKZG_DEV Fp fp_neg_lazy(const Fp &a) {
+    Fp r;
+#pragma unroll
+    for (int i = 0; i < kFpLimbs; i++) r.v[i] = cfp::P4S[i] - a.v[i];
+    return r;
+}
+
+// a*b + c*d with one Montgomery reduction: the two products and m*p share
+// fp_mul's column accumulators, so the second product costs its 196 limb
+// products and no reduction of its own.  Contract: a and b lazy (limbs below
+// 3 * 2^28, values below 6p), c normalised (limbs below 2^28, value below
+// 2p), d limbs below 2^29 and value below 4p.  A column then stays below
+// 14 * (9 + 2 + 1) * 2^56 + carry < 2^63.4, and a*b + c*d < 44p^2 keeps the
+// result below p + 44p^2/R < 2p.
+VKZG_DEV Fp fp_mul_sum(const Fp &a, const Fp &b, const Fp &c, const Fp &d) {
+    uint32_t m[kFpLimbs];
+    uint64_t acc = 0;
+#pragma unroll
+    for (int i = 0; i < kFpLimbs; i++) {
+#pragma unroll
+        for (int j = 0; j < i; j++) {
+            fp_mac(acc, a.v[j], b.v[i - j]);
+            fp_mac(acc, c.v[j], d.v[i - j]);
+            fp_mac(acc, m[j], cfp::P[i - j]);
+        }
+        fp_mac(acc, a.v[i], b.v[0]);
+        fp_mac(acc, c.v[i], d.v[0]);
+        m[i] = ((uint32_t)acc * cfp::N0) & kFpMask;
+        fp_mac(acc, m[i], cfp::P[0]);
+        acc >>= 28;
+    }
+    Fp r;
+#pragma unroll
+    for (int i = kFpLimbs; i < 2 * kFpLimbs; i++) {
+#pragma unroll
+        for (int j = i - kFpLimbs + 1; j < kFpLimbs; j++) {
+            fp_mac(acc, a.v[j], b.v[i - j]);
+            fp_mac(acc, c.v[j], d.v[i - j]);
+            fp_mac(acc, m[j], cfp::P[i - j]);
+        }
+        r.v[i - kFpLimbs] = (uint32_t)acc & kFpMask;
+        acc >>= 28;
+    }
+    return r;
+}
 
 // a + b in [0, 2p): s = a + b and t = s - 2p with their own signed carry
 // chains; t when non-negative.
