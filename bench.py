@@ -372,7 +372,7 @@ def main():
     prover.set_stage_timing(False)
     stage_ms = {k: v / a.steps for k, v in stage_tot.items()}
 
-    items_per_output = info["phase_b_items"] / 2.0  # each output uses one half's item list
+    items_per_output = info["phase_b_items"] / 128.0  # one blob's items over its 128 outputs
     work = stage_work(info, nb, items_per_output)
     frac = {k: (work[k] / (stage_ms[k] * 1e-3)) / PEAK_INT for k in work if stage_ms.get(k)}
     dom = max(stage_ms, key=stage_ms.get)

@@ -128,7 +128,7 @@ typedef struct {
     uint64_t graph_replays;    /* calls served by replaying a captured graph */
     uint32_t autotuned;        /* 1 after vkzg_amd_autotune */
     double autotune_ms;        /* its wall time */
-    uint32_t phase_b_items;    /* bucket items of both phase-B halves (each output uses one half) */
+    uint32_t phase_b_items;    /* bucket items (point additions) one blob's 128 phase-B outputs add up */
     uint64_t resident_table_bytes; /* the main phase-A table plus any smaller ones the autotuner kept */
     uint32_t extra_table_windows;  /* bit c set: a smaller phase-A table of window c is resident */
 } vkzg_amd_info;

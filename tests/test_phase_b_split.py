@@ -1,0 +1,137 @@
+"""Phase B's three convolutions, restated on the CPU and checked against the
+definition (no GPU): the fold of k_fold (Fc, Fn, F-), the taps and signed
+8-bit GLV digit items of build_phase_b_items, the slot and wrap rules of
+k_phase_b, the bucket weights of k_bucket_reduce and the recombination of
+k_finalize, on scalars standing in for points (phase B is linear; phi is
+multiplication by lambda).  The result must equal FFT_128(trunc_64(IFFT_128(u))),
+the map the reference's split circulant computes.  A wrong tap, slot, wrap
+sign or recombination sign fails here; the GPU tests then check the proofs
+bit-exactly."""
+import random
+
+R = 0x73EDA753299D7D483339D80809A1D80553BDA402FFFE5BFEFFFFFFFF00000001
+LAMBDA = 0xAC45A4010001A40200000000FFFFFFFF
+LAD_POS = 17
+
+
+def circulant_taps():
+    w = pow(7, (R - 1) // 128, R)
+
+    def K(e):
+        if e == 0:
+            return pow(2, -1, R)
+        if e % 2 == 0:
+            return 0
+        return pow(64 * (1 - pow(w, e, R)), -1, R)
+
+    half = pow(2, -1, R)
+    kp = [(K(e) + K(e + 64)) * half % R for e in range(64)]
+    km = [(K(e) - K(e + 64)) * half % R for e in range(64)]
+    return kp, km
+
+
+def segment_taps(kp, km):
+    half = pow(2, -1, R)
+    kc = [(kp[e] + kp[e + 32]) * half % R for e in range(32)]
+    kn = [(kp[e] - kp[e + 32]) * half % R for e in range(32)]
+    return [kc, kn, km]
+
+
+def items_of(taps):
+    """(bucket k) -> [(e, pos, neg)]: build_phase_b_items' GLV split and digits."""
+    lists = [[] for _ in range(128)]
+    for e, t in enumerate(taps):
+        k1, k2 = t % LAMBDA, t // LAMBDA
+        assert k1 < 1 << 128 and k2 < 1 << 128
+        for g, part in enumerate((k1, k2)):
+            carry = 0
+            for d in range(LAD_POS):
+                v = ((part >> (8 * d)) & 0xFF if d < 16 else 0) + carry
+                carry = 0
+                if v > 128:
+                    v -= 256
+                    carry = 1
+                if v:
+                    lists[abs(v) - 1].append((e, d + g * LAD_POS, v < 0))
+        assert carry == 0
+    return lists
+
+
+def output_rules(a):
+    """k_phase_b's PhaseBOut: (segment, aa, base)."""
+    seg = 0 if a < 32 else (1 if a < 64 else 2)
+    aa = a & 31 if a < 64 else a & 63
+    base = a & 32 if a < 64 else 0
+    return seg, aa, base
+
+
+def phase_b_model(u):
+    kp, km = circulant_taps()
+    taps = segment_taps(kp, km)
+    lists = [items_of(t) for t in taps]
+    fp = [(u[j] + u[j + 64]) % R for j in range(64)]
+    fm = [(u[j] - u[j + 64]) % R for j in range(64)]
+    # k_fold: block 0 = Fc (slots 0..31), Fn (32..63); block 1 = F-
+    block = [[(fp[i] + fp[i + 32]) % R for i in range(32)] + [(fp[i] - fp[i + 32]) % R for i in range(32)], fm]
+    # ladder positions: 2^(8d) x and phi(2^(8d) x) = lambda 2^(8d) x
+    lad = [[[x * pow(2, 8 * d, R) % R for d in range(LAD_POS)] for x in blk] for blk in block]
+    cvals = []
+    for a in range(128):
+        seg, aa, base = output_rules(a)
+        pts = lad[1 if seg == 2 else 0]
+        total = 0
+        for k in range(128):
+            bucket = 0
+            for e, pos, neg in lists[seg][k]:
+                if seg != 0 and e > aa:
+                    neg = not neg
+                slot = base + ((aa - e) & (63 if seg == 2 else 31))
+                x = pts[slot][pos] if pos < LAD_POS else pts[slot][pos - LAD_POS] * LAMBDA % R
+                bucket += -x if neg else x
+            total += (k + 1) * bucket
+        cvals.append(total % R)
+    out = [0] * 128
+    for a in range(64):
+        yc, yn = cvals[a & 31], cvals[32 + (a & 31)]
+        cp = (yc + yn) % R if a < 32 else (yc - yn) % R
+        cm = cvals[64 + a]
+        out[a] = (cp + cm) % R
+        out[a + 64] = (cp - cm) % R
+    return out, taps
+
+
+def test_phase_b_segments_equal_ifft_trunc_fft():
+    rng = random.Random(8)
+    w = pow(7, (R - 1) // 128, R)
+    inv128 = pow(128, -1, R)
+    u = [rng.randrange(R) for _ in range(128)]
+    h = [sum(u[j] * pow(w, -j * t, R) for j in range(128)) * inv128 % R for t in range(128)]
+    h = h[:64] + [0] * 64
+    want = [sum(h[t] * pow(w, t * k, R) for t in range(128)) % R for k in range(128)]
+    got, taps = phase_b_model(u)
+    assert got == want
+    # the cyclic half's sub-convolutions keep 17 nonzero taps each (e = 0, odd e)
+    assert [sum(1 for x in t if x) for t in taps] == [17, 17, 33]
+
+
+def test_phase_b_groups_cover_every_output_once():
+    """k_phase_b's lane -> (blob, a) map: pair groups of segments 0 and 1,
+    then one group per blob for segment 2; odd blob counts leave the last
+    pair's upper half idle."""
+    for nblobs in range(1, 6):
+        npairs = (nblobs + 1) // 2
+        seen = []
+        for grp in range(2 * npairs + nblobs):
+            segs = set()
+            for lane in range(64):
+                if grp < 2 * npairs:
+                    blob = 2 * (grp % npairs) + (lane >> 5)
+                    a = (grp // npairs) * 32 + (lane & 31)
+                else:
+                    blob = grp - 2 * npairs
+                    a = 64 + lane
+                if blob < nblobs:
+                    seen.append((blob, a))
+                    segs.add(output_rules(a)[0])
+            assert len(segs) == 1  # one item list per wave
+        assert sorted(seen) == [(b, a) for b in range(nblobs) for a in range(128)]

@@ -64,19 +64,34 @@ __global__ __launch_bounds__(256) void k_ladder(G1J *lad, const G1X *part, uint3
     }
 }
 
-// fold[blob][h][j][d] = L[j][d] + (h ? -1 : 1) * L[j+64][d]  (k_fold_ladder.comp)
+// The split-circulant folds of the ladder, position by position
+// (k_fold_ladder.comp), with the cyclic half split once more:
+//   F+[j] = L[j] + L[j+64], F-[j] = L[j] - L[j+64] (j < 64),
+//   Fc[i] = F+[i] + F+[i+32], Fn[i] = F+[i] - F+[i+32] (i < 32),
+// stored fold[blob][0][i][d] = Fc[i], fold[blob][0][32+i][d] = Fn[i],
+// fold[blob][1][j][d] = F-[j] (the three phase-B segments' points).
 __global__ void __launch_bounds__(64) k_fold(G1J *fold, const G1J *lad, uint32_t nblobs) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= nblobs * 64 * kLadPos) return;
-    const uint32_t blob = g / (64 * kLadPos);
-    const uint32_t r = g % (64 * kLadPos);
-    const uint32_t j = r / kLadPos, d = r % kLadPos;
-    const G1J *L = lad + (size_t)blob * kLadPos * 128;
-    const G1J a = L[d * 128 + j];
-    const G1J b = L[d * 128 + j + 64];
-    G1J *F = fold + (size_t)blob * 2 * 64 * kLadPos;
-    F[(0 * 64 + j) * kLadPos + d] = g1j_add(a, b);
-    F[(1 * 64 + j) * kLadPos + d] = g1j_add(a, g1j_neg(b));
+    if (g >= nblobs * 32 * kLadPos) return;
+    const uint32_t blob = g / (32 * kLadPos);
+    const uint32_t r = g % (32 * kLadPos);
+    const uint32_t i = r / kLadPos, d = r % kLadPos;
+    const G1J *L = lad + (size_t)blob * kLadPos * 128 + d * 128;
+    G1J *F = fold + (size_t)blob * 2 * 64 * kLadPos + d;
+    G1J p0;
+    {
+        const G1J a = L[i], b = L[i + 64];
+        p0 = g1j_add(a, b);
+        F[(64 + i) * kLadPos] = g1j_add(a, g1j_neg(b));
+    }
+    G1J p1;
+    {
+        const G1J a = L[i + 32], b = L[i + 96];
+        p1 = g1j_add(a, b);
+        F[(96 + i) * kLadPos] = g1j_add(a, g1j_neg(b));
+    }
+    F[i * kLadPos] = g1j_add(p0, p1);
+    F[(32 + i) * kLadPos] = g1j_add(p0, g1j_neg(p1));
 }
 
 // phi[k] = (beta x, y) of pts[k]: the endomorphism [lambda], lambda = z^2 - 1
@@ -107,7 +122,7 @@ hipError_t launch_ladder(G1J *lad, const G1X *part, uint32_t groups, uint32_t sp
 }
 
 hipError_t launch_fold(G1J *fold, const G1J *lad, uint32_t nblobs, hipStream_t s) {
-    hipLaunchKernelGGL(k_fold, dim3(blocks_for(nblobs * 64ull * kLadPos, 64)), dim3(64), 0, s, fold,
+    hipLaunchKernelGGL(k_fold, dim3(blocks_for(nblobs * 32ull * kLadPos, 64)), dim3(64), 0, s, fold,
                        lad, nblobs);
     return hipGetLastError();
 }

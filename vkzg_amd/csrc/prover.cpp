@@ -72,16 +72,86 @@ void glv_split(const uint32_t k[8], unsigned __int128 &k1, unsigned __int128 &k2
     k2 = q;
 }
 
-// Phase-B item lists: for each half and bucket k, the (e, d, sign) with
-// signed 8-bit digit of magnitude k+1 at position d of the scalar
-// K[half][e] (k_phase_b_split.comp:25-61, 8-bit signed digits), where d < 17
-// indexes the digits of k1 and 17 <= d < 34 those of k2.
+// Canonical Fr arithmetic on the host (little-endian 32-bit limbs).
+void fr_host_add(uint32_t r[8], const uint32_t a[8], const uint32_t b[8]) {
+    uint32_t t[8], u[8];
+    uint64_t c = 0;
+    for (int i = 0; i < 8; i++) {
+        c += (uint64_t)a[i] + b[i];
+        t[i] = (uint32_t)c;
+        c >>= 32;
+    }
+    int64_t br = 0;
+    for (int i = 0; i < 8; i++) {
+        const int64_t d = (int64_t)t[i] - cfr::P[i] + br;
+        u[i] = (uint32_t)d;
+        br = d >> 32;
+    }
+    const bool keep = br < 0;  // a + b < r (a + b < 2^256: no carry out)
+    for (int i = 0; i < 8; i++) r[i] = keep ? t[i] : u[i];
+}
+void fr_host_neg(uint32_t r[8], const uint32_t a[8]) {
+    bool zero = true;
+    for (int i = 0; i < 8; i++) zero = zero && a[i] == 0;
+    int64_t br = 0;
+    for (int i = 0; i < 8; i++) {
+        const int64_t d = (int64_t)cfr::P[i] - a[i] + br;
+        r[i] = zero ? 0u : (uint32_t)d;
+        br = d >> 32;
+    }
+}
+// a / 2 mod r: (a + r) / 2 for odd a (r is odd).
+void fr_host_half(uint32_t r[8], const uint32_t a[8]) {
+    uint32_t t[9];
+    const bool odd = (a[0] & 1u) != 0;
+    uint64_t c = 0;
+    for (int i = 0; i < 8; i++) {
+        c += (uint64_t)a[i] + (odd ? cfr::P[i] : 0u);
+        t[i] = (uint32_t)c;
+        c >>= 32;
+    }
+    t[8] = (uint32_t)c;
+    for (int i = 0; i < 8; i++) r[i] = (t[i] >> 1) | (t[i + 1] << 31);
+}
+
+// Phase-B taps of the three convolutions (canonical), from the split
+// circulant K+ (cyclic, 64 taps) and K- (negacyclic, 64 taps) of
+// k_phase_b_scalars.  The cyclic half splits once more by
+// X^64 - 1 = (X^32 - 1)(X^32 + 1):
+//   segment 0: Kc[e] = (K+[e] + K+[e+32]) / 2, cyclic over Fc = F+lo + F+hi,
+//   segment 1: Kn[e] = (K+[e] - K+[e+32]) / 2, negacyclic over Fn = F+lo - F+hi,
+//   segment 2: K-[e], negacyclic over F- (64 taps),
+// so that C+[a] = Yc[a] + Yn[a] and C+[a+32] = Yc[a] - Yn[a] for a < 32
+// (k_finalize).  K+ has only 33 nonzero taps (e = 0 and odd e), Kc and Kn 17
+// each: the cyclic half costs 64 x 17 scalar-point terms instead of 64 x 33.
+void phase_b_taps(const Fr *kpm, std::vector<Fr> taps[3]) {
+    taps[0].assign(32, Fr{});
+    taps[1].assign(32, Fr{});
+    for (int e = 0; e < 32; e++) {
+        uint32_t sum[8], neg[8], diff[8];
+        fr_host_add(sum, kpm[e].v, kpm[e + 32].v);
+        fr_host_neg(neg, kpm[e + 32].v);
+        fr_host_add(diff, kpm[e].v, neg);
+        fr_host_half(taps[0][e].v, sum);
+        fr_host_half(taps[1][e].v, diff);
+    }
+    taps[2].assign(kpm + 64, kpm + 128);
+}
+
+// Phase-B item lists: for each segment s and bucket k, the (e, d, sign) with
+// signed 8-bit digit of magnitude k+1 at position d of the tap taps[s][e]
+// (k_phase_b_split.comp:25-61, 8-bit signed digits), where d < 17 indexes
+// the digits of k1 and 17 <= d < 34 those of k2.
 void build_phase_b_items(const Fr *kpm, std::vector<uint32_t> &items, std::vector<uint32_t> &offsets) {
-    std::vector<uint32_t> lists[2][128];
-    for (int half = 0; half < 2; half++) {
-        for (uint32_t e = 0; e < 64; e++) {
+    std::vector<Fr> taps[3];
+    phase_b_taps(kpm, taps);
+    items.clear();
+    offsets.assign(3 * 129, 0);
+    for (int seg = 0; seg < 3; seg++) {
+        std::vector<uint32_t> lists[128];
+        for (uint32_t e = 0; e < (uint32_t)taps[seg].size(); e++) {
             unsigned __int128 parts[2];
-            glv_split(kpm[half * 64 + e].v, parts[0], parts[1]);
+            glv_split(taps[seg][e].v, parts[0], parts[1]);
             for (int g = 0; g < 2; g++) {
                 int carry = 0;
                 for (uint32_t d = 0; d < kLadPos; d++) {
@@ -95,19 +165,15 @@ void build_phase_b_items(const Fr *kpm, std::vector<uint32_t> &items, std::vecto
                     if (v == 0) continue;
                     const uint32_t mag = (uint32_t)(v < 0 ? -v : v);
                     const uint32_t pos = d + (uint32_t)g * kLadPos;
-                    lists[half][mag - 1].push_back(e | (pos << 8) | ((v < 0 ? 1u : 0u) << 16));
+                    lists[mag - 1].push_back(e | (pos << 8) | ((v < 0 ? 1u : 0u) << 16));
                 }
             }
         }
-    }
-    items.clear();
-    offsets.assign(2 * 129, 0);
-    for (int half = 0; half < 2; half++) {
         for (int k = 0; k < 128; k++) {
-            offsets[half * 129 + k] = (uint32_t)items.size();
-            items.insert(items.end(), lists[half][k].begin(), lists[half][k].end());
+            offsets[seg * 129 + k] = (uint32_t)items.size();
+            items.insert(items.end(), lists[k].begin(), lists[k].end());
         }
-        offsets[half * 129 + 128] = (uint32_t)items.size();
+        offsets[seg * 129 + 128] = (uint32_t)items.size();
     }
 }
 
@@ -194,6 +260,10 @@ void build_tables(vkzg_prover *p, const uint8_t *setup) {
     std::vector<uint32_t> items, offsets;
     build_phase_b_items(kpm.data(), items, offsets);
     p->kb_count = (uint32_t)items.size();
+    // segments 0 and 1 serve 32 outputs per blob each, segment 2 serves 64
+    p->kb_per_blob = 0;
+    for (int seg = 0; seg < 3; seg++)
+        p->kb_per_blob += (seg == 2 ? 64u : 32u) * (offsets[seg * 129 + 128] - offsets[seg * 129]);
     p->kb_items = p->arena.dalloc<uint32_t>(items.size());
     p->kb_offsets = p->arena.dalloc<uint32_t>(offsets.size());
     ck(hipMemcpyAsync(p->kb_items, items.data(), items.size() * 4, hipMemcpyHostToDevice, s));
@@ -1064,7 +1134,7 @@ vkzg_result vkzg_amd_prover_info(const vkzg_prover *p, vkzg_amd_info *out) {
     out->graph_replays = p->graphs.replays;
     out->autotuned = p->autotuned ? 1u : 0u;
     out->autotune_ms = p->autotune_ms;
-    out->phase_b_items = p->kb_count;
+    out->phase_b_items = p->kb_per_blob;
     out->max_batch_size = p->max_batch;
     out->setup_ms = p->setup_ms;
     return VKZG_OK;

@@ -136,13 +136,18 @@ VKZG_DEV void g1x_pair_to_affine(G1A &ra, G1A &rb, const G1X &a, const G1X &b) {
 }
 
 // out[a] = C+[a] + C-[a], out[a+64] = C+[a] - C-[a] (k_combine_split.comp),
-// compressed and written at brev7(k) (k_compress_proofs.comp).
+// compressed and written at brev7(k) (k_compress_proofs.comp).  The cyclic
+// half comes from its two sub-convolutions: C+[a] = Yc[a] + Yn[a] and
+// C+[a+32] = Yc[a] - Yn[a] for a < 32 (cvals[..][a] = Yc[a],
+// cvals[..][32+a] = Yn[a], cvals[..][64+a] = C-[a]).
 __global__ void __launch_bounds__(64) k_finalize(uint8_t *proofs, const G1X *cvals, uint32_t nblobs) {
     __builtin_amdgcn_s_setprio(3);  // dependent inversion chain: issue first
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= nblobs * 64) return;
     const uint32_t blob = g / 64, a = g % 64;
-    const G1X cp = cvals[blob * 128 + a];
+    const G1X yc = cvals[blob * 128 + (a & 31u)];
+    const G1X yn = cvals[blob * 128 + 32 + (a & 31u)];
+    const G1X cp = g1x_add(yc, a < 32 ? yn : g1x_neg(yn));
     const G1X cm = cvals[blob * 128 + a + 64];
     G1A p0, p1;
     g1x_pair_to_affine(p0, p1, g1x_add(cp, cm), g1x_add(cp, g1x_neg(cm)));

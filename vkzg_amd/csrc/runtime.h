@@ -267,6 +267,7 @@ struct vkzg_prover {
     bool extra_named[vkzg_amd::kExtraWindowMax - vkzg_amd::kExtraWindowMin + 1] = {};
     uint32_t *kb_items = nullptr, *kb_offsets = nullptr;
     uint32_t kb_count = 0;
+    uint32_t kb_per_blob = 0;  // items the 128 phase-B outputs of one blob add up
 
     // Working set of the synchronous entry points and its pinned staging.
     vkzg_amd::WorkSet ws;
