@@ -43,12 +43,13 @@ def seeded_blobs(n, seed):
 def stage_work(info, nb, items_per_output):
     """Algorithmic limb-MACs per launch of each proof stage (formula counts)."""
     W = info["windows"]
-    madd, add, dbl_j, add_j = 10, 14, 7, 16  # Fp multiplications per operation
+    madd, add, dbl_j = 10, 14, 7  # Fp multiplications per operation
     return {
         "fk20_scalars": nb * (12 * 2048 + 4096 + 64 * 448 + 8192 + 4096) * FR_M,
         "phase_a": nb * 128 * (64 * W * madd + 15 * add) * FP_M,
-        "ladder": nb * 128 * (LAD_POS - 1) * 8 * dbl_j * FP_M,
-        "fold": nb * 64 * LAD_POS * 2 * add_j * FP_M,
+        # 128 chains of 8 (LAD_POS - 1) doublings, after the split-circulant
+        # folds of u (3 additions for each of 64 chains, 1 for the other 64)
+        "ladder": nb * (128 * (LAD_POS - 1) * 8 * dbl_j + 256 * add) * FP_M,
         "phase_b": nb * 128 * items_per_output * madd * FP_M,
         "bucket_reduce": nb * 128 * (2 * 128 + 8) * add * FP_M,
     }

@@ -1,5 +1,5 @@
 """Phase B's three convolutions, restated on the CPU and checked against the
-definition (no GPU): the fold of k_fold (Fc, Fn, F-), the taps and signed
+definition (no GPU): the folds k_ladder doubles (Fc, Fn, F-), the taps and signed
 8-bit GLV digit items of build_phase_b_items, the slot and wrap rules of
 k_phase_b, the bucket weights of k_bucket_reduce and the recombination of
 k_finalize, on scalars standing in for points (phase B is linear; phi is
@@ -61,7 +61,7 @@ def output_rules(a):
     """k_phase_b's PhaseBOut: (segment, aa, base)."""
     seg = 0 if a < 32 else (1 if a < 64 else 2)
     aa = a & 31 if a < 64 else a & 63
-    base = a & 32 if a < 64 else 0
+    base = a & 32 if a < 64 else 64
     return seg, aa, base
 
 
@@ -71,14 +71,13 @@ def phase_b_model(u):
     lists = [items_of(t) for t in taps]
     fp = [(u[j] + u[j + 64]) % R for j in range(64)]
     fm = [(u[j] - u[j + 64]) % R for j in range(64)]
-    # k_fold: block 0 = Fc (slots 0..31), Fn (32..63); block 1 = F-
-    block = [[(fp[i] + fp[i + 32]) % R for i in range(32)] + [(fp[i] - fp[i + 32]) % R for i in range(32)], fm]
+    # k_ladder's chains: Fc (slots 0..31), Fn (32..63), F- (64..127)
+    f = [(fp[i] + fp[i + 32]) % R for i in range(32)] + [(fp[i] - fp[i + 32]) % R for i in range(32)] + fm
     # ladder positions: 2^(8d) x and phi(2^(8d) x) = lambda 2^(8d) x
-    lad = [[[x * pow(2, 8 * d, R) % R for d in range(LAD_POS)] for x in blk] for blk in block]
+    pts = [[x * pow(2, 8 * d, R) % R for d in range(LAD_POS)] for x in f]
     cvals = []
     for a in range(128):
         seg, aa, base = output_rules(a)
-        pts = lad[1 if seg == 2 else 0]
         total = 0
         for k in range(128):
             bucket = 0

@@ -13,12 +13,13 @@
 //                             c = window bits, W = ceil(256/c), H = 2^(c-1)
 //   Fr scal[B][128][64]       phase-A scalars, CANONICAL (not Montgomery)
 //   G1X u[B][128][G]          phase-A partial sums, one per window group g < G
-//   G1J lad[B][17][128]       2^(8d) * u[j], d < kLadPos
-//   G1J fold[B][2][64][17]    L+ / L- ladders, normalised into G1A lad_aff with
-//                             the same layout; G1A lad_phi: phi(lad_aff), where
+//   G1J lad[B][17][128]       2^(8d) * f[j], d < kLadPos, for the folds f of u:
+//                             Fc (j < 32), Fn (32 <= j < 64), F- (j >= 64);
+//                             normalised into G1A lad_aff with the same
+//                             layout; G1A lad_phi: phi(lad_aff), where
 //                             phi(x, y) = (beta x, y) = lambda * P (GLV)
 //   G1X buckets[B][128][128]  phase-B buckets, [output a][bucket k]
-//   G1X cvals[B][128]         C+ (a < 64) and C- (a >= 64)
+//   G1X cvals[B][128]         Yc (a < 32), Yn (32 <= a < 64), C- (a >= 64)
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -75,7 +76,6 @@ hipError_t launch_phase_a(G1X *part, const Fr *scal, const G1AT *table, uint32_t
                           uint32_t force_checked, hipStream_t s);
 hipError_t launch_ladder(G1J *lad, const G1X *part, uint32_t groups, uint32_t splits, uint32_t nblobs,
                          hipStream_t s);
-hipError_t launch_fold(G1J *fold, const G1J *lad, uint32_t nblobs, hipStream_t s);
 // Also clears the work counters of the phase-B and bucket-reduction launches
 // that follow (queue_b[0], queue_r[0]).
 hipError_t launch_glv_phi(G1A *phi, const G1A *pts, uint64_t count, uint32_t *queue_b, uint32_t *queue_r,

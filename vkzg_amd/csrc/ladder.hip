@@ -1,4 +1,4 @@
-// FK20 phase B inputs: doubling ladders of u[j] and the split-circulant folds.
+// FK20 phase B inputs: doubling ladders of the split-circulant folds of u.
 #include "proof_common.h"
 
 namespace vkzg_amd {
@@ -30,10 +30,23 @@ VKZG_DEV G1J g1j_dbl_pair(const G1J &p, uint32_t q) {
     return r;
 }
 
-// lad[blob][d][j] = 2^(8d) * u[blob][j], d < kLadPos.  With `splits` = M
-// segments, pair (blob, m, j) starts from phase A's 2^(8 start_m) u (the
-// sum of its `groups` window-group partials part[((blob*M + m)*128 + j)*G + g])
-// and writes positions [start_m, start_{m+1}): chains of 128 / M doublings.
+// The phase-B points are linear in u, and so are the doublings, so the
+// split-circulant folds (k_fold_ladder.comp) are taken before the ladder:
+//   F+[i] = u[i] + u[i+64], F-[i] = u[i] - u[i+64] (i < 64),
+//   Fc[i] = F+[i] + F+[i+32], Fn[i] = F+[i] - F+[i+32] (i < 32),
+// and chain j of a blob doubles f[j] = Fc[j] (j < 32), Fn[j-32] (j < 64),
+// F-[j-64] (j >= 64): lad[blob][d][j] = 2^(8d) * f[j], d < kLadPos.  With
+// `splits` = M segments, pair (blob, m, j) starts from the same folds of
+// phase A's 2^(8 start_m) u (each u the sum of its `groups` window-group
+// partials part[((blob*M + m)*128 + k)*G + g]) and writes positions
+// [start_m, start_{m+1}): chains of 128 / M doublings.
+VKZG_DEV G1X msm_sum(const G1X *part, size_t msm, uint32_t groups) {
+    G1X u = part[msm * groups];
+#pragma unroll 1
+    for (uint32_t k = 1; k < groups; k++) u = g1x_add(u, part[msm * groups + k]);
+    return u;
+}
+
 __global__ __launch_bounds__(256) void k_ladder(G1J *lad, const G1X *part, uint32_t groups, uint32_t splits, uint32_t nblobs) {
     // A long dependent chain: give these waves issue priority over the
     // throughput kernels of another sub-batch sharing the SIMD.
@@ -43,10 +56,18 @@ __global__ __launch_bounds__(256) void k_ladder(G1J *lad, const G1X *part, uint3
     const uint32_t g = tid >> 1, q = tid & 1u;
     if (g >= nblobs * 128 * splits) return;
     const uint32_t j = g % 128, m = (g / 128) % splits, blob = g / (128 * splits);
-    G1X u = part[(size_t)g * groups];
-#pragma unroll 1
-    for (uint32_t k = 1; k < groups; k++) u = g1x_add(u, part[(size_t)g * groups + k]);
-    G1J p = g1j_from_xyzz(u);
+    const size_t base = (size_t)(blob * splits + m) * 128;
+    G1X f;
+    if (j < 64) {
+        const uint32_t i = j & 31u;
+        const G1X lo = g1x_add(msm_sum(part, base + i, groups), msm_sum(part, base + i + 64, groups));
+        const G1X hi = g1x_add(msm_sum(part, base + i + 32, groups), msm_sum(part, base + i + 96, groups));
+        f = g1x_add(lo, j < 32 ? hi : g1x_neg(hi));
+    } else {
+        const uint32_t i = j - 64;
+        f = g1x_add(msm_sum(part, base + i, groups), g1x_neg(msm_sum(part, base + i + 64, groups)));
+    }
+    G1J p = g1j_from_xyzz(f);
     const uint32_t seg = 16 / splits;  // positions per segment; the last one also takes d = 16
     const uint32_t d0 = m * seg, d1 = m + 1 == splits ? kLadPos : d0 + seg;
     G1J *dst = lad + (size_t)blob * kLadPos * 128 + j;
@@ -62,36 +83,6 @@ __global__ __launch_bounds__(256) void k_ladder(G1J *lad, const G1X *part, uint3
 #pragma unroll 1
         for (int k = 0; k < 8; k++) p = g1j_dbl_pair(p, q);
     }
-}
-
-// The split-circulant folds of the ladder, position by position
-// (k_fold_ladder.comp), with the cyclic half split once more:
-//   F+[j] = L[j] + L[j+64], F-[j] = L[j] - L[j+64] (j < 64),
-//   Fc[i] = F+[i] + F+[i+32], Fn[i] = F+[i] - F+[i+32] (i < 32),
-// stored fold[blob][0][i][d] = Fc[i], fold[blob][0][32+i][d] = Fn[i],
-// fold[blob][1][j][d] = F-[j] (the three phase-B segments' points).
-__global__ void __launch_bounds__(64) k_fold(G1J *fold, const G1J *lad, uint32_t nblobs) {
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= nblobs * 32 * kLadPos) return;
-    const uint32_t blob = g / (32 * kLadPos);
-    const uint32_t r = g % (32 * kLadPos);
-    const uint32_t i = r / kLadPos, d = r % kLadPos;
-    const G1J *L = lad + (size_t)blob * kLadPos * 128 + d * 128;
-    G1J *F = fold + (size_t)blob * 2 * 64 * kLadPos + d;
-    G1J p0;
-    {
-        const G1J a = L[i], b = L[i + 64];
-        p0 = g1j_add(a, b);
-        F[(64 + i) * kLadPos] = g1j_add(a, g1j_neg(b));
-    }
-    G1J p1;
-    {
-        const G1J a = L[i + 32], b = L[i + 96];
-        p1 = g1j_add(a, b);
-        F[(96 + i) * kLadPos] = g1j_add(a, g1j_neg(b));
-    }
-    F[i * kLadPos] = g1j_add(p0, p1);
-    F[(32 + i) * kLadPos] = g1j_add(p0, g1j_neg(p1));
 }
 
 // phi[k] = (beta x, y) of pts[k]: the endomorphism [lambda], lambda = z^2 - 1
@@ -118,12 +109,6 @@ hipError_t launch_ladder(G1J *lad, const G1X *part, uint32_t groups, uint32_t sp
     // chain is issue-bound, so a shared SIMD halves its rate).
     hipLaunchKernelGGL(k_ladder, dim3(blocks_for(nblobs * 128ull * splits * 2, 256)), dim3(256), 0, s, lad,
                        part, groups, splits, nblobs);
-    return hipGetLastError();
-}
-
-hipError_t launch_fold(G1J *fold, const G1J *lad, uint32_t nblobs, hipStream_t s) {
-    hipLaunchKernelGGL(k_fold, dim3(blocks_for(nblobs * 32ull * kLadPos, 64)), dim3(64), 0, s, fold,
-                       lad, nblobs);
     return hipGetLastError();
 }
 

@@ -7,19 +7,18 @@ namespace {
 // Output a (< 128) of a blob belongs to segment 0 (a < 32: cyclic over the
 // 32 points Fc), 1 (32 <= a < 64: negacyclic over the 32 points Fn) or 2
 // (a >= 64: negacyclic over the 64 points F-); the ladder slots of the
-// first two are 0..31 and 32..63 of the blob's first 64-slot block, those of
-// segment 2 the second block (k_fold).  An item (e, d, sign) of segment s
+// three are 0..31, 32..63 and 64..127 of the blob's ladder (k_ladder), laid
+// out [d][slot]: the 64 lanes of a group read 64 consecutive points.  An item (e, d, sign) of segment s
 // adds +-2^(8d) F[a - e] for d < 17 and +-2^(8(d-17)) phi(F[a - e]) above
 // (the two GLV halves of the tap), with the index cyclic in the segment and
 // the sign flipped when a negacyclic tap wraps (k_phase_b_split.comp:25-61).
 struct PhaseBOut {
     uint32_t seg, aa, base;
     VKZG_DEV explicit PhaseBOut(uint32_t a)
-        : seg(a < 32 ? 0u : (a < 64 ? 1u : 2u)), aa(a < 64 ? (a & 31u) : (a & 63u)), base(a < 64 ? (a & 32u) : 0u) {}
+        : seg(a < 32 ? 0u : (a < 64 ? 1u : 2u)), aa(a < 64 ? (a & 31u) : (a & 63u)), base(a < 64 ? (a & 32u) : 64u) {}
     // ladder slot and sign of tap e
     VKZG_DEV uint32_t slot(uint32_t e) const { return base + ((aa - e) & (seg == 2 ? 63u : 31u)); }
     VKZG_DEV bool wraps(uint32_t e) const { return seg != 0 && e > aa; }
-    VKZG_DEV uint32_t block() const { return seg == 2 ? 1u : 0u; }
 };
 
 // One bucket recomputed with the checked point operations (see k_phase_b).
@@ -33,8 +32,8 @@ __device__ __attribute__((noinline)) void phase_b_slow(G1X *out, const G1A *L, c
         const uint32_t e = packed & 0xffu, d = (packed >> 8) & 0xffu;
         bool neg = ((packed >> 16) & 1u) != 0;
         if (o.wraps(e)) neg = !neg;
-        const uint32_t src = o.slot(e) * kLadPos;
-        const G1A pt = d < kLadPos ? L[src + d] : Phi[src + d - kLadPos];
+        const uint32_t src = o.slot(e);
+        const G1A pt = d < kLadPos ? L[d * 128 + src] : Phi[(d - kLadPos) * 128 + src];
         acc = g1x_madd(acc, g1a_neg_if(pt, neg));
     }
     *out = acc;
@@ -78,8 +77,8 @@ __global__ void __launch_bounds__(256, 2)
         if (!live) blob = nblobs - 1;
         const PhaseBOut o(a);
         const uint32_t lo = offsets[o.seg * 129 + k], hi = offsets[o.seg * 129 + k + 1];
-        const G1A *L = lad_aff + (size_t)(blob * 2 + o.block()) * 64 * kLadPos;
-        const G1A *Phi = lad_phi + (size_t)(blob * 2 + o.block()) * 64 * kLadPos;
+        const G1A *L = lad_aff + (size_t)blob * kLadPos * 128;
+        const G1A *Phi = lad_phi + (size_t)blob * kLadPos * 128;
         G1X acc = g1x_identity();
         bool exc = force_checked != 0, started = false;
 #pragma unroll 1
@@ -89,8 +88,8 @@ __global__ void __launch_bounds__(256, 2)
             const uint32_t d = (packed >> 8) & 0xffu;
             bool neg = ((packed >> 16) & 1u) != 0;
             if (o.wraps(e)) neg = !neg;
-            const uint32_t src = o.slot(e) * kLadPos;
-            const G1A pt = d < kLadPos ? L[src + d] : Phi[src + d - kLadPos];
+            const uint32_t src = o.slot(e);
+            const G1A pt = d < kLadPos ? L[d * 128 + src] : Phi[(d - kLadPos) * 128 + src];
             if (started) {
                 acc = g1x_madd_acc(acc, pt, neg, exc);
             } else {
@@ -116,8 +115,8 @@ __global__ void __launch_bounds__(256) k_phase_b_fix(G1X *buckets, const uint8_t
     const uint32_t blob = ba / 128, a = ba % 128;
     const PhaseBOut o(a);
     const uint32_t lo = offsets[o.seg * 129 + k], hi = offsets[o.seg * 129 + k + 1];
-    const G1A *L = lad_aff + (size_t)(blob * 2 + o.block()) * 64 * kLadPos;
-    const G1A *Phi = lad_phi + (size_t)(blob * 2 + o.block()) * 64 * kLadPos;
+    const G1A *L = lad_aff + (size_t)blob * kLadPos * 128;
+    const G1A *Phi = lad_phi + (size_t)blob * kLadPos * 128;
     phase_b_slow(&buckets[idx], L, Phi, items, lo, hi, a);
 }
 

@@ -309,7 +309,6 @@ void WorkSet::allocate(uint32_t B) {
     scal = arena.dalloc<Fr>((size_t)B * 128 * 64 * kMaxLadSplits);
     u = arena.dalloc<G1X>((size_t)B * 128 * kMaxGroups * kMaxLadSplits);
     lad = arena.dalloc<G1J>((size_t)B * 128 * kLadPos);
-    fold = arena.dalloc<G1J>((size_t)B * 128 * kLadPos);
     lad_aff = arena.dalloc<G1A>((size_t)B * 128 * kLadPos);
     lad_phi = arena.dalloc<G1A>((size_t)B * 128 * kLadPos);
     prefix = arena.dalloc<Fp>((size_t)B * 128 * kLadPos);
@@ -409,7 +408,7 @@ void proofs_range(vkzg_prover *p, WorkSet &ws, uint32_t off, uint32_t nb, uint8_
     Fr *scal = ws.scal + (size_t)off * 8192 * kMaxLadSplits;
     G1X *u = ws.u + (size_t)off * 128 * kMaxGroups * kMaxLadSplits;
     const size_t npos = (size_t)128 * kLadPos;  // ladder points per blob
-    G1J *lad = ws.lad + off * npos, *fold = ws.fold + off * npos;
+    G1J *lad = ws.lad + off * npos;
     G1A *lad_aff = ws.lad_aff + off * npos, *lad_phi = ws.lad_phi + off * npos;
     Fp *prefix = ws.prefix + off * npos;
     G1X *buckets = ws.buckets + (size_t)off * 128 * 128, *cvals = ws.cvals + (size_t)off * 128;
@@ -422,10 +421,8 @@ void proofs_range(vkzg_prover *p, WorkSet &ws, uint32_t off, uint32_t nb, uint8_
     clk.mark("phase_a");
     ck(launch_ladder(lad, u, tn.groups, tn.ladder_splits, nb, s));
     clk.mark("ladder");
-    ck(launch_fold(fold, lad, nb, s));
-    clk.mark("fold");
     const uint32_t chunk = tn.norm_chunk ? tn.norm_chunk : fit_norm_chunk((uint64_t)nb * npos, p->cus);
-    ck(launch_jac_to_affine(lad_aff, fold, prefix, (uint64_t)nb * npos, chunk, s));
+    ck(launch_jac_to_affine(lad_aff, lad, prefix, (uint64_t)nb * npos, chunk, s));
     ck(launch_glv_phi(lad_phi, lad_aff, (uint64_t)nb * npos, ws.queue_b + off, ws.queue_r + off, s));
     clk.mark("normalize");
     ck(launch_phase_b(buckets, ws.bucket_flags + off * 128 * 128, lad_aff, lad_phi, p->kb_items,

@@ -124,7 +124,7 @@ struct WorkSet {
     G1X *u = nullptr, *buckets = nullptr, *cvals = nullptr;
     uint8_t *bucket_flags = nullptr;  // phase-B buckets to recompute with checked operations
     uint8_t *cval_flags = nullptr;    // reduced outputs to recompute likewise
-    G1J *lad = nullptr, *fold = nullptr;
+    G1J *lad = nullptr;
     G1A *lad_aff = nullptr, *lad_phi = nullptr;
     Fp *prefix = nullptr;
     // recovery
