@@ -249,3 +249,35 @@ def test_recovery_maps_match_oracle(oracle_mod, radix, consistent):
     assert got == want
     if consistent:
         assert got == cells
+
+
+def nth_set_bit(w, k):
+    """recover.hip's nth_set_bit: binary search on the low halves' popcounts."""
+    pos = 0
+    for half in (32, 16, 8, 4, 2, 1):
+        c = bin(w & ((1 << half) - 1)).count("1")
+        if k >= c:
+            k -= c
+            w >>= half
+            pos += half
+    return pos
+
+
+def test_vanishing_parts_cover_wanted_cells():
+    """vanishing_product's work split: the two ballots' 128-bit mask, part p
+    of a value taking the wanted cells of rank p, p + PARTS, ... -- every
+    wanted cell exactly once, no other cell."""
+    rng = random.Random(12)
+    for trial in range(300):
+        present = [rng.random() < rng.random() for _ in range(128)]
+        for want in (False, True):
+            m0 = sum(1 << l for l in range(64) if present[l] == want)
+            m1 = sum(1 << l for l in range(64) if present[64 + l] == want)
+            n0 = bin(m0).count("1")
+            n = n0 + bin(m1).count("1")
+            for parts in (8, 16):
+                got = []
+                for p in range(parts):
+                    for t in range(p, n, parts):
+                        got.append(nth_set_bit(m0, t) if t < n0 else 64 + nth_set_bit(m1, t - n0))
+                assert sorted(got) == [c for c in range(128) if present[c] == want]

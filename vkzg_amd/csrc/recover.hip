@@ -235,34 +235,45 @@ VKZG_DEV uint32_t s_index(uint32_t n1, uint32_t k2) {
     return ((k2 & 63u) >> 2) * 512u + n1 * 8u + (k2 & 3u) + 4u * (k2 >> 6);
 }
 
-// Product over the cells c with (present[c] != 0) == want of (x - w128^brev7(c)),
-// for 64 / PARTS values per wave: lane L computes part L % PARTS (cells
-// [128 / PARTS * part, +128 / PARTS)) of value L / PARTS; the PARTS lanes of
-// a value combine by xor shuffles, so each of them ends with the whole
-// product.
-template <int PARTS>
-VKZG_DEV Fr vanishing_product(Fr x, const uint8_t *present, bool want, const Fr *roots, uint32_t lane) {
-    constexpr int kCells = 128 / PARTS;  // 16 or 8
-    const uint32_t part = lane & (PARTS - 1u);
-    uint32_t w[kCells / 4];
-    if constexpr (kCells == 16) {
-        const uint4 bits = *reinterpret_cast<const uint4 *>(present + kCells * part);
-        w[0] = bits.x, w[1] = bits.y, w[2] = bits.z, w[3] = bits.w;
-    } else {
-        const uint2 bits = *reinterpret_cast<const uint2 *>(present + kCells * part);
-        w[0] = bits.x, w[1] = bits.y;
-    }
-    // four independent chains (cells k mod 4), so a lane's products overlap
-    Fr acc[4] = {fr_one(), fr_one(), fr_one(), fr_one()};
+// Position of the k-th (from 0) set bit of w, for k < popcount(w): a binary
+// search on the popcounts of the low halves.
+VKZG_DEV uint32_t nth_set_bit(uint64_t w, uint32_t k) {
+    uint32_t pos = 0;
 #pragma unroll
-    for (int k = 0; k < kCells; k++) {
-        const bool pres = ((w[k >> 2] >> (8 * (k & 3))) & 0xffu) != 0;
-        if (pres == want) {
-            const uint32_t cell = kCells * part + (uint32_t)k;
-            acc[k & 3] = fr_mul(acc[k & 3], fr_sub(x, roots[(64u * brev_bits(cell, 7)) & 8191u]));
+    for (uint32_t half = 32; half >= 1; half >>= 1) {
+        const uint32_t c = __popcll(w & ((1ull << half) - 1ull));
+        if (k >= c) {
+            k -= c;
+            w >>= half;
+            pos += half;
         }
     }
-    Fr prod = fr_mul(fr_mul(acc[0], acc[1]), fr_mul(acc[2], acc[3]));
+    return pos;
+}
+
+// Product over the cells c with (present[c] != 0) == want of (x - w128^brev7(c)),
+// for 64 / PARTS values per wave.  Two ballots give the wanted cells as a
+// 128-bit mask, the same in every lane; part p = lane % PARTS of a value
+// takes the wanted cells of rank p, p + PARTS, p + 2 PARTS, ... in two
+// interleaved chains, so only wanted cells cost a product, and the PARTS
+// lanes of a value combine by xor shuffles, each ending with the whole
+// product.  Called by whole waves (the ballots).
+template <int PARTS>
+VKZG_DEV Fr vanishing_product(Fr x, const uint8_t *present, bool want, const Fr *roots, uint32_t lane) {
+    const uint64_t m0 = __ballot((present[lane] != 0) == want);
+    const uint64_t m1 = __ballot((present[64u + lane] != 0) == want);
+    const uint32_t n0 = __popcll(m0), n = n0 + __popcll(m1);
+    const auto factor = [&](uint32_t t) {
+        const uint32_t cell = t < n0 ? nth_set_bit(m0, t) : 64u + nth_set_bit(m1, t - n0);
+        return fr_sub(x, roots[(64u * brev_bits(cell, 7)) & 8191u]);
+    };
+    Fr acc0 = fr_one(), acc1 = fr_one();
+#pragma unroll 1
+    for (uint32_t t = lane & (PARTS - 1u); t < n; t += 2 * PARTS) {
+        acc0 = fr_mul(acc0, factor(t));
+        if (t + PARTS < n) acc1 = fr_mul(acc1, factor(t + PARTS));
+    }
+    Fr prod = fr_mul(acc0, acc1);
 #pragma unroll
     for (int m = 1; m < PARTS; m <<= 1) prod = fr_mul(prod, fr_shfl_xor(prod, m));
     return prod;
