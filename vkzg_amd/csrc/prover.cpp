@@ -290,7 +290,7 @@ uint64_t WorkSet::bytes_per_blob() {
     const uint64_t npos = 128ull * kLadPos;
     return kBlobBytes + kProofBytes + 4 * 4 + 128 + kMaxLadSplits * 128 * 64 * sizeof(Fr) +
            kMaxLadSplits * 128 * kMaxGroups * sizeof(G1X) +
-           2 * npos * sizeof(G1J) + 2 * npos * sizeof(G1A) + npos * sizeof(Fp) +
+           npos * sizeof(G1J) + 2 * npos * sizeof(G1A) + npos * sizeof(Fp) +
            128 * 128 * (sizeof(G1X) + 1) + 128 * sizeof(G1X) + 2 * kCellsBytes + 128 +
            2 * 8192 * sizeof(Fr);
 }
