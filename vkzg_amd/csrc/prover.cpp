@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <thread>
 
 #include "../../include/vkzg_amd.h"
 #include "runtime.h"
@@ -538,6 +539,66 @@ void enqueue_cells_and_proofs(vkzg_prover *p, WorkSet &ws, uint8_t *d_cells, uin
     enqueue_maybe_graph(p, *graphs, key, s, off_clock, body);
 }
 
+// Host-buffer staging of the synchronous entry points.  A large copy is
+// split over kCopyThreads host threads (one host thread copies pageable
+// memory at about a third of what the PCIe DMA takes); an upload also sends
+// each thread's part by DMA in kCopySlice slices as soon as a slice is
+// staged, so the copy of one slice overlaps the DMA of the previous one.
+// Small copies stay on the calling thread.
+constexpr unsigned kCopyThreads = 4;
+constexpr size_t kCopySlice = (size_t)2 << 20;
+constexpr size_t kThreadedCopyMin = (size_t)4 << 20;
+
+void parallel_copy(uint8_t *dst, const uint8_t *src, size_t bytes) {
+    if (bytes < kThreadedCopyMin) {
+        std::memcpy(dst, src, bytes);
+        return;
+    }
+    const size_t part = (bytes + kCopyThreads - 1) / kCopyThreads;
+    std::thread helpers[kCopyThreads - 1];
+    for (unsigned k = 1; k < kCopyThreads; k++) {
+        const size_t off = k * part;
+        if (off < bytes) helpers[k - 1] = std::thread([=] { std::memcpy(dst + off, src + off, std::min(part, bytes - off)); });
+    }
+    std::memcpy(dst, src, std::min(part, bytes));
+    for (std::thread &t : helpers)
+        if (t.joinable()) t.join();
+}
+
+// host (pageable) -> pinned staging -> dev, enqueued on s.
+void stage_upload(int device, uint8_t *dev, uint8_t *pinned, const uint8_t *host, size_t bytes, hipStream_t s) {
+    if (bytes < kThreadedCopyMin) {
+        std::memcpy(pinned, host, bytes);
+        ck(hipMemcpyAsync(dev, pinned, bytes, hipMemcpyHostToDevice, s));
+        return;
+    }
+    // parts in whole slices, so every slice but the last is full
+    const size_t slices = (bytes + kCopySlice - 1) / kCopySlice;
+    const size_t per = (slices + kCopyThreads - 1) / kCopyThreads;
+    std::atomic<bool> failed{false};
+    const auto work = [&](size_t first, bool own_thread) {
+        if (own_thread && hipSetDevice(device) != hipSuccess) {
+            failed = true;
+            return;
+        }
+        for (size_t k = first; k < std::min(slices, first + per); k++) {
+            const size_t off = k * kCopySlice, n = std::min(kCopySlice, bytes - off);
+            std::memcpy(pinned + off, host + off, n);
+            if (hipMemcpyAsync(dev + off, pinned + off, n, hipMemcpyHostToDevice, s) != hipSuccess) {
+                failed = true;
+                return;
+            }
+        }
+    };
+    std::thread helpers[kCopyThreads - 1];
+    for (unsigned t = 1; t < kCopyThreads; t++)
+        if (t * per < slices) helpers[t - 1] = std::thread(work, t * per, true);
+    work(0, false);
+    for (std::thread &t : helpers)
+        if (t.joinable()) t.join();
+    if (failed) throw HipFail{};
+}
+
 bool chunk_missing_ok(const uint8_t *present, uint32_t nb) {
     for (uint32_t b = 0; b < nb; b++) {
         uint32_t missing = 0;
@@ -827,8 +888,7 @@ vkzg_result vkzg_compute_proofs_batch(vkzg_prover *p, uint8_t *proofs, const uin
         for (size_t done = 0; done < num_blobs;) {
             const uint32_t nb = (uint32_t)std::min<size_t>(p->max_batch, num_blobs - done);
             hipStream_t s = p->stream;
-            std::memcpy(p->h_in, blobs + done * kBlobBytes, nb * kBlobBytes);
-            ck(hipMemcpyAsync(p->ws.blobs, p->h_in, nb * kBlobBytes, hipMemcpyHostToDevice, s));
+            stage_upload(p->device, p->ws.blobs, p->h_in, blobs + done * kBlobBytes, nb * kBlobBytes, s);
             StageClock clk(p, s, true);
             const Tuning tn = tuning_for(p, nb);
             p->last_splits = tn.splits;
@@ -862,9 +922,8 @@ vkzg_result vkzg_recover_cells_batch(vkzg_prover *p, uint8_t *cells_out, const u
             const uint8_t *pm = cell_present + done * 128;
             if (!chunk_missing_ok(pm, nb)) return VKZG_ERR_BADARGS;
             hipStream_t s = p->stream;
-            std::memcpy(p->h_in, cells + done * kCellsBytes, nb * kCellsBytes);
+            stage_upload(p->device, p->ws.cells, p->h_in, cells + done * kCellsBytes, nb * kCellsBytes, s);
             std::memcpy(p->h_present, pm, nb * 128);
-            ck(hipMemcpyAsync(p->ws.cells, p->h_in, nb * kCellsBytes, hipMemcpyHostToDevice, s));
             ck(hipMemcpyAsync(p->ws.present, p->h_present, nb * 128, hipMemcpyHostToDevice, s));
             StageClock clk(p, s, true);
             const Tuning tn = tuning_for(p, nb);
@@ -874,7 +933,7 @@ vkzg_result vkzg_recover_cells_batch(vkzg_prover *p, uint8_t *cells_out, const u
             ck(hipMemcpyAsync(p->h_out, p->ws.cells_out, nb * kCellsBytes, hipMemcpyDeviceToHost, s));
             ck(hipStreamSynchronize(s));
             clk.finish();
-            std::memcpy(cells_out + done * kCellsBytes, p->h_out, nb * kCellsBytes);
+            parallel_copy(cells_out + done * kCellsBytes, p->h_out, nb * kCellsBytes);
             done += nb;
         }
     } catch (HipFail &) {
@@ -894,8 +953,7 @@ vkzg_result vkzg_amd_compute_cells_and_proofs(vkzg_prover *p, uint8_t *cells, ui
         for (size_t done = 0; done < num_blobs;) {
             const uint32_t nb = (uint32_t)std::min<size_t>(p->max_batch, num_blobs - done);
             hipStream_t s = p->stream;
-            std::memcpy(p->h_in, blobs + done * kBlobBytes, nb * kBlobBytes);
-            ck(hipMemcpyAsync(p->ws.blobs, p->h_in, nb * kBlobBytes, hipMemcpyHostToDevice, s));
+            stage_upload(p->device, p->ws.blobs, p->h_in, blobs + done * kBlobBytes, nb * kBlobBytes, s);
             const Tuning tn = tuning_for(p, nb);
             Tuning one = tn;
             one.splits = 1;
@@ -909,7 +967,7 @@ vkzg_result vkzg_amd_compute_cells_and_proofs(vkzg_prover *p, uint8_t *cells, ui
             ck(hipStreamSynchronize(s));
             for (uint32_t b = 0; b < nb; b++)
                 if (p->h_err[b]) return VKZG_ERR_INVALID_BLOB;
-            if (cells) std::memcpy(cells + done * kCellsBytes, p->h_out, nb * kCellsBytes);
+            if (cells) parallel_copy(cells + done * kCellsBytes, p->h_out, nb * kCellsBytes);
             if (proofs) std::memcpy(proofs + done * kProofBytes, p->h_in, nb * kProofBytes);
             done += nb;
         }
@@ -933,9 +991,8 @@ vkzg_result vkzg_amd_recover_cells_and_proofs(vkzg_prover *p, uint8_t *cells_out
             const uint8_t *pm = cell_present + done * 128;
             if (!chunk_missing_ok(pm, nb)) return VKZG_ERR_BADARGS;
             hipStream_t s = p->stream;
-            std::memcpy(p->h_in, cells + done * kCellsBytes, nb * kCellsBytes);
+            stage_upload(p->device, p->ws.cells, p->h_in, cells + done * kCellsBytes, nb * kCellsBytes, s);
             std::memcpy(p->h_present, pm, nb * 128);
-            ck(hipMemcpyAsync(p->ws.cells, p->h_in, nb * kCellsBytes, hipMemcpyHostToDevice, s));
             ck(hipMemcpyAsync(p->ws.present, p->h_present, nb * 128, hipMemcpyHostToDevice, s));
             const Tuning tn = tuning_for(p, nb);
             Tuning one = tn;
@@ -950,7 +1007,7 @@ vkzg_result vkzg_amd_recover_cells_and_proofs(vkzg_prover *p, uint8_t *cells_out
             ck(hipMemcpyAsync(p->h_out, p->ws.cells_out, nb * kCellsBytes, hipMemcpyDeviceToHost, s));
             ck(hipMemcpyAsync(p->h_in, p->ws.proofs, nb * kProofBytes, hipMemcpyDeviceToHost, s));
             ck(hipStreamSynchronize(s));
-            std::memcpy(cells_out + done * kCellsBytes, p->h_out, nb * kCellsBytes);
+            parallel_copy(cells_out + done * kCellsBytes, p->h_out, nb * kCellsBytes);
             std::memcpy(proofs_out + done * kProofBytes, p->h_in, nb * kProofBytes);
             done += nb;
         }
