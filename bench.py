@@ -46,7 +46,8 @@ def stage_work(info, nb, items_per_output):
     madd, add, dbl_j = 10, 14, 7  # Fp multiplications per operation
     return {
         "fk20_scalars": nb * (12 * 2048 + 4096 + 64 * 448 + 8192 + 4096) * FR_M,
-        "phase_a": nb * 128 * (64 * W * madd + 15 * add) * FP_M,
+        # 8 lanes per MSM, 8 terms each, then a 3-level lane tree (7 adds)
+        "phase_a": nb * 128 * (64 * W * madd + 7 * add) * FP_M,
         # 128 chains of 8 (LAD_POS - 1) doublings, after the split-circulant
         # folds of u (3 additions for each of 64 chains, 1 for the other 64)
         "ladder": nb * (128 * (LAD_POS - 1) * 8 * dbl_j + 256 * add) * FP_M,

@@ -14,7 +14,7 @@ VKZG_DEV uint32_t window_bits(const Fr &s, uint32_t w, uint32_t c) {
     return lo & ((1u << c) - 1u);
 }
 
-constexpr int kMsmLanes = 16;  // lanes per tile (64 terms / 16 = 4 each)
+constexpr int kMsmLanes = 8;  // lanes per tile (64 terms / 8 = 8 each)
 constexpr int kTilesPerItem = 256 / kMsmLanes;  // tiles one 256-thread block takes per queue item
 
 // Windows [w0, w1) of the 64 terms of one MSM, recomputed with the checked
@@ -46,7 +46,7 @@ __device__ __attribute__((noinline)) void phase_a_slow(G1X *out, const Fr *scal,
 // k_phase_a_sort.comp:31-40 with a c-bit window) and
 // table[j][i][w][|v|-1] = |v| * 2^(cw) * xext[j][i].  Tile (msm, g) writes its
 // partial sum to part[msm * G + g]; the ladder adds the G partials.  A block
-// takes 16 consecutive tiles per queue item (16 lanes per tile, 4 terms per
+// takes 32 consecutive tiles per queue item (8 lanes per tile, 8 terms per
 // lane) from the counter q[0] until the queue is drained; every block exits
 // after one failed take, so the grid always drains.  q[0] is cleared by the
 // kernel that runs before this one.
