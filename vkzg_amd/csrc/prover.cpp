@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <system_error>
 #include <thread>
 
 #include "../../include/vkzg_amd.h"
@@ -555,12 +556,18 @@ void parallel_copy(uint8_t *dst, const uint8_t *src, size_t bytes) {
         return;
     }
     const size_t part = (bytes + kCopyThreads - 1) / kCopyThreads;
+    const auto copy_part = [=](size_t off) { std::memcpy(dst + off, src + off, std::min(part, bytes - off)); };
     std::thread helpers[kCopyThreads - 1];
     for (unsigned k = 1; k < kCopyThreads; k++) {
         const size_t off = k * part;
-        if (off < bytes) helpers[k - 1] = std::thread([=] { std::memcpy(dst + off, src + off, std::min(part, bytes - off)); });
+        if (off >= bytes) break;
+        try {
+            helpers[k - 1] = std::thread(copy_part, off);
+        } catch (const std::system_error &) {
+            copy_part(off);  // no thread to spare: copy it here
+        }
     }
-    std::memcpy(dst, src, std::min(part, bytes));
+    copy_part(0);
     for (std::thread &t : helpers)
         if (t.joinable()) t.join();
 }
@@ -591,8 +598,14 @@ void stage_upload(int device, uint8_t *dev, uint8_t *pinned, const uint8_t *host
         }
     };
     std::thread helpers[kCopyThreads - 1];
-    for (unsigned t = 1; t < kCopyThreads; t++)
-        if (t * per < slices) helpers[t - 1] = std::thread(work, t * per, true);
+    for (unsigned t = 1; t < kCopyThreads; t++) {
+        if (t * per >= slices) break;
+        try {
+            helpers[t - 1] = std::thread(work, t * per, true);
+        } catch (const std::system_error &) {
+            work(t * per, false);  // no thread to spare: stage it here
+        }
+    }
     work(0, false);
     for (std::thread &t : helpers)
         if (t.joinable()) t.join();
