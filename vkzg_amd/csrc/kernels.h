@@ -97,18 +97,26 @@ hipError_t launch_finalize(uint8_t *proofs, const G1X *cvals, uint32_t nblobs, h
 // over one-wave work items: 16 per blob with 8 values per lane (radix 8) or
 // 32 per blob with 4 (radix 4); `waves` (1, 2 or 4) items per block.  A and
 // B are per-blob Fr[8192] intermediates in the layout of the chosen radix.
+// Radix 8 computes in the 8 x 32-bit Fr form and reads the Fr tables;
+// radix 4 computes in FrL (9 x 29-bit limbs, Montgomery radix 2^261) and
+// reads FrL tables of 48-byte entries (launch_frl_table).
+struct RecTables {
+    const Fr *roots, *roots_inv, *shift_fwd, *shift_inv;          // w^k, w^-k, 7^k / 8192, 7^-k / 8192
+    const uint4 *roots4, *roots_inv4, *shift_fwd4, *shift_inv4;  // the same, FrL
+};
+// out[k] = in[k] as a 48-byte FrL entry (in: the 8 x 32-bit Montgomery form).
+hipError_t launch_frl_table(uint4 *out, const Fr *in, uint32_t n, hipStream_t s);
 // in_mode 0: `in` is the cells, the first inverse pass with the vanishing
 // factor; in_mode 1: `in` is Fr coefficients in natural order (Montgomery),
 // the first forward pass of their evaluation (followed by launch_rec_last).
-hipError_t launch_rec_first(Fr *out, const void *in, int in_mode, const uint8_t *present, const Fr *roots,
-                            const Fr *roots_inv, uint32_t nblobs, uint32_t radix, uint32_t waves, hipStream_t s);
-// stage 0: * shift (7^k / 8192); stage 1: * 1/Z on the coset (from present);
-// stage 2: * shift (7^-k / 8192).
-hipError_t launch_rec_mid(Fr *out, const Fr *in, int stage, const uint8_t *present, const Fr *roots,
-                          const Fr *roots_inv, const Fr *shift, uint32_t nblobs, uint32_t radix,
-                          uint32_t waves, hipStream_t s);
+hipError_t launch_rec_first(Fr *out, const void *in, int in_mode, const uint8_t *present, const RecTables &t,
+                            uint32_t nblobs, uint32_t radix, uint32_t waves, hipStream_t s);
+// stage 0: * 7^k / 8192; stage 1: * 1/Z on the coset (from present);
+// stage 2: * 7^-k / 8192.
+hipError_t launch_rec_mid(Fr *out, const Fr *in, int stage, const uint8_t *present, const RecTables &t,
+                          uint32_t nblobs, uint32_t radix, uint32_t waves, hipStream_t s);
 // Canonical big-endian cell bytes.
-hipError_t launch_rec_last(uint8_t *cells, const Fr *in, const Fr *roots, uint32_t nblobs, uint32_t radix,
+hipError_t launch_rec_last(uint8_t *cells, const Fr *in, const RecTables &t, uint32_t nblobs, uint32_t radix,
                            uint32_t waves, hipStream_t s);
 
 // ------------------------------------------------------------ test hooks

@@ -232,6 +232,12 @@ void build_tables(vkzg_prover *p, const uint8_t *setup) {
     p->shift_fwd = p->arena.dalloc<Fr>(8192);
     p->shift_inv = p->arena.dalloc<Fr>(8192);
     ck(launch_roots(p->roots, p->roots_inv, p->shift_fwd, p->shift_inv, s));
+    // the same tables as FrL entries, for the radix-4 recovery kernels
+    Fr *const fr_tabs[4] = {p->roots, p->roots_inv, p->shift_fwd, p->shift_inv};
+    for (int k = 0; k < 4; k++) {
+        p->frl_tabs[k] = p->arena.dalloc<uint4>(3 * 8192);
+        ck(launch_frl_table(p->frl_tabs[k], fr_tabs[k], 8192, s));
+    }
 
     // Decompress the setup.
     uint8_t *d_setup = p->arena.dalloc<uint8_t>(kSetupBytes);
@@ -378,24 +384,29 @@ bool tuning_valid(const Tuning &t) {
            (t.window_bits == 0 || (t.window_bits >= kExtraWindowMin && t.window_bits <= kExtraWindowMax));
 }
 
+RecTables rec_tables(const vkzg_prover *p) {
+    return RecTables{p->roots,       p->roots_inv,   p->shift_fwd,   p->shift_inv,
+                     p->frl_tabs[0], p->frl_tabs[1], p->frl_tabs[2], p->frl_tabs[3]};
+}
+
 void enqueue_recovery_into(vkzg_prover *p, Fr *A, Fr *B, uint8_t *d_out, const uint8_t *d_cells,
                            const uint8_t *d_present, uint32_t nb, hipStream_t s, StageClock &clk,
                            uint32_t radix, uint32_t waves) {
-    const Fr *r = p->roots, *ri = p->roots_inv;
+    const RecTables t = rec_tables(p);
     // T1 pass A: (E * Z) -> inverse
-    ck(launch_rec_first(A, d_cells, 0, d_present, r, ri, nb, radix, waves, s));
+    ck(launch_rec_first(A, d_cells, 0, d_present, t, nb, radix, waves, s));
     clk.mark("t1_a");
     // T1 pass B, * 7^k / 8192 (coefficients on the coset), T2 pass A
-    ck(launch_rec_mid(B, A, 0, d_present, r, ri, p->shift_fwd, nb, radix, waves, s));
+    ck(launch_rec_mid(B, A, 0, d_present, t, nb, radix, waves, s));
     clk.mark("t1_b_t2_a");
     // T2 pass B (coset evaluations), / Z on the coset, T3 pass A
-    ck(launch_rec_mid(A, B, 1, d_present, r, ri, nullptr, nb, radix, waves, s));
+    ck(launch_rec_mid(A, B, 1, d_present, t, nb, radix, waves, s));
     clk.mark("t2_b_t3_a");
     // T3 pass B, * 7^-k / 8192 (coefficients of P), T4 pass A
-    ck(launch_rec_mid(B, A, 2, d_present, r, ri, p->shift_inv, nb, radix, waves, s));
+    ck(launch_rec_mid(B, A, 2, d_present, t, nb, radix, waves, s));
     clk.mark("t3_b_t4_a");
     // T4 pass B: P on the whole domain, as cells
-    ck(launch_rec_last(d_out, B, r, nb, radix, waves, s));
+    ck(launch_rec_last(d_out, B, t, nb, radix, waves, s));
     clk.mark("t4_b");
 }
 
@@ -529,8 +540,9 @@ void enqueue_cells_and_proofs(vkzg_prover *p, WorkSet &ws, uint8_t *d_cells, uin
         proofs_range(p, ws, 0, nb, d_proofs, d_blobs, q, off_clock, tn, blob_stride,
                      d_cells ? ws.bufA : nullptr);
         if (d_cells) {
-            ck(launch_rec_first(ws.bufB, ws.bufA, 1, nullptr, p->roots, p->roots_inv, nb, tn.rec_radix, tn.rec_waves, q));
-            ck(launch_rec_last(d_cells, ws.bufB, p->roots, nb, tn.rec_radix, tn.rec_waves, q));
+            const RecTables t = rec_tables(p);
+            ck(launch_rec_first(ws.bufB, ws.bufA, 1, nullptr, t, nb, tn.rec_radix, tn.rec_waves, q));
+            ck(launch_rec_last(d_cells, ws.bufB, t, nb, tn.rec_radix, tn.rec_waves, q));
         }
     };
     if (!graphs) return body(s);

@@ -37,6 +37,7 @@
 // evaluation at natural index brev13(w); with i = n1 + 64 n2 that is
 // w = 128 brev6(n1) + brev7(n2), so a pass-A column is two whole cells, and
 // a pass-B column k2 is the one cell brev7(k2).
+#include "field_frl.h"
 #include "kernels.h"
 #include "ntt.h"
 
@@ -429,110 +430,199 @@ __global__ void __launch_bounds__(256, 2)
 // S4[q][n1][h][j] with k2 = 2q + j + 64h (q < 32, h, j < 2).
 constexpr int kItemsPerBlob4 = 32;
 
-struct Exch4 {
-    uint4 *base;
-    VKZG_DEV static uint32_t pos(uint32_t idx) { return idx ^ ((idx >> 5) & 7u); }
-    VKZG_DEV void put(uint32_t idx, const Fr &v) const {
-        uint4 *p = base + 2 * pos(idx);
-        p[0] = make_uint4(v.v[0], v.v[1], v.v[2], v.v[3]);
-        p[1] = make_uint4(v.v[4], v.v[5], v.v[6], v.v[7]);
-    }
-    VKZG_DEV Fr get(uint32_t idx) const {
-        const uint4 *p = base + 2 * pos(idx);
-        const uint4 a = p[0], b = p[1];
-        Fr r;
-        r.v[0] = a.x, r.v[1] = a.y, r.v[2] = a.z, r.v[3] = a.w;
-        r.v[4] = b.x, r.v[5] = b.y, r.v[6] = b.z, r.v[7] = b.w;
-        return r;
-    }
-};
+// ------------------------------------------------------------ radix 4, FrL
+//
+// The radix-4 kernels compute in FrL (field_frl.h: 9 x 29-bit limbs,
+// Montgomery radix 2^261, lazy sums).  Their intermediates in HBM (A, B) are
+// packed back into 8 words (values below 2r < 2^256, normalised) in the same
+// S layout; their twiddle tables are FrL tables of 48-byte entries
+// (launch_frl_tables).  Value bounds, in multiples of r, are noted where they
+// matter: a product's first operand stays below 36r (64r / 2 for the 1/Z
+// product) and every limb below 3 * 2^30 (field_frl.h).
 
 VKZG_DEV uint32_t s4_index(uint32_t n1, uint32_t k2) {
     return ((k2 & 63u) >> 1) * 256u + n1 * 4u + (k2 & 1u) + 2u * (k2 >> 6);
 }
 
+// A 48-byte FrL table entry (limbs 0-3, 4-7, 8 and padding).
+VKZG_DEV FrL ld_frl_tab(const uint4 *tab, uint32_t k) {
+    const uint4 *q = tab + 3u * k;
+    const uint4 a = q[0], b = q[1];
+    const uint32_t c = q[2].x;
+    FrL r;
+    r.v[0] = a.x, r.v[1] = a.y, r.v[2] = a.z, r.v[3] = a.w;
+    r.v[4] = b.x, r.v[5] = b.y, r.v[6] = b.z, r.v[7] = b.w;
+    r.v[8] = c;
+    return r;
+}
+// Packed intermediates: 8 words per value.
+VKZG_DEV FrL ld_frl(const Fr *p) {
+    const uint4 *q = reinterpret_cast<const uint4 *>(p);
+    const uint4 a = q[0], b = q[1];
+    const uint32_t w[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    return frl_unpack(w);
+}
+VKZG_DEV void st_frl(Fr *p, const FrL &v) {
+    uint32_t w[8];
+    frl_pack(w, v);
+    uint4 *q = reinterpret_cast<uint4 *>(p);
+    q[0] = make_uint4(w[0], w[1], w[2], w[3]);
+    q[1] = make_uint4(w[4], w[5], w[6], w[7]);
+}
+
+// A wave's 256-value FrL exchange region (9 KiB): limbs 0-3 and 4-7 as two
+// planes of 16-byte slots (bank-spread as Exch) and limb 8 as a plane of
+// words.
+struct Exch4 {
+    uint4 *base;
+    VKZG_DEV static uint32_t pos(uint32_t idx) { return idx ^ ((idx >> 5) & 7u); }
+    VKZG_DEV void put(uint32_t idx, const FrL &v) const {
+        const uint32_t p = pos(idx);
+        base[p] = make_uint4(v.v[0], v.v[1], v.v[2], v.v[3]);
+        base[256 + p] = make_uint4(v.v[4], v.v[5], v.v[6], v.v[7]);
+        reinterpret_cast<uint32_t *>(base + 512)[p] = v.v[8];
+    }
+    VKZG_DEV FrL get(uint32_t idx) const {
+        const uint32_t p = pos(idx);
+        const uint4 a = base[p], b = base[256 + p];
+        FrL r;
+        r.v[0] = a.x, r.v[1] = a.y, r.v[2] = a.z, r.v[3] = a.w;
+        r.v[4] = b.x, r.v[5] = b.y, r.v[6] = b.z, r.v[7] = b.w;
+        r.v[8] = reinterpret_cast<const uint32_t *>(base + 512)[p];
+        return r;
+    }
+};
+constexpr uint32_t kExch4Uint4 = 576;  // 9 KiB per wave
+
+// The transform's direction: tab[k] = w^k (FrL table), w4 = w^2048.
+struct Dir4 {
+    const uint4 *tab;
+    FrL w4;
+    VKZG_DEV explicit Dir4(const uint4 *t) : tab(t) { w4 = ld_frl_tab(tab, 2048); }
+    VKZG_DEV FrL pow(uint32_t k) const { return ld_frl_tab(tab, k & 8191u); }
+};
+
+// out[e] = sum_b v[b] w4^(b e) (natural order in and out).  In: v0
+// normalised (< V r), v1..v3 normalised below 2r.  Out: limbs below
+// 3 * 2^30, values below (V + 10) r.  One product.
+VKZG_DEV void dft4(FrL &v0, FrL &v1, FrL &v2, FrL &v3, const FrL &w4) {
+    const FrL a = frl_add(v0, v2), b = frl_sub4(v0, v2);  // < V+2, < V+4
+    const FrL c = frl_add(v1, v3), d = frl_sub4(v1, v3);  // < 4 (limbs < 2^30), < 6
+    const FrL tw = frl_mul(d, w4);                        // < 2
+    v0 = frl_add(a, c);                                   // < V+6
+    v2 = frl_sub8(a, c);                                  // < V+10
+    v1 = frl_add(b, tw);                                  // < V+6
+    v3 = frl_sub4(b, tw);                                 // < V+8
+}
+
 // One radix-4 Stockham stage of a column at LDS offset `col` (length N,
 // N / 4 threads, thread j): v <- d[j + r N/4] twiddled and transformed; the
-// caller stores v at (j / Ns) 4 Ns + j mod Ns + r Ns.
+// caller stores v at (j / Ns) 4 Ns + j mod Ns + r Ns.  v[0] is carried
+// untwiddled, so it is normalised first; the twiddle products take v[1..3]
+// back below 2r.
 template <int N, int NS>
-VKZG_DEV void stage4_load(Fr (&v)[4], const Dir &d, const Exch4 &x, uint32_t col, uint32_t j) {
+VKZG_DEV void stage4_load(FrL (&v)[4], const Dir4 &d, const Exch4 &x, uint32_t col, uint32_t j) {
 #pragma unroll
     for (int r = 0; r < 4; r++) v[r] = x.get(col + j + (uint32_t)r * (N / 4));
-    if (NS > 1) {
-        const uint32_t k = j % NS;
+    const uint32_t k = j % NS;
+    v[0] = frl_norm(v[0]);
 #pragma unroll
-        for (int r = 1; r < 4; r++) v[r] = fr_mul(v[r], d.pow((8192u / (NS * 4)) * k * (uint32_t)r));
-    }
-    dft4(v[0], v[1], v[2], v[3], d.w8_2);
+    for (int r = 1; r < 4; r++) v[r] = frl_mul(v[r], d.pow((8192u / (NS * 4)) * k * (uint32_t)r));
+    dft4(v[0], v[1], v[2], v[3], d.w4);
 }
 template <int NS>
-VKZG_DEV void stage4_store(const Fr (&v)[4], const Exch4 &x, uint32_t col, uint32_t j) {
+VKZG_DEV void stage4_store(const FrL (&v)[4], const Exch4 &x, uint32_t col, uint32_t j) {
     const uint32_t base = (j / NS) * NS * 4u + j % NS;
 #pragma unroll
     for (int r = 0; r < 4; r++) x.put(col + base + (uint32_t)r * NS, v[r]);
 }
 
 // Pass B (DFT_64) of the wave's 4 columns cc (k2 = 2q + (cc & 1) + 64 (cc >> 1)):
-// lane L = 4j + cc enters with v[r] = y[j + 16r][cc] and leaves with
-// v[r] = X[j + 16r] of its column.
-VKZG_DEV void pass_b4(Fr (&v)[4], const Dir &d, const Exch4 &x, uint32_t lane) {
+// lane L = 4j + cc enters with v[r] = y[j + 16r][cc] (normalised, below 2r)
+// and leaves with v[r] = X[j + 16r] of its column (below 32r).
+VKZG_DEV void pass_b4(FrL (&v)[4], const Dir4 &d, const Exch4 &x, uint32_t lane) {
     const uint32_t cc = lane & 3u, j = lane >> 2, col = cc * 64u;
-    dft4(v[0], v[1], v[2], v[3], d.w8_2);
+    dft4(v[0], v[1], v[2], v[3], d.w4);  // < 12
     exch_sync();
     stage4_store<1>(v, x, col, j);
     exch_sync();
-    stage4_load<64, 4>(v, d, x, col, j);
+    stage4_load<64, 4>(v, d, x, col, j);  // < 22
     exch_sync();
     stage4_store<4>(v, x, col, j);
     exch_sync();
-    stage4_load<64, 16>(v, d, x, col, j);  // NS = 16: output index j + 16 r
+    stage4_load<64, 16>(v, d, x, col, j);  // < 32; NS = 16: output index j + 16 r
 }
 
 // Pass A (DFT_128) of the wave's 2 columns c (n1 = 2q + c), with its
-// four-step twiddle: lane L = 32c + j enters with v[r] = x[j + 32r] and
-// leaves with v[t] = y[n1][k2] for k2 = pass_a4_k2(j, t).
+// four-step twiddle: lane L = 32c + j enters with v[r] = x[j + 32r]
+// (normalised, below 2r) and leaves with v[t] = y[n1][k2] for
+// k2 = pass_a4_k2(j, t), normalised and below 2r.
 VKZG_DEV uint32_t pass_a4_k2(uint32_t j, int t) {
     // t = 0, 1: j' = j, outputs j', j' + 64; t = 2, 3: j' = j + 32
     return j + 32u * (uint32_t)(t >> 1) + 64u * (uint32_t)(t & 1);
 }
-VKZG_DEV void pass_a4(Fr (&v)[4], const Dir &d, const Exch4 &x, uint32_t lane, uint32_t n1) {
+VKZG_DEV void pass_a4(FrL (&v)[4], const Dir4 &d, const Exch4 &x, uint32_t lane, uint32_t n1) {
     const uint32_t c = lane >> 5, j = lane & 31u, col = c * 128u;
-    dft4(v[0], v[1], v[2], v[3], d.w8_2);
+    dft4(v[0], v[1], v[2], v[3], d.w4);  // < 12
     exch_sync();
     stage4_store<1>(v, x, col, j);
     exch_sync();
-    stage4_load<128, 4>(v, d, x, col, j);
+    stage4_load<128, 4>(v, d, x, col, j);  // < 22
     exch_sync();
     stage4_store<4>(v, x, col, j);
     exch_sync();
-    stage4_load<128, 16>(v, d, x, col, j);
+    stage4_load<128, 16>(v, d, x, col, j);  // < 32
     exch_sync();
     stage4_store<16>(v, x, col, j);
     exch_sync();
     // radix-2 stage (Ns = 64): j' = j and j + 32, d[j'] and d[j' + 64],
     // twiddle w128^j' = w^(64 j')
-    Fr u0 = x.get(col + j), u1 = x.get(col + j + 64u);
-    Fr u2 = x.get(col + j + 32u), u3 = x.get(col + j + 96u);
-    bfly(u0, u1, d.pow(64u * j));
-    bfly(u2, u3, d.pow(64u * (j + 32u)));
-    v[0] = fr_mul(u0, d.pow(n1 * pass_a4_k2(j, 0)));
-    v[1] = fr_mul(u1, d.pow(n1 * pass_a4_k2(j, 1)));
-    v[2] = fr_mul(u2, d.pow(n1 * pass_a4_k2(j, 2)));
-    v[3] = fr_mul(u3, d.pow(n1 * pass_a4_k2(j, 3)));
+    const FrL u0 = frl_norm(x.get(col + j)), u2 = frl_norm(x.get(col + j + 32u));
+    const FrL t1 = frl_mul(x.get(col + j + 64u), d.pow(64u * j));           // < 2
+    const FrL t3 = frl_mul(x.get(col + j + 96u), d.pow(64u * (j + 32u)));  // < 2
+    v[0] = frl_mul(frl_add(u0, t1), d.pow(n1 * pass_a4_k2(j, 0)));   // first operands < 34
+    v[1] = frl_mul(frl_sub4(u0, t1), d.pow(n1 * pass_a4_k2(j, 1)));  // < 36
+    v[2] = frl_mul(frl_add(u2, t3), d.pow(n1 * pass_a4_k2(j, 2)));
+    v[3] = frl_mul(frl_sub4(u2, t3), d.pow(n1 * pass_a4_k2(j, 3)));
+}
+
+// Both vanishing values' work split, in FrL: the product over the cells c
+// with (present[c] != 0) == want of (x - w128^brev7(c)), for 4 values per
+// wave (16 lanes each; see vanishing_product).  x normalised below 2r.
+VKZG_DEV FrL vanishing_product4(const FrL &x, const uint8_t *present, bool want, const uint4 *roots,
+                                uint32_t lane) {
+    constexpr uint32_t kParts = 16;
+    const uint64_t m0 = __ballot((present[lane] != 0) == want);
+    const uint64_t m1 = __ballot((present[64u + lane] != 0) == want);
+    const uint32_t n0 = __popcll(m0), n = n0 + __popcll(m1);
+    const auto factor = [&](uint32_t t) {  // below 6r, limbs below 2^29 + 2^30
+        const uint32_t cell = t < n0 ? nth_set_bit(m0, t) : 64u + nth_set_bit(m1, t - n0);
+        return frl_sub4(x, ld_frl_tab(roots, (64u * brev_bits(cell, 7)) & 8191u));
+    };
+    FrL acc0 = frl_const(cfrl::ONE_M), acc1 = frl_const(cfrl::ONE_M);
+#pragma unroll 1
+    for (uint32_t t = lane & (kParts - 1u); t < n; t += 2 * kParts) {
+        acc0 = frl_mul(factor(t), acc0);
+        if (t + kParts < n) acc1 = frl_mul(factor(t + kParts), acc1);
+    }
+    FrL prod = frl_mul(acc0, acc1);
+#pragma unroll
+    for (int m = 1; m < (int)kParts; m <<= 1) prod = frl_mul(prod, frl_shfl_xor(prod, m));
+    return prod;
 }
 
 template <int IN_MODE>
 __global__ void __launch_bounds__(256, 2)
-    k_rec4_first(Fr *out, const void *in, const uint8_t *present, const Fr *roots, const Fr *roots_inv,
+    k_rec4_first(Fr *out, const void *in, const uint8_t *present, const uint4 *roots, const uint4 *roots_inv,
                  uint32_t nitems) {
-    extern __shared__ uint4 lds[];  // 8 KiB per wave
+    extern __shared__ uint4 lds[];  // 9 KiB per wave
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const uint32_t item = blockIdx.x * (blockDim.x >> 6) + wave;
     if (item >= nitems) return;
     const uint32_t blob = item / kItemsPerBlob4, q = item % kItemsPerBlob4;
-    const Exch4 x{lds + 512 * wave};
+    const Exch4 x{lds + kExch4Uint4 * wave};
     const uint32_t c = lane >> 5, j = lane & 31u, n1 = 2u * q + c;
-    Fr v[4];
+    FrL v[4];
     if (IN_MODE == 0) {
         // Z(omega^i) for i & 127 = n1 + 64 (j & 1): value vv = c + 2 (j & 1)
         // of the 4 this wave computes (k = 2q + (vv & 1) + 64 (vv >> 1)).
@@ -549,62 +639,67 @@ __global__ void __launch_bounds__(256, 2)
         }
         const uint32_t vv = lane >> 4;
         const uint32_t k = 2u * q + (vv & 1u) + 64u * (vv >> 1);
-        const Fr zd = vanishing_product<16>(roots[(64u * k) & 8191u], present + 128ull * blob, false, roots, lane);
-        const Fr zr = fr_mul(fr_shfl(zd, (int)((c + 2u * (j & 1u)) * 16u)), fr_const(cfr::R2));
+        const FrL zd = vanishing_product4(ld_frl_tab(roots, (64u * k) & 8191u), present + 128ull * blob, false,
+                                          roots, lane);
+        // times 2^522: one product takes a plain cell element to E Z in Montgomery form
+        const FrL zr = frl_mul(frl_shfl(zd, (int)((c + 2u * (j & 1u)) * 16u)), frl_const(cfrl::R2));
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             const uint4 lo = raw[r][0], hi = raw[r][1];
             const uint32_t words[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
-            Fr e = fr_from_be_words(words);
-            fr_reduce_once(e.v, e.v);
-            fr_reduce_once(e.v, e.v);
-            v[r] = fr_mul(e, zr);
+            const Fr e = fr_from_be_words(words);  // any 256-bit value: below 3r, no reduction needed
+            v[r] = frl_mul(frl_unpack(e.v), zr);
         }
-        pass_a4(v, Dir(roots_inv), x, lane, n1);
+        pass_a4(v, Dir4(roots_inv), x, lane, n1);
     } else {
+        // Fr coefficients in the 8 x 32-bit Montgomery form (x 2^256)
         const Fr *src = reinterpret_cast<const Fr *>(in) + (size_t)blob * 8192;
+        const FrL conv = frl_const(cfrl::FROM_R256);
 #pragma unroll
-        for (int r = 0; r < 4; r++) v[r] = ld_fr(src + n1 + 64u * (j + 32u * (uint32_t)r));
-        pass_a4(v, Dir(roots), x, lane, n1);
+        for (int r = 0; r < 4; r++) {
+            const Fr e = ld_fr(src + n1 + 64u * (j + 32u * (uint32_t)r));
+            v[r] = frl_mul(frl_unpack(e.v), conv);
+        }
+        pass_a4(v, Dir4(roots), x, lane, n1);
     }
     Fr *dst = out + (size_t)blob * 8192;
 #pragma unroll
-    for (int t = 0; t < 4; t++) st_fr(dst + s4_index(n1, pass_a4_k2(j, t)), v[t]);
+    for (int t = 0; t < 4; t++) st_frl(dst + s4_index(n1, pass_a4_k2(j, t)), v[t]);
 }
 
 template <int STAGE>
 __global__ void __launch_bounds__(256, 2)
-    k_rec4_mid(Fr *out, const Fr *in, const uint8_t *present, const Fr *roots, const Fr *roots_inv,
-               const Fr *shift, uint32_t nitems) {
+    k_rec4_mid(Fr *out, const Fr *in, const uint8_t *present, const uint4 *roots, const uint4 *roots_inv,
+               const uint4 *shift, uint32_t nitems) {
     extern __shared__ uint4 lds[];
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const uint32_t item = blockIdx.x * (blockDim.x >> 6) + wave;
     if (item >= nitems) return;
     const uint32_t blob = item / kItemsPerBlob4, q = item % kItemsPerBlob4;
-    const Exch4 x{lds + 512 * wave};
+    const Exch4 x{lds + kExch4Uint4 * wave};
     const Fr *src = in + (size_t)blob * 8192 + q * 256u;
-    Fr v[4];
+    FrL v[4];
 #pragma unroll
-    for (int r = 0; r < 4; r++) v[r] = ld_fr(src + 64u * (uint32_t)r + lane);
+    for (int r = 0; r < 4; r++) v[r] = ld_frl(src + 64u * (uint32_t)r + lane);
     const bool b_inverse = STAGE != 1;
     // lane L = 4j + cc ends pass B with X[k2 + 128 (j + 16 r)], k2 = 2q + (cc & 1) + 64 (cc >> 1)
     const uint32_t cc = lane & 3u, jb = lane >> 2;
     const uint32_t k2 = 2u * q + (cc & 1u) + 64u * (cc >> 1);
-    Fr izc;
+    FrL izc;
     if (STAGE == 1) {  // before pass B: overlaps the loads above
         const uint32_t vv = lane >> 4;
         const uint32_t kv = 2u * q + (vv & 1u) + 64u * (vv >> 1);
-        const Fr y = fr_mul(roots[(64u * kv) & 8191u], fr_const(cfr::SEVEN_POW64_M));
-        const Fr p = vanishing_product<16>(y, present + 128ull * blob, true, roots, lane);
-        izc = fr_mul(fr_shfl(p, (int)(cc * 16u)), fr_const(cfr::INV_SEVEN_POW8192_MINUS_1_M));
+        const FrL y = frl_mul(ld_frl_tab(roots, (64u * kv) & 8191u), frl_const(cfrl::SEVEN_POW64_M));
+        const FrL p = vanishing_product4(y, present + 128ull * blob, true, roots, lane);
+        izc = frl_mul(frl_shfl(p, (int)(cc * 16u)), frl_const(cfrl::INV_SEVEN_POW8192_MINUS_1_M));
     }
-    pass_b4(v, Dir(b_inverse ? roots_inv : roots), x, lane);
+    pass_b4(v, Dir4(b_inverse ? roots_inv : roots), x, lane);  // < 32
     if (STAGE == 1) {
 #pragma unroll
-        for (int r = 0; r < 4; r++) v[r] = fr_mul(v[r], izc);
+        for (int r = 0; r < 4; r++) v[r] = frl_mul(v[r], izc);  // 32 x 2 < 70
     } else {
 #pragma unroll
-        for (int r = 0; r < 4; r++) v[r] = fr_mul(v[r], shift[k2 + 128u * (jb + 16u * (uint32_t)r)]);
+        for (int r = 0; r < 4; r++) v[r] = frl_mul(v[r], ld_frl_tab(shift, k2 + 128u * (jb + 16u * (uint32_t)r)));
     }
     // next pass A column c = cc & 1 takes x[n2] = X'[k1 = n2 >> 1] of the
     // pass-B column with h = n2 & 1 = cc >> 1
@@ -615,40 +710,55 @@ __global__ void __launch_bounds__(256, 2)
     const uint32_t c = lane >> 5, j = lane & 31u, n1 = 2u * q + c;
 #pragma unroll
     for (int r = 0; r < 4; r++) v[r] = x.get(c * 128u + j + 32u * (uint32_t)r);
-    pass_a4(v, Dir(b_inverse ? roots : roots_inv), x, lane, n1);
+    pass_a4(v, Dir4(b_inverse ? roots : roots_inv), x, lane, n1);
     Fr *dst = out + (size_t)blob * 8192;
 #pragma unroll
-    for (int t = 0; t < 4; t++) st_fr(dst + s4_index(n1, pass_a4_k2(j, t)), v[t]);
+    for (int t = 0; t < 4; t++) st_frl(dst + s4_index(n1, pass_a4_k2(j, t)), v[t]);
 }
 
 __global__ void __launch_bounds__(256, 2)
-    k_rec4_last(uint8_t *cells, const Fr *in, const Fr *roots, uint32_t nitems) {
+    k_rec4_last(uint8_t *cells, const Fr *in, const uint4 *roots, uint32_t nitems) {
     extern __shared__ uint4 lds[];
     const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
     const uint32_t item = blockIdx.x * (blockDim.x >> 6) + wave;
     if (item >= nitems) return;
     const uint32_t blob = item / kItemsPerBlob4, q = item % kItemsPerBlob4;
-    const Exch4 x{lds + 512 * wave};
+    const Exch4 x{lds + kExch4Uint4 * wave};
     const Fr *src = in + (size_t)blob * 8192 + q * 256u;
-    Fr v[4];
+    FrL v[4];
 #pragma unroll
-    for (int r = 0; r < 4; r++) v[r] = ld_fr(src + 64u * (uint32_t)r + lane);
-    pass_b4(v, Dir(roots), x, lane);
+    for (int r = 0; r < 4; r++) v[r] = ld_frl(src + 64u * (uint32_t)r + lane);
+    pass_b4(v, Dir4(roots), x, lane);  // < 32
     // X[k1 = j + 16r] of cell brev7(k2) at element 4 brev4(j) + brev2(r)
     const uint32_t cc = lane & 3u, j = lane >> 2;
     const uint32_t k2 = 2u * q + (cc & 1u) + 64u * (cc >> 1);
     uint32_t *dst = reinterpret_cast<uint32_t *>(cells) +
                     ((size_t)blob * 8192 + 64u * brev_bits(k2, 7) + 4u * brev_bits(j, 4)) * 8;
-    Fr one = fr_zero();
-    one.v[0] = 1;
+    const FrL one = frl_const(cfrl::ONE_PLAIN);
 #pragma unroll
     for (int r = 0; r < 4; r++) {
+        // Montgomery -> plain: at most r (r only for 0), then canonical
+        Fr e;
+        frl_pack(e.v, frl_mul(v[r], one));
+        fr_reduce_once(e.v, e.v);
         uint32_t w[8];
-        fr_to_be_words(w, fr_mul(v[r], one));  // Montgomery -> canonical
+        fr_to_be_words(w, e);
         uint4 *p = reinterpret_cast<uint4 *>(dst + 8u * brev_bits((uint32_t)r, 2));
         p[0] = make_uint4(w[0], w[1], w[2], w[3]);
         p[1] = make_uint4(w[4], w[5], w[6], w[7]);
     }
+}
+
+// FrL table from an Fr (8 x 32-bit Montgomery) one: out[k] = in[k] 2^5, as
+// 48-byte entries.
+__global__ void k_frl_table(uint4 *out, const Fr *in, uint32_t n) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const Fr t = fr_mul(in[k], fr_const(cfrl::TO_R261));  // canonical x 2^261
+    const FrL l = frl_unpack(t.v);
+    out[3 * k] = make_uint4(l.v[0], l.v[1], l.v[2], l.v[3]);
+    out[3 * k + 1] = make_uint4(l.v[4], l.v[5], l.v[6], l.v[7]);
+    out[3 * k + 2] = make_uint4(l.v[8], 0u, 0u, 0u);
 }
 
 uint32_t rec_grid(uint32_t nitems, uint32_t waves) { return (nitems + waves - 1) / waves; }
@@ -656,56 +766,66 @@ constexpr uint32_t kExchBytes = 1024 * sizeof(uint4);  // LDS per wave
 
 }  // namespace
 
-hipError_t launch_rec_first(Fr *out, const void *in, int in_mode, const uint8_t *present, const Fr *roots,
-                            const Fr *roots_inv, uint32_t nblobs, uint32_t radix, uint32_t waves, hipStream_t s) {
-    const bool r4 = radix == 4;
-    const uint32_t nitems = nblobs * (r4 ? kItemsPerBlob4 : kItemsPerBlob);
-    const dim3 grid(rec_grid(nitems, waves)), block(64 * waves);
-    const uint32_t lds = (r4 ? kExchBytes / 2 : kExchBytes) * waves;
-    if (r4 && in_mode == 0)
-        hipLaunchKernelGGL(k_rec4_first<0>, grid, block, lds, s, out, in, present, roots, roots_inv, nitems);
-    else if (r4)
-        hipLaunchKernelGGL(k_rec4_first<1>, grid, block, lds, s, out, in, present, roots, roots_inv, nitems);
-    else if (in_mode == 0)
-        hipLaunchKernelGGL(k_rec_first<0>, grid, block, lds, s, out, in, present, roots, roots_inv, nitems);
-    else
-        hipLaunchKernelGGL(k_rec_first<1>, grid, block, lds, s, out, in, present, roots, roots_inv, nitems);
+hipError_t launch_frl_table(uint4 *out, const Fr *in, uint32_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_frl_table, dim3((n + 255) / 256), dim3(256), 0, s, out, in, n);
     return hipGetLastError();
 }
 
-hipError_t launch_rec_mid(Fr *out, const Fr *in, int stage, const uint8_t *present, const Fr *roots,
-                          const Fr *roots_inv, const Fr *shift, uint32_t nblobs, uint32_t radix,
-                          uint32_t waves, hipStream_t s) {
+hipError_t launch_rec_first(Fr *out, const void *in, int in_mode, const uint8_t *present, const RecTables &t,
+                            uint32_t nblobs, uint32_t radix, uint32_t waves, hipStream_t s) {
     const bool r4 = radix == 4;
     const uint32_t nitems = nblobs * (r4 ? kItemsPerBlob4 : kItemsPerBlob);
     const dim3 grid(rec_grid(nitems, waves)), block(64 * waves);
-    const uint32_t lds = (r4 ? kExchBytes / 2 : kExchBytes) * waves;
+    const uint32_t lds = (r4 ? kExch4Uint4 * sizeof(uint4) : kExchBytes) * waves;
+    if (r4 && in_mode == 0)
+        hipLaunchKernelGGL(k_rec4_first<0>, grid, block, lds, s, out, in, present, t.roots4, t.roots_inv4, nitems);
+    else if (r4)
+        hipLaunchKernelGGL(k_rec4_first<1>, grid, block, lds, s, out, in, present, t.roots4, t.roots_inv4, nitems);
+    else if (in_mode == 0)
+        hipLaunchKernelGGL(k_rec_first<0>, grid, block, lds, s, out, in, present, t.roots, t.roots_inv, nitems);
+    else
+        hipLaunchKernelGGL(k_rec_first<1>, grid, block, lds, s, out, in, present, t.roots, t.roots_inv, nitems);
+    return hipGetLastError();
+}
+
+hipError_t launch_rec_mid(Fr *out, const Fr *in, int stage, const uint8_t *present, const RecTables &t,
+                          uint32_t nblobs, uint32_t radix, uint32_t waves, hipStream_t s) {
+    const bool r4 = radix == 4;
+    const uint32_t nitems = nblobs * (r4 ? kItemsPerBlob4 : kItemsPerBlob);
+    const dim3 grid(rec_grid(nitems, waves)), block(64 * waves);
+    const uint32_t lds = (r4 ? kExch4Uint4 * sizeof(uint4) : kExchBytes) * waves;
     if (r4) {
         if (stage == 0)
-            hipLaunchKernelGGL(k_rec4_mid<0>, grid, block, lds, s, out, in, present, roots, roots_inv, shift, nitems);
+            hipLaunchKernelGGL(k_rec4_mid<0>, grid, block, lds, s, out, in, present, t.roots4, t.roots_inv4,
+                               t.shift_fwd4, nitems);
         else if (stage == 1)
-            hipLaunchKernelGGL(k_rec4_mid<1>, grid, block, lds, s, out, in, present, roots, roots_inv, shift, nitems);
+            hipLaunchKernelGGL(k_rec4_mid<1>, grid, block, lds, s, out, in, present, t.roots4, t.roots_inv4,
+                               (const uint4 *)nullptr, nitems);
         else
-            hipLaunchKernelGGL(k_rec4_mid<2>, grid, block, lds, s, out, in, present, roots, roots_inv, shift, nitems);
+            hipLaunchKernelGGL(k_rec4_mid<2>, grid, block, lds, s, out, in, present, t.roots4, t.roots_inv4,
+                               t.shift_inv4, nitems);
     } else {
         if (stage == 0)
-            hipLaunchKernelGGL(k_rec_mid<0>, grid, block, lds, s, out, in, present, roots, roots_inv, shift, nitems);
+            hipLaunchKernelGGL(k_rec_mid<0>, grid, block, lds, s, out, in, present, t.roots, t.roots_inv,
+                               t.shift_fwd, nitems);
         else if (stage == 1)
-            hipLaunchKernelGGL(k_rec_mid<1>, grid, block, lds, s, out, in, present, roots, roots_inv, shift, nitems);
+            hipLaunchKernelGGL(k_rec_mid<1>, grid, block, lds, s, out, in, present, t.roots, t.roots_inv,
+                               (const Fr *)nullptr, nitems);
         else
-            hipLaunchKernelGGL(k_rec_mid<2>, grid, block, lds, s, out, in, present, roots, roots_inv, shift, nitems);
+            hipLaunchKernelGGL(k_rec_mid<2>, grid, block, lds, s, out, in, present, t.roots, t.roots_inv,
+                               t.shift_inv, nitems);
     }
     return hipGetLastError();
 }
 
-hipError_t launch_rec_last(uint8_t *cells, const Fr *in, const Fr *roots, uint32_t nblobs, uint32_t radix,
+hipError_t launch_rec_last(uint8_t *cells, const Fr *in, const RecTables &t, uint32_t nblobs, uint32_t radix,
                            uint32_t waves, hipStream_t s) {
     const bool r4 = radix == 4;
     const uint32_t nitems = nblobs * (r4 ? kItemsPerBlob4 : kItemsPerBlob);
     const dim3 grid(rec_grid(nitems, waves)), block(64 * waves);
-    const uint32_t lds = (r4 ? kExchBytes / 2 : kExchBytes) * waves;
-    if (r4) hipLaunchKernelGGL(k_rec4_last, grid, block, lds, s, cells, in, roots, nitems);
-    else hipLaunchKernelGGL(k_rec_last, grid, block, lds, s, cells, in, roots, nitems);
+    const uint32_t lds = (r4 ? kExch4Uint4 * sizeof(uint4) : kExchBytes) * waves;
+    if (r4) hipLaunchKernelGGL(k_rec4_last, grid, block, lds, s, cells, in, t.roots4, nitems);
+    else hipLaunchKernelGGL(k_rec_last, grid, block, lds, s, cells, in, t.roots, nitems);
     return hipGetLastError();
 }
 

@@ -255,6 +255,7 @@ struct vkzg_prover {
     // Resident tables.
     vkzg_amd::DeviceArena arena;
     vkzg_amd::Fr *roots = nullptr, *roots_inv = nullptr, *shift_fwd = nullptr, *shift_inv = nullptr;
+    uint4 *frl_tabs[4] = {};  // the four tables above as FrL entries (radix-4 recovery)
     vkzg_amd::G1A *srs = nullptr, *xext = nullptr;
     vkzg_amd::G1AT *table = nullptr;  // the main phase-A table (window c)
     uint32_t c = 0, W = 0, H = 0;
