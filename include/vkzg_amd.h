@@ -142,7 +142,7 @@ vkzg_result vkzg_amd_prover_info(const vkzg_prover *p, vkzg_amd_info *out);
  * bit-identical results.
  * No reference counterpart: the reference fixes one launch shape. */
 typedef struct {
-    uint32_t phase_a_groups;        /* window groups (work-queue tiles) per phase-A MSM, 1-10 */
+    uint32_t phase_a_groups;        /* window groups (work-queue tiles) per phase-A MSM, 1-20 */
     uint32_t phase_a_blocks_per_cu; /* persistent phase-A grid in blocks per CU, 1-2 */
     uint32_t normalize_chunk;       /* points per batched inversion, 1-256; 0 = the smallest chunk
                                        whose lanes fit in one wave per SIMD (the default) */

@@ -22,7 +22,7 @@ def restore_tuning(gpu_prover):
         gpu_prover.set_tuning(n, **t)
 
 
-@pytest.mark.parametrize("groups", [1, 2, 4, 5, 10])
+@pytest.mark.parametrize("groups", [1, 2, 4, 5, 10, 20])
 def test_phase_a_groups_bit_exact(gpu_prover, vectors, restore_tuning, groups):
     blobs = b"".join(vectors.blob(n) for n in NAMES)
     want = b"".join(vectors.proofs(n) for n in NAMES)
@@ -92,7 +92,7 @@ def test_tuning_rejects_out_of_range(gpu_prover, restore_tuning):
     for bad in (dict(phase_a_groups=0), dict(phase_a_blocks_per_cu=3),
                 dict(normalize_chunk=257), dict(splits=5),
                 dict(recovery_waves=3), dict(recovery_waves=8), dict(recovery_radix=2), dict(ladder_splits=3),
-                dict(phase_a_groups=11), dict(reduce_lanes=2), dict(reduce_lanes=64)):
+                dict(phase_a_groups=21), dict(reduce_lanes=2), dict(reduce_lanes=64)):
         with pytest.raises(vkzg_amd.VkzgError) as e:
             gpu_prover.set_tuning(3, **bad)
         assert e.value.code == vkzg_amd.ERR_BADARGS
@@ -105,7 +105,7 @@ def test_autotune_keeps_results(gpu_prover_small_batch, vectors):
     assert info["autotuned"] == 1 and ms > 0
     for n in (1, 4):
         t = p.get_tuning(n)
-        assert 1 <= t["phase_a_groups"] <= 10 and 1 <= t["splits"] <= 4 and t["ladder_splits"] in (1, 2, 4)
+        assert 1 <= t["phase_a_groups"] <= 20 and 1 <= t["splits"] <= 4 and t["ladder_splits"] in (1, 2, 4)
         assert t["reduce_lanes"] in (4, 8, 16, 32)
     names = [NAMES[i % 3] for i in range(6)]  # chunks of 4 + 2
     blobs = b"".join(vectors.blob(n) for n in names)

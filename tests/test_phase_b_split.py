@@ -134,3 +134,58 @@ def test_phase_b_groups_cover_every_output_once():
                     segs.add(output_rules(a)[0])
             assert len(segs) == 1  # one item list per wave
         assert sorted(seen) == [(b, a) for b in range(nblobs) for a in range(128)]
+
+
+def reduce_lane_model(B, per, l, handle_doubling):
+    """k_bucket_reduce's per-lane running sums on scalars standing in for
+    points (0 = the identity): returns (rs, s, flagged), flagged when an
+    addition meets a == +-b with both non-identity (the case the fast
+    additions flag)."""
+    flagged = False
+
+    def add(a, b):
+        nonlocal flagged
+        if a and b and (a == b or (a + b) % R == 0):
+            flagged = True
+        return (a + b) % R
+
+    s = rs = 0
+    single = False
+    for i in range(per - 1, -1, -1):
+        b = B[l * per + i]
+        s = add(s, b)
+        if handle_doubling and b == 0 and single:
+            rs = 2 * rs % R
+            single = False
+        else:
+            single = rs == 0 and s != 0
+            rs = add(rs, s)
+    return rs, s, flagged
+
+
+def test_reduction_empty_buckets_take_the_doubling():
+    """The segments' item lists leave some buckets empty, the same ones for
+    every blob.  Without the doubling step some lane count meets a running sum
+    equal to s (and every output of the segment would go to the checked
+    recompute); with it no lane count flags, and the weighted sum is exact."""
+    kp, km = circulant_taps()
+    taps = segment_taps(kp, km)
+    lists = [items_of(t) for t in taps]
+    empties = [[k for k in range(128) if not lists[s][k]] for s in range(3)]
+    assert any(empties), "the split item lists have empty buckets"
+    rng = random.Random(9)
+    naive_flags = set()
+    for seg in range(3):
+        B = [0 if k in empties[seg] else rng.randrange(1, R) for k in range(128)]
+        want = sum((k + 1) * B[k] for k in range(128)) % R
+        for lanes in (4, 8, 16, 32):
+            per = 128 // lanes
+            got = 0
+            for l in range(lanes):
+                rs, s, flagged = reduce_lane_model(B, per, l, True)
+                assert not flagged, (seg, lanes, l)
+                got += rs + per * l * s
+                if reduce_lane_model(B, per, l, False)[2]:
+                    naive_flags.add(lanes)
+            assert got % R == want
+    assert naive_flags  # the case the doubling step exists for

@@ -74,8 +74,11 @@ hipError_t launch_fk20_scalars(Fr *scal, const uint8_t *blobs, uint64_t blob_str
 hipError_t launch_phase_a(G1X *part, const Fr *scal, const G1AT *table, uint32_t c, uint32_t W,
                           uint32_t nmsm, uint32_t groups, uint32_t grid, uint32_t *queue,
                           uint32_t force_checked, hipStream_t s);
-hipError_t launch_ladder(G1J *lad, const G1X *part, uint32_t groups, uint32_t splits, uint32_t nblobs,
-                         hipStream_t s);
+// usum[msm] = sum over g < groups of part[msm * groups + g] (groups <= 24,
+// nmsm a multiple of 32).
+hipError_t launch_sum_groups(G1X *usum, const G1X *part, uint32_t groups, uint32_t nmsm, hipStream_t s);
+// u: one sum per MSM, [nblobs][splits][128].
+hipError_t launch_ladder(G1J *lad, const G1X *u, uint32_t splits, uint32_t nblobs, hipStream_t s);
 // Also clears the work counters of the phase-B and bucket-reduction launches
 // that follow (queue_b[0], queue_r[0]).
 hipError_t launch_glv_phi(G1A *phi, const G1A *pts, uint64_t count, uint32_t *queue_b, uint32_t *queue_r,

@@ -37,17 +37,10 @@ VKZG_DEV G1J g1j_dbl_pair(const G1J &p, uint32_t q) {
 // and chain j of a blob doubles f[j] = Fc[j] (j < 32), Fn[j-32] (j < 64),
 // F-[j-64] (j >= 64): lad[blob][d][j] = 2^(8d) * f[j], d < kLadPos.  With
 // `splits` = M segments, pair (blob, m, j) starts from the same folds of
-// phase A's 2^(8 start_m) u (each u the sum of its `groups` window-group
-// partials part[((blob*M + m)*128 + k)*G + g]) and writes positions
-// [start_m, start_{m+1}): chains of 128 / M doublings.
-VKZG_DEV G1X msm_sum(const G1X *part, size_t msm, uint32_t groups) {
-    G1X u = part[msm * groups];
-#pragma unroll 1
-    for (uint32_t k = 1; k < groups; k++) u = g1x_add(u, part[msm * groups + k]);
-    return u;
-}
-
-__global__ __launch_bounds__(256) void k_ladder(G1J *lad, const G1X *part, uint32_t groups, uint32_t splits, uint32_t nblobs) {
+// phase A's 2^(8 start_m) u (u[(blob*M + m)*128 + k], phase A's MSMs with
+// their window groups summed) and writes positions [start_m, start_{m+1}):
+// chains of 128 / M doublings.
+__global__ __launch_bounds__(256) void k_ladder(G1J *lad, const G1X *u, uint32_t splits, uint32_t nblobs) {
     // A long dependent chain: give these waves issue priority over the
     // throughput kernels of another sub-batch sharing the SIMD.
     __builtin_amdgcn_s_setprio(3);
@@ -60,12 +53,12 @@ __global__ __launch_bounds__(256) void k_ladder(G1J *lad, const G1X *part, uint3
     G1X f;
     if (j < 64) {
         const uint32_t i = j & 31u;
-        const G1X lo = g1x_add(msm_sum(part, base + i, groups), msm_sum(part, base + i + 64, groups));
-        const G1X hi = g1x_add(msm_sum(part, base + i + 32, groups), msm_sum(part, base + i + 96, groups));
+        const G1X lo = g1x_add(u[base + i], u[base + i + 64]);
+        const G1X hi = g1x_add(u[base + i + 32], u[base + i + 96]);
         f = g1x_add(lo, j < 32 ? hi : g1x_neg(hi));
     } else {
         const uint32_t i = j - 64;
-        f = g1x_add(msm_sum(part, base + i, groups), g1x_neg(msm_sum(part, base + i + 64, groups)));
+        f = g1x_add(u[base + i], g1x_neg(u[base + i + 64]));
     }
     G1J p = g1j_from_xyzz(f);
     const uint32_t seg = 16 / splits;  // positions per segment; the last one also takes d = 16
@@ -85,6 +78,24 @@ __global__ __launch_bounds__(256) void k_ladder(G1J *lad, const G1X *part, uint3
     }
 }
 
+// The window-group partials of phase A summed per MSM:
+// usum[msm] = sum_g part[msm * G + g].  Eight lanes per MSM: lane t adds
+// partials t, t + 8 and t + 16, then a three-level lane tree.  With G > 1 the
+// ladder reads these sums, so its prologue (a dependent chain per pair of
+// lanes) adds no partials of its own.  nmsm is a multiple of 128, so the grid
+// holds whole MSMs and no lane leaves before the shuffles.
+__global__ __launch_bounds__(256) void k_sum_groups(G1X *usum, const G1X *part, uint32_t groups, uint32_t nmsm) {
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t msm = tid / 8, t = tid % 8;
+    if (msm >= nmsm) return;
+    G1X acc = g1x_identity();
+#pragma unroll 1
+    for (uint32_t g = t; g < groups; g += 8) acc = g1x_add(acc, part[(size_t)msm * groups + g]);
+#pragma unroll 1
+    for (int m = 1; m < 8; m <<= 1) acc = g1x_add(acc, g1x_shfl_xor(acc, m));
+    if (t == 0) usum[msm] = acc;
+}
+
 // phi[k] = (beta x, y) of pts[k]: the endomorphism [lambda], lambda = z^2 - 1
 // (the identity (0, 0) maps to itself).
 __global__ void k_glv_phi(G1A *phi, const G1A *pts, uint64_t count, uint32_t *queue_b, uint32_t *queue_r) {
@@ -101,14 +112,20 @@ __global__ void k_glv_phi(G1A *phi, const G1A *pts, uint64_t count, uint32_t *qu
 
 }  // namespace
 
-hipError_t launch_ladder(G1J *lad, const G1X *part, uint32_t groups, uint32_t splits, uint32_t nblobs,
-                         hipStream_t s) {
+hipError_t launch_ladder(G1J *lad, const G1X *u, uint32_t splits, uint32_t nblobs, hipStream_t s) {
     // Workgroups of four waves: the dispatcher spreads a workgroup's waves over
     // a CU's four SIMDs, where one-wave workgroups doubled up on some SIMDs
     // while others idled (1.6 ms instead of 0.9 ms for 256 blobs; each wave's
     // chain is issue-bound, so a shared SIMD halves its rate).
     hipLaunchKernelGGL(k_ladder, dim3(blocks_for(nblobs * 128ull * splits * 2, 256)), dim3(256), 0, s, lad,
-                       part, groups, splits, nblobs);
+                       u, splits, nblobs);
+    return hipGetLastError();
+}
+
+hipError_t launch_sum_groups(G1X *usum, const G1X *part, uint32_t groups, uint32_t nmsm, hipStream_t s) {
+    if (nmsm % 32 != 0 || groups > 24) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_sum_groups, dim3(blocks_for(nmsm * 8ull, 256)), dim3(256), 0, s, usum, part, groups,
+                       nmsm);
     return hipGetLastError();
 }
 

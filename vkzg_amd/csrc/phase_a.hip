@@ -45,11 +45,14 @@ __device__ __attribute__((noinline)) void phase_a_slow(G1X *out, const Fr *scal,
 // with signed c-bit digits v in [-(2^(c-1)-1), 2^(c-1)] (the recoding of
 // k_phase_a_sort.comp:31-40 with a c-bit window) and
 // table[j][i][w][|v|-1] = |v| * 2^(cw) * xext[j][i].  Tile (msm, g) writes its
-// partial sum to part[msm * G + g]; the ladder adds the G partials.  A block
-// takes 32 consecutive tiles per queue item (8 lanes per tile, 8 terms per
-// lane) from the counter q[0] until the queue is drained; every block exits
-// after one failed take, so the grid always drains.  q[0] is cleared by the
-// kernel that runs before this one.
+// partial sum to part[msm * G + g]; k_sum_groups adds the G partials.  Tiles
+// are numbered group-major (tile = g * nmsm + msm): nmsm is a multiple of 128,
+// so the 8 tiles of a wave share g and run the same windows -- mixed groups
+// in one wave would make it step through the union of their windows.  A
+// block takes 32 consecutive tiles per queue item (8 lanes per tile, 8 terms
+// per lane) from the counter q[0] until the queue is drained; every block
+// exits after one failed take, so the grid always drains.  q[0] is cleared by
+// the kernel that runs before this one.
 __global__ void __launch_bounds__(256, 2)
     k_phase_a(G1X *part, const Fr *scal, const G1AT *table, uint32_t c, uint32_t W, uint32_t nmsm,
               uint32_t groups, uint32_t *q, uint32_t force_checked) {
@@ -67,8 +70,8 @@ __global__ void __launch_bounds__(256, 2)
         if (item >= nitems) break;
         const uint32_t tile = item * kTilesPerItem + threadIdx.x / kMsmLanes;
         const bool live = tile < ntiles;
-        const uint32_t msm = live ? tile / groups : 0u;
-        const uint32_t g = live ? tile % groups : 0u;
+        const uint32_t g = live ? tile / nmsm : 0u;
+        const uint32_t msm = live ? tile % nmsm : 0u;
         const uint32_t j = msm % 128;
         const uint32_t w0 = g * W / groups, w1 = (g + 1) * W / groups;
         G1X acc = g1x_identity();
@@ -97,18 +100,18 @@ __global__ void __launch_bounds__(256, 2)
                 }
             }
         }
-        // Sum the 16 partial results of this tile.
+        // Sum the 8 lanes' partial results of this tile.
 #pragma unroll 1
         for (int m = 1; m < kMsmLanes; m <<= 1) acc = g1x_add_fast(acc, g1x_shfl_xor(acc, m), exc);
         const uint64_t flagged = __ballot(exc);
         const uint32_t sub = (threadIdx.x & 63u) / kMsmLanes;
         const bool redo = ((flagged >> (sub * kMsmLanes)) & ((1ull << kMsmLanes) - 1)) != 0;
         if (live && t == 0) {
+            G1X *out = &part[(size_t)msm * groups + g];
             if (redo)
-                phase_a_slow(&part[tile], scal + (size_t)msm * 64, table + (size_t)j * 64 * W * H, c,
-                             W, w0, w1);
+                phase_a_slow(out, scal + (size_t)msm * 64, table + (size_t)j * 64 * W * H, c, W, w0, w1);
             else
-                part[tile] = acc;
+                *out = acc;
         }
     }
 }

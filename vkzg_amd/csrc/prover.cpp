@@ -297,7 +297,7 @@ void build_tables(vkzg_prover *p, const uint8_t *setup) {
 uint64_t WorkSet::bytes_per_blob() {
     const uint64_t npos = 128ull * kLadPos;
     return kBlobBytes + kProofBytes + 4 * 4 + 128 + kMaxLadSplits * 128 * 64 * sizeof(Fr) +
-           kMaxLadSplits * 128 * kMaxGroups * sizeof(G1X) +
+           kMaxLadSplits * 128 * (kMaxGroups + 1) * sizeof(G1X) +
            npos * sizeof(G1J) + 2 * npos * sizeof(G1A) + npos * sizeof(Fp) +
            128 * 128 * (sizeof(G1X) + 1) + 128 * sizeof(G1X) + 2 * kCellsBytes + 128 +
            2 * 8192 * sizeof(Fr);
@@ -315,7 +315,7 @@ void WorkSet::allocate(uint32_t B) {
     queue_r = arena.dalloc<uint32_t>(B);
     ck(hipMemset(queue_r, 0, (size_t)B * sizeof(uint32_t)));
     scal = arena.dalloc<Fr>((size_t)B * 128 * 64 * kMaxLadSplits);
-    u = arena.dalloc<G1X>((size_t)B * 128 * kMaxGroups * kMaxLadSplits);
+    u = arena.dalloc<G1X>((size_t)B * 128 * (kMaxGroups + 1) * kMaxLadSplits);
     lad = arena.dalloc<G1J>((size_t)B * 128 * kLadPos);
     lad_aff = arena.dalloc<G1A>((size_t)B * 128 * kLadPos);
     lad_phi = arena.dalloc<G1A>((size_t)B * 128 * kLadPos);
@@ -353,13 +353,19 @@ Tuning tuning_for(const vkzg_prover *p, uint32_t nb) {
 }
 
 // Defaults before (or without) autotuning: small batches split each MSM into
-// more window groups, so that more lanes work on the few MSMs there are; the
-// normalisation chunk is fitted to the chip (norm_chunk 0).
+// more window groups and the ladder into segments, so that more lanes work on
+// the few MSMs and chains there are (one wave per SIMD already saturates
+// phase A at 64 blobs); the normalisation chunk is fitted to the chip
+// (norm_chunk 0).
 void default_tunings(vkzg_prover *p) {
-    static const uint32_t groups[kTuneClasses] = {4, 4, 2, 1, 1};
+    static const uint32_t groups[kTuneClasses] = {10, 10, 4, 1, 1};
+    static const uint32_t ladder_splits[kTuneClasses] = {4, 2, 1, 1, 1};
+    static const uint32_t reduce_lanes[kTuneClasses] = {32, 32, 32, 16, 8};
     for (int c = 0; c < kTuneClasses; c++) {
         p->tune[c] = Tuning{};
         p->tune[c].groups = groups[c];
+        p->tune[c].ladder_splits = ladder_splits[c];
+        p->tune[c].reduce_lanes = reduce_lanes[c];
         p->tune[c].norm_chunk = 0;
     }
 }
@@ -419,7 +425,9 @@ void proofs_range(vkzg_prover *p, WorkSet &ws, uint32_t off, uint32_t nb, uint8_
     uint32_t *err = ws.err + off;
     uint32_t *queue = ws.queue + off;
     Fr *scal = ws.scal + (size_t)off * 8192 * kMaxLadSplits;
-    G1X *u = ws.u + (size_t)off * 128 * kMaxGroups * kMaxLadSplits;
+    // phase-A partials [nb][splits][128][groups], then their sums [nb][splits][128]
+    G1X *u = ws.u + (size_t)off * 128 * (kMaxGroups + 1) * kMaxLadSplits;
+    G1X *usum = u + (size_t)nb * 128 * kMaxGroups * kMaxLadSplits;
     const size_t npos = (size_t)128 * kLadPos;  // ladder points per blob
     G1J *lad = ws.lad + off * npos;
     G1A *lad_aff = ws.lad_aff + off * npos, *lad_phi = ws.lad_phi + off * npos;
@@ -432,7 +440,11 @@ void proofs_range(vkzg_prover *p, WorkSet &ws, uint32_t off, uint32_t nb, uint8_
     ck(launch_phase_a(u, scal, tab.table, tab.c, tab.W, nb * 128 * tn.ladder_splits, tn.groups,
                       p->cus * tn.blocks_per_cu, queue, tn.force_checked, s));
     clk.mark("phase_a");
-    ck(launch_ladder(lad, u, tn.groups, tn.ladder_splits, nb, s));
+    if (tn.groups > 1) {
+        ck(launch_sum_groups(usum, u, tn.groups, nb * 128 * tn.ladder_splits, s));
+        clk.mark("sum_groups");
+    }
+    ck(launch_ladder(lad, tn.groups > 1 ? usum : u, tn.ladder_splits, nb, s));
     clk.mark("ladder");
     const uint32_t chunk = tn.norm_chunk ? tn.norm_chunk : fit_norm_chunk((uint64_t)nb * npos, p->cus);
     ck(launch_jac_to_affine(lad_aff, lad, prefix, (uint64_t)nb * npos, chunk, s));
@@ -703,7 +715,7 @@ void autotune(vkzg_prover *p) {
                 best = cand;
             }
         };
-        for (uint32_t g : {1u, 2u, 4u, 5u, 10u}) { Tuning t = best; t.groups = g; consider(t); }
+        for (uint32_t g : {1u, 2u, 4u, 5u, 10u, 20u}) { Tuning t = best; t.groups = g; consider(t); }
         if (nb <= 64)
             for (uint32_t m : {2u, 4u}) { Tuning t = best; t.ladder_splits = m; consider(t); }
         for (uint32_t r : {4u, 16u, 32u}) { Tuning t = best; t.reduce_lanes = r; consider(t); }

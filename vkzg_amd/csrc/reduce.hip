@@ -76,10 +76,26 @@ __global__ void __launch_bounds__(256, 2)
         bool exc = force_checked != 0;
         if (o < nout) {
             const G1X *B = buckets + (size_t)o * 128 + l * per;
+            // The item lists leave some buckets empty (the identity), the
+            // same ones for every blob.  An empty bucket right after the
+            // lane's first non-empty one leaves s unchanged while the running
+            // sum still equals s: that update is a doubling, taken here
+            // rather than flagged.  `single` records running sum == s.
+            bool single = false;
 #pragma unroll 1
             for (int i = per - 1; i >= 0; i--) {
-                s = g1x_add_fast(s, B[i], exc);
-                stash.put(g1x_add_fast(stash.get(), s, exc));
+                const G1X b = B[i];
+                const bool empty = fp_is_zero(b.zz);
+                s = g1x_add_fast(s, b, exc);
+                G1X rs = stash.get();
+                if (empty && single) {
+                    rs = g1x_dbl_core(rs);
+                    single = false;
+                } else {
+                    single = fp_is_zero(rs.zz) && !fp_is_zero(s.zz);
+                    rs = g1x_add_fast(rs, s, exc);
+                }
+                stash.put(rs);
             }
         }
         // sum_l per l s_l = per * sum_{m >= 1} S_m, with the suffix sums

@@ -20,7 +20,7 @@ constexpr size_t kBlobBytes = VKZG_BYTES_PER_BLOB;
 constexpr size_t kProofBytes = (size_t)VKZG_NUM_CELL_PROOFS * VKZG_BYTES_PER_PROOF;
 constexpr size_t kCellsBytes = (size_t)VKZG_NUM_CELL_PROOFS * VKZG_BYTES_PER_CELL;
 constexpr int kMaxSplits = 4;  // sub-streams one batch may be spread over
-constexpr uint32_t kMaxGroups = 10;  // phase-A window groups per MSM
+constexpr uint32_t kMaxGroups = 20;  // phase-A window groups per MSM
 
 // Launch shape of the proof pipeline for one batch-size class.  Every choice
 // gives bit-identical proofs; only the time differs.
