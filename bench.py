@@ -30,7 +30,9 @@ FR_M = 136                    # limb-MACs per Fr multiplication (8-limb CIOS)
 # Algorithmic work of one blob's recovery (lean form: 6 NTT_8192, pointwise
 # products, batch-inverted division, vanishing polynomial, I/O), in limb-MACs.
 W_REC = 6.38e7
-LAD_POS = 17                  # ladder positions 2^(8d) u, d < 17 (GLV halves of the phase-B scalars)
+LAD_POS = 17                  # ladder positions 2^(8d) u, d < 17 (8-bit digits of the GLV halves of the phase-B scalars)
+DIGIT_BITS = 8                # doublings between ladder positions
+BUCKETS = 128                 # phase-B buckets per output (digit magnitudes 1..128)
 METRIC = "proof blobs/s @256-blob batch; recovery blobs/s; p50/p99 serve latency"
 HERE = os.path.dirname(os.path.abspath(__file__))
 
@@ -50,9 +52,9 @@ def stage_work(info, nb, items_per_output):
         "phase_a": nb * 128 * (64 * W * madd + 7 * add) * FP_M,
         # 128 chains of 8 (LAD_POS - 1) doublings, after the split-circulant
         # folds of u (3 additions for each of 64 chains, 1 for the other 64)
-        "ladder": nb * (128 * (LAD_POS - 1) * 8 * dbl_j + 256 * add) * FP_M,
+        "ladder": nb * (128 * (LAD_POS - 1) * DIGIT_BITS * dbl_j + 256 * add) * FP_M,
         "phase_b": nb * 128 * items_per_output * madd * FP_M,
-        "bucket_reduce": nb * 128 * (2 * 128 + 8) * add * FP_M,
+        "bucket_reduce": nb * 128 * (2 * BUCKETS + 8) * add * FP_M,
     }
 
 

@@ -11,7 +11,9 @@ import random
 
 R = 0x73EDA753299D7D483339D80809A1D80553BDA402FFFE5BFEFFFFFFFF00000001
 LAMBDA = 0xAC45A4010001A40200000000FFFFFFFF
-LAD_POS = 17
+DIGIT_BITS = 8
+LAD_POS = 17  # signed bytes of a 128-bit half and a carry position
+BUCKETS = 128
 
 
 def circulant_taps():
@@ -38,22 +40,24 @@ def segment_taps(kp, km):
 
 
 def items_of(taps):
-    """(bucket k) -> [(e, pos, neg)]: build_phase_b_items' GLV split and digits."""
-    lists = [[] for _ in range(128)]
+    """(bucket k) -> [(e, pos, neg)]: build_phase_b_items' GLV split and
+    signed 8-bit digits."""
+    lists = [[] for _ in range(BUCKETS)]
+    win = 1 << DIGIT_BITS
     for e, t in enumerate(taps):
         k1, k2 = t % LAMBDA, t // LAMBDA
         assert k1 < 1 << 128 and k2 < 1 << 128
         for g, part in enumerate((k1, k2)):
             carry = 0
             for d in range(LAD_POS):
-                v = ((part >> (8 * d)) & 0xFF if d < 16 else 0) + carry
+                v = ((part >> (DIGIT_BITS * d)) & (win - 1)) + carry
                 carry = 0
-                if v > 128:
-                    v -= 256
+                if v > win // 2:
+                    v -= win
                     carry = 1
                 if v:
                     lists[abs(v) - 1].append((e, d + g * LAD_POS, v < 0))
-        assert carry == 0
+            assert carry == 0
     return lists
 
 
@@ -74,12 +78,12 @@ def phase_b_model(u):
     # k_ladder's chains: Fc (slots 0..31), Fn (32..63), F- (64..127)
     f = [(fp[i] + fp[i + 32]) % R for i in range(32)] + [(fp[i] - fp[i + 32]) % R for i in range(32)] + fm
     # ladder positions: 2^(8d) x and phi(2^(8d) x) = lambda 2^(8d) x
-    pts = [[x * pow(2, 8 * d, R) % R for d in range(LAD_POS)] for x in f]
+    pts = [[x * pow(2, DIGIT_BITS * d, R) % R for d in range(LAD_POS)] for x in f]
     cvals = []
     for a in range(128):
         seg, aa, base = output_rules(a)
         total = 0
-        for k in range(128):
+        for k in range(BUCKETS):
             bucket = 0
             for e, pos, neg in lists[seg][k]:
                 if seg != 0 and e > aa:
@@ -113,27 +117,72 @@ def test_phase_b_segments_equal_ifft_trunc_fft():
     assert [sum(1 for x in t if x) for t in taps] == [17, 17, 33]
 
 
-def test_phase_b_groups_cover_every_output_once():
-    """k_phase_b's lane -> (blob, a) map: pair groups of segments 0 and 1,
-    then one group per blob for segment 2; odd blob counts leave the last
-    pair's upper half idle."""
-    for nblobs in range(1, 6):
-        npairs = (nblobs + 1) // 2
+def work_order(lists):
+    """build_phase_b_items' class order: (segment, bucket) by item count,
+    longest first (stable), and the segment-2 classes before each class."""
+    cls = sorted(((seg, k) for seg in range(3) for k in range(BUCKETS)),
+                 key=lambda c: -len(lists[c[0]][c[1]]))
+    n2 = [0]
+    for seg, _ in cls:
+        n2.append(n2[-1] + (seg == 2))
+    return cls, n2
+
+
+def phase_b_item(item, nblobs, cls, n2, band=32):
+    """k_phase_b's item -> (segment, bucket, [(lane, blob, a)]) in bands of
+    `band` blobs, classes longest first within a band."""
+    band_items = BUCKETS * 2 * band
+    nfull, rest = divmod(nblobs, band)
+    b, j = divmod(item, band_items)
+    b0 = b * band
+    bb = band if b < nfull else rest
+    pb = (bb + 1) // 2
+    start = lambda c: n2[c] * bb + (c - n2[c]) * pb
+    c = 0
+    step = 1 << (len(cls).bit_length() - 1)
+    while step:
+        t = c + step
+        if t < len(cls) and start(t) <= j:
+            c = t
+        step >>= 1
+    assert start(c) <= j and (c + 1 == len(cls) or start(c + 1) > j)
+    g = j - start(c)
+    seg, k = cls[c]
+    lanes = []
+    for lane in range(64):
+        if seg == 2:
+            blob, a = b0 + g, 64 + lane
+        else:
+            blob, a = b0 + 2 * g + (lane >> 5), seg * 32 + (lane & 31)
+        lanes.append((lane, blob, a))
+    return seg, k, lanes
+
+
+def test_phase_b_work_order_covers_every_bucket_once():
+    """k_phase_b's item -> (blob, output, bucket) map over bands of 32 blobs
+    with the classes longest first: every bucket of every output exactly
+    once, one item list per wave, odd blob counts leaving the last pair's
+    upper half idle, and within a band the item lengths never increase."""
+    kp, km = circulant_taps()
+    lists = [items_of(t) for t in segment_taps(kp, km)]
+    cls, n2 = work_order(lists)
+    for nblobs in (1, 2, 3, 5, 31, 32, 33, 64, 67):
+        rest = nblobs % 32
+        nitems = (nblobs // 32) * BUCKETS * 64 + BUCKETS * (2 * ((rest + 1) // 2) + rest)
         seen = []
-        for grp in range(2 * npairs + nblobs):
-            segs = set()
-            for lane in range(64):
-                if grp < 2 * npairs:
-                    blob = 2 * (grp % npairs) + (lane >> 5)
-                    a = (grp // npairs) * 32 + (lane & 31)
-                else:
-                    blob = grp - 2 * npairs
-                    a = 64 + lane
+        prev = None
+        for item in range(nitems):
+            seg, k, lanes = phase_b_item(item, nblobs, cls, n2)
+            for lane, blob, a in lanes:
                 if blob < nblobs:
-                    seen.append((blob, a))
-                    segs.add(output_rules(a)[0])
-            assert len(segs) == 1  # one item list per wave
-        assert sorted(seen) == [(b, a) for b in range(nblobs) for a in range(128)]
+                    assert output_rules(a)[0] == seg
+                    seen.append((blob, a, k))
+            band, j = divmod(item, BUCKETS * 64)
+            n = len(lists[seg][k])
+            if j and prev is not None:
+                assert n <= prev
+            prev = n
+        assert sorted(seen) == [(b, a, k) for b in range(nblobs) for a in range(128) for k in range(BUCKETS)]
 
 
 def reduce_lane_model(B, per, l, handle_doubling):
@@ -164,28 +213,31 @@ def reduce_lane_model(B, per, l, handle_doubling):
 
 
 def test_reduction_empty_buckets_take_the_doubling():
-    """The segments' item lists leave some buckets empty, the same ones for
-    every blob.  Without the doubling step some lane count meets a running sum
-    equal to s (and every output of the segment would go to the checked
-    recompute); with it no lane count flags, and the weighted sum is exact."""
+    """Empty buckets (the identity) are the same for every blob, since the
+    item lists are.  An empty bucket right after a lane's first non-empty one
+    leaves the running sum equal to s: without the doubling step the fast
+    additions would flag it and send every output of the segment to the
+    checked recompute.  With it no lane count flags, on the real lists'
+    empty buckets and on an empty bucket at every position, and the weighted
+    sum stays exact."""
     kp, km = circulant_taps()
     taps = segment_taps(kp, km)
     lists = [items_of(t) for t in taps]
-    empties = [[k for k in range(128) if not lists[s][k]] for s in range(3)]
-    assert any(empties), "the split item lists have empty buckets"
+    patterns = [{k for k in range(BUCKETS) if not lists[s][k]} for s in range(3)]
+    patterns += [{k} for k in range(BUCKETS)] + [{k, k + 1} for k in range(BUCKETS - 1)]
     rng = random.Random(9)
     naive_flags = set()
-    for seg in range(3):
-        B = [0 if k in empties[seg] else rng.randrange(1, R) for k in range(128)]
-        want = sum((k + 1) * B[k] for k in range(128)) % R
+    for empty in patterns:
+        B = [0 if k in empty else rng.randrange(1, R) for k in range(BUCKETS)]
+        want = sum((k + 1) * B[k] for k in range(BUCKETS)) % R
         for lanes in (4, 8, 16, 32):
-            per = 128 // lanes
+            per = BUCKETS // lanes
             got = 0
             for l in range(lanes):
                 rs, s, flagged = reduce_lane_model(B, per, l, True)
-                assert not flagged, (seg, lanes, l)
+                assert not flagged, (sorted(empty), lanes, l)
                 got += rs + per * l * s
                 if reduce_lane_model(B, per, l, False)[2]:
                     naive_flags.add(lanes)
             assert got % R == want
-    assert naive_flags  # the case the doubling step exists for
+    assert naive_flags == {4, 8, 16, 32}  # the case the doubling step exists for

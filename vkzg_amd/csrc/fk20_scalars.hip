@@ -4,8 +4,9 @@
 //                    columns -> NTT_128 -> phase-A scalars (one block per blob,
 //                    the whole polynomial resident in LDS)
 //   k_phase_a        u[j] = sum_i scal[j][i] * xext[j][i]: fixed-base windowed
-//                    MSM against the resident multiples table, 16 lanes per MSM
-//   k_ladder/k_fold  2^(8d) * u[j] and the split-circulant inputs L+-
+//                    MSM against the resident multiples table, 8 lanes per tile
+//   k_sum_groups     the window-group partials of each MSM summed
+//   k_ladder         2^(8d) times the split-circulant folds of u
 //   k_phase_b        phase-B buckets, one lane per (blob, output, bucket)
 //   k_bucket_reduce  sum_k k * B[k-1] per output
 //   k_finalize       C+ +- C-, affine, compressed, bit-reversed order
@@ -78,15 +79,18 @@ __global__ void __launch_bounds__(kScalarThreads)
     const int lane = tid & 63;
     uint32_t *col = cols + wave * 128;
     Fr *out = scal + (size_t)blob * splits * 128 * 64;
-    // Scalar sets of a split ladder: segment m starts at 2^(8 start_m) u.
+    // Scalar sets of a split ladder: segment m starts at 2^(kDigitBits
+    // lad_start(M, m)) u; the multipliers by doubling one (under 8 kLadPos Fr
+    // additions per thread, once per blob).
     Fr mult[kMaxLadSplits];
     mult[0] = fr_one();
-    if (splits == 2) {
-        mult[1] = fr_const(cfr::TWO_POW_64_M);
-    } else if (splits == 4) {
-        mult[1] = fr_const(cfr::TWO_POW_32_M);
-        mult[2] = fr_const(cfr::TWO_POW_64_M);
-        mult[3] = fr_const(cfr::TWO_POW_96_M);
+#pragma unroll 1
+    for (uint32_t m = 1; m < splits; m++) {
+        Fr x = mult[m - 1];
+#pragma unroll 1
+        for (uint32_t k = kDigitBits * lad_start(splits, m - 1); k < kDigitBits * lad_start(splits, m); k++)
+            x = fr_add(x, x);
+        mult[m] = x;
     }
 #pragma unroll 1
     for (int round = 0; round < 64 / kScalarWaves; round++) {

@@ -35,11 +35,11 @@ VKZG_DEV G1J g1j_dbl_pair(const G1J &p, uint32_t q) {
 //   F+[i] = u[i] + u[i+64], F-[i] = u[i] - u[i+64] (i < 64),
 //   Fc[i] = F+[i] + F+[i+32], Fn[i] = F+[i] - F+[i+32] (i < 32),
 // and chain j of a blob doubles f[j] = Fc[j] (j < 32), Fn[j-32] (j < 64),
-// F-[j-64] (j >= 64): lad[blob][d][j] = 2^(8d) * f[j], d < kLadPos.  With
-// `splits` = M segments, pair (blob, m, j) starts from the same folds of
-// phase A's 2^(8 start_m) u (u[(blob*M + m)*128 + k], phase A's MSMs with
-// their window groups summed) and writes positions [start_m, start_{m+1}):
-// chains of 128 / M doublings.
+// F-[j-64] (j >= 64): lad[blob][d][j] = 2^(kDigitBits d) * f[j], d < kLadPos.
+// With `splits` = M segments, pair (blob, m, j) starts from the same folds of
+// phase A's 2^(kDigitBits start_m) u (u[(blob*M + m)*128 + k], phase A's MSMs with
+// their window groups summed), start_m = lad_start(M, m), and writes
+// positions [start_m, start_{m+1}).
 __global__ __launch_bounds__(256) void k_ladder(G1J *lad, const G1X *u, uint32_t splits, uint32_t nblobs) {
     // A long dependent chain: give these waves issue priority over the
     // throughput kernels of another sub-batch sharing the SIMD.
@@ -61,8 +61,7 @@ __global__ __launch_bounds__(256) void k_ladder(G1J *lad, const G1X *u, uint32_t
         f = g1x_add(u[base + i], g1x_neg(u[base + i + 64]));
     }
     G1J p = g1j_from_xyzz(f);
-    const uint32_t seg = 16 / splits;  // positions per segment; the last one also takes d = 16
-    const uint32_t d0 = m * seg, d1 = m + 1 == splits ? kLadPos : d0 + seg;
+    const uint32_t d0 = lad_start(splits, m), d1 = m + 1 == splits ? kLadPos : lad_start(splits, m + 1);
     G1J *dst = lad + (size_t)blob * kLadPos * 128 + j;
 #pragma unroll 1
     for (uint32_t d = d0; d < d1; d++) {
@@ -74,7 +73,7 @@ __global__ __launch_bounds__(256) void k_ladder(G1J *lad, const G1X *u, uint32_t
         }
         if (d + 1 == d1) break;
 #pragma unroll 1
-        for (int k = 0; k < 8; k++) p = g1j_dbl_pair(p, q);
+        for (uint32_t k = 0; k < kDigitBits; k++) p = g1j_dbl_pair(p, q);
     }
 }
 

@@ -9,7 +9,7 @@ namespace {
 // (a >= 64: negacyclic over the 64 points F-); the ladder slots of the
 // three are 0..31, 32..63 and 64..127 of the blob's ladder (k_ladder), laid
 // out [d][slot]: the 64 lanes of a group read 64 consecutive points.  An item (e, d, sign) of segment s
-// adds +-2^(8d) F[a - e] for d < 17 and +-2^(8(d-17)) phi(F[a - e]) above
+// adds +-2^(8d) F[a - e] for d < kLadPos and +-2^(8(d-kLadPos)) phi(F[a - e]) above
 // (the two GLV halves of the tap), with the index cyclic in the segment and
 // the sign flipped when a negacyclic tap wraps (k_phase_b_split.comp:25-61).
 struct PhaseBOut {
@@ -41,42 +41,67 @@ __device__ __attribute__((noinline)) void phase_b_slow(G1X *out, const G1A *L, c
 
 // Lane (blob, a, k): bucket k of output a.  The item list depends only on
 // (segment, k), so work groups are formed per segment: a 64-lane group of
-// segment 0 or 1 holds the 32 outputs of two blobs (pair bp: blobs 2bp and
-// 2bp + 1), one of segment 2 the 64 outputs of one blob.  Groups
-// [0, npairs) are segment 0, [npairs, 2 npairs) segment 1, the rest
-// segment 2.
+// segment 0 or 1 holds the 32 outputs of two blobs (blobs 2p and 2p + 1 of
+// pair p), one of segment 2 the 64 outputs of one blob.  A work item is one
+// (group, bucket).
+//
+// Work order: the blobs go in bands of kBand; within a band, class by class
+// (a class is a (segment, bucket) pair, longest item list first: the host's
+// order in offsets[]), every group of the band takes the class's bucket.
+// Bucket lengths vary several-fold (magnitude 1 also takes the top
+// positions' carries), and a long bucket taken last would run alone.  Class
+// c starts at item start(c) = n2(c) bb + (c - n2(c)) pb of its band, with
+// bb the band's blobs, pb its pairs and n2(c) the segment-2 classes before c.
 //
 // Persistent: every wave of the `grid` blocks takes work items from the
 // counter q[0] (cleared by k_glv_phi, the kernel before) until the queue is
-// drained, and leaves after one failed take; waves share no barrier.  Item i
-// is bucket k = i % 128 of group i / 128.  A flagged bucket is recomputed by
-// k_phase_b_fix, which keeps this kernel free of calls.
+// drained, and leaves after one failed take; waves share no barrier.  A
+// flagged bucket is recomputed by k_phase_b_fix, which keeps this kernel
+// free of calls.
+constexpr uint32_t kBand = 32;       // blobs per band (even)
+constexpr uint32_t kClasses = 3 * kBuckets;
+constexpr uint32_t top_pow2(uint32_t n) { return n < 2 ? 1u : 2u * top_pow2(n / 2); }
+constexpr uint32_t kSearchTop = top_pow2(kClasses);  // largest power of two <= kClasses
+
 __global__ void __launch_bounds__(256, 2)
     k_phase_b(G1X *buckets, uint8_t *flags, const G1A *lad_aff, const G1A *lad_phi, const uint32_t *items,
               const uint32_t *offsets, uint32_t nblobs, uint32_t *q, uint32_t force_checked) {
     const uint32_t lane = threadIdx.x % 64;
-    const uint32_t npairs = (nblobs + 1) / 2;
-    const uint32_t nitems = 128u * (2 * npairs + nblobs);
+    const uint32_t *cls = offsets + 3 * (kBuckets + 1);
+    const uint32_t *n2 = cls + kClasses;
+    const uint32_t band_items = kBuckets * 2 * kBand;  // a full band: kBand / 2 pairs x 2 + kBand
+    const uint32_t nfull = nblobs / kBand, rest = nblobs % kBand;
+    const uint32_t nitems = nfull * band_items + kBuckets * (2 * ((rest + 1) / 2) + rest);
 #pragma unroll 1
     for (;;) {
         uint32_t item = 0;
         if (lane == 0) item = atomicAdd(q, 1u);
         item = __shfl(item, 0);
         if (item >= nitems) break;
-        const uint32_t k = item % 128;  // bucket
-        const uint32_t grp = item / 128;
+        const uint32_t band = item / band_items, j = item % band_items;
+        const uint32_t b0 = band * kBand;
+        const uint32_t bb = band < nfull ? kBand : rest, pb = (bb + 1) / 2;
+        // the last class c with start(c) <= j
+        uint32_t c = 0;
+#pragma unroll 1
+        for (uint32_t step = kSearchTop; step; step >>= 1) {
+            const uint32_t t = c + step;
+            if (t < kClasses && n2[t] * bb + (t - n2[t]) * pb <= j) c = t;
+        }
+        const uint32_t g = j - (n2[c] * bb + (c - n2[c]) * pb);
+        const uint32_t seg = cls[c] >> 16, k = cls[c] & 0xffffu;
         uint32_t blob, a;
-        if (grp < 2 * npairs) {
-            blob = 2 * (grp % npairs) + (lane >> 5);
-            a = (grp / npairs) * 32 + (lane & 31u);
-        } else {
-            blob = grp - 2 * npairs;
+        if (seg == 2) {
+            blob = b0 + g;
             a = 64 + lane;
+        } else {
+            blob = b0 + 2 * g + (lane >> 5);
+            a = seg * 32 + (lane & 31u);
         }
         const bool live = blob < nblobs;  // the odd blob count's last pair
         if (!live) blob = nblobs - 1;
         const PhaseBOut o(a);
-        const uint32_t lo = offsets[o.seg * 129 + k], hi = offsets[o.seg * 129 + k + 1];
+        const uint32_t lo = offsets[o.seg * (kBuckets + 1) + k], hi = offsets[o.seg * (kBuckets + 1) + k + 1];
         const G1A *L = lad_aff + (size_t)blob * kLadPos * 128;
         const G1A *Phi = lad_phi + (size_t)blob * kLadPos * 128;
         G1X acc = g1x_identity();
@@ -99,8 +124,8 @@ __global__ void __launch_bounds__(256, 2)
         }
         if (live) {
             const size_t ba = (size_t)blob * 128 + a;
-            buckets[ba * 128 + k] = acc;
-            flags[ba * 128 + k] = exc ? 1 : 0;
+            buckets[ba * kBuckets + k] = acc;
+            flags[ba * kBuckets + k] = exc ? 1 : 0;
         }
     }
 }
@@ -111,10 +136,10 @@ __global__ void __launch_bounds__(256) k_phase_b_fix(G1X *buckets, const uint8_t
                               const uint32_t *items, const uint32_t *offsets, uint32_t nbuckets) {
     const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= nbuckets || !flags[idx]) return;
-    const uint32_t ba = idx / 128, k = idx % 128;
+    const uint32_t ba = idx / kBuckets, k = idx % kBuckets;
     const uint32_t blob = ba / 128, a = ba % 128;
     const PhaseBOut o(a);
-    const uint32_t lo = offsets[o.seg * 129 + k], hi = offsets[o.seg * 129 + k + 1];
+    const uint32_t lo = offsets[o.seg * (kBuckets + 1) + k], hi = offsets[o.seg * (kBuckets + 1) + k + 1];
     const G1A *L = lad_aff + (size_t)blob * kLadPos * 128;
     const G1A *Phi = lad_phi + (size_t)blob * kLadPos * 128;
     phase_b_slow(&buckets[idx], L, Phi, items, lo, hi, a);
@@ -125,13 +150,13 @@ __global__ void __launch_bounds__(256) k_phase_b_fix(G1X *buckets, const uint8_t
 hipError_t launch_phase_b(G1X *buckets, uint8_t *flags, const G1A *lad_aff, const G1A *lad_phi,
                           const uint32_t *items, const uint32_t *offsets, uint32_t nblobs, uint32_t grid,
                           uint32_t *queue, uint32_t force_checked, hipStream_t s) {
-    // 64-lane groups: two per pair of blobs (segments 0 and 1), one per blob
-    const uint32_t ngroups = 2 * ((nblobs + 1) / 2) + nblobs;
-    const uint32_t nitems = 128u * ngroups;
+    // 64-lane groups: two per pair of blobs of a band (segments 0 and 1), one per blob
+    const uint32_t rest = nblobs % kBand;
+    const uint32_t nitems = (nblobs / kBand) * kBuckets * 2 * kBand + kBuckets * (2 * ((rest + 1) / 2) + rest);
     if (grid > (nitems + 3) / 4) grid = (nitems + 3) / 4;
     hipLaunchKernelGGL(k_phase_b, dim3(grid), dim3(256), 0, s, buckets, flags, lad_aff, lad_phi, items,
                        offsets, nblobs, queue, force_checked);
-    const uint32_t nbuckets = nblobs * 128 * 128;
+    const uint32_t nbuckets = nblobs * 128 * kBuckets;
     hipLaunchKernelGGL(k_phase_b_fix, dim3(blocks_for(nbuckets, 256)), dim3(256), 0, s, buckets, flags,
                        lad_aff, lad_phi, items, offsets, nbuckets);
     return hipGetLastError();

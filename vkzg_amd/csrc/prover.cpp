@@ -51,8 +51,8 @@ uint32_t choose_window(size_t free_bytes) {
     return 8;
 }
 
-// Signed 8-bit digits of the 66 phase-B scalars, grouped by bucket |v|-1:
-// item = e | d << 8 | (v < 0) << 16, offsets[half][129] (absolute).
+// GLV split of a phase-B scalar (its halves are written in signed digits
+// below, item = e | d << 8 | (v < 0) << 16, grouped by bucket |v|-1):
 // k = k1 + lambda k2 with k1 = k mod lambda and k2 = k div lambda, for the
 // G1 endomorphism phi(x, y) = (beta x, y) = [lambda] P, lambda = z^2 - 1 =
 // 0xac45a4010001a40200000000ffffffff.  k < r = lambda^2 + lambda + 1 keeps
@@ -141,27 +141,30 @@ void phase_b_taps(const Fr *kpm, std::vector<Fr> taps[3]) {
 }
 
 // Phase-B item lists: for each segment s and bucket k, the (e, d, sign) with
-// signed 8-bit digit of magnitude k+1 at position d of the tap taps[s][e]
-// (k_phase_b_split.comp:25-61, 8-bit signed digits), where d < 17 indexes
-// the digits of k1 and 17 <= d < 34 those of k2.
+// signed kDigitBits-bit digit of magnitude k+1 at position d of the tap
+// taps[s][e] (the byte-plus-carry recoding of k_phase_b_split.comp:25-61),
+// where d < kLadPos indexes the digits of k1 and kLadPos <= d < 2 kLadPos
+// those of k2.  Offsets: kBuckets + 1 per segment.
 void build_phase_b_items(const Fr *kpm, std::vector<uint32_t> &items, std::vector<uint32_t> &offsets) {
+    constexpr uint32_t kWin = 1u << kDigitBits, kHalf = kWin / 2;
     std::vector<Fr> taps[3];
     phase_b_taps(kpm, taps);
     items.clear();
-    offsets.assign(3 * 129, 0);
+    offsets.assign(3 * (kBuckets + 1), 0);
     for (int seg = 0; seg < 3; seg++) {
-        std::vector<uint32_t> lists[128];
+        std::vector<uint32_t> lists[kBuckets];
         for (uint32_t e = 0; e < (uint32_t)taps[seg].size(); e++) {
             unsigned __int128 parts[2];
             glv_split(taps[seg][e].v, parts[0], parts[1]);
             for (int g = 0; g < 2; g++) {
                 int carry = 0;
                 for (uint32_t d = 0; d < kLadPos; d++) {
-                    const uint32_t byte = d < 16 ? (uint32_t)(parts[g] >> (8 * d)) & 0xffu : 0u;
-                    int v = (int)byte + carry;
+                    const uint32_t sh = kDigitBits * d;  // the top position may start at bit 128
+                    const uint32_t bits = sh < 128 ? (uint32_t)(parts[g] >> sh) & (kWin - 1) : 0u;
+                    int v = (int)bits + carry;
                     carry = 0;
-                    if (v > 128) {
-                        v -= 256;
+                    if (v > (int)kHalf) {
+                        v -= (int)kWin;
                         carry = 1;
                     }
                     if (v == 0) continue;
@@ -169,14 +172,34 @@ void build_phase_b_items(const Fr *kpm, std::vector<uint32_t> &items, std::vecto
                     const uint32_t pos = d + (uint32_t)g * kLadPos;
                     lists[mag - 1].push_back(e | (pos << 8) | ((v < 0 ? 1u : 0u) << 16));
                 }
+                if (carry) throw VKZG_ERR_SETUP;  // cannot happen: halves are below 2^128
             }
         }
-        for (int k = 0; k < 128; k++) {
-            offsets[seg * 129 + k] = (uint32_t)items.size();
+        for (uint32_t k = 0; k < kBuckets; k++) {
+            offsets[seg * (kBuckets + 1) + k] = (uint32_t)items.size();
             items.insert(items.end(), lists[k].begin(), lists[k].end());
         }
-        offsets[seg * 129 + 128] = (uint32_t)items.size();
+        offsets[seg * (kBuckets + 1) + kBuckets] = (uint32_t)items.size();
     }
+    // Work order of k_phase_b: the 3 kBuckets (segment, bucket) classes by
+    // item count, longest first (so that a band of blobs ends on its
+    // shortest buckets), as seg << 16 | k, then the number of segment-2
+    // classes before each class (3 kBuckets + 1 entries).
+    std::vector<uint32_t> cls;
+    for (uint32_t seg = 0; seg < 3; seg++)
+        for (uint32_t k = 0; k < kBuckets; k++) cls.push_back(seg << 16 | k);
+    auto len = [&](uint32_t c) {
+        const uint32_t seg = c >> 16, k = c & 0xffffu;
+        return offsets[seg * (kBuckets + 1) + k + 1] - offsets[seg * (kBuckets + 1) + k];
+    };
+    std::stable_sort(cls.begin(), cls.end(), [&](uint32_t x, uint32_t y) { return len(x) > len(y); });
+    offsets.insert(offsets.end(), cls.begin(), cls.end());
+    uint32_t n2 = 0;
+    for (uint32_t c : cls) {
+        offsets.push_back(n2);
+        n2 += (c >> 16) == 2 ? 1u : 0u;
+    }
+    offsets.push_back(n2);
 }
 
 // Fills t with a freshly built phase-A table of window c: the W chains
@@ -271,7 +294,8 @@ void build_tables(vkzg_prover *p, const uint8_t *setup) {
     // segments 0 and 1 serve 32 outputs per blob each, segment 2 serves 64
     p->kb_per_blob = 0;
     for (int seg = 0; seg < 3; seg++)
-        p->kb_per_blob += (seg == 2 ? 64u : 32u) * (offsets[seg * 129 + 128] - offsets[seg * 129]);
+        p->kb_per_blob += (seg == 2 ? 64u : 32u) *
+                          (offsets[seg * (kBuckets + 1) + kBuckets] - offsets[seg * (kBuckets + 1)]);
     p->kb_items = p->arena.dalloc<uint32_t>(items.size());
     p->kb_offsets = p->arena.dalloc<uint32_t>(offsets.size());
     ck(hipMemcpyAsync(p->kb_items, items.data(), items.size() * 4, hipMemcpyHostToDevice, s));
@@ -299,7 +323,7 @@ uint64_t WorkSet::bytes_per_blob() {
     return kBlobBytes + kProofBytes + 4 * 4 + 128 + kMaxLadSplits * 128 * 64 * sizeof(Fr) +
            kMaxLadSplits * 128 * (kMaxGroups + 1) * sizeof(G1X) +
            npos * sizeof(G1J) + 2 * npos * sizeof(G1A) + npos * sizeof(Fp) +
-           128 * 128 * (sizeof(G1X) + 1) + 128 * sizeof(G1X) + 2 * kCellsBytes + 128 +
+           128 * kBuckets * (sizeof(G1X) + 1) + 128 * sizeof(G1X) + 2 * kCellsBytes + 128 +
            2 * 8192 * sizeof(Fr);
 }
 
@@ -320,8 +344,8 @@ void WorkSet::allocate(uint32_t B) {
     lad_aff = arena.dalloc<G1A>((size_t)B * 128 * kLadPos);
     lad_phi = arena.dalloc<G1A>((size_t)B * 128 * kLadPos);
     prefix = arena.dalloc<Fp>((size_t)B * 128 * kLadPos);
-    buckets = arena.dalloc<G1X>((size_t)B * 128 * 128);
-    bucket_flags = arena.dalloc<uint8_t>((size_t)B * 128 * 128);
+    buckets = arena.dalloc<G1X>((size_t)B * 128 * kBuckets);
+    bucket_flags = arena.dalloc<uint8_t>((size_t)B * 128 * kBuckets);
     cval_flags = arena.dalloc<uint8_t>((size_t)B * 128);
     cvals = arena.dalloc<G1X>((size_t)B * 128);
     cells = arena.dalloc<uint8_t>((size_t)B * kCellsBytes);
@@ -432,7 +456,7 @@ void proofs_range(vkzg_prover *p, WorkSet &ws, uint32_t off, uint32_t nb, uint8_
     G1J *lad = ws.lad + off * npos;
     G1A *lad_aff = ws.lad_aff + off * npos, *lad_phi = ws.lad_phi + off * npos;
     Fp *prefix = ws.prefix + off * npos;
-    G1X *buckets = ws.buckets + (size_t)off * 128 * 128, *cvals = ws.cvals + (size_t)off * 128;
+    G1X *buckets = ws.buckets + (size_t)off * 128 * kBuckets, *cvals = ws.cvals + (size_t)off * 128;
     ck(launch_fk20_scalars(scal, d_blobs, blob_stride, err, p->roots, p->roots_inv, queue, coeffs,
                            tn.ladder_splits, nb, s));
     clk.mark("fk20_scalars");
@@ -450,7 +474,7 @@ void proofs_range(vkzg_prover *p, WorkSet &ws, uint32_t off, uint32_t nb, uint8_
     ck(launch_jac_to_affine(lad_aff, lad, prefix, (uint64_t)nb * npos, chunk, s));
     ck(launch_glv_phi(lad_phi, lad_aff, (uint64_t)nb * npos, ws.queue_b + off, ws.queue_r + off, s));
     clk.mark("normalize");
-    ck(launch_phase_b(buckets, ws.bucket_flags + off * 128 * 128, lad_aff, lad_phi, p->kb_items,
+    ck(launch_phase_b(buckets, ws.bucket_flags + (size_t)off * 128 * kBuckets, lad_aff, lad_phi, p->kb_items,
                       p->kb_offsets, nb, p->cus * 2, ws.queue_b + off, tn.force_checked, s));
     clk.mark("phase_b");
     ck(launch_bucket_reduce(cvals, ws.cval_flags + off * 128, buckets, nb, tn.reduce_lanes, p->cus * 2,

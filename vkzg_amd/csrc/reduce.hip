@@ -7,7 +7,7 @@ namespace {
 // One output's reduction recomputed with the checked point operations.
 __device__ __attribute__((noinline)) void reduce_slow(G1X *out, const G1X *B) {
     G1X run = g1x_identity(), sum = g1x_identity();
-    for (int k = 127; k >= 0; k--) {
+    for (int k = (int)kBuckets - 1; k >= 0; k--) {
         run = g1x_add(run, B[k]);
         sum = g1x_add(sum, run);
     }
@@ -41,9 +41,9 @@ struct G1XStash {
     }
 };
 
-// cvals[blob][a] = sum_{k=1..128} k * buckets[blob][a][k-1]
+// cvals[blob][a] = sum_{k=1..kBuckets} k * buckets[blob][a][k-1]
 // (field.glsl.inc:478-517).  A team of LANES lanes shares an output; lane l
-// owns buckets [per l, per l + per) with per = 128 / LANES:
+// owns buckets [per l, per l + per) with per = kBuckets / LANES:
 // rs_l = sum_i (i+1) B[per l + i] by running sums, s_l = sum_i B[per l + i];
 // the output is sum_l (rs_l + per l * s_l).  More lanes shorten each lane's
 // dependent chain and add total work (the scan over the team's lanes).  The
@@ -57,7 +57,7 @@ template <int LANES>
 __global__ void __launch_bounds__(256, 2)
     k_bucket_reduce(G1X *cvals, uint8_t *flags, const G1X *buckets, uint32_t nout, uint32_t *q,
                     uint32_t force_checked) {
-    constexpr uint32_t per = 128 / LANES;
+    constexpr uint32_t per = kBuckets / LANES;
     constexpr uint32_t teams = 64 / LANES;
     __shared__ uint32_t stash_words[4 * kFpLimbs * 256];
     const G1XStash stash{stash_words};
@@ -75,7 +75,7 @@ __global__ void __launch_bounds__(256, 2)
         stash.put(g1x_identity());
         bool exc = force_checked != 0;
         if (o < nout) {
-            const G1X *B = buckets + (size_t)o * 128 + l * per;
+            const G1X *B = buckets + (size_t)o * kBuckets + l * per;
             // The item lists leave some buckets empty (the identity), the
             // same ones for every blob.  An empty bucket right after the
             // lane's first non-empty one leaves s unchanged while the running
@@ -128,7 +128,7 @@ __global__ void __launch_bounds__(256, 2)
 __global__ void __launch_bounds__(256) k_reduce_fix(G1X *cvals, const uint8_t *flags, const G1X *buckets, uint32_t nout) {
     const uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
     if (o >= nout || !flags[o]) return;
-    reduce_slow(&cvals[o], buckets + (size_t)o * 128);
+    reduce_slow(&cvals[o], buckets + (size_t)o * kBuckets);
 }
 
 // XYZZ -> affine for two points with one inversion.
