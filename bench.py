@@ -384,7 +384,8 @@ def main():
     traffic = None
     if os.path.exists(a.traffic_file):
         t = json.load(open(a.traffic_file))
-        if t.get("kernel") == dom and t.get("blobs") == nb and t.get("window_bits") == info["window_bits"]:
+        if (t.get("kernel") == dom and t.get("blobs") == nb and t.get("window_bits") == info["window_bits"]
+                and t.get("windows") == info["windows"]):
             traffic = t["fetch_bytes"] + t["write_bytes"]
     roofline = {
         "bound": "int-valu",

@@ -134,6 +134,12 @@ def test_device_entry_point_matches_host(gpu_prover, vectors):
 def test_prover_info(gpu_prover, gpu_prover_small_batch):
     info = gpu_prover.info()
     print("prover info:", info, gpu_prover.device_name())
-    assert info["windows"] * info["window_bits"] >= 256
+    # window plan (c, W, k): W windows of c bits, the top k one bit wider,
+    # covering the 255 bits of a signed scalar's magnitude and sign
+    c, W = info["window_bits"], info["windows"]
+    entries = info["table_bytes"] // (8192 * 128)
+    k = entries // (1 << (c - 1)) - W
+    assert entries == (W + k) << (c - 1) and 0 <= k <= W
+    assert W * c + k >= 255 and (W - 1) * c + max(k - 1, 0) < 255
     assert info["max_batch_size"] == 256
     assert gpu_prover_small_batch.info()["max_batch_size"] == 4

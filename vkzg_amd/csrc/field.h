@@ -386,6 +386,27 @@ VKZG_DEV Fr fr_pow_const(const Fr &a, const uint32_t (&e)[NE]) {
 VKZG_DEV Fr fr_inv(const Fr &a) { return fr_pow_const<8>(a, cfr::P_MINUS_2); }
 
 // 32 big-endian bytes (as 8 big-endian words) -> little-endian limbs
+// A canonical a < r as sign and magnitude: r - a with bit 255 set when
+// r - a < a, else a.  The magnitude is at most (r - 1) / 2 < 2^254; bit 255 is
+// free since r < 2^255.
+VKZG_DEV Fr fr_signed_magnitude(const Fr &a) {
+    Fr n;
+    uint32_t br = 0, lt = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const uint64_t d = (uint64_t)cfr::P[i] - a.v[i] - br;
+        n.v[i] = (uint32_t)d;
+        br = (uint32_t)(d >> 32) & 1u;
+    }
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+        const uint64_t d = (uint64_t)n.v[i] - a.v[i] - lt;
+        lt = (uint32_t)(d >> 32) & 1u;
+    }
+    if (!lt) return a;
+    n.v[7] |= 0x80000000u;
+    return n;
+}
 VKZG_DEV Fr fr_from_be_words(const uint32_t *w) {
     Fr r;
 #pragma unroll

@@ -121,10 +121,10 @@ __global__ void __launch_bounds__(kScalarThreads)
         for (int h = 0; h < 2; h++) {
             const int j = lane + 64 * h;
             const Fr v = lds_load_fr(col, kColPitch, j);
-            out[j * 64 + i] = fr_to_canonical(v);
+            out[j * 64 + i] = fr_signed_magnitude(fr_to_canonical(v));
 #pragma unroll 1
             for (uint32_t m = 1; m < splits; m++)
-                out[(size_t)m * 8192 + j * 64 + i] = fr_to_canonical(fr_mul(v, mult[m]));
+                out[(size_t)m * 8192 + j * 64 + i] = fr_signed_magnitude(fr_to_canonical(fr_mul(v, mult[m])));
         }
         __syncthreads();
     }

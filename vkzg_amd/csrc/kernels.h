@@ -8,10 +8,12 @@
 //   Fr shift_fwd[8192]        7^k / 8192;    shift_inv: 7^-k / 8192
 //   G1A srs[4096]             the G1 monomial setup, affine
 //   G1A xext[128][64]         x_ext_fft[j][i] (FK20 setup in the Fourier domain)
-//   G1AT table[128][64][W][H] table[j][i][w][d-1] = d * 2^(c*w) * xext[j][i], one
-//                             128-byte line per entry,
-//                             c = window bits, W = ceil(256/c), H = 2^(c-1)
-//   Fr scal[B][128][64]       phase-A scalars, CANONICAL (not Montgomery)
+//   G1AT table[128][64][E]    for the window plan (c, W, k) (WindowPlan):
+//                             table[j][i][entry(w) + d - 1] =
+//                             d * 2^bit(w) * xext[j][i], d <= 2^(width(w)-1),
+//                             one 128-byte line per entry
+//   Fr scal[B][128][64]       phase-A scalars, CANONICAL (not Montgomery), as
+//                             sign (bit 255) and magnitude (< 2^254)
 //   G1X u[B][128][G]          phase-A partial sums, one per window group g < G
 //   G1J lad[B][17][128]       2^(8d) * f[j], d < kLadPos, for the folds f of u:
 //                             Fc (j < 32), Fn (32 <= j < 64), F- (j >= 64);
@@ -48,6 +50,27 @@ __host__ __device__ constexpr uint32_t lad_start(uint32_t M, uint32_t m) { retur
 
 struct Fr;
 struct Fp;
+
+// Phase-A window plan: W signed-digit windows over a scalar's magnitude
+// (below 2^254, with the sign in bit 255: 255 bits always suffice, and the top
+// window never carries out), the top k of them c + 1 bits wide and the
+// others c bits wide.  Window w starts at bit bit(w); its multiples 1..2^(width-1)
+// sit at entries [entry(w), entry(w) + 2^(width-1)) of a base's entries().
+struct WindowPlan {
+    uint32_t c, W, k;
+    __host__ __device__ uint32_t H() const { return 1u << (c - 1); }
+    __host__ __device__ uint32_t wide_below(uint32_t w) const { return w > W - k ? w - (W - k) : 0u; }
+    __host__ __device__ uint32_t wide(uint32_t w) const { return w >= W - k ? 1u : 0u; }
+    __host__ __device__ uint32_t bit(uint32_t w) const { return w * c + wide_below(w); }
+    __host__ __device__ uint32_t entry(uint32_t w) const { return (w + wide_below(w)) * H(); }
+    __host__ __device__ uint32_t entries() const { return (W + k) * H(); }
+};
+// The fewest windows of c bits, k of them widened by one bit, that cover 255
+// bits: (14, 0) is 19 windows, (14, 3) 18 (15 of 14 bits and 3 of 15).
+inline WindowPlan window_plan(uint32_t c, uint32_t k) {
+    const uint32_t W = (255 - k + c - 1) / c;
+    return WindowPlan{c, W, k < W ? k : W};
+}
 struct G1A;
 struct G1AT;
 struct G1J;
@@ -59,10 +82,10 @@ hipError_t launch_decompress(G1A *out, const uint8_t *in, uint32_t n, uint32_t *
 hipError_t launch_xext_fft(G1J *out, const G1A *srs, const Fr *roots, hipStream_t s);
 hipError_t launch_jac_to_affine(G1A *out, const G1J *in, Fp *prefix, uint64_t count, uint32_t chunk,
                                 hipStream_t s);
-hipError_t launch_table_chains(G1J *out, const G1A *xext, uint32_t c, uint32_t W, hipStream_t s);
+hipError_t launch_table_chains(G1J *out, const G1A *xext, WindowPlan plan, hipStream_t s);
 // The reference's v1 position table (8192 x 32 entries of 24 words) from pw = 2^(8d) xext.
 hipError_t launch_v1_positions(uint32_t *out, const G1A *pw, hipStream_t s);
-hipError_t launch_table_multiples(G1AT *table, const G1A *pw, uint32_t c, uint32_t W,
+hipError_t launch_table_multiples(G1AT *table, const G1A *pw, WindowPlan plan,
                                   uint64_t thread_begin, uint32_t thread_count, G1J *scratch,
                                   Fp *prefix, hipStream_t s);
 hipError_t launch_phase_b_scalars(Fr *out, const Fr *roots, hipStream_t s);
@@ -73,7 +96,8 @@ hipError_t launch_phase_b_scalars(Fr *out, const Fr *roots, hipStream_t s);
 // writes the blob polynomial's monomial coefficients there (Fr[B][8192],
 // Montgomery, zero above 4096).
 // With `splits` > 1 it writes `splits` scalar sets per blob,
-// scal[blob][m][j][i] = scal[blob][0][j][i] * 2^(8 start_m) mod r.
+// scal[blob][m][j][i] = scal[blob][0][j][i] * 2^(8 start_m) mod r (each as
+// sign and magnitude).
 hipError_t launch_fk20_scalars(Fr *scal, const uint8_t *blobs, uint64_t blob_stride, uint32_t *err,
                                const Fr *roots, const Fr *roots_inv, uint32_t *queue, Fr *coeffs,
                                uint32_t splits, uint32_t nblobs, hipStream_t s);
@@ -81,7 +105,7 @@ hipError_t launch_fk20_scalars(Fr *scal, const uint8_t *blobs, uint64_t blob_str
 // force_checked != 0 (a test switch) sends every tile down the checked
 // recompute, as if the fast additions had met a doubling or cancellation; the
 // same holds for launch_phase_b and launch_bucket_reduce.
-hipError_t launch_phase_a(G1X *part, const Fr *scal, const G1AT *table, uint32_t c, uint32_t W,
+hipError_t launch_phase_a(G1X *part, const Fr *scal, const G1AT *table, WindowPlan plan,
                           uint32_t nmsm, uint32_t groups, uint32_t grid, uint32_t *queue,
                           uint32_t force_checked, hipStream_t s);
 // usum[msm] = sum over g < groups of part[msm * groups + g] (groups <= 24,

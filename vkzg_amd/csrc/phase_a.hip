@@ -4,14 +4,21 @@
 namespace vkzg_amd {
 
 namespace {
-// c-bit window `w` of a canonical 256-bit scalar.
-VKZG_DEV uint32_t window_bits(const Fr &s, uint32_t w, uint32_t c) {
-    const uint32_t off = w * c;
+// The `width` bits of s from bit `off` (bits above 255 read as zero).
+VKZG_DEV uint32_t window_bits(const Fr &s, uint32_t off, uint32_t width) {
     const uint32_t li = off >> 5;
     const uint32_t sh = off & 31u;
     uint32_t lo = li < 8 ? s.v[li] >> sh : 0u;
     if (sh && li + 1 < 8) lo |= s.v[li + 1] << (32u - sh);
-    return lo & ((1u << c) - 1u);
+    return lo & ((1u << width) - 1u);
+}
+
+// A phase-A scalar's magnitude (bit 255 cleared) and its sign.
+VKZG_DEV Fr scalar_magnitude(const Fr &s, bool &neg) {
+    Fr m = s;
+    neg = (s.v[7] >> 31) != 0;
+    m.v[7] &= 0x7fffffffu;
+    return m;
 }
 
 constexpr int kMsmLanes = 8;  // lanes per tile (64 terms / 8 = 8 each)
@@ -21,19 +28,20 @@ constexpr int kTilesPerItem = 256 / kMsmLanes;  // tiles one 256-thread block ta
 // point operations: reached only when the fast path flagged a doubling or
 // cancellation case in this tile.
 __device__ __attribute__((noinline)) void phase_a_slow(G1X *out, const Fr *scal, const G1AT *tab_j,
-                                                       uint32_t c, uint32_t W, uint32_t w0,
-                                                       uint32_t w1) {
-    const uint32_t H = 1u << (c - 1);
+                                                       WindowPlan plan, uint32_t w0, uint32_t w1) {
     G1X acc = g1x_identity();
     for (uint32_t i = 0; i < 64; i++) {
-        const Fr s = scal[i];
-        const G1AT *tab = tab_j + (size_t)i * W * H;
+        bool sneg;
+        const Fr s = scalar_magnitude(scal[i], sneg);
+        const G1AT *tab = tab_j + (size_t)i * plan.entries();
         uint32_t carry = 0;
         for (uint32_t w = 0; w < w1; w++) {
-            const uint32_t raw = window_bits(s, w, c) + carry;
-            carry = raw > H ? 1u : 0u;
-            const uint32_t mag = carry ? (2u * H - raw) : raw;
-            if (w >= w0 && mag) acc = g1x_madd(acc, g1a_neg_if(tab[(size_t)w * H + (mag - 1)].p, carry != 0));
+            const uint32_t Hw = plan.H() << plan.wide(w);
+            const uint32_t raw = window_bits(s, plan.bit(w), plan.c + plan.wide(w)) + carry;
+            carry = raw > Hw ? 1u : 0u;
+            const uint32_t mag = carry ? (2u * Hw - raw) : raw;
+            if (w >= w0 && mag)
+                acc = g1x_madd(acc, g1a_neg_if(tab[plan.entry(w) + (mag - 1)].p, (carry != 0) != sneg));
         }
     }
     *out = acc;
@@ -42,9 +50,11 @@ __device__ __attribute__((noinline)) void phase_a_slow(G1X *out, const Fr *scal,
 // Persistent phase A.  The work is a queue of tiles (msm, g): windows
 // [g W / G, (g+1) W / G) of MSM msm = (blob * splits + m) * 128 + j, where
 //   u[msm] = sum_{i<64} scal[msm][i] * xext[j][i]
-// with signed c-bit digits v in [-(2^(c-1)-1), 2^(c-1)] (the recoding of
-// k_phase_a_sort.comp:31-40 with a c-bit window) and
-// table[j][i][w][|v|-1] = |v| * 2^(cw) * xext[j][i].  Tile (msm, g) writes its
+// with the scalar as sign and magnitude (k_fk20_scalars) and the magnitude in
+// signed digits v in [-(2^(b-1)-1), 2^(b-1)] of each window's width b (the
+// recoding of k_phase_a_sort.comp:31-40 with the plan's windows), each digit's
+// sign flipped for a negative scalar, and
+// table[j][i][entry(w) + |v| - 1] = |v| * 2^bit(w) * xext[j][i].  Tile (msm, g) writes its
 // partial sum to part[msm * G + g]; k_sum_groups adds the G partials.  Tiles
 // are numbered group-major (tile = g * nmsm + msm): nmsm is a multiple of 128,
 // so the 8 tiles of a wave share g and run the same windows -- mixed groups
@@ -54,11 +64,11 @@ __device__ __attribute__((noinline)) void phase_a_slow(G1X *out, const Fr *scal,
 // exits after one failed take, so the grid always drains.  q[0] is cleared by
 // the kernel that runs before this one.
 __global__ void __launch_bounds__(256, 2)
-    k_phase_a(G1X *part, const Fr *scal, const G1AT *table, uint32_t c, uint32_t W, uint32_t nmsm,
+    k_phase_a(G1X *part, const Fr *scal, const G1AT *table, WindowPlan plan, uint32_t nmsm,
               uint32_t groups, uint32_t *q, uint32_t force_checked) {
     __shared__ uint32_t item_sh;
     const uint32_t t = threadIdx.x % kMsmLanes;
-    const uint32_t H = 1u << (c - 1);
+    const uint32_t W = plan.W, H = plan.H(), narrow = plan.W - plan.k;
     const uint32_t ntiles = nmsm * groups;
     const uint32_t nitems = (ntiles + kTilesPerItem - 1) / kTilesPerItem;
 #pragma unroll 1
@@ -80,17 +90,21 @@ __global__ void __launch_bounds__(256, 2)
 #pragma unroll 1
             for (uint32_t qq = 0; qq < 64 / kMsmLanes; qq++) {
                 const uint32_t i = t * (64 / kMsmLanes) + qq;
-                const Fr s = scal[(size_t)msm * 64 + i];
-                const G1AT *tab = table + ((size_t)(j * 64 + i) * W) * H;
+                bool sneg;
+                const Fr s = scalar_magnitude(scal[(size_t)msm * 64 + i], sneg);
+                const G1AT *tab = table + (size_t)(j * 64 + i) * plan.entries();
                 uint32_t carry = 0;
 #pragma unroll 1
                 for (uint32_t w = 0; w < w1; w++) {
-                    const uint32_t raw = window_bits(s, w, c) + carry;
-                    carry = raw > H ? 1u : 0u;
-                    const bool neg = carry != 0;
-                    const uint32_t mag = neg ? (2u * H - raw) : raw;
+                    // windows at or above `narrow` are one bit wider
+                    const uint32_t wide = w >= narrow ? 1u : 0u, below = w > narrow ? w - narrow : 0u;
+                    const uint32_t Hw = H << wide;
+                    const uint32_t raw = window_bits(s, w * plan.c + below, plan.c + wide) + carry;
+                    carry = raw > Hw ? 1u : 0u;
+                    const bool neg = (carry != 0) != sneg;
+                    const uint32_t mag = carry ? (2u * Hw - raw) : raw;
                     if (w < w0 || mag == 0) continue;
-                    const G1A pt = tab[(size_t)w * H + (mag - 1)].p;
+                    const G1A pt = tab[(w + below) * H + (mag - 1)].p;
                     if (started) {
                         acc = g1x_madd_acc(acc, pt, neg, exc);
                     } else {
@@ -109,7 +123,7 @@ __global__ void __launch_bounds__(256, 2)
         if (live && t == 0) {
             G1X *out = &part[(size_t)msm * groups + g];
             if (redo)
-                phase_a_slow(out, scal + (size_t)msm * 64, table + (size_t)j * 64 * W * H, c, W, w0, w1);
+                phase_a_slow(out, scal + (size_t)msm * 64, table + (size_t)j * 64 * plan.entries(), plan, w0, w1);
             else
                 *out = acc;
         }
@@ -118,13 +132,13 @@ __global__ void __launch_bounds__(256, 2)
 
 }  // namespace
 
-hipError_t launch_phase_a(G1X *part, const Fr *scal, const G1AT *table, uint32_t c, uint32_t W,
+hipError_t launch_phase_a(G1X *part, const Fr *scal, const G1AT *table, WindowPlan plan,
                           uint32_t nmsm, uint32_t groups, uint32_t grid, uint32_t *queue,
                           uint32_t force_checked, hipStream_t s) {
     const uint32_t nitems = (nmsm * groups + kTilesPerItem - 1) / kTilesPerItem;
     if (grid > nitems) grid = nitems;
     if (grid == 0) grid = 1;
-    hipLaunchKernelGGL(k_phase_a, dim3(grid), dim3(256), 0, s, part, scal, table, c, W, nmsm, groups,
+    hipLaunchKernelGGL(k_phase_a, dim3(grid), dim3(256), 0, s, part, scal, table, plan, nmsm, groups,
                        queue, force_checked);
     return hipGetLastError();
 }

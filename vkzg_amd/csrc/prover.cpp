@@ -30,25 +30,26 @@ namespace vkzg_amd {
 constexpr size_t kSetupBytes = 4096 * 48;
 constexpr uint32_t kDefaultBatch = 256;
 
-uint64_t table_bytes_for(uint32_t c) {
-    const uint64_t W = (256 + c - 1) / c;
-    const uint64_t H = 1ull << (c - 1);
-    return 8192ull * W * H * sizeof(G1AT);
+uint64_t table_bytes_for(uint32_t c, uint32_t k = 0) {
+    return 8192ull * window_plan(c, k).entries() * sizeof(G1AT);
 }
 
-// Window bits of the resident phase-A table: $VKZG_AMD_WINDOW if set (6..14),
-// else the largest c <= 14 whose table fits in 60% of the free memory.  On a
-// 288 GB MI355X that is c = 14 (W = 19, a 152 GiB table; c = 13 is W = 20 in
-// 80 GiB); a second prover in the same process gets a smaller window.
-uint32_t choose_window(size_t free_bytes) {
+// Window plan of the resident phase-A table (c bits, the top k windows one
+// bit wider): $VKZG_AMD_WINDOW = c if set (6..14, k = 0); else (14, 3) -- 18
+// windows, a 168 GiB table -- when it fits in 65% of the free memory, else the
+// largest c <= 14 whose plain table fits in 60%.  On a fresh 288 GB MI355X
+// that is (14, 3); (14, 0) is 19 windows in 152 GiB, c = 13 20 in 80 GiB; a
+// second prover in the same process gets a smaller window.
+WindowPlan choose_window(size_t free_bytes) {
     if (const char *env = std::getenv("VKZG_AMD_WINDOW")) {
         const int c = std::atoi(env);
-        if (c >= 6 && c <= 14) return (uint32_t)c;
+        if (c >= 6 && c <= 14) return window_plan((uint32_t)c, 0);
     }
+    if (table_bytes_for(14, 3) <= free_bytes / 20 * 13) return window_plan(14, 3);
     for (uint32_t c = 14; c > 8; c--) {
-        if (table_bytes_for(c) <= free_bytes / 5 * 3) return c;
+        if (table_bytes_for(c) <= free_bytes / 5 * 3) return window_plan(c, 0);
     }
-    return 8;
+    return window_plan(8, 0);
 }
 
 // GLV split of a phase-B scalar (its halves are written in signed digits
@@ -202,31 +203,34 @@ void build_phase_b_items(const Fr *kpm, std::vector<uint32_t> &items, std::vecto
     offsets.push_back(n2);
 }
 
-// Fills t with a freshly built phase-A table of window c: the W chains
-// 2^(cw) xext (affine), then every multiple d * 2^(cw) xext, d <= 2^(c-1).
-void build_phase_a_table(vkzg_prover *p, PhaseATable &t, uint32_t c) {
+// Fills t with a freshly built phase-A table of the window plan: the W chains
+// 2^bit(w) xext (affine), then every multiple d * 2^bit(w) xext,
+// d <= 2^(width(w)-1).
+void build_phase_a_table(vkzg_prover *p, PhaseATable &t, WindowPlan plan) {
     hipStream_t s = p->stream;
+    const uint32_t c = plan.c;
     t.c = c;
-    t.W = (256 + c - 1) / c;
-    t.H = 1u << (c - 1);
-    t.bytes = table_bytes_for(c);
+    t.W = plan.W;
+    t.H = plan.H();
+    t.k = plan.k;
+    t.bytes = 8192ull * plan.entries() * sizeof(G1AT);
     t.table = p->arena.dalloc<G1AT>(t.bytes / sizeof(G1AT));
     G1J *pwj = p->arena.dalloc<G1J>((size_t)t.W * 8192);
     Fp *pw_prefix = p->arena.dalloc<Fp>((size_t)t.W * 8192);
     G1A *pw = p->arena.dalloc<G1A>((size_t)t.W * 8192);
-    ck(launch_table_chains(pwj, p->xext, c, t.W, s));
+    ck(launch_table_chains(pwj, p->xext, plan, s));
     ck(launch_jac_to_affine(pw, pwj, pw_prefix, (uint64_t)t.W * 8192, 16, s));
     ck(hipStreamSynchronize(s));
     p->arena.dfree(pwj);
     p->arena.dfree(pw_prefix);
     const uint32_t S = t.H < 64 ? t.H : 64;
-    const uint64_t total_threads = 8192ull * t.W * (t.H / S);
+    const uint64_t total_threads = 8192ull * (t.W + t.k) * (t.H / S);  // S entries each
     const uint32_t slice = 1u << 18;
     G1J *scratch = p->arena.dalloc<G1J>((size_t)S * slice);
     Fp *pref = p->arena.dalloc<Fp>((size_t)S * slice);
     for (uint64_t t0 = 0; t0 < total_threads; t0 += slice) {
         const uint32_t n = (uint32_t)std::min<uint64_t>(slice, total_threads - t0);
-        ck(launch_table_multiples(t.table, pw, c, t.W, t0, n, scratch, pref, s));
+        ck(launch_table_multiples(t.table, pw, plan, t0, n, scratch, pref, s));
     }
     ck(hipStreamSynchronize(s));
     p->arena.dfree(scratch);
@@ -243,6 +247,7 @@ PhaseATable phase_a_table(const vkzg_prover *p, uint32_t window_bits) {
     m.c = p->c;
     m.W = p->W;
     m.H = p->H;
+    m.k = p->wide;
     m.bytes = p->table_bytes;
     m.table = p->table;
     return m;
@@ -309,6 +314,7 @@ void build_tables(vkzg_prover *p, const uint8_t *setup) {
     p->c = main_table.c;
     p->W = main_table.W;
     p->H = main_table.H;
+    p->wide = main_table.k;
     p->table_bytes = main_table.bytes;
     p->table = main_table.table;
     p->arena.dfree(xj);
@@ -461,7 +467,7 @@ void proofs_range(vkzg_prover *p, WorkSet &ws, uint32_t off, uint32_t nb, uint8_
                            tn.ladder_splits, nb, s));
     clk.mark("fk20_scalars");
     const PhaseATable tab = phase_a_table(p, tn.window_bits);
-    ck(launch_phase_a(u, scal, tab.table, tab.c, tab.W, nb * 128 * tn.ladder_splits, tn.groups,
+    ck(launch_phase_a(u, scal, tab.table, tab.plan(), nb * 128 * tn.ladder_splits, tn.groups,
                       p->cus * tn.blocks_per_cu, queue, tn.force_checked, s));
     clk.mark("phase_a");
     if (tn.groups > 1) {
@@ -718,7 +724,7 @@ void autotune(vkzg_prover *p) {
             size_t free_b = 0, total_b = 0;
             ck(hipMemGetInfo(&free_b, &total_b));
             if (table_bytes_for(w) > free_b / 2) continue;
-            build_phase_a_table(p, slot, w);
+            build_phase_a_table(p, slot, window_plan(w, 0));
         }
         windows.push_back(w);
     }
@@ -1194,7 +1200,7 @@ vkzg_result vkzg_amd_load_window_table(vkzg_prover *p, uint32_t window_bits) {
         size_t free_b = 0, total_b = 0;
         ck(hipMemGetInfo(&free_b, &total_b));
         if (table_bytes_for(window_bits) > free_b) return VKZG_ERR_MALLOC;
-        build_phase_a_table(p, slot, window_bits);
+        build_phase_a_table(p, slot, window_plan(window_bits, 0));
     } catch (HipFail &) {
         if (slot.table) p->arena.dfree(slot.table);
         slot = PhaseATable{};

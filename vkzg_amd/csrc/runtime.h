@@ -44,9 +44,10 @@ struct Tuning {
 // A resident phase-A multiples table of window c: W = ceil(256 / c) windows
 // of H = 2^(c-1) multiples (table[128][64][W][H], one 128-byte line each).
 struct PhaseATable {
-    uint32_t c = 0, W = 0, H = 0;
+    uint32_t c = 0, W = 0, H = 0, k = 0;  // window plan (c, W, k), H = 2^(c-1)
     uint64_t bytes = 0;
     G1AT *table = nullptr;
+    WindowPlan plan() const { return WindowPlan{c, W, k}; }
 };
 // Smaller tables the autotuner may keep beside the main one, by window.
 constexpr uint32_t kExtraWindowMin = 10, kExtraWindowMax = 13;
@@ -258,7 +259,7 @@ struct vkzg_prover {
     uint4 *frl_tabs[4] = {};  // the four tables above as FrL entries (radix-4 recovery)
     vkzg_amd::G1A *srs = nullptr, *xext = nullptr;
     vkzg_amd::G1AT *table = nullptr;  // the main phase-A table (window c)
-    uint32_t c = 0, W = 0, H = 0;
+    uint32_t c = 0, W = 0, H = 0, wide = 0;  // its window plan (c, W, wide)
     uint64_t table_bytes = 0;
     // Smaller phase-A tables kept by the autotuner (window kExtraWindowMin + i).
     // A slot is filled under mu before any tuning names it and is freed only

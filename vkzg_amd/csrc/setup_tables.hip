@@ -40,16 +40,16 @@ __global__ void __launch_bounds__(256) k_jac_to_affine(G1A *out, const G1J *in, 
     }
 }
 
-// out[w][b] = 2^(c*w) * xext[b] for the 8192 bases.
-__global__ void k_table_chains(G1J *out, const G1A *xext, uint32_t c, uint32_t W) {
+// out[w][b] = 2^bit(w) * xext[b] for the 8192 bases.
+__global__ void k_table_chains(G1J *out, const G1A *xext, WindowPlan plan) {
     const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= 8192) return;
     G1J p = g1j_from_affine(xext[b]);
 #pragma unroll 1
-    for (uint32_t w = 0; w < W; w++) {
+    for (uint32_t w = 0; w < plan.W; w++) {
         out[(size_t)w * 8192 + b] = p;
 #pragma unroll 1
-        for (uint32_t k = 0; k < c; k++) p = g1j_dbl(p);
+        for (uint32_t k = 0; k < plan.c + plan.wide(w); k++) p = g1j_dbl(p);
     }
 }
 
@@ -73,24 +73,30 @@ __device__ __attribute__((noinline)) void table_run_slow(G1J *scratch, Fp *prefi
     *acc_out = acc;
 }
 
-// Thread g = (b*W + w)*nseg + seg writes table[b][w][d-1] for d in
-// [seg*S+1, seg*S+S]: a running sum of the affine base P = pw[w][b],
-// normalised with one inversion per thread.
-__global__ void __launch_bounds__(64) k_table_multiples(G1AT *table, const G1A *pw, uint32_t c, uint32_t W,
+// Thread g = (b * slots + v) * nseg + seg writes the S entries of base b
+// from entry v H + seg S: the multiples d in [d0, d0 + S) of P = pw[w][b] for
+// the window w that owns slot v (d0 = seg S + 1, plus H in a wide window's
+// second slot), a running sum normalised with one inversion per thread.
+__global__ void __launch_bounds__(64) k_table_multiples(G1AT *table, const G1A *pw, WindowPlan plan,
                                   uint64_t thread_begin, uint32_t thread_count, G1J *scratch,
                                   Fp *prefix) {
     const uint32_t lt = blockIdx.x * blockDim.x + threadIdx.x;
     if (lt >= thread_count) return;
     const uint64_t g = thread_begin + lt;
-    const uint32_t H = 1u << (c - 1);
+    // A base's entries() split into slots of H entries: one per c-bit window,
+    // two per wide window; a slot into nseg runs of S consecutive multiples.
+    const uint32_t H = plan.H();
     const uint32_t S = H < 64u ? H : 64u;
     const uint32_t nseg = H / S;
+    const uint32_t slots = plan.W + plan.k, narrow = plan.W - plan.k;
     const uint32_t seg = (uint32_t)(g % nseg);
-    const uint64_t bw = g / nseg;  // b*W + w
-    const uint32_t w = (uint32_t)(bw % W);
-    const uint32_t b = (uint32_t)(bw / W);
+    const uint64_t bv = g / nseg;  // b * slots + v
+    const uint32_t v = (uint32_t)(bv % slots);
+    const uint32_t b = (uint32_t)(bv / slots);
+    const uint32_t w = v < narrow ? v : narrow + (v - narrow) / 2;
+    const uint32_t half = v < narrow ? 0u : (v - narrow) % 2;
     const G1A P = pw[(size_t)w * 8192 + b];
-    const uint32_t d0 = seg * S + 1;
+    const uint32_t d0 = (half * nseg + seg) * S + 1;
 
     // d0 * P by double-and-add (no addition there meets equal points), then
     // the running sum; the one doubling case, 1*P + P, is taken explicitly.
@@ -113,7 +119,7 @@ __global__ void __launch_bounds__(64) k_table_multiples(G1AT *table, const G1A *
     }
     if (exc) table_run_slow(scratch, prefix, &acc, P, d0, S, thread_count, lt);
     Fp inv = fp_inv(acc);
-    G1AT *dst = table + bw * H + (d0 - 1);
+    G1AT *dst = table + bv * H + seg * S;  // entry(w) + d0 - 1 of base b
 #pragma unroll 1
     for (int m = (int)S - 1; m >= 0; m--) {
         const size_t slot = (size_t)m * thread_count + lt;
@@ -144,8 +150,8 @@ hipError_t launch_jac_to_affine(G1A *out, const G1J *in, Fp *prefix, uint64_t co
     return hipGetLastError();
 }
 
-hipError_t launch_table_chains(G1J *out, const G1A *xext, uint32_t c, uint32_t W, hipStream_t s) {
-    hipLaunchKernelGGL(k_table_chains, dim3(8192 / 64), dim3(64), 0, s, out, xext, c, W);
+hipError_t launch_table_chains(G1J *out, const G1A *xext, WindowPlan plan, hipStream_t s) {
+    hipLaunchKernelGGL(k_table_chains, dim3(8192 / 64), dim3(64), 0, s, out, xext, plan);
     return hipGetLastError();
 }
 
@@ -176,11 +182,11 @@ hipError_t launch_v1_positions(uint32_t *out, const G1A *pw, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_table_multiples(G1AT *table, const G1A *pw, uint32_t c, uint32_t W,
+hipError_t launch_table_multiples(G1AT *table, const G1A *pw, WindowPlan plan,
                                   uint64_t thread_begin, uint32_t thread_count, G1J *scratch,
                                   Fp *prefix, hipStream_t s) {
     hipLaunchKernelGGL(k_table_multiples, dim3(blocks_for(thread_count, 64)), dim3(64), 0, s, table,
-                       pw, c, W, thread_begin, thread_count, scratch, prefix);
+                       pw, plan, thread_begin, thread_count, scratch, prefix);
     return hipGetLastError();
 }
 

@@ -91,7 +91,8 @@ extern "C" vkzg_result vkzg_amd_write_v1_tables(vkzg_prover *p, const char *path
         G1A *pw = tmp.dalloc<G1A>((size_t)kV1Positions * 8192);
         uint32_t *words = tmp.dalloc<uint32_t>(positions.size());
         Fr *d_kpm = tmp.dalloc<Fr>(128);
-        ck(launch_table_chains(pwj, p->xext, 8, kV1Positions, s));
+        static_assert(kV1Positions == 32, "2^(8d) positions d < 32: the (8, 0) window plan");
+        ck(launch_table_chains(pwj, p->xext, window_plan(8, 0), s));
         ck(launch_jac_to_affine(pw, pwj, prefix, (uint64_t)kV1Positions * 8192, 16, s));
         ck(launch_v1_positions(words, pw, s));
         ck(launch_phase_b_scalars(d_kpm, p->roots, s));
