@@ -30,7 +30,7 @@ FR_M = 136                    # limb-MACs per Fr multiplication (8-limb CIOS)
 # Algorithmic work of one blob's recovery (lean form: 6 NTT_8192, pointwise
 # products, batch-inverted division, vanishing polynomial, I/O), in limb-MACs.
 W_REC = 6.38e7
-LAD_POS = 17                  # ladder positions 2^(8d) u, d < 17 (8-bit digits of the GLV halves of the phase-B scalars)
+LAD_POS = 16                  # ladder positions 2^(8d) u, d < 16 (8-bit digits of the balanced GLV halves of the phase-B scalars)
 DIGIT_BITS = 8                # doublings between ladder positions
 BUCKETS = 128                 # phase-B buckets per output (digit magnitudes 1..128)
 METRIC = "proof blobs/s @256-blob batch; recovery blobs/s; p50/p99 serve latency"

@@ -12,7 +12,7 @@ import random
 R = 0x73EDA753299D7D483339D80809A1D80553BDA402FFFE5BFEFFFFFFFF00000001
 LAMBDA = 0xAC45A4010001A40200000000FFFFFFFF
 DIGIT_BITS = 8
-LAD_POS = 17  # signed bytes of a 128-bit half and a carry position
+LAD_POS = 16  # signed bytes of a balanced GLV half's magnitude (below 2^127: no carry position)
 BUCKETS = 128
 
 
@@ -39,15 +39,28 @@ def segment_taps(kp, km):
     return [kc, kn, km]
 
 
+def glv_split_signed(k):
+    """glv_split_signed: balanced signed halves with k = h1 + lambda h2 mod r."""
+    k1, k2 = k % LAMBDA, k // LAMBDA
+    half = LAMBDA // 2
+    if k1 > half:
+        k1, k2 = k1 - LAMBDA, k2 + 1
+    if k2 > half:
+        k1, k2 = k1 - 1, k2 - (LAMBDA + 1)
+    assert (k1 + LAMBDA * k2 - k) % R == 0
+    assert abs(k1) <= half + 1 and abs(k2) <= half + 1 and half + 1 < 1 << 127
+    return k1, k2
+
+
 def items_of(taps):
-    """(bucket k) -> [(e, pos, neg)]: build_phase_b_items' GLV split and
-    signed 8-bit digits."""
+    """(bucket k) -> [(e, pos, neg)]: build_phase_b_items' balanced GLV split
+    and signed 8-bit digits of the halves' magnitudes, each digit's sign
+    flipped with its half's."""
     lists = [[] for _ in range(BUCKETS)]
     win = 1 << DIGIT_BITS
     for e, t in enumerate(taps):
-        k1, k2 = t % LAMBDA, t // LAMBDA
-        assert k1 < 1 << 128 and k2 < 1 << 128
-        for g, part in enumerate((k1, k2)):
+        for g, h in enumerate(glv_split_signed(t)):
+            part, hneg = abs(h), h < 0
             carry = 0
             for d in range(LAD_POS):
                 v = ((part >> (DIGIT_BITS * d)) & (win - 1)) + carry
@@ -56,7 +69,7 @@ def items_of(taps):
                     v -= win
                     carry = 1
                 if v:
-                    lists[abs(v) - 1].append((e, d + g * LAD_POS, v < 0))
+                    lists[abs(v) - 1].append((e, d + g * LAD_POS, (v < 0) != hneg))
             assert carry == 0
     return lists
 
@@ -241,3 +254,14 @@ def test_reduction_empty_buckets_take_the_doubling():
                     naive_flags.add(lanes)
             assert got % R == want
     assert naive_flags == {4, 8, 16, 32}  # the case the doubling step exists for
+
+
+def test_balanced_glv_split_edges():
+    """glv_split_signed on the edge scalars and random ones: the halves
+    rebuild k mod r and stay below 2^127 (16 signed bytes, no carry)."""
+    rng = random.Random(12)
+    half = LAMBDA // 2
+    edge = [0, 1, half, half + 1, LAMBDA - 1, LAMBDA, LAMBDA + 1, LAMBDA * LAMBDA, LAMBDA * (LAMBDA + 1),
+            R - 1, R - 2, LAMBDA * half, LAMBDA * (half + 1) + half + 1]
+    for k in edge + [rng.randrange(R) for _ in range(2000)]:
+        glv_split_signed(k % R)

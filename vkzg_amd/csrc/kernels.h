@@ -30,20 +30,22 @@
 
 namespace vkzg_amd {
 
-// Phase-B scalars are split as k = k1 + lambda k2 (k1, k2 < 2^128) with the
-// G1 endomorphism phi = [lambda] and written in signed kDigitBits-bit digits:
-// kLadPos of them per 128-bit half (16 bytes and a carry position for 8-bit
-// digits), so each ladder needs 2^(kDigitBits d) u for d < kLadPos only.
+// Phase-B scalars are split as k = k1 + lambda k2 with the G1 endomorphism
+// phi = [lambda], balanced and signed (|k1|, |k2| < 2^126.5), and the
+// magnitudes written in signed kDigitBits-bit digits: kLadPos of them per half
+// (16 bytes, the top one below 2^6.5, so no carry position), so each ladder
+// needs 2^(kDigitBits d) u for d < kLadPos only.
 // Digit magnitudes 1..kBuckets are the buckets of every phase-B output.
-// 7-bit digits (19 positions, 64 buckets) halve the bucket reduction but
-// cost phase B more than that on the MI355X (19.6-19.8k against 19.9-20.0k
-// blobs/s at 256 blobs, on one box).
+// 7-bit digits (64 buckets) halve the bucket reduction but cost phase B more
+// than that on the MI355X (19.6-19.8k against 19.9-20.0k blobs/s at 256
+// blobs, on one box).
 constexpr uint32_t kDigitBits = 8;
-constexpr uint32_t kLadPos = (128 + kDigitBits - 1) / kDigitBits + (128 % kDigitBits == 0 ? 1 : 0);
+constexpr uint32_t kLadPos = (127 + kDigitBits - 1) / kDigitBits;
+static_assert(kDigitBits == 8 && kLadPos == 16, "the top byte of a magnitude below 2^126.5 never carries");
 constexpr uint32_t kBuckets = 1u << (kDigitBits - 1);
 // The ladder may be split into 1, 2 or 4 segments; segment m of M covers
-// positions [lad_start(M, m), lad_start(M, m + 1)) (chains of at most 8 and
-// 4 steps of kDigitBits doublings for M = 2 and 4), and phase A then also
+// positions [lad_start(M, m), lad_start(M, m + 1)) (chains of 7 and 3 steps
+// of kDigitBits doublings for M = 2 and 4), and phase A then also
 // computes 2^(kDigitBits start) u directly.
 constexpr uint32_t kMaxLadSplits = 4;
 __host__ __device__ constexpr uint32_t lad_start(uint32_t M, uint32_t m) { return (kLadPos * m + M - 1) / M; }

@@ -75,6 +75,37 @@ void glv_split(const uint32_t k[8], unsigned __int128 &k1, unsigned __int128 &k2
     k2 = q;
 }
 
+// Balanced signed GLV halves: k = s1 m1 + lambda s2 m2 (mod r), signs s1, s2
+// and magnitudes m1, m2 <= lambda/2 + 1 < 2^127 (so 16 signed bytes hold each
+// half with no carry position).  From glv_split's k1 < lambda, k2 <= lambda + 1:
+// k1 > lambda/2 takes k1 - lambda and k2 + 1; then k2 > lambda/2 takes
+// k2 - (lambda + 1) and k1 - 1, since lambda (lambda + 1) = r - 1 = -1 mod r.
+void glv_split_signed(const uint32_t k[8], unsigned __int128 m[2], bool neg[2]) {
+    const unsigned __int128 lambda =
+        ((unsigned __int128)0xac45a4010001a402ull << 64) | (unsigned __int128)0x00000000ffffffffull;
+    const unsigned __int128 half = lambda >> 1;
+    glv_split(k, m[0], m[1]);
+    neg[0] = neg[1] = false;
+    if (m[0] > half) {
+        m[0] = lambda - m[0];
+        neg[0] = true;
+        m[1] += 1;  // k2 <= lambda here: k < r
+    }
+    if (m[1] > half) {
+        m[1] = lambda + 1 - m[1];  // k2 <= lambda + 1
+        neg[1] = true;
+        if (neg[0]) {
+            m[0] += 1;
+        } else if (m[0] == 0) {
+            m[0] = 1;
+            neg[0] = true;
+        } else {
+            m[0] -= 1;
+        }
+    }
+    if ((m[0] >> 127) != 0 || (m[1] >> 127) != 0) throw VKZG_ERR_SETUP;  // cannot happen
+}
+
 // Canonical Fr arithmetic on the host (little-endian 32-bit limbs).
 void fr_host_add(uint32_t r[8], const uint32_t a[8], const uint32_t b[8]) {
     uint32_t t[8], u[8];
@@ -143,9 +174,10 @@ void phase_b_taps(const Fr *kpm, std::vector<Fr> taps[3]) {
 
 // Phase-B item lists: for each segment s and bucket k, the (e, d, sign) with
 // signed kDigitBits-bit digit of magnitude k+1 at position d of the tap
-// taps[s][e] (the byte-plus-carry recoding of k_phase_b_split.comp:25-61),
-// where d < kLadPos indexes the digits of k1 and kLadPos <= d < 2 kLadPos
-// those of k2.  Offsets: kBuckets + 1 per segment.
+// taps[s][e] (the byte-plus-carry recoding of k_phase_b_split.comp:25-61, on
+// the magnitudes of the balanced signed GLV halves, each digit's sign flipped
+// with its half's), where d < kLadPos indexes the digits of k1 and
+// kLadPos <= d < 2 kLadPos those of k2.  Offsets: kBuckets + 1 per segment.
 void build_phase_b_items(const Fr *kpm, std::vector<uint32_t> &items, std::vector<uint32_t> &offsets) {
     constexpr uint32_t kWin = 1u << kDigitBits, kHalf = kWin / 2;
     std::vector<Fr> taps[3];
@@ -156,7 +188,8 @@ void build_phase_b_items(const Fr *kpm, std::vector<uint32_t> &items, std::vecto
         std::vector<uint32_t> lists[kBuckets];
         for (uint32_t e = 0; e < (uint32_t)taps[seg].size(); e++) {
             unsigned __int128 parts[2];
-            glv_split(taps[seg][e].v, parts[0], parts[1]);
+            bool half_neg[2];
+            glv_split_signed(taps[seg][e].v, parts, half_neg);
             for (int g = 0; g < 2; g++) {
                 int carry = 0;
                 for (uint32_t d = 0; d < kLadPos; d++) {
@@ -171,9 +204,10 @@ void build_phase_b_items(const Fr *kpm, std::vector<uint32_t> &items, std::vecto
                     if (v == 0) continue;
                     const uint32_t mag = (uint32_t)(v < 0 ? -v : v);
                     const uint32_t pos = d + (uint32_t)g * kLadPos;
-                    lists[mag - 1].push_back(e | (pos << 8) | ((v < 0 ? 1u : 0u) << 16));
+                    const bool dneg = (v < 0) != half_neg[g];
+                    lists[mag - 1].push_back(e | (pos << 8) | ((dneg ? 1u : 0u) << 16));
                 }
-                if (carry) throw VKZG_ERR_SETUP;  // cannot happen: halves are below 2^128
+                if (carry) throw VKZG_ERR_SETUP;  // cannot happen: magnitudes are below 2^127
             }
         }
         for (uint32_t k = 0; k < kBuckets; k++) {
