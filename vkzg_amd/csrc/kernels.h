@@ -15,7 +15,7 @@
 //   Fr scal[B][128][64]       phase-A scalars, CANONICAL (not Montgomery), as
 //                             sign (bit 255) and magnitude (< 2^254)
 //   G1X u[B][128][G]          phase-A partial sums, one per window group g < G
-//   G1J lad[B][17][128]       2^(8d) * f[j], d < kLadPos, for the folds f of u:
+//   G1J lad[B][16][128]       2^(8d) * f[j], d < kLadPos, for the folds f of u:
 //                             Fc (j < 32), Fn (32 <= j < 64), F- (j >= 64);
 //                             normalised into G1A lad_aff with the same
 //                             layout; G1A lad_phi: phi(lad_aff), where
