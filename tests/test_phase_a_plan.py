@@ -77,10 +77,27 @@ def test_plans_cover_255_bits():
         assert p.entry(p.W - 1) + (p.H << p.wide(p.W - 1)) == p.entries()
 
 
+def test_chooser_plans():
+    """choose_window's candidates, fewest windows first: c = floor(255 / W)
+    with the top k = 255 - W c windows widened; each is a valid plan of W
+    windows, and the table shrinks as W grows."""
+    prev = None
+    for W in range(18, 37):
+        c, k = 255 // W, 255 - W * (255 // W)
+        p = Plan(c, k)
+        assert (p.W, p.k) == (W, k) and p.W * c + p.k == 255
+        size = 8192 * p.entries() * 128
+        assert prev is None or size < prev
+        prev = size
+    assert 8192 * Plan(13, 8).entries() * 128 == 108 << 30
+    assert 8192 * Plan(12, 3).entries() * 128 == 48 << 30
+
+
 def test_signed_digits_rebuild_the_scalar():
     rng = random.Random(11)
     edge = [0, 1, 2, (R - 1) // 2, (R + 1) // 2, R - 2, R - 1, 1 << 253, (1 << 254) - 1]
-    for plan in (Plan(14, 3), Plan(14, 0), Plan(12, 0), Plan(10, 0), Plan(13, 8)):
+    for plan in (Plan(14, 3), Plan(14, 0), Plan(12, 0), Plan(10, 0), Plan(13, 8), Plan(12, 3), Plan(11, 2),
+                 Plan(9, 21), Plan(7, 31)):
         for a in edge + [rng.randrange(R) for _ in range(400)]:
             s = signed_magnitude(a)
             assert (s & ((1 << 255) - 1)) < 1 << 254

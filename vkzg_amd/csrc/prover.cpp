@@ -34,22 +34,23 @@ uint64_t table_bytes_for(uint32_t c, uint32_t k = 0) {
     return 8192ull * window_plan(c, k).entries() * sizeof(G1AT);
 }
 
-// Window plan of the resident phase-A table (c bits, the top k windows one
-// bit wider): $VKZG_AMD_WINDOW = c if set (6..14, k = 0); else (14, 3) -- 18
-// windows, a 168 GiB table -- when it fits in 65% of the free memory, else the
-// largest c <= 14 whose plain table fits in 60%.  On a fresh 288 GB MI355X
-// that is (14, 3); (14, 0) is 19 windows in 152 GiB, c = 13 20 in 80 GiB; a
-// second prover in the same process gets a smaller window.
+// Window plan of the resident phase-A table: $VKZG_AMD_WINDOW = c if set
+// (6..14, all windows c bits); else the fewest windows W whose cheapest plan
+// fits in 65% of the free memory, c = floor(255 / W) bits with the top
+// k = 255 - W c windows one bit wider.  On a fresh 288 GB MI355X that is
+// W = 18 (14 and 15 bits, 168 GiB); W = 19 is 108 GiB, 20 is 70 GiB, 21 is
+// 48 GiB, 23 is 25 GiB: a second prover in the same process gets more windows.
 WindowPlan choose_window(size_t free_bytes) {
     if (const char *env = std::getenv("VKZG_AMD_WINDOW")) {
         const int c = std::atoi(env);
         if (c >= 6 && c <= 14) return window_plan((uint32_t)c, 0);
     }
-    if (table_bytes_for(14, 3) <= free_bytes / 20 * 13) return window_plan(14, 3);
-    for (uint32_t c = 14; c > 8; c--) {
-        if (table_bytes_for(c) <= free_bytes / 5 * 3) return window_plan(c, 0);
+    for (uint32_t W = 18; W <= 36; W++) {
+        const uint32_t c = 255 / W, k = 255 - W * c;
+        if (c > 14 || c < 6) continue;
+        if (table_bytes_for(c, k) <= free_bytes / 20 * 13) return window_plan(c, k);
     }
-    return window_plan(8, 0);
+    return window_plan(6, 0);
 }
 
 // GLV split of a phase-B scalar (its halves are written in signed digits
