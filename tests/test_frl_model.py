@@ -86,6 +86,7 @@ def test_constants_match_definitions():
     assert C["OFF4"][8] >= (2 * R - 1) >> 232 and C["OFF8"][8] >= 2 * ((2 * R - 1) >> 232)
     assert C["SEVEN_POW64_M"] == limbs(pow(7, 64, R) * RL % R)
     assert C["INV_SEVEN_POW8192_MINUS_1_M"] == limbs(pow(K, -1, R) * RL % R)
+    assert C["SIXTY_FOUR_INV_SEVEN_POW8192_MINUS_1_M"] == limbs(64 * pow(K, -1, R) * RL % R)
     assert C["ONE_M"] == limbs(RL % R) and C["ONE_PLAIN"] == limbs(1)
     assert C["R2"] == limbs(RL * RL % R)
     assert C["FROM_R256"] == limbs(pow(2, 266, R))

@@ -72,6 +72,29 @@ def test_recover_inconsistent_cells_match_oracle(gpu_prover, oracle_mod):
     assert gpu_prover.recover_cells_batch(cells, present, n) == want
 
 
+@pytest.mark.parametrize("radix", [2, 8])
+def test_recover_modes_match_oracle_on_inconsistent_cells(gpu_prover, oracle_mod, radix):
+    """The three-kernel (radix 2) and radix-8 recoveries on present cells that
+    are not a codeword, uneven present counts (64..127, all present, all but
+    one pattern per blob): the reference's linear map, bit for bit."""
+    rng = random.Random(40 + radix)
+    n = 7
+    cells = rng.randbytes(CELLS * n)
+    masks = []
+    for b in range(n):
+        keep = set(rng.sample(range(128), [64, 65, 90, 127, 128, 64, 100][b]))
+        masks.append(bytes(1 if k in keep else 0 for k in range(128)))
+    present = b"".join(masks)
+    rc, want = oracle_mod.recover_cells(cells, present, n)
+    assert rc == 0
+    saved = gpu_prover.get_tuning(n)
+    try:
+        gpu_prover.set_tuning(n, recovery_radix=radix)
+        assert gpu_prover.recover_cells_batch(cells, present, n) == want
+    finally:
+        gpu_prover.set_tuning(n, **saved)
+
+
 def test_recover_chunking_and_bad_chunk(gpu_prover_small_batch, vectors, cells_of):
     rng = random.Random(23)
     src = cells_of("compute_cells_and_kzg_proofs_case_valid_2")

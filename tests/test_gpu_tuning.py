@@ -41,7 +41,7 @@ def test_splits_and_odd_sizes_bit_exact(gpu_prover, vectors, restore_tuning):
     assert gpu_prover.compute_cell_kzg_proofs_batch(blobs, 37) == want
 
 
-@pytest.mark.parametrize("radix,waves", [(4, 1), (4, 2), (4, 4), (8, 1), (8, 2), (8, 4)])
+@pytest.mark.parametrize("radix,waves", [(2, 1), (2, 2), (2, 4), (4, 1), (4, 2), (4, 4), (8, 1), (8, 2), (8, 4)])
 def test_recovery_radix_and_waves_bit_exact(gpu_prover, vectors, cells_of, restore_tuning, radix, waves):
     rng = random.Random(radix * waves)
     names = [NAMES[i % 3] for i in range(3)]
@@ -91,7 +91,7 @@ def test_smaller_window_table_bit_exact(gpu_prover_small_batch, gpu_seeded_prove
 def test_tuning_rejects_out_of_range(gpu_prover, restore_tuning):
     for bad in (dict(phase_a_groups=0), dict(phase_a_blocks_per_cu=3),
                 dict(normalize_chunk=257), dict(splits=5),
-                dict(recovery_waves=3), dict(recovery_waves=8), dict(recovery_radix=2), dict(ladder_splits=3),
+                dict(recovery_waves=3), dict(recovery_waves=8), dict(recovery_radix=16), dict(ladder_splits=3),
                 dict(phase_a_groups=21), dict(reduce_lanes=2), dict(reduce_lanes=64)):
         with pytest.raises(vkzg_amd.VkzgError) as e:
             gpu_prover.set_tuning(3, **bad)

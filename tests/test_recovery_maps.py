@@ -251,6 +251,104 @@ def test_recovery_maps_match_oracle(oracle_mod, radix, consistent):
         assert got == cells
 
 
+# ------------------------------------------------------------------ the three kernels
+
+def r4_pass_af(v, tab, fac):
+    """r4_pass_a with output (L, t) multiplied by fac(L, t) instead of the twiddle."""
+    out = r4_pass_a(v, tab, lambda L: 0)
+    return [[out[L][t] * fac(L, t) % R for t in range(4)] for L in range(64)]
+
+
+def m3_index(c1, k2):
+    return ((c1 >> 1) * 32 + (k2 >> 2)) * 8 + (c1 & 1) * 4 + (k2 & 3)
+
+
+def recover3_model(cells, present):
+    """k_rec3_in, k_rec3_cols, k_rec3_out with their lane and tile maps."""
+    E = [int.from_bytes(cells[32 * w:32 * w + 32], "big") % R for w in range(8192)]
+    M = [None] * 8192
+    izc = [None] * 128
+    for q in range(32):  # k_rec3_in: cells k2 = 4q + cc
+        ks = [4 * q + vv for vv in range(4)]
+        zd = vanish(present, [ROOTS[64 * k] for k in ks], False)
+        pc = vanish(present, [SEVEN64 * ROOTS[64 * k] % R for k in ks], True)
+        for vv in range(4):
+            izc[4 * q + vv] = 64 * pc[vv] * KINV % R
+        v = []
+        for L in range(64):
+            cc, j = L & 3, L >> 2
+            k2 = 4 * q + cc
+            # element brev6(k1) of cell brev7(k2), k1 = j + 16 r
+            v.append([E[64 * brev(k2, 7) + brev(j + 16 * r, 6)] * zd[cc] % R for r in range(4)])
+        X = r4_pass_b(v, ROOTS_INV)
+        for L in range(64):
+            cc, j = L & 3, L >> 2
+            k2 = 4 * q + cc
+            for r in range(4):
+                c1 = j + 16 * r
+                M[m3_index(c1, k2)] = X[L][r] * ROOTS_INV[k2 * c1 % 8192] % R
+    for q in range(32):  # k_rec3_cols: columns c1 = 2q + c
+        def c1_of(L, q=q):
+            return 2 * q + (L >> 5)
+        v = [[M[m3_index(c1_of(L), (L & 31) + 32 * r)] for r in range(4)] for L in range(64)]
+
+        def reorder(o):
+            return [[row[0], row[2], row[1], row[3]] for row in o]
+        o = r4_pass_af(v, ROOTS_INV, lambda L, t: SHIFT_FWD[c1_of(L) + 64 * r4_k2(L & 31, t)])
+        o = r4_pass_af(reorder(o), ROOTS, lambda L, t: izc[r4_k2(L & 31, t)])
+        o = r4_pass_af(reorder(o), ROOTS_INV, lambda L, t: SHIFT_INV[c1_of(L) + 64 * r4_k2(L & 31, t)])
+        o = r4_pass_af(reorder(o), ROOTS, lambda L, t: ROOTS[c1_of(L) * r4_k2(L & 31, t) % 8192])
+        for L in range(64):
+            for t in range(4):
+                M[m3_index(c1_of(L), r4_k2(L & 31, t))] = o[L][t]
+    res = bytearray(CELLS)
+    for q in range(32):  # k_rec3_out
+        v = [[M[m3_index((L >> 2) + 16 * r, 4 * q + (L & 3))] for r in range(4)] for L in range(64)]
+        X = r4_pass_b(v, ROOTS)
+        for L in range(64):
+            cc, j = L & 3, L >> 2
+            k2 = 4 * q + cc
+            for r in range(4):
+                w = 64 * brev(k2, 7) + brev(j + 16 * r, 6)
+                res[32 * w:32 * w + 32] = X[L][r].to_bytes(32, "big")
+    return bytes(res)
+
+
+@pytest.mark.parametrize("consistent", [True, False])
+def test_three_kernel_recovery_matches_oracle(oracle_mod, consistent):
+    """The cell-first split: T2's per-cell DFT_64, the division by the
+    per-cell constant Z on the coset and T3's per-cell inverse DFT_64 cancel
+    to the factor 64; the rest is per cell, per column, per cell."""
+    rng = random.Random(70 + consistent)
+    if consistent:
+        rc, cells = oracle_mod.blob_to_cells(random_blob(rng))
+        assert rc == 0
+        keep = set(rng.sample(range(128), 64))
+    else:
+        cells = rng.randbytes(CELLS)
+        keep = set(rng.sample(range(128), 97))
+    present = bytes(1 if c in keep else 0 for c in range(128))
+    masked = bytearray(rng.randbytes(CELLS))
+    for c in keep:
+        masked[2048 * c:2048 * (c + 1)] = cells[2048 * c:2048 * (c + 1)]
+    rc, want = oracle_mod.recover_cells(bytes(masked), present, 1)
+    assert rc == 0
+    got = recover3_model(bytes(masked), present)
+    assert got == want
+    if consistent:
+        assert got == cells
+
+
+def test_m3_tiles():
+    """Every (c1, k2) has its own slot; a column pair is one contiguous
+    256-value block and 4 cells x 2 columns one 8-value tile."""
+    idx = sorted(m3_index(c1, k2) for c1 in range(64) for k2 in range(128))
+    assert idx == list(range(8192))
+    for q in range(32):
+        assert sorted(m3_index(2 * q + c, k2) for c in range(2) for k2 in range(128)) == \
+            list(range(256 * q, 256 * q + 256))
+
+
 def nth_set_bit(w, k):
     """recover.hip's nth_set_bit: binary search on the low halves' popcounts."""
     pos = 0

@@ -47,6 +47,11 @@ constexpr uint32_t SEVEN_POW64_M[9] = {0x127eef80u, 0x18bc375fu, 0x1bc9599bu, 0x
 constexpr uint32_t INV_SEVEN_POW8192_MINUS_1_M[9] = {0x0d4f86b8u, 0x0e2526d6u, 0x18645494u, 0x1116bbdbu,
                                                      0x014c330au, 0x1cbbdb75u, 0x1bb0daccu, 0x0f5f5d89u,
                                                      0x00305b29u};
+// 64 / (7^8192 - 1): the three-kernel recovery's coset factor with the 64 of
+// the cancelled per-cell DFT_64 pair
+constexpr uint32_t SIXTY_FOUR_INV_SEVEN_POW8192_MINUS_1_M[9] = {
+    0x13e1ade6u, 0x0949b66au, 0x03bf2b9cu, 0x005e5af1u, 0x14041a8eu,
+    0x168e600au, 0x06e1e3b9u, 0x099022a0u, 0x0050a760u};
 // 2^261 mod r: one in Montgomery form
 constexpr uint32_t ONE_M[9] = {0x1fffffbau, 0x0000022fu, 0x1cb61180u, 0x0a4e5c00u, 0x0ee8b1a2u,
                                0x16e6aedfu, 0x1907f8bbu, 0x0853ddf7u, 0x004d043fu};

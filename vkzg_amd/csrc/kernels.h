@@ -155,6 +155,11 @@ hipError_t launch_rec_first(Fr *out, const void *in, int in_mode, const uint8_t 
 hipError_t launch_rec_mid(Fr *out, const Fr *in, int stage, const uint8_t *present, const RecTables &t,
                           uint32_t nblobs, uint32_t radix, uint32_t waves, hipStream_t s);
 // Canonical big-endian cell bytes.
+// The three-kernel recovery (cells -> M, M in place, M -> cells; step 0, 1,
+// 2), radix 4 and `waves` per block: M and izc are per-blob Fr[8192] buffers
+// (izc: its first 128 values).
+hipError_t launch_rec3(uint8_t *cells_out, Fr *M, Fr *izc, const uint8_t *cells, const uint8_t *present,
+                       const RecTables &t, uint32_t nblobs, uint32_t waves, hipStream_t s, int step);
 hipError_t launch_rec_last(uint8_t *cells, const Fr *in, const RecTables &t, uint32_t nblobs, uint32_t radix,
                            uint32_t waves, hipStream_t s);
 

@@ -449,7 +449,7 @@ uint32_t fit_norm_chunk(uint64_t count, uint32_t cus) {
 bool tuning_valid(const Tuning &t) {
     return t.groups >= 1 && t.groups <= kMaxGroups && t.blocks_per_cu >= 1 && t.blocks_per_cu <= 2 &&
            t.norm_chunk <= 256 && t.splits >= 1 && t.splits <= (uint32_t)kMaxSplits &&
-           (t.rec_waves == 1 || t.rec_waves == 2 || t.rec_waves == 4) && (t.rec_radix == 4 || t.rec_radix == 8) &&
+           (t.rec_waves == 1 || t.rec_waves == 2 || t.rec_waves == 4) && (t.rec_radix == 2 || t.rec_radix == 4 || t.rec_radix == 8) &&
            (t.ladder_splits == 1 || t.ladder_splits == 2 || t.ladder_splits == 4) &&
            (t.reduce_lanes == 4 || t.reduce_lanes == 8 || t.reduce_lanes == 16 || t.reduce_lanes == 32) &&
            (t.window_bits == 0 || (t.window_bits >= kExtraWindowMin && t.window_bits <= kExtraWindowMax));
@@ -464,6 +464,17 @@ void enqueue_recovery_into(vkzg_prover *p, Fr *A, Fr *B, uint8_t *d_out, const u
                            const uint8_t *d_present, uint32_t nb, hipStream_t s, StageClock &clk,
                            uint32_t radix, uint32_t waves) {
     const RecTables t = rec_tables(p);
+    if (radix == 2) {
+        // three kernels: per cell, per column (in place in A), per cell;
+        // B holds the coset factors
+        ck(launch_rec3(d_out, A, B, d_cells, d_present, t, nb, waves, s, 0));
+        clk.mark("cells_in");
+        ck(launch_rec3(d_out, A, B, d_cells, d_present, t, nb, waves, s, 1));
+        clk.mark("columns");
+        ck(launch_rec3(d_out, A, B, d_cells, d_present, t, nb, waves, s, 2));
+        clk.mark("cells_out");
+        return;
+    }
     // T1 pass A: (E * Z) -> inverse
     ck(launch_rec_first(A, d_cells, 0, d_present, t, nb, radix, waves, s));
     clk.mark("t1_a");
@@ -840,7 +851,7 @@ void autotune(vkzg_prover *p) {
         Tuning t = p->tune[c];
         float best_ms = 1e30f;
         uint32_t best_waves = t.rec_waves, best_radix = t.rec_radix;
-        for (uint32_t radix : {4u, 8u}) {
+        for (uint32_t radix : {2u, 4u, 8u}) {
             for (uint32_t waves : {1u, 2u, 4u}) {
                 t.rec_radix = radix;
                 t.rec_waves = waves;

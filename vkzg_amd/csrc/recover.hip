@@ -561,7 +561,10 @@ VKZG_DEV uint32_t pass_a4_k2(uint32_t j, int t) {
     // t = 0, 1: j' = j, outputs j', j' + 64; t = 2, 3: j' = j + 32
     return j + 32u * (uint32_t)(t >> 1) + 64u * (uint32_t)(t & 1);
 }
-VKZG_DEV void pass_a4(FrL (&v)[4], const Dir4 &d, const Exch4 &x, uint32_t lane, uint32_t n1) {
+// The same DFT_128 with output v[t] (index pass_a4_k2(j, t)) multiplied by
+// fac(t) (normalised, below 2r) instead of the four-step twiddle.
+template <class F>
+VKZG_DEV void pass_a4f(FrL (&v)[4], const Dir4 &d, const Exch4 &x, uint32_t lane, const F &fac) {
     const uint32_t c = lane >> 5, j = lane & 31u, col = c * 128u;
     dft4(v[0], v[1], v[2], v[3], d.w4);  // < 12
     exch_sync();
@@ -580,10 +583,14 @@ VKZG_DEV void pass_a4(FrL (&v)[4], const Dir4 &d, const Exch4 &x, uint32_t lane,
     const FrL u0 = frl_norm(x.get(col + j)), u2 = frl_norm(x.get(col + j + 32u));
     const FrL t1 = frl_mul(x.get(col + j + 64u), d.pow(64u * j));           // < 2
     const FrL t3 = frl_mul(x.get(col + j + 96u), d.pow(64u * (j + 32u)));  // < 2
-    v[0] = frl_mul(frl_add(u0, t1), d.pow(n1 * pass_a4_k2(j, 0)));   // first operands < 34
-    v[1] = frl_mul(frl_sub4(u0, t1), d.pow(n1 * pass_a4_k2(j, 1)));  // < 36
-    v[2] = frl_mul(frl_add(u2, t3), d.pow(n1 * pass_a4_k2(j, 2)));
-    v[3] = frl_mul(frl_sub4(u2, t3), d.pow(n1 * pass_a4_k2(j, 3)));
+    v[0] = frl_mul(frl_add(u0, t1), fac(0));   // first operands < 34
+    v[1] = frl_mul(frl_sub4(u0, t1), fac(1));  // < 36
+    v[2] = frl_mul(frl_add(u2, t3), fac(2));
+    v[3] = frl_mul(frl_sub4(u2, t3), fac(3));
+}
+VKZG_DEV void pass_a4(FrL (&v)[4], const Dir4 &d, const Exch4 &x, uint32_t lane, uint32_t n1) {
+    const uint32_t j = lane & 31u;
+    pass_a4f(v, d, x, lane, [&](int t) { return d.pow(n1 * pass_a4_k2(j, t)); });
 }
 
 // Both vanishing values' work split, in FrL: the product over the cells c
@@ -749,6 +756,153 @@ __global__ void __launch_bounds__(256, 2)
     }
 }
 
+// ------------------------------------------------------------ three kernels
+//
+// A cell is the coset of mu_64 at the natural indices k = k2 + 128 k1
+// (k1 < 64: cell brev7(k2), element brev6(k1)), and Z is constant on it.
+// With the four-step split taken cell first (k = k2 + 128 k1, coefficient
+// c = c1 + 64 c2), an 8192-point transform is a per-cell DFT_64 over k1, the
+// twiddle w^(k2 c1) and a per-column DFT_128 over k2 (forward transforms run
+// the same steps in reverse).  T2's per-cell DFT_64, the division by the
+// per-cell constant Z(7 w^k) and T3's per-cell inverse DFT_64, with their
+// twiddles, then cancel to a factor 64, and the recovery is
+//   k_rec3_in    per cell: E * Z(w^k), inverse DFT_64, * w^-(k2 c1)
+//   k_rec3_cols  per column c1: inverse DFT_128, * 7^c / 8192, DFT_128,
+//                * 64 / Z(7 w^k), inverse DFT_128, * 7^-c / 8192, DFT_128,
+//                * w^(k2 c1)
+//   k_rec3_out   per cell: DFT_64, canonical cell bytes
+// -- the same linear map as the five kernels (the same bytes on any input)
+// with two twiddle layers and two rounds of DFT_64 fewer.  Items are waves,
+// 32 per blob in each kernel: 4 cells (k2 = 4q + cc) or 2 columns
+// (c1 = 2q + c).  M[c1][k2] sits per blob in 256-byte tiles of 2 columns x
+// 4 cells (m3_index), so cell-wise and column-wise waves both move whole
+// tiles; k_rec3_cols updates its 2 columns (one contiguous 256-value block)
+// in place.  k_rec3_in also leaves 64 / Z(7 w^k) for the blob's 128 classes
+// k2 at the start of the blob's B.
+VKZG_DEV uint32_t m3_index(uint32_t c1, uint32_t k2) {
+    return ((c1 >> 1) * 32u + (k2 >> 2)) * 8u + (c1 & 1u) * 4u + (k2 & 3u);
+}
+
+__global__ void __launch_bounds__(256, 2)
+    k_rec3_in(Fr *M, Fr *izc, const uint8_t *cells, const uint8_t *present, const uint4 *roots,
+              const uint4 *roots_inv, uint32_t nitems) {
+    extern __shared__ uint4 lds[];
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint32_t item = blockIdx.x * (blockDim.x >> 6) + wave;
+    if (item >= nitems) return;
+    const uint32_t blob = item / kItemsPerBlob4, q = item % kItemsPerBlob4;
+    const Exch4 x{lds + kExch4Uint4 * wave};
+    const uint32_t cc = lane & 3u, j = lane >> 2, k2 = 4u * q + cc;
+    // E[k2 + 128 k1] for k1 = j + 16 r: element brev6(k1) = 4 brev4(j) + brev2(r)
+    // of cell brev7(k2); the loads go out first
+    const uint32_t *cw = reinterpret_cast<const uint32_t *>(cells) +
+                         ((size_t)blob * 8192 + 64u * brev_bits(k2, 7) + 4u * brev_bits(j, 4)) * 8;
+    uint4 raw[4][2];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const uint32_t w = brev_bits((uint32_t)r, 2);
+        raw[r][0] = *reinterpret_cast<const uint4 *>(cw + 8 * w);
+        raw[r][1] = *reinterpret_cast<const uint4 *>(cw + 8 * w + 4);
+    }
+    // Z(w^k) = z(w128^k2) and 64 / Z(7 w^k) for the wave's 4 classes
+    // (16 lanes each: value vv = lane >> 4, class 4q + vv)
+    const uint8_t *pres = present + 128ull * blob;
+    const uint32_t kv = 4u * q + (lane >> 4);
+    const FrL xd = ld_frl_tab(roots, (64u * kv) & 8191u);
+    const FrL zd = vanishing_product4(xd, pres, false, roots, lane);
+    const FrL pc = vanishing_product4(frl_mul(xd, frl_const(cfrl::SEVEN_POW64_M)), pres, true, roots, lane);
+    if ((lane & 15u) == 0)
+        st_frl(izc + (size_t)blob * 8192 + kv, frl_mul(pc, frl_const(cfrl::SIXTY_FOUR_INV_SEVEN_POW8192_MINUS_1_M)));
+    // times 2^522: one product takes a plain cell element to E Z in Montgomery form
+    const FrL zr = frl_mul(frl_shfl(zd, (int)(16u * cc)), frl_const(cfrl::R2));
+    FrL v[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const uint4 lo = raw[r][0], hi = raw[r][1];
+        const uint32_t words[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+        const Fr e = fr_from_be_words(words);  // any 256-bit value: below 3r, no reduction needed
+        v[r] = frl_mul(frl_unpack(e.v), zr);
+    }
+    pass_b4(v, Dir4(roots_inv), x, lane);  // inverse DFT_64: v[r] at c1 = j + 16 r, < 32
+    Fr *dst = M + (size_t)blob * 8192;
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        const uint32_t c1 = j + 16u * (uint32_t)r;
+        st_frl(dst + m3_index(c1, k2), frl_mul(v[r], ld_frl_tab(roots_inv, (k2 * c1) & 8191u)));
+    }
+}
+
+__global__ void __launch_bounds__(256, 2)
+    k_rec3_cols(Fr *M, const Fr *izc, const uint4 *roots, const uint4 *roots_inv, const uint4 *shift_fwd,
+                const uint4 *shift_inv, uint32_t nitems) {
+    extern __shared__ uint4 lds[];
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint32_t item = blockIdx.x * (blockDim.x >> 6) + wave;
+    if (item >= nitems) return;
+    const uint32_t blob = item / kItemsPerBlob4, q = item % kItemsPerBlob4;
+    const Exch4 x{lds + kExch4Uint4 * wave};
+    const uint32_t c = lane >> 5, j = lane & 31u, c1 = 2u * q + c;
+    Fr *blk = M + (size_t)blob * 8192;
+    FrL v[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) v[r] = ld_frl(blk + m3_index(c1, j + 32u * (uint32_t)r));
+    // 64 / Z(7 w^k) at this lane's outputs k2 = pass_a4_k2(j, t)
+    FrL iz[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) iz[t] = ld_frl(izc + (size_t)blob * 8192 + pass_a4_k2(j, t));
+    const Dir4 inv(roots_inv), fwd(roots);
+    // pass_a4f leaves v[t] at index pass_a4_k2(j, t) = j, j + 64, j + 32, j + 96;
+    // the next one takes v[r] at j + 32 r
+    const auto reorder = [&]() {
+        const FrL s = v[1];
+        v[1] = v[2];
+        v[2] = s;
+    };
+    // T1: inverse DFT_128 over k2 -> c2, * 7^c / 8192
+    pass_a4f(v, inv, x, lane, [&](int t) { return ld_frl_tab(shift_fwd, c1 + 64u * pass_a4_k2(j, t)); });
+    reorder();
+    // T2: DFT_128 over c2 -> k2, * 64 / Z(7 w^k)
+    pass_a4f(v, fwd, x, lane, [&](int t) { return iz[t]; });
+    reorder();
+    // T3: inverse DFT_128 over k2 -> c2, * 7^-c / 8192
+    pass_a4f(v, inv, x, lane, [&](int t) { return ld_frl_tab(shift_inv, c1 + 64u * pass_a4_k2(j, t)); });
+    reorder();
+    // T4: DFT_128 over c2 -> k2, * w^(k2 c1)
+    pass_a4f(v, fwd, x, lane, [&](int t) { return fwd.pow(c1 * pass_a4_k2(j, t)); });
+#pragma unroll
+    for (int t = 0; t < 4; t++) st_frl(blk + m3_index(c1, pass_a4_k2(j, t)), v[t]);
+}
+
+__global__ void __launch_bounds__(256, 2)
+    k_rec3_out(uint8_t *cells, const Fr *M, const uint4 *roots, uint32_t nitems) {
+    extern __shared__ uint4 lds[];
+    const uint32_t wave = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint32_t item = blockIdx.x * (blockDim.x >> 6) + wave;
+    if (item >= nitems) return;
+    const uint32_t blob = item / kItemsPerBlob4, q = item % kItemsPerBlob4;
+    const Exch4 x{lds + kExch4Uint4 * wave};
+    const uint32_t cc = lane & 3u, j = lane >> 2, k2 = 4u * q + cc;
+    const Fr *src = M + (size_t)blob * 8192;
+    FrL v[4];
+#pragma unroll
+    for (int r = 0; r < 4; r++) v[r] = ld_frl(src + m3_index(j + 16u * (uint32_t)r, k2));
+    pass_b4(v, Dir4(roots), x, lane);  // DFT_64 over c1: v[r] at k1 = j + 16 r, < 32
+    uint32_t *dst = reinterpret_cast<uint32_t *>(cells) +
+                    ((size_t)blob * 8192 + 64u * brev_bits(k2, 7) + 4u * brev_bits(j, 4)) * 8;
+    const FrL one = frl_const(cfrl::ONE_PLAIN);
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+        Fr e;
+        frl_pack(e.v, frl_mul(v[r], one));  // Montgomery -> plain: at most r (r only for 0)
+        fr_reduce_once(e.v, e.v);
+        uint32_t w[8];
+        fr_to_be_words(w, e);
+        uint4 *p = reinterpret_cast<uint4 *>(dst + 8u * brev_bits((uint32_t)r, 2));
+        p[0] = make_uint4(w[0], w[1], w[2], w[3]);
+        p[1] = make_uint4(w[4], w[5], w[6], w[7]);
+    }
+}
+
 // FrL table from an Fr (8 x 32-bit Montgomery) one: out[k] = in[k] 2^5, as
 // 48-byte entries.
 __global__ void k_frl_table(uint4 *out, const Fr *in, uint32_t n) {
@@ -773,7 +927,7 @@ hipError_t launch_frl_table(uint4 *out, const Fr *in, uint32_t n, hipStream_t s)
 
 hipError_t launch_rec_first(Fr *out, const void *in, int in_mode, const uint8_t *present, const RecTables &t,
                             uint32_t nblobs, uint32_t radix, uint32_t waves, hipStream_t s) {
-    const bool r4 = radix == 4;
+    const bool r4 = radix != 8;  // radix 2 (the three kernels) evaluates cells like radix 4
     const uint32_t nitems = nblobs * (r4 ? kItemsPerBlob4 : kItemsPerBlob);
     const dim3 grid(rec_grid(nitems, waves)), block(64 * waves);
     const uint32_t lds = (r4 ? kExch4Uint4 * sizeof(uint4) : kExchBytes) * waves;
@@ -790,7 +944,7 @@ hipError_t launch_rec_first(Fr *out, const void *in, int in_mode, const uint8_t 
 
 hipError_t launch_rec_mid(Fr *out, const Fr *in, int stage, const uint8_t *present, const RecTables &t,
                           uint32_t nblobs, uint32_t radix, uint32_t waves, hipStream_t s) {
-    const bool r4 = radix == 4;
+    const bool r4 = radix != 8;  // radix 2 (the three kernels) evaluates cells like radix 4
     const uint32_t nitems = nblobs * (r4 ? kItemsPerBlob4 : kItemsPerBlob);
     const dim3 grid(rec_grid(nitems, waves)), block(64 * waves);
     const uint32_t lds = (r4 ? kExch4Uint4 * sizeof(uint4) : kExchBytes) * waves;
@@ -818,9 +972,24 @@ hipError_t launch_rec_mid(Fr *out, const Fr *in, int stage, const uint8_t *prese
     return hipGetLastError();
 }
 
+hipError_t launch_rec3(uint8_t *cells_out, Fr *M, Fr *izc, const uint8_t *cells, const uint8_t *present,
+                       const RecTables &t, uint32_t nblobs, uint32_t waves, hipStream_t s, int step) {
+    const uint32_t nitems = nblobs * kItemsPerBlob4;
+    const dim3 grid(rec_grid(nitems, waves)), block(64 * waves);
+    const uint32_t lds = kExch4Uint4 * sizeof(uint4) * waves;
+    if (step == 0)
+        hipLaunchKernelGGL(k_rec3_in, grid, block, lds, s, M, izc, cells, present, t.roots4, t.roots_inv4, nitems);
+    else if (step == 1)
+        hipLaunchKernelGGL(k_rec3_cols, grid, block, lds, s, M, izc, t.roots4, t.roots_inv4, t.shift_fwd4,
+                           t.shift_inv4, nitems);
+    else
+        hipLaunchKernelGGL(k_rec3_out, grid, block, lds, s, cells_out, M, t.roots4, nitems);
+    return hipGetLastError();
+}
+
 hipError_t launch_rec_last(uint8_t *cells, const Fr *in, const RecTables &t, uint32_t nblobs, uint32_t radix,
                            uint32_t waves, hipStream_t s) {
-    const bool r4 = radix == 4;
+    const bool r4 = radix != 8;  // radix 2 (the three kernels) evaluates cells like radix 4
     const uint32_t nitems = nblobs * (r4 ? kItemsPerBlob4 : kItemsPerBlob);
     const dim3 grid(rec_grid(nitems, waves)), block(64 * waves);
     const uint32_t lds = (r4 ? kExch4Uint4 * sizeof(uint4) : kExchBytes) * waves;

@@ -30,7 +30,7 @@ struct Tuning {
     uint32_t norm_chunk = 0;     // points per batched inversion in the normalisation (0: fitted to the chip)
     uint32_t splits = 1;         // sub-streams, 1..kMaxSplits
     uint32_t rec_waves = 1;      // recovery work items (waves) per block: 1, 2 or 4
-    uint32_t rec_radix = 4;      // recovery register radix: 4 (4 values per lane) or 8 (8 values)
+    uint32_t rec_radix = 2;      // recovery: 2 (three kernels, 4 values per lane), 4 or 8 (five kernels)
     uint32_t ladder_splits = 1;  // ladder segments (phase A also computes their starts): 1, 2 or 4
     uint32_t reduce_lanes = 8;   // bucket-reduction lanes per output: 4, 8, 16 or 32
     uint32_t window_bits = 0;    // phase-A table: 0 = the prover's main table, else a resident
