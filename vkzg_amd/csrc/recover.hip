@@ -23,6 +23,9 @@
 //   k_rec_last           T4 pass B -> canonical cell bytes
 // The vanishing values each wave needs (8 of Z on the domain in the first
 // kernel, 8 of 1/Z on the coset in k_rec_mid<1>) are computed by that wave.
+// The default form, k_rec3_* ("three kernels" below), splits the other way
+// round and needs only three kernels; the five-kernel forms remain as tuning
+// options (radix 4 and 8).
 //
 // Work items are waves: item (blob, q) owns the pass-A columns n1 in
 // [4q, 4q + 4) and the pass-B columns k2 in [4q, 4q + 4) and
