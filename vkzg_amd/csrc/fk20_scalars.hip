@@ -21,15 +21,20 @@ constexpr int kScalarWaves = kScalarThreads / 64;
 constexpr int kPolyPitch = 4096;                 // LDS pitch of the polynomial (SoA limbs)
 constexpr int kColPitch = kScalarWaves * 128;    // LDS pitch of the per-wave circulant columns
 
-// One block per blob, 8 waves (2 per SIMD).  LDS: 8 x 4096 words of
-// polynomial + 8 x 1024 words of circulant columns = 160 KiB, all of a CU.
+// `parts` blocks per blob (1, 2, 4 or 8), 8 waves each (2 per SIMD).  LDS:
+// 8 x 4096 words of polynomial + 8 x 1024 words of circulant columns =
+// 160 KiB, all of a CU.  Every part parses the blob and runs the inverse
+// NTT_4096 in its own LDS; part p then takes the circulant columns of rounds
+// [p R, (p + 1) R), R = 8 / parts.  Small batches leave most CUs idle, so
+// they split a blob over more blocks: a shorter chain for the same CUs'
+// time.  Part 0 alone owns the blob's error flag and its coefficients.
 __global__ void __launch_bounds__(kScalarThreads)
     k_fk20_scalars(Fr *scal, const uint8_t *blobs, uint64_t blob_stride, uint32_t *err,
                    const Fr *roots, const Fr *roots_inv, uint32_t *queue, Fr *coeffs,
-                   uint32_t splits) {
+                   uint32_t splits, uint32_t parts) {
     __shared__ uint32_t poly[8 * kPolyPitch];
     __shared__ uint32_t cols[8 * kColPitch];
-    const uint32_t blob = blockIdx.x;
+    const uint32_t blob = blockIdx.x / parts, part = blockIdx.x % parts;
     const int tid = threadIdx.x;
     const uint32_t *src = reinterpret_cast<const uint32_t *>(blobs + blob_stride * blob);
 
@@ -50,11 +55,11 @@ __global__ void __launch_bounds__(kScalarThreads)
     // non-canonical element, so err needs no clearing before the launch (a
     // barrier orders the clear before the sets; __syncthreads_or would take
     // LDS beyond the 160 KiB the arrays use).
-    if (tid == 0) err[blob] = 0u;
+    if (part == 0 && tid == 0) err[blob] = 0u;
     __syncthreads();
-    if (bad) atomicOr(&err[blob], 1u);
+    if (part == 0 && bad) atomicOr(&err[blob], 1u);
     // Clear the tile counter of the persistent phase-A launch that follows.
-    if (blob == 0 && tid == 0) queue[0] = 0u;
+    if (blockIdx.x == 0 && tid == 0) queue[0] = 0u;
 
     // 2. Inverse NTT_4096 -> monomial coefficients, scaled by 1/4096.
     lds_ntt_dit<12>(poly, kPolyPitch, 1, roots_inv, tid, kScalarThreads);
@@ -65,7 +70,7 @@ __global__ void __launch_bounds__(kScalarThreads)
     __syncthreads();
     // Optionally the monomial coefficients, zero-padded to 8192, for the cell
     // extension (compute_cells_and_kzg_proofs).
-    if (coeffs) {
+    if (coeffs && part == 0) {
         Fr *cdst = coeffs + (size_t)blob * 8192;
 #pragma unroll 1
         for (int i = tid; i < 8192; i += kScalarThreads)
@@ -92,8 +97,9 @@ __global__ void __launch_bounds__(kScalarThreads)
             x = fr_add(x, x);
         mult[m] = x;
     }
+    const int rounds = 64 / kScalarWaves / (int)parts;
 #pragma unroll 1
-    for (int round = 0; round < 64 / kScalarWaves; round++) {
+    for (int round = (int)part * rounds; round < ((int)part + 1) * rounds; round++) {
         const int i = round * kScalarWaves + wave;
 #pragma unroll
         for (int h = 0; h < 2; h++) {
@@ -135,8 +141,11 @@ __global__ void __launch_bounds__(kScalarThreads)
 hipError_t launch_fk20_scalars(Fr *scal, const uint8_t *blobs, uint64_t blob_stride, uint32_t *err,
                                const Fr *roots, const Fr *roots_inv, uint32_t *queue, Fr *coeffs,
                                uint32_t splits, uint32_t nblobs, hipStream_t s) {
-    hipLaunchKernelGGL(k_fk20_scalars, dim3(nblobs), dim3(kScalarThreads), 0, s, scal, blobs,
-                       blob_stride, err, roots, roots_inv, queue, coeffs, splits);
+    // blocks of a blob: up to 8 while the batch fills fewer than 256 blocks
+    uint32_t parts = 1;
+    while (parts < 8 && nblobs * parts * 2 <= 256) parts *= 2;
+    hipLaunchKernelGGL(k_fk20_scalars, dim3(nblobs * parts), dim3(kScalarThreads), 0, s, scal, blobs,
+                       blob_stride, err, roots, roots_inv, queue, coeffs, splits, parts);
     return hipGetLastError();
 }
 
