@@ -1,8 +1,8 @@
 // FK20 cell-proof kernels (the reference's computeBatch, src/vkzg.cpp:899-1039).
 //
 //   k_fk20_scalars   blob bytes -> canonical check -> iNTT_4096 -> circulant
-//                    columns -> NTT_128 -> phase-A scalars (one block per blob,
-//                    the whole polynomial resident in LDS)
+//                    columns -> NTT_128 -> phase-A scalars (1-8 blocks per
+//                    blob, each with the whole polynomial resident in LDS)
 //   k_phase_a        u[j] = sum_i scal[j][i] * xext[j][i]: fixed-base windowed
 //                    MSM against the resident multiples table, 8 lanes per tile
 //   k_sum_groups     the window-group partials of each MSM summed
