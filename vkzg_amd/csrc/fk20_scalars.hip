@@ -86,15 +86,18 @@ __global__ void __launch_bounds__(kScalarThreads)
     Fr *out = scal + (size_t)blob * splits * 128 * 64;
     // Scalar sets of a split ladder: segment m starts at 2^(kDigitBits
     // lad_start(M, m)) u; the multipliers by doubling one (under 8 kLadPos Fr
-    // additions per thread, once per blob).
+    // additions per thread, once per blob).  Indexed by constants only, so
+    // they stay in registers.
     Fr mult[kMaxLadSplits];
     mult[0] = fr_one();
-#pragma unroll 1
-    for (uint32_t m = 1; m < splits; m++) {
+#pragma unroll
+    for (uint32_t m = 1; m < kMaxLadSplits; m++) {
         Fr x = mult[m - 1];
+        if (m < splits) {
 #pragma unroll 1
-        for (uint32_t k = kDigitBits * lad_start(splits, m - 1); k < kDigitBits * lad_start(splits, m); k++)
-            x = fr_add(x, x);
+            for (uint32_t k = kDigitBits * lad_start(splits, m - 1); k < kDigitBits * lad_start(splits, m); k++)
+                x = fr_add(x, x);
+        }
         mult[m] = x;
     }
     const int rounds = 64 / kScalarWaves / (int)parts;
@@ -128,9 +131,10 @@ __global__ void __launch_bounds__(kScalarThreads)
             const int j = lane + 64 * h;
             const Fr v = lds_load_fr(col, kColPitch, j);
             out[j * 64 + i] = fr_signed_magnitude(fr_to_canonical(v));
-#pragma unroll 1
-            for (uint32_t m = 1; m < splits; m++)
-                out[(size_t)m * 8192 + j * 64 + i] = fr_signed_magnitude(fr_to_canonical(fr_mul(v, mult[m])));
+#pragma unroll
+            for (uint32_t m = 1; m < kMaxLadSplits; m++)
+                if (m < splits)
+                    out[(size_t)m * 8192 + j * 64 + i] = fr_signed_magnitude(fr_to_canonical(fr_mul(v, mult[m])));
         }
         __syncthreads();
     }

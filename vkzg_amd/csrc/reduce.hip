@@ -18,28 +18,43 @@ __device__ __attribute__((noinline)) void reduce_slow(G1X *out, const G1X *B) {
 // k * 256 + t), so the running sum waits there while the bucket sum is
 // updated: three live points would not fit 256 VGPRs without spilling.
 struct G1XStash {
-    uint32_t *w;
+    uint32_t *w;  // this thread's word 0
     VKZG_DEV void put(const G1X &p) const {
 #pragma unroll
         for (int i = 0; i < kFpLimbs; i++) {
-            w[(0 * kFpLimbs + i) * 256 + threadIdx.x] = p.x.v[i];
-            w[(1 * kFpLimbs + i) * 256 + threadIdx.x] = p.y.v[i];
-            w[(2 * kFpLimbs + i) * 256 + threadIdx.x] = p.zz.v[i];
-            w[(3 * kFpLimbs + i) * 256 + threadIdx.x] = p.zzz.v[i];
+            w[(0 * kFpLimbs + i) * 256] = p.x.v[i];
+            w[(1 * kFpLimbs + i) * 256] = p.y.v[i];
+            w[(2 * kFpLimbs + i) * 256] = p.zz.v[i];
+            w[(3 * kFpLimbs + i) * 256] = p.zzz.v[i];
         }
     }
     VKZG_DEV G1X get() const {
         G1X p;
 #pragma unroll
         for (int i = 0; i < kFpLimbs; i++) {
-            p.x.v[i] = w[(0 * kFpLimbs + i) * 256 + threadIdx.x];
-            p.y.v[i] = w[(1 * kFpLimbs + i) * 256 + threadIdx.x];
-            p.zz.v[i] = w[(2 * kFpLimbs + i) * 256 + threadIdx.x];
-            p.zzz.v[i] = w[(3 * kFpLimbs + i) * 256 + threadIdx.x];
+            p.x.v[i] = w[(0 * kFpLimbs + i) * 256];
+            p.y.v[i] = w[(1 * kFpLimbs + i) * 256];
+            p.zz.v[i] = w[(2 * kFpLimbs + i) * 256];
+            p.zzz.v[i] = w[(3 * kFpLimbs + i) * 256];
         }
         return p;
     }
 };
+
+// The lane index, recomputed where it is called: the asm is opaque, so what
+// derives from it is not hoisted out of a loop and held (or spilled) across
+// the loop's point chains.
+VKZG_DEV uint32_t lane_here() {
+    uint32_t l;
+    asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(l));
+    return l;
+}
+
+// A uniform value made opaque at this point (same reason).
+VKZG_DEV uint32_t uniform_here(uint32_t v) {
+    asm volatile("" : "+s"(v));
+    return v;
+}
 
 // cvals[blob][a] = sum_{k=1..kBuckets} k * buckets[blob][a][k-1]
 // (field.glsl.inc:478-517).  A team of LANES lanes shares an output; lane l
@@ -60,20 +75,23 @@ __global__ void __launch_bounds__(256, 2)
     constexpr uint32_t per = kBuckets / LANES;
     constexpr uint32_t teams = 64 / LANES;
     __shared__ uint32_t stash_words[4 * kFpLimbs * 256];
-    const G1XStash stash{stash_words};
-    const uint32_t lane = threadIdx.x % 64;
-    const uint32_t l = lane % LANES;
+    const uint32_t wave_base = __builtin_amdgcn_readfirstlane(threadIdx.x) & ~63u;
     const uint32_t nitems = (nout + teams - 1) / teams;
+    // Lane-dependent values are rederived per item from lane_here(), so none
+    // is held across the chains.
 #pragma unroll 1
     for (;;) {
+        const uint32_t lane = lane_here();
         uint32_t item = 0;
         if (lane == 0) item = atomicAdd(q, 1u);
         item = __shfl(item, 0);
         if (item >= nitems) break;
+        const uint32_t l = lane % LANES;
         const uint32_t o = item * teams + lane / LANES;
+        const G1XStash stash{stash_words + wave_base + lane};
         G1X s = g1x_identity();
         stash.put(g1x_identity());
-        bool exc = force_checked != 0;
+        bool exc = uniform_here(force_checked) != 0;
         if (o < nout) {
             const G1X *B = buckets + (size_t)o * kBuckets + l * per;
             // The item lists leave some buckets empty (the identity), the
@@ -101,25 +119,27 @@ __global__ void __launch_bounds__(256, 2)
         // sum_l per l s_l = per * sum_{m >= 1} S_m, with the suffix sums
         // S_m = sum_{l >= m} s_l of the team (a log-step scan over shuffles),
         // then log2(per) doublings: no lane-dependent double-and-add chain.
+        const uint32_t l2 = lane_here() % LANES;
         G1X x = s;
 #pragma unroll 1
         for (uint32_t d = 1; d < LANES; d <<= 1) {
             const G1X y = g1x_shfl_down(x, d);
-            if (l + d < LANES) x = g1x_add_fast(x, y, exc);
+            if (l2 + d < LANES) x = g1x_add_fast(x, y, exc);
         }
-        if (l == 0) x = g1x_identity();
+        if (l2 == 0) x = g1x_identity();
 #pragma unroll 1
         for (uint32_t k = 1; k < per; k <<= 1) x = g1x_dbl_core(x);
         G1X v = g1x_add_fast(stash.get(), x, exc);
 #pragma unroll 1
         for (int m = 1; m < LANES; m <<= 1) v = g1x_add_fast(v, g1x_shfl_xor(v, m), exc);
         const uint64_t flagged = __ballot(exc);
-        const uint32_t team = lane / LANES;
+        const uint32_t lane2 = lane_here();
+        const uint32_t o2 = item * teams + lane2 / LANES;
         const uint64_t team_mask = LANES == 64 ? ~0ull : ((1ull << LANES) - 1);
-        const bool redo = ((flagged >> (team * LANES)) & team_mask) != 0;
-        if (o < nout && l == 0) {
-            cvals[o] = v;
-            flags[o] = redo ? 1 : 0;
+        const bool redo = ((flagged >> (lane2 / LANES * LANES)) & team_mask) != 0;
+        if (o2 < nout && lane2 % LANES == 0) {
+            cvals[o2] = v;
+            flags[o2] = redo ? 1 : 0;
         }
     }
 }
