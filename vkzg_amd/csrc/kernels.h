@@ -114,7 +114,8 @@ hipError_t launch_phase_a(G1X *part, const Fr *scal, const G1AT *table, WindowPl
 // nmsm a multiple of 32).
 hipError_t launch_sum_groups(G1X *usum, const G1X *part, uint32_t groups, uint32_t nmsm, hipStream_t s);
 // u: one sum per MSM, [nblobs][splits][128].
-hipError_t launch_ladder(G1J *lad, const G1X *u, uint32_t splits, uint32_t nblobs, hipStream_t s);
+hipError_t launch_ladder(G1J *lad, const G1X *u, uint32_t splits, uint32_t nblobs, uint32_t lanes,
+                         hipStream_t s);
 // Also clears the work counters of the phase-B and bucket-reduction launches
 // that follow (queue_b[0], queue_r[0]).
 hipError_t launch_glv_phi(G1A *phi, const G1A *pts, uint64_t count, uint32_t *queue_b, uint32_t *queue_r,

@@ -30,6 +30,30 @@ VKZG_DEV G1J g1j_dbl_pair(const G1J &p, uint32_t q) {
     return r;
 }
 
+// The same doubling on the four lanes of a quad: E^2 needs only A, so it
+// joins round two on a third lane, and a doubling waits on two squarings and
+// one product -- {X^2, Y^2}, {B^2, (X+B)^2, E^2}, {E(D - X3), YZ} -- at twice
+// the lanes per chain.  Operands and lazy sums are g1j_dbl_pair's, so the
+// results are the same field elements.  For batches whose pair ladder leaves
+// SIMDs idle: there the chain's length, not the issue rate, sets the time.
+VKZG_DEV G1J g1j_dbl_quad(const G1J &p, uint32_t q) {
+    if (fp_is_zero(p.z)) return p;  // uniform in the quad
+    const bool odd = (q & 1u) != 0;
+    const Fp r1 = fp_sqr(fp_select(odd, p.y, p.x));
+    const Fp A = fp_quad_bcast<0>(r1), B = fp_quad_bcast<1>(r1);
+    const Fp E = fp_triple_lazy(A);  // feeds products only
+    const Fp r2 = fp_sqr(fp_select(q == 2, E, fp_select(q == 0, B, fp_add_lazy(p.x, B))));
+    const Fp C = fp_quad_bcast<0>(r2), T = fp_quad_bcast<1>(r2), F = fp_quad_bcast<2>(r2);
+    const Fp D = fp_dbl(fp_sub(fp_sub(T, A), C));
+    G1J r;
+    r.x = fp_sub(F, fp_dbl(D));
+    const Fp C8 = fp_dbl(fp_dbl(fp_dbl(C)));
+    const Fp r4 = fp_mul(fp_select(odd, p.y, E), fp_select(odd, p.z, fp_sub_lazy(D, r.x)));
+    r.y = fp_sub(fp_quad_bcast<0>(r4), C8);
+    r.z = fp_dbl(fp_quad_bcast<1>(r4));
+    return r;
+}
+
 // The phase-B points are linear in u, and so are the doublings, so the
 // split-circulant folds (k_fold_ladder.comp) are taken before the ladder:
 //   F+[i] = u[i] + u[i+64], F-[i] = u[i] - u[i+64] (i < 64),
@@ -39,14 +63,15 @@ VKZG_DEV G1J g1j_dbl_pair(const G1J &p, uint32_t q) {
 // With `splits` = M segments, pair (blob, m, j) starts from the same folds of
 // phase A's 2^(kDigitBits start_m) u (u[(blob*M + m)*128 + k], phase A's MSMs with
 // their window groups summed), start_m = lad_start(M, m), and writes
-// positions [start_m, start_{m+1}).
+// positions [start_m, start_{m+1}).  LANES (2 or 4) lanes share a chain
+// (g1j_dbl_pair or g1j_dbl_quad) and exit together.
+template <int LANES>
 __global__ __launch_bounds__(256) void k_ladder(G1J *lad, const G1X *u, uint32_t splits, uint32_t nblobs) {
     // A long dependent chain: give these waves issue priority over the
     // throughput kernels of another sub-batch sharing the SIMD.
     __builtin_amdgcn_s_setprio(3);
-    // two lanes per ladder (g1j_dbl_pair); pairs exit together
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t g = tid >> 1, q = tid & 1u;
+    const uint32_t g = tid / LANES, q = tid % LANES;
     if (g >= nblobs * 128 * splits) return;
     const uint32_t j = g % 128, m = (g / 128) % splits, blob = g / (128 * splits);
     const size_t base = (size_t)(blob * splits + m) * 128;
@@ -73,7 +98,7 @@ __global__ __launch_bounds__(256) void k_ladder(G1J *lad, const G1X *u, uint32_t
         }
         if (d + 1 == d1) break;
 #pragma unroll 1
-        for (uint32_t k = 0; k < kDigitBits; k++) p = g1j_dbl_pair(p, q);
+        for (uint32_t k = 0; k < kDigitBits; k++) p = LANES == 4 ? g1j_dbl_quad(p, q) : g1j_dbl_pair(p, q);
     }
 }
 
@@ -111,13 +136,20 @@ __global__ void k_glv_phi(G1A *phi, const G1A *pts, uint64_t count, uint32_t *qu
 
 }  // namespace
 
-hipError_t launch_ladder(G1J *lad, const G1X *u, uint32_t splits, uint32_t nblobs, hipStream_t s) {
+hipError_t launch_ladder(G1J *lad, const G1X *u, uint32_t splits, uint32_t nblobs, uint32_t lanes,
+                         hipStream_t s) {
     // Workgroups of four waves: the dispatcher spreads a workgroup's waves over
     // a CU's four SIMDs, where one-wave workgroups doubled up on some SIMDs
     // while others idled (1.6 ms instead of 0.9 ms for 256 blobs; each wave's
-    // chain is issue-bound, so a shared SIMD halves its rate).
-    hipLaunchKernelGGL(k_ladder, dim3(blocks_for(nblobs * 128ull * splits * 2, 256)), dim3(256), 0, s, lad,
-                       u, splits, nblobs);
+    // chain is issue-bound, so a shared SIMD halves its rate).  Whole groups
+    // of LANES lanes fill each 256-thread block.
+    const uint64_t threads = nblobs * 128ull * splits * lanes;
+    if (lanes == 4)
+        hipLaunchKernelGGL(k_ladder<4>, dim3(blocks_for(threads, 256)), dim3(256), 0, s, lad, u, splits, nblobs);
+    else if (lanes == 2)
+        hipLaunchKernelGGL(k_ladder<2>, dim3(blocks_for(threads, 256)), dim3(256), 0, s, lad, u, splits, nblobs);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

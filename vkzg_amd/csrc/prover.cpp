@@ -440,6 +440,13 @@ void default_tunings(vkzg_prover *p) {
 // plus 7 products per point, issued at one wave per SIMD; a second wave on a
 // SIMD doubles the stage, so 557,056 points (256 blobs) take chunks of 9 on
 // 1024 SIMDs, not 8.
+// Lanes per ladder chain: four while four-lane chains still fit one wave per
+// SIMD (the chain's length sets the time there), two above that (the issue
+// rate does).
+uint32_t ladder_lanes(uint64_t chains, uint32_t cus) {
+    return chains * 4 <= (uint64_t)cus * 4 * 64 ? 4u : 2u;
+}
+
 uint32_t fit_norm_chunk(uint64_t count, uint32_t cus) {
     const uint64_t lanes = (uint64_t)cus * 4 * 64;
     const uint64_t c = (count + lanes - 1) / lanes;
@@ -520,7 +527,8 @@ void proofs_range(vkzg_prover *p, WorkSet &ws, uint32_t off, uint32_t nb, uint8_
         ck(launch_sum_groups(usum, u, tn.groups, nb * 128 * tn.ladder_splits, s));
         clk.mark("sum_groups");
     }
-    ck(launch_ladder(lad, tn.groups > 1 ? usum : u, tn.ladder_splits, nb, s));
+    ck(launch_ladder(lad, tn.groups > 1 ? usum : u, tn.ladder_splits, nb,
+                     ladder_lanes((uint64_t)nb * 128 * tn.ladder_splits, p->cus), s));
     clk.mark("ladder");
     const uint32_t chunk = tn.norm_chunk ? tn.norm_chunk : fit_norm_chunk((uint64_t)nb * npos, p->cus);
     ck(launch_jac_to_affine(lad_aff, lad, prefix, (uint64_t)nb * npos, chunk, s));
