@@ -150,7 +150,8 @@ typedef struct {
     uint32_t recovery_waves;        /* recovery work items (one wave each) per block: 1, 2 or 4 */
     uint32_t ladder_splits;         /* doubling-ladder segments, 1, 2 or 4; phase A computes the start
                                        of each (more phase-A work, shorter dependent chains) */
-    uint32_t reduce_lanes;          /* bucket-reduction lanes per output: 4, 8, 16 or 32 */
+    uint32_t reduce_lanes;          /* bucket-reduction lanes per output: 4, 8, 16 or 32 (one lane per
+                                       bucket range), or 64 (16 ranges, four lanes sharing each addition) */
     uint32_t recovery_radix;        /* recovery: 2 (three kernels, 4 values per lane), 4 or 8 (five kernels, 4 or 8 values per lane) */
     uint32_t phase_a_window_bits;   /* phase-A table window: 0 = the main table, else a smaller
                                        resident one (vkzg_amd_info.extra_table_windows) */

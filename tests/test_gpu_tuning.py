@@ -26,11 +26,30 @@ def restore_tuning(gpu_prover):
 def test_phase_a_groups_bit_exact(gpu_prover, vectors, restore_tuning, groups):
     blobs = b"".join(vectors.blob(n) for n in NAMES)
     want = b"".join(vectors.proofs(n) for n in NAMES)
-    for bpc, chunk, splits, lanes in ((1, 1, 1, 8), (2, 64, 2, 16), (2, 8, 4, 32), (2, 0, 1, 4)):
+    for bpc, chunk, splits, lanes in ((1, 1, 1, 8), (2, 64, 2, 16), (2, 8, 4, 32), (2, 0, 1, 4), (1, 0, 4, 64)):
         gpu_prover.set_tuning(3, phase_a_groups=groups, phase_a_blocks_per_cu=bpc, normalize_chunk=chunk,
                               ladder_splits=splits, reduce_lanes=lanes)
         assert gpu_prover.get_tuning(3)["phase_a_groups"] == groups
         assert gpu_prover.compute_cell_kzg_proofs_batch(blobs, 3) == want, (bpc, chunk, splits)
+
+
+@pytest.mark.parametrize("lanes", [4, 8, 16, 32, 64])
+def test_reduce_lanes_bit_exact(gpu_prover, vectors, restore_tuning, lanes):
+    """Every bucket-reduction shape (64: 16 bucket ranges, four lanes sharing
+    each addition), on the fast path and with every output forced through the
+    checked recompute, gives the consensus proofs."""
+    names = vectors.valid_compute()
+    blobs = b"".join(vectors.blob(n) for n in names)
+    want = b"".join(vectors.proofs(n) for n in names)
+    gpu_prover.set_tuning(len(names), reduce_lanes=lanes)
+    assert gpu_prover.get_tuning(len(names))["reduce_lanes"] == lanes
+    assert gpu_prover.compute_cell_kzg_proofs_batch(blobs, len(names)) == want
+    gpu_prover.debug_force_checked(True)
+    try:
+        got = gpu_prover.compute_cell_kzg_proofs_batch(blobs, len(names))
+    finally:
+        gpu_prover.debug_force_checked(False)
+    assert got == want
 
 
 def test_splits_and_odd_sizes_bit_exact(gpu_prover, vectors, restore_tuning):
@@ -92,7 +111,7 @@ def test_tuning_rejects_out_of_range(gpu_prover, restore_tuning):
     for bad in (dict(phase_a_groups=0), dict(phase_a_blocks_per_cu=3),
                 dict(normalize_chunk=257), dict(splits=5),
                 dict(recovery_waves=3), dict(recovery_waves=8), dict(recovery_radix=16), dict(ladder_splits=3),
-                dict(phase_a_groups=21), dict(reduce_lanes=2), dict(reduce_lanes=64)):
+                dict(phase_a_groups=21), dict(reduce_lanes=2), dict(reduce_lanes=12), dict(reduce_lanes=128)):
         with pytest.raises(vkzg_amd.VkzgError) as e:
             gpu_prover.set_tuning(3, **bad)
         assert e.value.code == vkzg_amd.ERR_BADARGS
@@ -106,7 +125,7 @@ def test_autotune_keeps_results(gpu_prover_small_batch, vectors):
     for n in (1, 4):
         t = p.get_tuning(n)
         assert 1 <= t["phase_a_groups"] <= 20 and 1 <= t["splits"] <= 4 and t["ladder_splits"] in (1, 2, 4)
-        assert t["reduce_lanes"] in (4, 8, 16, 32)
+        assert t["reduce_lanes"] in (4, 8, 16, 32, 64)
     names = [NAMES[i % 3] for i in range(6)]  # chunks of 4 + 2
     blobs = b"".join(vectors.blob(n) for n in names)
     assert p.compute_cell_kzg_proofs_batch(blobs, 6) == b"".join(vectors.proofs(n) for n in names)

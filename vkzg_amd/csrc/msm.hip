@@ -337,52 +337,6 @@ VKZG_DEV_COLD void combine_checked(G1X *acc, const G1X *ws, uint32_t W, uint32_t
     *acc = a;
 }
 
-// The four lanes of a quad hold the same point(s) and share one operation:
-// its products are split into dependent rounds of at most four, and DPP quad
-// broadcasts give every lane each round's products.  Same formulas and
-// operand order as g1x_dbl (3 rounds instead of 9 products) and g1x_add_fast
-// (4 rounds instead of 14); sums that only feed products stay lazy.
-VKZG_DEV G1X g1x_dbl_quad(const G1X &p, uint32_t q) {
-    if (fp_is_zero(p.zz)) return p;  // uniform in the quad
-    const Fp U = fp_add_lazy(p.y, p.y);  // U and M feed products only
-    const Fp r1 = fp_sqr((q & 1u) ? p.x : U);
-    const Fp V = fp_quad_bcast<0>(r1), X2 = fp_quad_bcast<1>(r1);
-    const Fp M = fp_triple_lazy(X2);
-    const Fp r2 = fp_mul(q == 0 ? U : (q == 1 ? p.x : (q == 2 ? M : V)), q == 2 ? M : (q == 3 ? p.zz : V));
-    const Fp W = fp_quad_bcast<0>(r2), S = fp_quad_bcast<1>(r2);
-    G1X r;
-    r.x = fp_sub(fp_quad_bcast<2>(r2), fp_dbl(S));  // M^2 - 2S
-    r.zz = fp_quad_bcast<3>(r2);                    // V ZZ
-    const Fp r3 = fp_mul(q == 0 ? M : W, q == 0 ? fp_sub_lazy(S, r.x) : (q == 1 ? p.y : p.zzz));
-    r.y = fp_sub(fp_quad_bcast<0>(r3), fp_quad_bcast<1>(r3));  // M (S - X3) - W Y
-    r.zzz = fp_quad_bcast<2>(r3);                               // W ZZZ
-    return r;
-}
-
-VKZG_DEV G1X g1x_add_quad(const G1X &a, const G1X &b, uint32_t q, bool &exc) {
-    if (fp_is_zero(a.zz)) return b;  // uniform in the quad
-    if (fp_is_zero(b.zz)) return a;
-    const Fp r1 = fp_mul(q == 0 ? a.x : (q == 1 ? b.x : (q == 2 ? a.y : b.y)),
-                         q == 0 ? b.zz : (q == 1 ? a.zz : (q == 2 ? b.zzz : a.zzz)));
-    const Fp U1 = fp_quad_bcast<0>(r1), S1 = fp_quad_bcast<2>(r1);
-    const Fp P = fp_sub_lazy(fp_quad_bcast<1>(r1), U1);  // U2 - U1
-    const Fp R = fp_sub_lazy(fp_quad_bcast<3>(r1), S1);  // S2 - S1
-    const Fp r2 = fp_mul(q == 0 ? P : (q == 1 ? R : (q == 2 ? a.zz : a.zzz)),
-                         q == 0 ? P : (q == 1 ? R : (q == 2 ? b.zz : b.zzz)));
-    const Fp PP = fp_quad_bcast<0>(r2), RR = fp_quad_bcast<1>(r2);
-    const Fp ZZab = fp_quad_bcast<2>(r2), ZZZab = fp_quad_bcast<3>(r2);
-    exc |= fp_is_zero(PP);
-    const Fp r3 = fp_mul(q == 0 ? P : (q == 1 ? U1 : ZZab), PP);
-    const Fp PPP = fp_quad_bcast<0>(r3), Q = fp_quad_bcast<1>(r3);
-    G1X r;
-    r.zz = fp_quad_bcast<2>(r3);
-    r.x = fp_sub(fp_sub(RR, PPP), fp_dbl(Q));
-    const Fp r4 = fp_mul(q == 0 ? R : (q == 1 ? S1 : ZZZab), q == 0 ? fp_sub_lazy(Q, r.x) : PPP);
-    r.y = fp_sub(fp_quad_bcast<0>(r4), fp_quad_bcast<1>(r4));
-    r.zzz = fp_quad_bcast<2>(r4);
-    return r;
-}
-
 // out[m] = sum_w 2^(c w) wsum[m W + w], affine ((0, 0) for the identity),
 // by the quad m = tid / 4.  The doubling chain is this launch's critical
 // path, so the loop uses the call-free addition; an equal or opposite pair

@@ -425,7 +425,7 @@ Tuning tuning_for(const vkzg_prover *p, uint32_t nb) {
 void default_tunings(vkzg_prover *p) {
     static const uint32_t groups[kTuneClasses] = {10, 10, 4, 1, 1};
     static const uint32_t ladder_splits[kTuneClasses] = {4, 2, 1, 1, 1};
-    static const uint32_t reduce_lanes[kTuneClasses] = {32, 32, 32, 16, 8};
+    static const uint32_t reduce_lanes[kTuneClasses] = {64, 64, 32, 16, 4};
     for (int c = 0; c < kTuneClasses; c++) {
         p->tune[c] = Tuning{};
         p->tune[c].groups = groups[c];
@@ -458,7 +458,8 @@ bool tuning_valid(const Tuning &t) {
            t.norm_chunk <= 256 && t.splits >= 1 && t.splits <= (uint32_t)kMaxSplits &&
            (t.rec_waves == 1 || t.rec_waves == 2 || t.rec_waves == 4) && (t.rec_radix == 2 || t.rec_radix == 4 || t.rec_radix == 8) &&
            (t.ladder_splits == 1 || t.ladder_splits == 2 || t.ladder_splits == 4) &&
-           (t.reduce_lanes == 4 || t.reduce_lanes == 8 || t.reduce_lanes == 16 || t.reduce_lanes == 32) &&
+           (t.reduce_lanes == 4 || t.reduce_lanes == 8 || t.reduce_lanes == 16 || t.reduce_lanes == 32 ||
+            t.reduce_lanes == 64) &&
            (t.window_bits == 0 || (t.window_bits >= kExtraWindowMin && t.window_bits <= kExtraWindowMax));
 }
 
@@ -789,11 +790,19 @@ void autotune(vkzg_prover *p) {
             p->tune[c] = p->tune[c - 1];  // class beyond the batch size: never used alone
             continue;
         }
+        // A class below the largest is also timed at half its top size (32
+        // blobs for the 17-64 class), so that a shape which suits only the
+        // top size (one window group at 64 blobs, half the SIMDs idle at 32)
+        // does not take the whole class; the largest class is timed at its
+        // top only.
+        const uint32_t half = nb / 2;
+        const bool two = c + 1 < kTuneClasses && c > 0 && half > kTuneClassTop[c - 1];
+        auto time_class = [&](const Tuning &tn) { return time_one(nb, tn) + (two ? time_one(half, tn) : 0.0f); };
         Tuning best = p->tune[c];
-        float best_ms = time_one(nb, best);
+        float best_ms = time_class(best);
         auto consider = [&](Tuning cand) {
             if (!tuning_valid(cand)) return;
-            const float ms = time_one(nb, cand);
+            const float ms = time_class(cand);
             if (ms < best_ms * 0.98f) {
                 best_ms = ms;
                 best = cand;
@@ -802,7 +811,7 @@ void autotune(vkzg_prover *p) {
         for (uint32_t g : {1u, 2u, 4u, 5u, 10u, 20u}) { Tuning t = best; t.groups = g; consider(t); }
         if (nb <= 64)
             for (uint32_t m : {2u, 4u}) { Tuning t = best; t.ladder_splits = m; consider(t); }
-        for (uint32_t r : {4u, 16u, 32u}) { Tuning t = best; t.reduce_lanes = r; consider(t); }
+        for (uint32_t r : {4u, 16u, 32u, 64u}) { Tuning t = best; t.reduce_lanes = r; consider(t); }
         for (uint32_t b : {1u, 2u}) { Tuning t = best; t.blocks_per_cu = b; consider(t); }
         for (uint32_t ch : {0u, 8u, 16u, 32u, 64u}) { Tuning t = best; t.norm_chunk = ch; consider(t); }
         if (nb >= 8) { Tuning t = best; t.splits = 2; consider(t); }

@@ -56,13 +56,31 @@ VKZG_DEV uint32_t uniform_here(uint32_t v) {
     return v;
 }
 
+// One point operation of a bucket range: on one lane (Q = 1), or shared by
+// the four lanes of a quad that all hold the operands (Q = 4: its products
+// in dependent rounds of up to four, g1x_add_quad / g1x_dbl_quad).
+template <uint32_t Q>
+VKZG_DEV G1X range_add(const G1X &a, const G1X &b, bool &exc) {
+    if constexpr (Q == 4) return g1x_add_quad(a, b, lane_here() & 3u, exc);
+    else return g1x_add_fast(a, b, exc);
+}
+template <uint32_t Q>
+VKZG_DEV G1X range_dbl(const G1X &a) {
+    if constexpr (Q == 4) return g1x_dbl_quad(a, lane_here() & 3u);
+    else return g1x_dbl_core(a);
+}
+
 // cvals[blob][a] = sum_{k=1..kBuckets} k * buckets[blob][a][k-1]
-// (field.glsl.inc:478-517).  A team of LANES lanes shares an output; lane l
-// owns buckets [per l, per l + per) with per = kBuckets / LANES:
+// (field.glsl.inc:478-517).  LANES lanes share an output: 4, 8, 16 or 32
+// bucket ranges of one lane each, or (LANES = 64) 16 ranges of a quad each.
+// Range l owns buckets [per l, per l + per), per = kBuckets / ranges:
 // rs_l = sum_i (i+1) B[per l + i] by running sums, s_l = sum_i B[per l + i];
-// the output is sum_l (rs_l + per l * s_l).  More lanes shorten each lane's
-// dependent chain and add total work (the scan over the team's lanes).  The
-// running sum rs_l stays in LDS between its updates (G1XStash).
+// the output is sum_l (rs_l + per l * s_l).  More ranges shorten each
+// range's dependent chain and add total work (the scan over the ranges);
+// quads shorten it again (an addition waits on four rounds of products
+// instead of fourteen products) at four times the lanes, for batches that
+// leave SIMDs idle.  The running sum rs_l stays in LDS between its updates
+// (G1XStash).
 //
 // Persistent: every wave takes items of 64 / LANES consecutive outputs from
 // the counter q[0] (cleared by k_glv_phi) until the queue is drained and
@@ -72,7 +90,9 @@ template <int LANES>
 __global__ void __launch_bounds__(256, 2)
     k_bucket_reduce(G1X *cvals, uint8_t *flags, const G1X *buckets, uint32_t nout, uint32_t *q,
                     uint32_t force_checked) {
-    constexpr uint32_t per = kBuckets / LANES;
+    constexpr uint32_t Q = LANES == 64 ? 4u : 1u;  // lanes per bucket range
+    constexpr uint32_t ranges = LANES / Q;
+    constexpr uint32_t per = kBuckets / ranges;
     constexpr uint32_t teams = 64 / LANES;
     __shared__ uint32_t stash_words[4 * kFpLimbs * 256];
     const uint32_t wave_base = __builtin_amdgcn_readfirstlane(threadIdx.x) & ~63u;
@@ -86,7 +106,7 @@ __global__ void __launch_bounds__(256, 2)
         if (lane == 0) item = atomicAdd(q, 1u);
         item = __shfl(item, 0);
         if (item >= nitems) break;
-        const uint32_t l = lane % LANES;
+        const uint32_t l = (lane / Q) % ranges;
         const uint32_t o = item * teams + lane / LANES;
         const G1XStash stash{stash_words + wave_base + lane};
         G1X s = g1x_identity();
@@ -96,42 +116,43 @@ __global__ void __launch_bounds__(256, 2)
             const G1X *B = buckets + (size_t)o * kBuckets + l * per;
             // The item lists leave some buckets empty (the identity), the
             // same ones for every blob.  An empty bucket right after the
-            // lane's first non-empty one leaves s unchanged while the running
-            // sum still equals s: that update is a doubling, taken here
-            // rather than flagged.  `single` records running sum == s.
+            // range's first non-empty one leaves s unchanged while the
+            // running sum still equals s: that update is a doubling, taken
+            // here rather than flagged.  `single` records running sum == s.
             bool single = false;
 #pragma unroll 1
             for (int i = per - 1; i >= 0; i--) {
                 const G1X b = B[i];
                 const bool empty = fp_is_zero(b.zz);
-                s = g1x_add_fast(s, b, exc);
+                s = range_add<Q>(s, b, exc);
                 G1X rs = stash.get();
                 if (empty && single) {
-                    rs = g1x_dbl_core(rs);
+                    rs = range_dbl<Q>(rs);
                     single = false;
                 } else {
                     single = fp_is_zero(rs.zz) && !fp_is_zero(s.zz);
-                    rs = g1x_add_fast(rs, s, exc);
+                    rs = range_add<Q>(rs, s, exc);
                 }
                 stash.put(rs);
             }
         }
         // sum_l per l s_l = per * sum_{m >= 1} S_m, with the suffix sums
         // S_m = sum_{l >= m} s_l of the team (a log-step scan over shuffles),
-        // then log2(per) doublings: no lane-dependent double-and-add chain.
-        const uint32_t l2 = lane_here() % LANES;
+        // then log2(per) doublings: no range-dependent double-and-add chain.
+        // Every condition below is uniform in a quad.
+        const uint32_t l2 = (lane_here() / Q) % ranges;
         G1X x = s;
 #pragma unroll 1
-        for (uint32_t d = 1; d < LANES; d <<= 1) {
-            const G1X y = g1x_shfl_down(x, d);
-            if (l2 + d < LANES) x = g1x_add_fast(x, y, exc);
+        for (uint32_t d = 1; d < ranges; d <<= 1) {
+            const G1X y = g1x_shfl_down(x, d * Q);
+            if (l2 + d < ranges) x = range_add<Q>(x, y, exc);
         }
         if (l2 == 0) x = g1x_identity();
 #pragma unroll 1
-        for (uint32_t k = 1; k < per; k <<= 1) x = g1x_dbl_core(x);
-        G1X v = g1x_add_fast(stash.get(), x, exc);
+        for (uint32_t k = 1; k < per; k <<= 1) x = range_dbl<Q>(x);
+        G1X v = range_add<Q>(stash.get(), x, exc);
 #pragma unroll 1
-        for (int m = 1; m < LANES; m <<= 1) v = g1x_add_fast(v, g1x_shfl_xor(v, m), exc);
+        for (uint32_t m = 1; m < ranges; m <<= 1) v = range_add<Q>(v, g1x_shfl_xor(v, m * Q), exc);
         const uint64_t flagged = __ballot(exc);
         const uint32_t lane2 = lane_here();
         const uint32_t o2 = item * teams + lane2 / LANES;
@@ -204,7 +225,7 @@ hipError_t launch_bucket_reduce(G1X *cvals, uint8_t *flags, const G1X *buckets, 
                                 uint32_t lanes, uint32_t grid, uint32_t *queue, uint32_t force_checked,
                                 hipStream_t s) {
     const uint32_t nout = nblobs * 128;
-    const uint32_t teams = 64 / (lanes == 4 || lanes == 16 || lanes == 32 ? lanes : 8);
+    const uint32_t teams = 64 / (lanes == 4 || lanes == 16 || lanes == 32 || lanes == 64 ? lanes : 8);
     const uint32_t waves = (nout + teams - 1) / teams;
     if (grid > (waves + 3) / 4) grid = (waves + 3) / 4;
     switch (lanes) {
@@ -218,6 +239,10 @@ hipError_t launch_bucket_reduce(G1X *cvals, uint8_t *flags, const G1X *buckets, 
             break;
         case 32:
             hipLaunchKernelGGL(k_bucket_reduce<32>, dim3(grid), dim3(256), 0, s, cvals, flags, buckets, nout,
+                               queue, force_checked);
+            break;
+        case 64:
+            hipLaunchKernelGGL(k_bucket_reduce<64>, dim3(grid), dim3(256), 0, s, cvals, flags, buckets, nout,
                                queue, force_checked);
             break;
         default:

@@ -32,7 +32,8 @@ struct Tuning {
     uint32_t rec_waves = 1;      // recovery work items (waves) per block: 1, 2 or 4
     uint32_t rec_radix = 2;      // recovery: 2 (three kernels, 4 values per lane), 4 or 8 (five kernels)
     uint32_t ladder_splits = 1;  // ladder segments (phase A also computes their starts): 1, 2 or 4
-    uint32_t reduce_lanes = 8;   // bucket-reduction lanes per output: 4, 8, 16 or 32
+    uint32_t reduce_lanes = 8;   // bucket-reduction lanes per output: 4, 8, 16 or 32 (one per bucket
+                                 // range), or 64 (16 ranges of a quad each)
     uint32_t window_bits = 0;    // phase-A table: 0 = the prover's main table, else a resident
                                  // smaller one of this window (10..13, below the main one)
     // Not a tuning: the test switch that sends phase A, phase B and the bucket
