@@ -437,6 +437,31 @@ VKZG_DEV G1X g1x_add_quad(const G1X &a, const G1X &b, uint32_t q, bool &exc) {
     return r;
 }
 
+// g1x_madd_acc shared by the four lanes of a quad that all hold a, b and
+// neg: its products in four dependent rounds -- {U2, S2}, {P^2, R^2},
+// {P PP, X1 PP, ZZ1 PP}, {R (Q - X3), Y1 PPP, ZZZ1 PPP} -- instead of ten
+// products in a row; same operands and lazy sums, so the same field elements.
+VKZG_DEV G1X g1x_madd_acc_quad(const G1X &a, const G1A &b, bool neg, uint32_t q, bool &exc) {
+    exc |= limbs_zero<kFpLimbs>(b.y.v);
+    const bool odd = (q & 1u) != 0;
+    const Fp by = fp_neg_lazy_if(b.y, neg);
+    const Fp r1 = fp_mul(fp_select(odd, by, b.x), fp_select(odd, a.zzz, a.zz));
+    const Fp P = fp_sub_lazy(fp_quad_bcast<0>(r1), a.x);  // U2 - X1
+    const Fp R = fp_sub_lazy(fp_quad_bcast<1>(r1), a.y);  // S2 - Y1
+    const Fp r2 = fp_sqr(fp_select(odd, R, P));
+    const Fp PP = fp_quad_bcast<0>(r2), RR = fp_quad_bcast<1>(r2);
+    exc |= fp_is_zero(PP);
+    const Fp r3 = fp_mul(q == 0 ? P : (q == 1 ? a.x : a.zz), PP);
+    const Fp PPP = fp_quad_bcast<0>(r3), Q = fp_quad_bcast<1>(r3);
+    G1X r;
+    r.zz = fp_quad_bcast<2>(r3);
+    r.x = fp_sub(fp_sub(RR, PPP), fp_dbl(Q));
+    const Fp r4 = fp_mul(q == 0 ? R : (q == 1 ? a.y : a.zzz), q == 0 ? fp_sub_lazy(Q, r.x) : PPP);
+    r.y = fp_sub(fp_quad_bcast<0>(r4), fp_quad_bcast<1>(r4));  // R (Q - X3) - Y1 PPP
+    r.zzz = fp_quad_bcast<2>(r4);
+    return r;
+}
+
 VKZG_DEV G1X g1x_neg(const G1X &p) {
     G1X r = p;
     r.y = fp_neg(p.y);

@@ -121,10 +121,10 @@ hipError_t launch_ladder(G1J *lad, const G1X *u, uint32_t splits, uint32_t nblob
 hipError_t launch_glv_phi(G1A *phi, const G1A *pts, uint64_t count, uint32_t *queue_b, uint32_t *queue_r,
                           hipStream_t s);
 // Persistent (`grid` blocks drain queue[0]), then the recompute of flagged
-// buckets; flags[B][128][128] is scratch.
+// buckets; flags[B][128][128] is scratch.  quad: four lanes share a bucket.
 hipError_t launch_phase_b(G1X *buckets, uint8_t *flags, const G1A *lad_aff, const G1A *lad_phi,
                           const uint32_t *items, const uint32_t *offsets, uint32_t nblobs, uint32_t grid,
-                          uint32_t *queue, uint32_t force_checked, hipStream_t s);
+                          bool quad, uint32_t *queue, uint32_t force_checked, hipStream_t s);
 // `lanes` (4, 8, 16 or 32) lanes per output; persistent (`grid` blocks drain
 // queue[0]), then the recompute of flagged outputs; flags[B][128] is scratch.
 hipError_t launch_bucket_reduce(G1X *cvals, uint8_t *flags, const G1X *buckets, uint32_t nblobs,

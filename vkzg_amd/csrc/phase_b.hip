@@ -58,26 +58,34 @@ __device__ __attribute__((noinline)) void phase_b_slow(G1X *out, const G1A *L, c
 // drained, and leaves after one failed take; waves share no barrier.  A
 // flagged bucket is recomputed by k_phase_b_fix, which keeps this kernel
 // free of calls.
+//
+// Q = 4 (batches too small to fill the SIMDs, where the longest bucket's
+// chain sets the time): a quad of lanes shares each bucket
+// (g1x_madd_acc_quad), and a work item is a quarter of a group, 16 of its 64
+// buckets.
 constexpr uint32_t kBand = 32;       // blobs per band (even)
 constexpr uint32_t kClasses = 3 * kBuckets;
 constexpr uint32_t top_pow2(uint32_t n) { return n < 2 ? 1u : 2u * top_pow2(n / 2); }
 constexpr uint32_t kSearchTop = top_pow2(kClasses);  // largest power of two <= kClasses
 
+template <uint32_t Q>
 __global__ void __launch_bounds__(256, 2)
     k_phase_b(G1X *buckets, uint8_t *flags, const G1A *lad_aff, const G1A *lad_phi, const uint32_t *items,
               const uint32_t *offsets, uint32_t nblobs, uint32_t *q, uint32_t force_checked) {
-    const uint32_t lane = threadIdx.x % 64;
+    const uint32_t plane = threadIdx.x % 64;
     const uint32_t *cls = offsets + 3 * (kBuckets + 1);
     const uint32_t *n2 = cls + kClasses;
     const uint32_t band_items = kBuckets * 2 * kBand;  // a full band: kBand / 2 pairs x 2 + kBand
     const uint32_t nfull = nblobs / kBand, rest = nblobs % kBand;
-    const uint32_t nitems = nfull * band_items + kBuckets * (2 * ((rest + 1) / 2) + rest);
+    const uint32_t nitems = (nfull * band_items + kBuckets * (2 * ((rest + 1) / 2) + rest)) * Q;
 #pragma unroll 1
     for (;;) {
-        uint32_t item = 0;
-        if (lane == 0) item = atomicAdd(q, 1u);
-        item = __shfl(item, 0);
-        if (item >= nitems) break;
+        uint32_t qitem = 0;
+        if (plane == 0) qitem = atomicAdd(q, 1u);
+        qitem = __shfl(qitem, 0);
+        if (qitem >= nitems) break;
+        // the group's lane (bucket) this lane works on
+        const uint32_t item = qitem / Q, lane = (qitem % Q) * (64 / Q) + plane / Q;
         const uint32_t band = item / band_items, j = item % band_items;
         const uint32_t b0 = band * kBand;
         const uint32_t bb = band < nfull ? kBand : rest, pb = (bb + 1) / 2;
@@ -116,13 +124,14 @@ __global__ void __launch_bounds__(256, 2)
             const uint32_t src = o.slot(e);
             const G1A pt = d < kLadPos ? L[d * 128 + src] : Phi[(d - kLadPos) * 128 + src];
             if (started) {
-                acc = g1x_madd_acc(acc, pt, neg, exc);
+                if constexpr (Q == 4) acc = g1x_madd_acc_quad(acc, pt, neg, plane & 3u, exc);
+                else acc = g1x_madd_acc(acc, pt, neg, exc);
             } else {
                 acc = g1x_start(pt, neg, exc);
                 started = true;
             }
         }
-        if (live) {
+        if (live && plane % Q == 0) {
             const size_t ba = (size_t)blob * 128 + a;
             buckets[ba * kBuckets + k] = acc;
             flags[ba * kBuckets + k] = exc ? 1 : 0;
@@ -149,13 +158,18 @@ __global__ void __launch_bounds__(256) k_phase_b_fix(G1X *buckets, const uint8_t
 
 hipError_t launch_phase_b(G1X *buckets, uint8_t *flags, const G1A *lad_aff, const G1A *lad_phi,
                           const uint32_t *items, const uint32_t *offsets, uint32_t nblobs, uint32_t grid,
-                          uint32_t *queue, uint32_t force_checked, hipStream_t s) {
+                          bool quad, uint32_t *queue, uint32_t force_checked, hipStream_t s) {
     // 64-lane groups: two per pair of blobs of a band (segments 0 and 1), one per blob
     const uint32_t rest = nblobs % kBand;
-    const uint32_t nitems = (nblobs / kBand) * kBuckets * 2 * kBand + kBuckets * (2 * ((rest + 1) / 2) + rest);
+    const uint32_t ngroups = (nblobs / kBand) * kBuckets * 2 * kBand + kBuckets * (2 * ((rest + 1) / 2) + rest);
+    const uint32_t nitems = ngroups * (quad ? 4u : 1u);  // one wave each
     if (grid > (nitems + 3) / 4) grid = (nitems + 3) / 4;
-    hipLaunchKernelGGL(k_phase_b, dim3(grid), dim3(256), 0, s, buckets, flags, lad_aff, lad_phi, items,
-                       offsets, nblobs, queue, force_checked);
+    if (quad)
+        hipLaunchKernelGGL(k_phase_b<4>, dim3(grid), dim3(256), 0, s, buckets, flags, lad_aff, lad_phi, items,
+                           offsets, nblobs, queue, force_checked);
+    else
+        hipLaunchKernelGGL(k_phase_b<1>, dim3(grid), dim3(256), 0, s, buckets, flags, lad_aff, lad_phi, items,
+                           offsets, nblobs, queue, force_checked);
     const uint32_t nbuckets = nblobs * 128 * kBuckets;
     hipLaunchKernelGGL(k_phase_b_fix, dim3(blocks_for(nbuckets, 256)), dim3(256), 0, s, buckets, flags,
                        lad_aff, lad_phi, items, offsets, nbuckets);

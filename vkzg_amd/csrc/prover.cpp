@@ -447,6 +447,15 @@ uint32_t ladder_lanes(uint64_t chains, uint32_t cus) {
     return chains * 4 <= (uint64_t)cus * 4 * 64 ? 4u : 2u;
 }
 
+// Phase B on quads (four lanes per bucket) while the quad lanes come to at
+// most kPhaseBQuadWaves waves per SIMD (8 blobs on 256 CUs): there the
+// longest bucket's chain, more than the issue rate, sets the time.  Measured
+// limits of 1, 2, 4 and 8 waves: each size up to 8 blobs was fastest on quads.
+constexpr uint64_t kPhaseBQuadWaves = 8;
+bool phase_b_quad(uint32_t nb, uint32_t cus) {
+    return (uint64_t)nb * 128 * kBuckets * 4 <= (uint64_t)cus * 4 * 64 * kPhaseBQuadWaves;
+}
+
 uint32_t fit_norm_chunk(uint64_t count, uint32_t cus) {
     const uint64_t lanes = (uint64_t)cus * 4 * 64;
     const uint64_t c = (count + lanes - 1) / lanes;
@@ -536,7 +545,8 @@ void proofs_range(vkzg_prover *p, WorkSet &ws, uint32_t off, uint32_t nb, uint8_
     ck(launch_glv_phi(lad_phi, lad_aff, (uint64_t)nb * npos, ws.queue_b + off, ws.queue_r + off, s));
     clk.mark("normalize");
     ck(launch_phase_b(buckets, ws.bucket_flags + (size_t)off * 128 * kBuckets, lad_aff, lad_phi, p->kb_items,
-                      p->kb_offsets, nb, p->cus * 2, ws.queue_b + off, tn.force_checked, s));
+                      p->kb_offsets, nb, p->cus * 2, phase_b_quad(nb, p->cus), ws.queue_b + off,
+                      tn.force_checked, s));
     clk.mark("phase_b");
     ck(launch_bucket_reduce(cvals, ws.cval_flags + off * 128, buckets, nb, tn.reduce_lanes, p->cus * 2,
                             ws.queue_r + off, tn.force_checked, s));
