@@ -112,7 +112,9 @@ hipError_t launch_phase_a(G1X *part, const Fr *scal, const G1AT *table, WindowPl
                           uint32_t force_checked, hipStream_t s);
 // usum[msm] = sum over g < groups of part[msm * groups + g] (groups <= 24,
 // nmsm a multiple of 32).
-hipError_t launch_sum_groups(G1X *usum, const G1X *part, uint32_t groups, uint32_t nmsm, hipStream_t s);
+// quad: four lanes share each addition (force_checked: the checked recompute).
+hipError_t launch_sum_groups(G1X *usum, const G1X *part, uint32_t groups, uint32_t nmsm, bool quad,
+                             uint32_t force_checked, hipStream_t s);
 // u: one sum per MSM, [nblobs][splits][128].
 hipError_t launch_ladder(G1J *lad, const G1X *u, uint32_t splits, uint32_t nblobs, uint32_t lanes,
                          hipStream_t s);

@@ -103,21 +103,48 @@ __global__ __launch_bounds__(256) void k_ladder(G1J *lad, const G1X *u, uint32_t
 }
 
 // The window-group partials of phase A summed per MSM:
-// usum[msm] = sum_g part[msm * G + g].  Eight lanes per MSM: lane t adds
-// partials t, t + 8 and t + 16, then a three-level lane tree.  With G > 1 the
-// ladder reads these sums, so its prologue (a dependent chain per pair of
-// lanes) adds no partials of its own.  nmsm is a multiple of 128, so the grid
-// holds whole MSMs and no lane leaves before the shuffles.
-__global__ __launch_bounds__(256) void k_sum_groups(G1X *usum, const G1X *part, uint32_t groups, uint32_t nmsm) {
+// usum[msm] = sum_g part[msm * G + g].  Eight partial sums per MSM: sum t
+// adds partials t, t + 8 and t + 16, then a three-level tree.  Q = 1: a lane
+// per sum, with the checked additions; Q = 4 (small batches, where this
+// chain of additions is the kernel's time): a quad per sum sharing each
+// addition (g1x_add_quad, four rounds of products instead of fourteen), any
+// flagged doubling or cancellation redone checked by the quad's lane 0.
+// With G > 1 the ladder reads these sums, so its prologue (a dependent chain
+// per group of lanes) adds no partials of its own.  nmsm is a multiple of
+// 128, so the grid holds whole MSMs and no lane leaves before the shuffles.
+VKZG_DEV_COLD void sum_groups_checked(G1X *out, const G1X *part, uint32_t groups) {
+    G1X acc = g1x_identity();
+    for (uint32_t g = 0; g < groups; g++) acc = g1x_add(acc, part[g]);
+    *out = acc;
+}
+
+template <uint32_t Q>
+__global__ __launch_bounds__(256) void k_sum_groups(G1X *usum, const G1X *part, uint32_t groups, uint32_t nmsm,
+                                                    uint32_t force_checked) {
     const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t msm = tid / 8, t = tid % 8;
+    const uint32_t msm = tid / (8 * Q), t = (tid / Q) % 8, qd = tid % Q;
     if (msm >= nmsm) return;
     G1X acc = g1x_identity();
+    if constexpr (Q == 4) {
+        bool exc = force_checked != 0;
 #pragma unroll 1
-    for (uint32_t g = t; g < groups; g += 8) acc = g1x_add(acc, part[(size_t)msm * groups + g]);
+        for (uint32_t g = t; g < groups; g += 8) acc = g1x_add_quad(acc, part[(size_t)msm * groups + g], qd, exc);
 #pragma unroll 1
-    for (int m = 1; m < 8; m <<= 1) acc = g1x_add(acc, g1x_shfl_xor(acc, m));
-    if (t == 0) usum[msm] = acc;
+        for (uint32_t m = 1; m < 8; m <<= 1) acc = g1x_add_quad(acc, g1x_shfl_xor(acc, m * Q), qd, exc);
+        // a flag anywhere in the MSM's 32 lanes: lane 0 redoes the sum
+        const uint64_t flagged = __ballot(exc);
+        const uint32_t sub = (threadIdx.x & 63u) / 32u;
+        if (((flagged >> (32u * sub)) & 0xffffffffull) != 0) {
+            if (t == 0 && qd == 0) sum_groups_checked(&usum[msm], part + (size_t)msm * groups, groups);
+            return;
+        }
+    } else {
+#pragma unroll 1
+        for (uint32_t g = t; g < groups; g += 8) acc = g1x_add(acc, part[(size_t)msm * groups + g]);
+#pragma unroll 1
+        for (uint32_t m = 1; m < 8; m <<= 1) acc = g1x_add(acc, g1x_shfl_xor(acc, m));
+    }
+    if (t == 0 && qd == 0) usum[msm] = acc;
 }
 
 // phi[k] = (beta x, y) of pts[k]: the endomorphism [lambda], lambda = z^2 - 1
@@ -153,10 +180,15 @@ hipError_t launch_ladder(G1J *lad, const G1X *u, uint32_t splits, uint32_t nblob
     return hipGetLastError();
 }
 
-hipError_t launch_sum_groups(G1X *usum, const G1X *part, uint32_t groups, uint32_t nmsm, hipStream_t s) {
+hipError_t launch_sum_groups(G1X *usum, const G1X *part, uint32_t groups, uint32_t nmsm, bool quad,
+                             uint32_t force_checked, hipStream_t s) {
     if (nmsm % 32 != 0 || groups > 24) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_sum_groups, dim3(blocks_for(nmsm * 8ull, 256)), dim3(256), 0, s, usum, part, groups,
-                       nmsm);
+    if (quad)
+        hipLaunchKernelGGL(k_sum_groups<4>, dim3(blocks_for(nmsm * 32ull, 256)), dim3(256), 0, s, usum, part,
+                           groups, nmsm, force_checked);
+    else
+        hipLaunchKernelGGL(k_sum_groups<1>, dim3(blocks_for(nmsm * 8ull, 256)), dim3(256), 0, s, usum, part,
+                           groups, nmsm, force_checked);
     return hipGetLastError();
 }
 

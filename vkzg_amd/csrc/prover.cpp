@@ -534,7 +534,9 @@ void proofs_range(vkzg_prover *p, WorkSet &ws, uint32_t off, uint32_t nb, uint8_
                       p->cus * tn.blocks_per_cu, queue, tn.force_checked, s));
     clk.mark("phase_a");
     if (tn.groups > 1) {
-        ck(launch_sum_groups(usum, u, tn.groups, nb * 128 * tn.ladder_splits, s));
+        const uint64_t nsum = (uint64_t)nb * 128 * tn.ladder_splits;  // quads while they fit a wave per SIMD
+        ck(launch_sum_groups(usum, u, tn.groups, (uint32_t)nsum, nsum * 32 <= (uint64_t)p->cus * 4 * 64,
+                             tn.force_checked, s));
         clk.mark("sum_groups");
     }
     ck(launch_ladder(lad, tn.groups > 1 ? usum : u, tn.ladder_splits, nb,
