@@ -19,7 +19,9 @@
 //                             Fc (j < 32), Fn (32 <= j < 64), F- (j >= 64);
 //                             normalised into G1A lad_aff with the same
 //                             layout; G1A lad_phi: phi(lad_aff), where
-//                             phi(x, y) = (beta x, y) = lambda * P (GLV)
+//                             phi(x, y) = (beta x, y) = lambda * P (GLV);
+//                             small batches: G1X (XYZZ) points and their
+//                             images in the same two buffers' memory
 //   G1X buckets[B][128][128]  phase-B buckets, [output a][bucket k]
 //   G1X cvals[B][128]         Yc (a < 32), Yn (32 <= a < 64), C- (a >= 64)
 #pragma once
@@ -120,13 +122,18 @@ hipError_t launch_ladder(G1J *lad, const G1X *u, uint32_t splits, uint32_t nblob
                          hipStream_t s);
 // Also clears the work counters of the phase-B and bucket-reduction launches
 // that follow (queue_b[0], queue_r[0]).
+// The ladder points in XYZZ form and their endomorphism images (small
+// batches, whose phase B adds XYZZ points on quads); clears the same counters.
+hipError_t launch_ladder_xyzz(G1X *pts, G1X *phi, const G1J *lad, uint64_t count, uint32_t *queue_b,
+                              uint32_t *queue_r, hipStream_t s);
 hipError_t launch_glv_phi(G1A *phi, const G1A *pts, uint64_t count, uint32_t *queue_b, uint32_t *queue_r,
                           hipStream_t s);
 // Persistent (`grid` blocks drain queue[0]), then the recompute of flagged
-// buckets; flags[B][128][128] is scratch.  quad: four lanes share a bucket.
-hipError_t launch_phase_b(G1X *buckets, uint8_t *flags, const G1A *lad_aff, const G1A *lad_phi,
+// buckets; flags[B][128][128] is scratch.  quad: four lanes share a bucket;
+// xyzz (with quad): lad_pts / lad_phi are G1X (launch_ladder_xyzz), else G1A.
+hipError_t launch_phase_b(G1X *buckets, uint8_t *flags, const void *lad_pts, const void *lad_phi,
                           const uint32_t *items, const uint32_t *offsets, uint32_t nblobs, uint32_t grid,
-                          bool quad, uint32_t *queue, uint32_t force_checked, hipStream_t s);
+                          bool quad, bool xyzz, uint32_t *queue, uint32_t force_checked, hipStream_t s);
 // `lanes` (4, 8, 16 or 32) lanes per output; persistent (`grid` blocks drain
 // queue[0]), then the recompute of flagged outputs; flags[B][128] is scratch.
 hipError_t launch_bucket_reduce(G1X *cvals, uint8_t *flags, const G1X *buckets, uint32_t nblobs,

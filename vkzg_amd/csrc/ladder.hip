@@ -161,7 +161,38 @@ __global__ void k_glv_phi(G1A *phi, const G1A *pts, uint64_t count, uint32_t *qu
     phi[k] = q;
 }
 
+// Small batches: the ladder points in XYZZ form for phase B's quad
+// additions instead of normalised ones -- ZZ = Z^2, ZZZ = Z^3 keep X and Y
+// (Z = 0 gives the XYZZ identity) -- and their images under the
+// endomorphism, (beta X, Y, ZZ, ZZZ).  Two products per point, no inversion.
+// Clears the phase-B and bucket-reduction work counters, as k_glv_phi does.
+__global__ void k_ladder_xyzz(G1X *pts, G1X *phi, const G1J *lad, uint64_t count, uint32_t *queue_b,
+                              uint32_t *queue_r) {
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k == 0) {
+        queue_b[0] = 0u;
+        queue_r[0] = 0u;
+    }
+    if (k >= count) return;
+    const G1J p = lad[k];
+    G1X r;
+    r.x = p.x;
+    r.y = p.y;
+    r.zz = fp_sqr(p.z);
+    r.zzz = fp_mul(r.zz, p.z);
+    pts[k] = r;
+    r.x = fp_mul(p.x, fp_const(cfp::BETA_M));
+    phi[k] = r;
+}
+
 }  // namespace
+
+hipError_t launch_ladder_xyzz(G1X *pts, G1X *phi, const G1J *lad, uint64_t count, uint32_t *queue_b,
+                              uint32_t *queue_r, hipStream_t s) {
+    hipLaunchKernelGGL(k_ladder_xyzz, dim3(blocks_for(count, 256)), dim3(256), 0, s, pts, phi, lad, count,
+                       queue_b, queue_r);
+    return hipGetLastError();
+}
 
 hipError_t launch_ladder(G1J *lad, const G1X *u, uint32_t splits, uint32_t nblobs, uint32_t lanes,
                          hipStream_t s) {

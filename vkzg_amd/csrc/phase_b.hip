@@ -39,6 +39,24 @@ __device__ __attribute__((noinline)) void phase_b_slow(G1X *out, const G1A *L, c
     *out = acc;
 }
 
+// The same for ladder points in XYZZ form (the small-batch path).
+__device__ __attribute__((noinline)) void phase_b_slow_xyzz(G1X *out, const G1X *L, const G1X *Phi,
+                                                            const uint32_t *items, uint32_t lo,
+                                                            uint32_t hi, uint32_t a) {
+    const PhaseBOut o(a);
+    G1X acc = g1x_identity();
+    for (uint32_t it = lo; it < hi; it++) {
+        const uint32_t packed = items[it];
+        const uint32_t e = packed & 0xffu, d = (packed >> 8) & 0xffu;
+        bool neg = ((packed >> 16) & 1u) != 0;
+        if (o.wraps(e)) neg = !neg;
+        const uint32_t src = o.slot(e);
+        const G1X pt = d < kLadPos ? L[d * 128 + src] : Phi[(d - kLadPos) * 128 + src];
+        acc = g1x_add(acc, neg ? g1x_neg(pt) : pt);
+    }
+    *out = acc;
+}
+
 // Lane (blob, a, k): bucket k of output a.  The item list depends only on
 // (segment, k), so work groups are formed per segment: a 64-lane group of
 // segment 0 or 1 holds the 32 outputs of two blobs (blobs 2p and 2p + 1 of
@@ -62,15 +80,17 @@ __device__ __attribute__((noinline)) void phase_b_slow(G1X *out, const G1A *L, c
 // Q = 4 (batches too small to fill the SIMDs, where the longest bucket's
 // chain sets the time): a quad of lanes shares each bucket
 // (g1x_madd_acc_quad), and a work item is a quarter of a group, 16 of its 64
-// buckets.
+// buckets.  XZ (with Q = 4): the ladder points come in XYZZ form
+// (k_ladder_xyzz) and the quad adds them whole (g1x_add_quad, also four
+// rounds), so the batch needs no normalisation.
 constexpr uint32_t kBand = 32;       // blobs per band (even)
 constexpr uint32_t kClasses = 3 * kBuckets;
 constexpr uint32_t top_pow2(uint32_t n) { return n < 2 ? 1u : 2u * top_pow2(n / 2); }
 constexpr uint32_t kSearchTop = top_pow2(kClasses);  // largest power of two <= kClasses
 
-template <uint32_t Q>
+template <uint32_t Q, bool XZ>
 __global__ void __launch_bounds__(256, 2)
-    k_phase_b(G1X *buckets, uint8_t *flags, const G1A *lad_aff, const G1A *lad_phi, const uint32_t *items,
+    k_phase_b(G1X *buckets, uint8_t *flags, const void *lad_pts, const void *lad_phi, const uint32_t *items,
               const uint32_t *offsets, uint32_t nblobs, uint32_t *q, uint32_t force_checked) {
     const uint32_t plane = threadIdx.x % 64;
     const uint32_t *cls = offsets + 3 * (kBuckets + 1);
@@ -110,25 +130,47 @@ __global__ void __launch_bounds__(256, 2)
         if (!live) blob = nblobs - 1;
         const PhaseBOut o(a);
         const uint32_t lo = offsets[o.seg * (kBuckets + 1) + k], hi = offsets[o.seg * (kBuckets + 1) + k + 1];
-        const G1A *L = lad_aff + (size_t)blob * kLadPos * 128;
-        const G1A *Phi = lad_phi + (size_t)blob * kLadPos * 128;
         G1X acc = g1x_identity();
         bool exc = force_checked != 0, started = false;
+        if constexpr (XZ) {
+            const G1X *L = static_cast<const G1X *>(lad_pts) + (size_t)blob * kLadPos * 128;
+            const G1X *Phi = static_cast<const G1X *>(lad_phi) + (size_t)blob * kLadPos * 128;
 #pragma unroll 1
-        for (uint32_t it = lo; it < hi; it++) {
-            const uint32_t packed = items[it];
-            const uint32_t e = packed & 0xffu;
-            const uint32_t d = (packed >> 8) & 0xffu;
-            bool neg = ((packed >> 16) & 1u) != 0;
-            if (o.wraps(e)) neg = !neg;
-            const uint32_t src = o.slot(e);
-            const G1A pt = d < kLadPos ? L[d * 128 + src] : Phi[(d - kLadPos) * 128 + src];
-            if (started) {
-                if constexpr (Q == 4) acc = g1x_madd_acc_quad(acc, pt, neg, plane & 3u, exc);
-                else acc = g1x_madd_acc(acc, pt, neg, exc);
-            } else {
-                acc = g1x_start(pt, neg, exc);
-                started = true;
+            for (uint32_t it = lo; it < hi; it++) {
+                const uint32_t packed = items[it];
+                const uint32_t e = packed & 0xffu;
+                const uint32_t d = (packed >> 8) & 0xffu;
+                bool neg = ((packed >> 16) & 1u) != 0;
+                if (o.wraps(e)) neg = !neg;
+                const uint32_t src = o.slot(e);
+                G1X pt = d < kLadPos ? L[d * 128 + src] : Phi[(d - kLadPos) * 128 + src];
+                if (neg) pt = g1x_neg(pt);
+                if (started) {
+                    acc = g1x_add_quad(acc, pt, plane & 3u, exc);
+                } else {
+                    acc = pt;
+                    started = true;
+                }
+            }
+        } else {
+            const G1A *L = static_cast<const G1A *>(lad_pts) + (size_t)blob * kLadPos * 128;
+            const G1A *Phi = static_cast<const G1A *>(lad_phi) + (size_t)blob * kLadPos * 128;
+#pragma unroll 1
+            for (uint32_t it = lo; it < hi; it++) {
+                const uint32_t packed = items[it];
+                const uint32_t e = packed & 0xffu;
+                const uint32_t d = (packed >> 8) & 0xffu;
+                bool neg = ((packed >> 16) & 1u) != 0;
+                if (o.wraps(e)) neg = !neg;
+                const uint32_t src = o.slot(e);
+                const G1A pt = d < kLadPos ? L[d * 128 + src] : Phi[(d - kLadPos) * 128 + src];
+                if (started) {
+                    if constexpr (Q == 4) acc = g1x_madd_acc_quad(acc, pt, neg, plane & 3u, exc);
+                    else acc = g1x_madd_acc(acc, pt, neg, exc);
+                } else {
+                    acc = g1x_start(pt, neg, exc);
+                    started = true;
+                }
             }
         }
         if (live && plane % Q == 0) {
@@ -141,38 +183,52 @@ __global__ void __launch_bounds__(256, 2)
 
 // Recompute, with the checked point operations, the buckets k_phase_b flagged
 // (a doubling or cancellation inside the bucket; random data never has one).
-__global__ void __launch_bounds__(256) k_phase_b_fix(G1X *buckets, const uint8_t *flags, const G1A *lad_aff, const G1A *lad_phi,
-                              const uint32_t *items, const uint32_t *offsets, uint32_t nbuckets) {
+template <bool XZ>
+__global__ void __launch_bounds__(256) k_phase_b_fix(G1X *buckets, const uint8_t *flags, const void *lad_pts,
+                                                     const void *lad_phi, const uint32_t *items,
+                                                     const uint32_t *offsets, uint32_t nbuckets) {
     const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
     if (idx >= nbuckets || !flags[idx]) return;
     const uint32_t ba = idx / kBuckets, k = idx % kBuckets;
     const uint32_t blob = ba / 128, a = ba % 128;
     const PhaseBOut o(a);
     const uint32_t lo = offsets[o.seg * (kBuckets + 1) + k], hi = offsets[o.seg * (kBuckets + 1) + k + 1];
-    const G1A *L = lad_aff + (size_t)blob * kLadPos * 128;
-    const G1A *Phi = lad_phi + (size_t)blob * kLadPos * 128;
-    phase_b_slow(&buckets[idx], L, Phi, items, lo, hi, a);
+    const size_t base = (size_t)blob * kLadPos * 128;
+    if constexpr (XZ)
+        phase_b_slow_xyzz(&buckets[idx], static_cast<const G1X *>(lad_pts) + base,
+                          static_cast<const G1X *>(lad_phi) + base, items, lo, hi, a);
+    else
+        phase_b_slow(&buckets[idx], static_cast<const G1A *>(lad_pts) + base,
+                     static_cast<const G1A *>(lad_phi) + base, items, lo, hi, a);
 }
 
 }  // namespace
 
-hipError_t launch_phase_b(G1X *buckets, uint8_t *flags, const G1A *lad_aff, const G1A *lad_phi,
+hipError_t launch_phase_b(G1X *buckets, uint8_t *flags, const void *lad_pts, const void *lad_phi,
                           const uint32_t *items, const uint32_t *offsets, uint32_t nblobs, uint32_t grid,
-                          bool quad, uint32_t *queue, uint32_t force_checked, hipStream_t s) {
+                          bool quad, bool xyzz, uint32_t *queue, uint32_t force_checked, hipStream_t s) {
+    if (xyzz && !quad) return hipErrorInvalidValue;
     // 64-lane groups: two per pair of blobs of a band (segments 0 and 1), one per blob
     const uint32_t rest = nblobs % kBand;
     const uint32_t ngroups = (nblobs / kBand) * kBuckets * 2 * kBand + kBuckets * (2 * ((rest + 1) / 2) + rest);
     const uint32_t nitems = ngroups * (quad ? 4u : 1u);  // one wave each
     if (grid > (nitems + 3) / 4) grid = (nitems + 3) / 4;
-    if (quad)
-        hipLaunchKernelGGL(k_phase_b<4>, dim3(grid), dim3(256), 0, s, buckets, flags, lad_aff, lad_phi, items,
-                           offsets, nblobs, queue, force_checked);
+    if (xyzz)
+        hipLaunchKernelGGL((k_phase_b<4, true>), dim3(grid), dim3(256), 0, s, buckets, flags, lad_pts, lad_phi,
+                           items, offsets, nblobs, queue, force_checked);
+    else if (quad)
+        hipLaunchKernelGGL((k_phase_b<4, false>), dim3(grid), dim3(256), 0, s, buckets, flags, lad_pts, lad_phi,
+                           items, offsets, nblobs, queue, force_checked);
     else
-        hipLaunchKernelGGL(k_phase_b<1>, dim3(grid), dim3(256), 0, s, buckets, flags, lad_aff, lad_phi, items,
-                           offsets, nblobs, queue, force_checked);
+        hipLaunchKernelGGL((k_phase_b<1, false>), dim3(grid), dim3(256), 0, s, buckets, flags, lad_pts, lad_phi,
+                           items, offsets, nblobs, queue, force_checked);
     const uint32_t nbuckets = nblobs * 128 * kBuckets;
-    hipLaunchKernelGGL(k_phase_b_fix, dim3(blocks_for(nbuckets, 256)), dim3(256), 0, s, buckets, flags,
-                       lad_aff, lad_phi, items, offsets, nbuckets);
+    if (xyzz)
+        hipLaunchKernelGGL(k_phase_b_fix<true>, dim3(blocks_for(nbuckets, 256)), dim3(256), 0, s, buckets, flags,
+                           lad_pts, lad_phi, items, offsets, nbuckets);
+    else
+        hipLaunchKernelGGL(k_phase_b_fix<false>, dim3(blocks_for(nbuckets, 256)), dim3(256), 0, s, buckets, flags,
+                           lad_pts, lad_phi, items, offsets, nbuckets);
     return hipGetLastError();
 }
 

@@ -542,13 +542,27 @@ void proofs_range(vkzg_prover *p, WorkSet &ws, uint32_t off, uint32_t nb, uint8_
     ck(launch_ladder(lad, tn.groups > 1 ? usum : u, tn.ladder_splits, nb,
                      ladder_lanes((uint64_t)nb * 128 * tn.ladder_splits, p->cus), s));
     clk.mark("ladder");
-    const uint32_t chunk = tn.norm_chunk ? tn.norm_chunk : fit_norm_chunk((uint64_t)nb * npos, p->cus);
-    ck(launch_jac_to_affine(lad_aff, lad, prefix, (uint64_t)nb * npos, chunk, s));
-    ck(launch_glv_phi(lad_phi, lad_aff, (uint64_t)nb * npos, ws.queue_b + off, ws.queue_r + off, s));
-    clk.mark("normalize");
-    ck(launch_phase_b(buckets, ws.bucket_flags + (size_t)off * 128 * kBuckets, lad_aff, lad_phi, p->kb_items,
-                      p->kb_offsets, nb, p->cus * 2, phase_b_quad(nb, p->cus), ws.queue_b + off,
-                      tn.force_checked, s));
+    // Small unsplit batches skip the normalisation: phase B's quads add the
+    // ladder points in XYZZ form, which take twice the bytes of affine ones
+    // and sit in the affine buffers' memory (so 2 nb <= the workset's blobs).
+    const bool quad_b = phase_b_quad(nb, p->cus);
+    static_assert(sizeof(G1X) <= 2 * sizeof(G1A), "XYZZ ladder points must fit the affine buffers");
+    const bool xyzz = quad_b && tn.splits == 1 && off == 0 && 2 * nb <= ws.cap;
+    if (xyzz) {
+        G1X *lx = reinterpret_cast<G1X *>(ws.lad_aff), *lpx = reinterpret_cast<G1X *>(ws.lad_phi);
+        ck(launch_ladder_xyzz(lx, lpx, lad, (uint64_t)nb * npos, ws.queue_b + off, ws.queue_r + off, s));
+        clk.mark("normalize");
+        ck(launch_phase_b(buckets, ws.bucket_flags + (size_t)off * 128 * kBuckets, lx, lpx, p->kb_items,
+                          p->kb_offsets, nb, p->cus * 2, true, true, ws.queue_b + off, tn.force_checked, s));
+    } else {
+        const uint32_t chunk = tn.norm_chunk ? tn.norm_chunk : fit_norm_chunk((uint64_t)nb * npos, p->cus);
+        ck(launch_jac_to_affine(lad_aff, lad, prefix, (uint64_t)nb * npos, chunk, s));
+        ck(launch_glv_phi(lad_phi, lad_aff, (uint64_t)nb * npos, ws.queue_b + off, ws.queue_r + off, s));
+        clk.mark("normalize");
+        ck(launch_phase_b(buckets, ws.bucket_flags + (size_t)off * 128 * kBuckets, lad_aff, lad_phi,
+                          p->kb_items, p->kb_offsets, nb, p->cus * 2, quad_b, false, ws.queue_b + off,
+                          tn.force_checked, s));
+    }
     clk.mark("phase_b");
     ck(launch_bucket_reduce(cvals, ws.cval_flags + off * 128, buckets, nb, tn.reduce_lanes, p->cus * 2,
                             ws.queue_r + off, tn.force_checked, s));
