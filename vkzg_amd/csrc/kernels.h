@@ -139,7 +139,10 @@ hipError_t launch_phase_b(G1X *buckets, uint8_t *flags, const void *lad_pts, con
 hipError_t launch_bucket_reduce(G1X *cvals, uint8_t *flags, const G1X *buckets, uint32_t nblobs,
                                 uint32_t lanes, uint32_t grid, uint32_t *queue, uint32_t force_checked,
                                 hipStream_t s);
-hipError_t launch_finalize(uint8_t *proofs, const G1X *cvals, uint32_t nblobs, hipStream_t s);
+// quad: four lanes share each output pair's additions (force_checked: their
+// checked recompute).
+hipError_t launch_finalize(uint8_t *proofs, const G1X *cvals, uint32_t nblobs, bool quad,
+                           uint32_t force_checked, hipStream_t s);
 
 // --------------------------------------------------------------- recovery
 // The four transforms of a recovery as five kernels (recover.hip), each

@@ -567,7 +567,7 @@ void proofs_range(vkzg_prover *p, WorkSet &ws, uint32_t off, uint32_t nb, uint8_
     ck(launch_bucket_reduce(cvals, ws.cval_flags + off * 128, buckets, nb, tn.reduce_lanes, p->cus * 2,
                             ws.queue_r + off, tn.force_checked, s));
     clk.mark("bucket_reduce");
-    ck(launch_finalize(d_proofs, cvals, nb, s));
+    ck(launch_finalize(d_proofs, cvals, nb, (uint64_t)nb * 256 <= (uint64_t)p->cus * 4 * 64, tn.force_checked, s));
     clk.mark("finalize");
 }
 

@@ -197,17 +197,41 @@ VKZG_DEV void g1x_pair_to_affine(G1A &ra, G1A &rb, const G1X &a, const G1X &b) {
 // half comes from its two sub-convolutions: C+[a] = Yc[a] + Yn[a] and
 // C+[a+32] = Yc[a] - Yn[a] for a < 32 (cvals[..][a] = Yc[a],
 // cvals[..][32+a] = Yn[a], cvals[..][64+a] = C-[a]).
-__global__ void __launch_bounds__(64) k_finalize(uint8_t *proofs, const G1X *cvals, uint32_t nblobs) {
+//
+// Q = 4 (while the quads fit one wave per SIMD): the four lanes of a quad
+// share the three additions (g1x_add_quad, four rounds of products each);
+// a flagged doubling or cancellation redoes them checked, uniformly in the
+// quad.  The inversion is the same on every lane of the quad.
+template <uint32_t Q>
+__global__ void __launch_bounds__(64 * Q) k_finalize(uint8_t *proofs, const G1X *cvals, uint32_t nblobs,
+                                                     uint32_t force_checked) {
     __builtin_amdgcn_s_setprio(3);  // dependent inversion chain: issue first
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t g = tid / Q, qd = tid % Q;
     if (g >= nblobs * 64) return;
     const uint32_t blob = g / 64, a = g % 64;
     const G1X yc = cvals[blob * 128 + (a & 31u)];
     const G1X yn = cvals[blob * 128 + 32 + (a & 31u)];
-    const G1X cp = g1x_add(yc, a < 32 ? yn : g1x_neg(yn));
     const G1X cm = cvals[blob * 128 + a + 64];
+    G1X s0, s1;
+    if constexpr (Q == 4) {
+        bool exc = force_checked != 0;
+        const G1X cp = g1x_add_quad(yc, a < 32 ? yn : g1x_neg(yn), qd, exc);
+        s0 = g1x_add_quad(cp, cm, qd, exc);
+        s1 = g1x_add_quad(cp, g1x_neg(cm), qd, exc);
+        if (exc) {
+            const G1X cc = g1x_add(yc, a < 32 ? yn : g1x_neg(yn));
+            s0 = g1x_add(cc, cm);
+            s1 = g1x_add(cc, g1x_neg(cm));
+        }
+    } else {
+        const G1X cp = g1x_add(yc, a < 32 ? yn : g1x_neg(yn));
+        s0 = g1x_add(cp, cm);
+        s1 = g1x_add(cp, g1x_neg(cm));
+    }
+    if (qd != 0) return;  // the inversion and the stores: one lane of the quad
     G1A p0, p1;
-    g1x_pair_to_affine(p0, p1, g1x_add(cp, cm), g1x_add(cp, g1x_neg(cm)));
+    g1x_pair_to_affine(p0, p1, s0, s1);
     uint32_t *dst = reinterpret_cast<uint32_t *>(proofs + 6144ull * blob);
     uint32_t w[12];
     g1a_compress_words(w, p0);
@@ -255,9 +279,14 @@ hipError_t launch_bucket_reduce(G1X *cvals, uint8_t *flags, const G1X *buckets, 
     return hipGetLastError();
 }
 
-hipError_t launch_finalize(uint8_t *proofs, const G1X *cvals, uint32_t nblobs, hipStream_t s) {
-    hipLaunchKernelGGL(k_finalize, dim3(blocks_for(nblobs * 64ull, 64)), dim3(64), 0, s, proofs,
-                       cvals, nblobs);
+hipError_t launch_finalize(uint8_t *proofs, const G1X *cvals, uint32_t nblobs, bool quad,
+                           uint32_t force_checked, hipStream_t s) {
+    if (quad)
+        hipLaunchKernelGGL(k_finalize<4>, dim3(blocks_for(nblobs * 256ull, 256)), dim3(256), 0, s, proofs,
+                           cvals, nblobs, force_checked);
+    else
+        hipLaunchKernelGGL(k_finalize<1>, dim3(blocks_for(nblobs * 64ull, 64)), dim3(64), 0, s, proofs,
+                           cvals, nblobs, force_checked);
     return hipGetLastError();
 }
 
